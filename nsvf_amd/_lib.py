@@ -1,0 +1,64 @@
+"""ctypes binding of ``libnsvf_amd.so`` (the C ABI declared in
+``include/nsvf_amd.h``).
+
+The library is built in-tree by ``python -m nsvf_amd.build``.  There is no
+fallback: if the library is missing or lacks a symbol, importing an op raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnsvf_amd.so")
+
+_lock = threading.Lock()
+_LIB = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+
+# name -> (argtypes, restype); every entry point declared in include/nsvf_amd.h
+SIGNATURES = {
+    "nsvf_last_error": ([], ctypes.c_char_p),
+    "nsvf_version": ([], _I),
+    "nsvf_aabb_intersect": ([_P, _P, _P, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
+    "nsvf_svo_intersect": ([_P, _P, _P, _P, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
+    "nsvf_build_octree": ([_P, _P, _I, _I, ctypes.POINTER(_P), ctypes.POINTER(_I)], _I),
+    "nsvf_octree_take": ([_P, _P, _P], _I),
+    "nsvf_inverse_cdf_sampling": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P, _P], _I),
+    "nsvf_inverse_cdf_sampling_rows": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P, _P], _I),
+    "nsvf_uniform_ray_sampling": ([_P, _P, _P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P, _P], _I),
+}
+
+
+class NSVFError(RuntimeError):
+    """Error returned by the native library (bad argument or HIP failure)."""
+
+
+def lib():
+    """Load the shared library once; raise if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    with _lock:
+        if _LIB is not None:
+            return _LIB
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} not built; run `python -m nsvf_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (argtypes, restype) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argtypes
+            fn.restype = restype
+        _LIB = L
+        return L
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        msg = lib().nsvf_last_error()
+        raise NSVFError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

@@ -1,0 +1,559 @@
+"""CPU oracle for the NSVF sparse-voxel rendering path.
+
+TEST INFRASTRUCTURE ONLY.  Tests, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` import this module to check (or time) the
+HIP library; no product code path imports it.
+
+Two layers:
+
+* FP32 restatements, bit-for-bit in intent with the reference:
+  - the four native ops and the octree builder, in C++ (``cpu_ref.cpp``,
+    loaded here through ctypes);
+  - the Python glue around them, in NumPy float32 with a fixed evaluation
+    order (left-to-right sums, stable sorts).
+* FP64 restatements of the continuous parts (trilinear gather, field MLP,
+  compositing) used to state the error of every floating-point output.
+
+Reference files restated (line numbers in each docstring):
+  fairnr/clib/__init__.py, fairnr/clib/src/*.cu|*.cpp, fairnr/data/geometry.py,
+  fairnr/modules/encoder.py, fairnr/modules/renderer.py, fairnr/models/nsvf.py,
+  fairnr/models/nerf.py, fairnr/models/fairnr_model.py.
+
+Choices where the reference leaves the result to a library's internal order:
+  * ``sort`` of hit depths is stable (ties keep slot order);
+  * ``sum`` of segment lengths and of compositing terms runs left to right;
+  * the corner sum of the gather runs over corners 0..7;
+  * ``cumsum`` of free energy runs left to right;
+  * the octree residual is the float64 mean of (p - coord*h), rounded to fp32.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+MAX_DEPTH = 10000.0          # fairnr/clib/__init__.py:36
+FREE_ENERGY_SCALE = 7.0      # fairnr/modules/renderer.py:121
+BG_DEPTH = 5.0               # fairnr/modules/implicit.py:19 (background_depth)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    """Load ``liboracle_cpu.so`` (built by ``make -C oracle``)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = os.path.join(_HERE, "liboracle_cpu.so")
+    if not os.path.exists(path):
+        raise RuntimeError("oracle library missing: run `make -C oracle`")
+    L = ctypes.CDLL(path)
+    P = ctypes.c_void_p
+    i, f = ctypes.c_int, ctypes.c_float
+    L.orc_aabb_intersect.argtypes = [i, i, i, P, P, P, f, i, P, P, P]
+    L.orc_aabb_intersect.restype = None
+    L.orc_svo_intersect.argtypes = [i, i, i, P, P, P, P, f, i, P, P, P]
+    L.orc_svo_intersect.restype = i
+    L.orc_octree_build.argtypes = [P, P, i, i]
+    L.orc_octree_build.restype = P
+    L.orc_octree_num_nodes.argtypes = [P]
+    L.orc_octree_num_nodes.restype = i
+    L.orc_octree_num_leaves.argtypes = [P]
+    L.orc_octree_num_leaves.restype = i
+    L.orc_octree_copy.argtypes = [P, P, P]
+    L.orc_octree_copy.restype = None
+    L.orc_octree_free.argtypes = [P]
+    L.orc_octree_free.restype = None
+    L.orc_inverse_cdf_sampling.argtypes = [i, i, i, i, f, P, P, P, P, P, P, i, P, P, P, P]
+    L.orc_inverse_cdf_sampling.restype = None
+    L.orc_uniform_ray_sampling.argtypes = [i, i, i, i, f, P, P, P, P, P, P, P, P]
+    L.orc_uniform_ray_sampling.restype = None
+    _LIB = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------------------------------------------------------------------
+# Native ops, op level (the `_ext` boundary: fairnr/clib/src/binding.cpp:11-21)
+
+def aabb_intersect(ray_start, ray_dir, points, voxelsize, n_max):
+    """``_ext.aabb_intersect``: intersect.cpp:49-75, intersect_gpu.cu:125-167.
+
+    ray_start, ray_dir: [B,M,3] f32; points: [B,N,3] f32.
+    Returns idx [B,M,n_max] i32 (-1 pad), min/max depth f32 (0 pad)."""
+    rs, rd, pts = _c(ray_start, np.float32), _c(ray_dir, np.float32), _c(points, np.float32)
+    B, M = rs.shape[:2]
+    N = pts.shape[1]
+    n_max = int(n_max)
+    idx = np.zeros((B, M, n_max), np.int32)
+    lo = np.zeros((B, M, n_max), np.float32)
+    hi = np.zeros((B, M, n_max), np.float32)
+    lib().orc_aabb_intersect(B, M, N, _p(rs), _p(rd), _p(pts), float(voxelsize), n_max,
+                             _p(idx), _p(lo), _p(hi))
+    return idx, lo, hi
+
+
+def svo_intersect(ray_start, ray_dir, points, children, voxelsize, n_max):
+    """``_ext.svo_intersect``: intersect.cpp:84-112, intersect_gpu.cu:170-237.
+
+    points: [B,T,3] f32 node centres; children: [B,T,9] i32.  Root is node T-1."""
+    rs, rd = _c(ray_start, np.float32), _c(ray_dir, np.float32)
+    pts, ch = _c(points, np.float32), _c(children, np.int32)
+    B, M = rs.shape[:2]
+    T = pts.shape[1]
+    n_max = int(n_max)
+    idx = np.zeros((B, M, n_max), np.int32)
+    lo = np.zeros((B, M, n_max), np.float32)
+    hi = np.zeros((B, M, n_max), np.float32)
+    ovf = lib().orc_svo_intersect(B, M, T, _p(rs), _p(rd), _p(pts), _p(ch), float(voxelsize),
+                                  n_max, _p(idx), _p(lo), _p(hi))
+    if ovf:
+        raise RuntimeError(f"svo_intersect: DFS stack exceeded 256 entries on {ovf} rays")
+    return idx, lo, hi
+
+
+def build_octree(center, points, depth):
+    """``_ext.build_octree``: octree.cpp:125-136.
+
+    center: [3] int; points: [N,3] int (half-voxel units).  Returns
+    centers [T,3] i32 and children [T,9] i32 (-1 = none, [8] = 1<<(depth+1))."""
+    c = _c(np.asarray(center).reshape(3), np.int64)
+    pts = _c(points, np.int64).reshape(-1, 3)
+    if int(depth) < 0:
+        raise RuntimeError("build_octree: depth must be >= 0")
+    L = lib()
+    h = L.orc_octree_build(_p(c), _p(pts), pts.shape[0], int(depth))
+    try:
+        T = L.orc_octree_num_nodes(h)
+        centers = np.zeros((T, 3), np.int32)
+        children = np.zeros((T, 9), np.int32)
+        L.orc_octree_copy(h, _p(centers), _p(children))
+    finally:
+        L.orc_octree_free(h)
+    return centers, children
+
+
+def inverse_cdf_sampling_op(pts_idx, min_depth, max_depth, noise, probs, steps,
+                            fixed_step_size, compat=True):
+    """``_ext.inverse_cdf_sampling``: sample.cpp:58-95, sample_gpu.cu:108-202.
+
+    Inputs [B,R,P]; steps [B,R]; noise [B,R,S].  Outputs [B,R,S] with idx -1,
+    depth 0 and dists 0 in unused slots.  Returns (idx, depth, dists, flags)."""
+    pi = _c(pts_idx, np.int32)
+    mn, mx = _c(min_depth, np.float32), _c(max_depth, np.float32)
+    nz, pr, st = _c(noise, np.float32), _c(probs, np.float32), _c(steps, np.float32)
+    B, R, P = pi.shape
+    S = nz.shape[-1]
+    oi = np.full((B, R, S), -1, np.int32)
+    od = np.zeros((B, R, S), np.float32)
+    ot = np.zeros((B, R, S), np.float32)
+    flags = np.zeros(4, np.int64)
+    lib().orc_inverse_cdf_sampling(B, R, P, S, float(fixed_step_size), _p(pi), _p(mn), _p(mx),
+                                   _p(nz), _p(pr), _p(st), int(bool(compat)), _p(oi), _p(od),
+                                   _p(ot), _p(flags))
+    return oi, od, ot, flags
+
+
+def uniform_ray_sampling_op(pts_idx, min_depth, max_depth, noise, step_size, max_steps):
+    """``_ext.uniform_ray_sampling``: sample.cpp:23-55, sample_gpu.cu:15-106.
+
+    Returns (idx, depth, dists, flags); outputs [B,R,max_steps]."""
+    pi = _c(pts_idx, np.int32)
+    mn, mx = _c(min_depth, np.float32), _c(max_depth, np.float32)
+    nz = _c(noise, np.float32)
+    B, R, P = pi.shape
+    S = int(max_steps)
+    assert nz.shape == (B, R, S)
+    oi = np.full((B, R, S), -1, np.int32)
+    od = np.zeros((B, R, S), np.float32)
+    ot = np.zeros((B, R, S), np.float32)
+    flags = np.zeros(4, np.int64)
+    lib().orc_uniform_ray_sampling(B, R, P, S, float(step_size), _p(pi), _p(mn), _p(mx), _p(nz),
+                                   _p(oi), _p(od), _p(ot), _p(flags))
+    return oi, od, ot, flags
+
+
+# ---------------------------------------------------------------------------
+# Python-level wrappers (fairnr/clib/__init__.py)
+
+def aabb_ray_intersect(voxelsize, n_max, points, ray_start, ray_dir):
+    """``aabb_ray_intersect``: clib/__init__.py:58-95.  The wrapper's regrouping
+    (G blocks of K rays, repeated pad rays, trimmed afterwards) does not change
+    any ray's result, so the op runs once over [S, N] rays."""
+    return aabb_intersect(ray_start, ray_dir, points, voxelsize, n_max)
+
+
+def svo_ray_intersect(voxelsize, n_max, points, children, ray_start, ray_dir):
+    """``svo_ray_intersect``: clib/__init__.py:98-135 (same note as above)."""
+    return svo_intersect(ray_start, ray_dir, points, children, voxelsize, n_max)
+
+
+def inverse_cdf_sampling(pts_idx, min_depth, max_depth, probs, steps, fixed_step_size=-1.0,
+                         deterministic=True, noise=None, compat=True):
+    """``inverse_cdf_sampling``: clib/__init__.py:231-300.
+
+    Rows are padded to a multiple of G=200 with copies of row 0, regrouped to
+    [G, H/G, P] and processed 800 columns (4*G) per op call; the results are
+    concatenated, trimmed back to N rows and to the longest row.  In compat
+    mode the grouping decides which ray is "ray 0 of the row" and "the next
+    ray" for the reference's trailing-loop quirks.
+
+    noise: [N, max_steps] (only when not deterministic).  Returns
+    (idx [N,L] i32, depth [N,L] f32, dists [N,L] f32, max_steps, flags)."""
+    pts_idx = _c(pts_idx, np.int32)
+    N, P = pts_idx.shape
+    G = 200
+    H = int(math.ceil(N / G)) * G
+    arrs = [pts_idx, _c(min_depth, np.float32), _c(max_depth, np.float32), _c(probs, np.float32)]
+    st = _c(steps, np.float32)
+    if H > N:
+        arrs = [np.concatenate([a, np.repeat(a[:1], H - N, 0)], 0) for a in arrs]
+        st = np.concatenate([st, np.repeat(st[:1], H - N, 0)], 0)
+    Kc = H // G
+    arrs = [a.reshape(G, Kc, P) for a in arrs]
+    st = st.reshape(G, Kc)
+    max_steps = int(np.ceil(st).astype(np.int64).max()) + P
+    if deterministic:
+        nz = np.full((G, Kc, max_steps), 0.5, np.float32)
+    else:
+        assert noise is not None and noise.shape[1] >= max_steps
+        nz_full = _c(noise[:, :max_steps], np.float32)
+        if H > N:
+            nz_full = np.concatenate([nz_full, np.repeat(nz_full[:1], H - N, 0)], 0)
+        nz = nz_full.reshape(G, Kc, max_steps)
+    chunk = 4 * G
+    outs = []
+    flags = np.zeros(4, np.int64)
+    for i in range(0, Kc, chunk):
+        sl = slice(i, i + chunk)
+        oi, od, ot, fl = inverse_cdf_sampling_op(arrs[0][:, sl], arrs[1][:, sl], arrs[2][:, sl],
+                                                 nz[:, sl], arrs[3][:, sl], st[:, sl],
+                                                 fixed_step_size, compat)
+        outs.append((oi, od, ot))
+        flags += fl
+    si = np.concatenate([o[0] for o in outs], 1).reshape(H, -1)[:N]
+    sd = np.concatenate([o[1] for o in outs], 1).reshape(H, -1)[:N]
+    sx = np.concatenate([o[2] for o in outs], 1).reshape(H, -1)[:N]
+    max_len = int((si != -1).sum(-1).max()) if N > 0 else 0
+    return si[:, :max_len], sd[:, :max_len], sx[:, :max_len], max_steps, flags
+
+
+def uniform_ray_sampling(pts_idx, min_depth, max_depth, step_size, max_ray_length,
+                         deterministic=True, noise=None):
+    """``uniform_ray_sampling``: clib/__init__.py:178-228.
+
+    max_steps = int(max_ray_length / step_size) + 2P.  The G=256 regrouping is
+    contiguous, so every ray sees the same neighbours as in one [1,N,P] call
+    and the op runs once.  Returns (idx, depth, dists, max_steps, flags)."""
+    pts_idx = _c(pts_idx, np.int32)
+    N, P = pts_idx.shape
+    max_steps = int(max_ray_length / step_size) + P * 2
+    if deterministic:
+        nz = np.full((1, N, max_steps), 0.5, np.float32)
+    else:
+        nz = _c(noise[:, :max_steps], np.float32)[None]
+    oi, od, ot, fl = uniform_ray_sampling_op(pts_idx[None], min_depth[None], max_depth[None], nz,
+                                             step_size, max_steps)
+    oi, od, ot = oi[0], od[0], ot[0]
+    max_len = int((oi != -1).sum(-1).max()) if N > 0 else 0
+    return oi[:, :max_len], od[:, :max_len], ot[:, :max_len], max_steps, fl
+
+
+# ---------------------------------------------------------------------------
+# Geometry helpers (fairnr/data/geometry.py)
+
+def offset_points(point_xyz, quarter_voxel=1.0, offset_only=False, bits=2):
+    """geometry.py:229-238: lattice offsets (2i+1-bits)/(bits-1) * quarter_voxel,
+    i-major (x slowest, z fastest: meshgrid 'ij').  Integer inputs give float32
+    outputs with integer offsets, as the reference's type_as(long) does."""
+    c = np.arange(1, 2 * bits, 2)
+    ox, oy, oz = np.meshgrid(c, c, c, indexing="ij")
+    off_i = np.stack([ox.reshape(-1), oy.reshape(-1), oz.reshape(-1)], 1) - bits
+    p = np.asarray(point_xyz)
+    if p.dtype.kind in "iu":
+        off = (off_i / float(bits - 1)).astype(np.int64).astype(np.float32) * np.float32(quarter_voxel)
+        if offset_only:
+            return off
+        return p[:, None, :].astype(np.float32) + off[None]
+    dt = p.dtype.type
+    off = (off_i.astype(p.dtype) / dt(bits - 1)) * dt(quarter_voxel)
+    if offset_only:
+        return off
+    return p[:, None, :] + off[None]
+
+
+def discretize_points(voxel_points, voxel_size):
+    """geometry.py:241-247: integer coordinates relative to the minimum point,
+    and the mean residual (float64 mean, rounded to fp32)."""
+    vp = np.asarray(voxel_points, np.float32)
+    vs = np.float32(voxel_size)
+    mn = vp.min(0, keepdims=True)
+    coords = np.round((vp - mn) / vs).astype(np.int64)
+    residual = (vp.astype(np.float64) - (coords.astype(np.float32) * vs).astype(np.float64)).mean(0, keepdims=True)
+    return coords, residual.astype(np.float32)
+
+
+def build_easy_octree(points, half_voxel):
+    """geometry.py:320-327: octree over voxel centres in half-voxel units.
+
+    depth = ceil(log2(max range)) - 1 (0 for a single voxel, where the
+    reference is undefined); centre = (max + min) / 2."""
+    coords, residual = discretize_points(points, half_voxel)
+    ranges = coords.max(0) - coords.min(0)
+    rmax = int(ranges.max())
+    depth = int(math.ceil(math.log2(rmax))) - 1 if rmax > 0 else 0
+    depth = max(depth, 0)
+    center = (coords.max(0) + coords.min(0)) // 2
+    centers, children = build_octree(center, coords, depth)
+    centers_f = centers.astype(np.float32) * np.float32(half_voxel) + residual
+    return centers_f.astype(np.float32), children
+
+
+def voxel_keys(points, voxel_size):
+    """encoder.py:269-276: corner keys of every voxel, unique and sorted, and
+    each voxel's 8 corner ids (corner order = offset_points order)."""
+    coords, _ = discretize_points(points, np.float32(voxel_size) * np.float32(0.5))
+    keys0 = offset_points(coords, 1.0).reshape(-1, 3).astype(np.int64)
+    keys, inv = np.unique(keys0, axis=0, return_inverse=True)
+    return keys, inv.reshape(-1, 8).astype(np.int64)
+
+
+def shift_centers(points, voxel_size):
+    """encoder.py:383: x += voxel_size / 10 (fp32)."""
+    p = np.array(points, np.float32, copy=True)
+    p[:, 0] += np.float32(voxel_size) / np.float32(10)
+    return p
+
+
+# ---------------------------------------------------------------------------
+# Post-intersection glue (encoder.py:518-535, nsvf.py:65-74)
+
+def sort_hits(idx, min_depth, max_depth):
+    """encoder.py:519-524: pads -> MAX_DEPTH, stable sort by entry depth."""
+    idx = np.asarray(idx, np.int32)
+    pad = idx == -1
+    mn = np.where(pad, np.float32(MAX_DEPTH), min_depth).astype(np.float32)
+    mx = np.where(pad, np.float32(MAX_DEPTH), max_depth).astype(np.float32)
+    order = np.argsort(mn, axis=-1, kind="stable")
+    mn = np.take_along_axis(mn, order, -1)
+    mx = np.take_along_axis(mx, order, -1)
+    idx = np.take_along_axis(idx, order, -1)
+    hits = (idx != -1).any(-1)
+    return idx, mn, mx, hits
+
+
+def probs_steps(idx, min_depth, max_depth, step_size):
+    """nsvf.py:68-74: probs = seg / sum(seg), steps = sum(seg) / step_size, in
+    fp32 with a left-to-right sum."""
+    d = np.where(idx == -1, np.float32(0), (max_depth - min_depth).astype(np.float32)).astype(np.float32)
+    tot = np.zeros(d.shape[:-1], np.float32)
+    for k in range(d.shape[-1]):
+        tot = (tot + d[..., k]).astype(np.float32)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        probs = (d / tot[..., None]).astype(np.float32)
+        steps = (tot / np.float32(step_size)).astype(np.float32)
+    return probs, steps
+
+
+def sample_tail(idx, depth, dists):
+    """encoder.py:547-549: clamp dists at 0; pads -> MAX_DEPTH depth, 0 dist."""
+    dists = np.maximum(dists, np.float32(0)).astype(np.float32)
+    pad = idx == -1
+    depth = np.where(pad, np.float32(MAX_DEPTH), depth).astype(np.float32)
+    dists = np.where(pad, np.float32(0), dists).astype(np.float32)
+    return depth, dists
+
+
+# ---------------------------------------------------------------------------
+# Trilinear gather (encoder.py:582-590, geometry.py:195-200)
+
+def trilinear_gather(sample_idx, xyz, centers, feats, values, voxel_size, dtype=np.float64):
+    """emb = sum_c w_c * values[feats[idx][c]], p = (x - centre)/vs + 0.5,
+    w_c = prod_axis (p if bit else 1-p), corner c = 4x + 2y + z.
+
+    Evaluated in ``dtype`` (float64 for the oracle; float32 restates the
+    reference's FP32 arithmetic with the corner sum in corner order)."""
+    dt = np.dtype(dtype).type
+    idx = np.asarray(sample_idx, np.int64)
+    x = np.asarray(xyz, dtype)
+    c = np.asarray(centers, dtype)[idx]
+    p = ((x - c) / dt(voxel_size) + dt(0.5)).astype(dtype)
+    E = np.asarray(values, dtype)[np.asarray(feats)[idx]]          # [M,8,D]
+    out = np.zeros((x.shape[0], E.shape[-1]), dtype)
+    for corner in range(8):
+        bx, by, bz = (corner >> 2) & 1, (corner >> 1) & 1, corner & 1
+        wx = p[:, 0] if bx else (dt(1) - p[:, 0])
+        wy = p[:, 1] if by else (dt(1) - p[:, 1])
+        wz = p[:, 2] if bz else (dt(1) - p[:, 2])
+        w = ((wx * wy).astype(dtype) * wz).astype(dtype)
+        out = (out + (w[:, None] * E[:, corner]).astype(dtype)).astype(dtype)
+    return out
+
+
+def trilinear_weights(sample_idx, xyz, centers, voxel_size, dtype=np.float64):
+    """Per-sample corner weights [M,8] (same formula as trilinear_gather)."""
+    dt = np.dtype(dtype).type
+    idx = np.asarray(sample_idx, np.int64)
+    x = np.asarray(xyz, dtype)
+    c = np.asarray(centers, dtype)[idx]
+    p = ((x - c) / dt(voxel_size) + dt(0.5)).astype(dtype)
+    W = np.zeros((x.shape[0], 8), dtype)
+    for corner in range(8):
+        bx, by, bz = (corner >> 2) & 1, (corner >> 1) & 1, corner & 1
+        wx = p[:, 0] if bx else (dt(1) - p[:, 0])
+        wy = p[:, 1] if by else (dt(1) - p[:, 1])
+        wz = p[:, 2] if bz else (dt(1) - p[:, 2])
+        W[:, corner] = (wx * wy) * wz
+    return W
+
+
+# ---------------------------------------------------------------------------
+# Field MLP (the configs' own MLPs: L hidden ReLU layers, then a 4-wide head
+# giving sigma and RGB logits; RGB through a sigmoid as with min_color=0 in
+# fairnr/modules/field.py:275-276)
+
+@dataclass
+class MLPWeights:
+    weights: list   # [out, in] matrices, hidden layers then head
+    biases: list
+
+
+def mlp_forward(x, w: MLPWeights, dtype=np.float64):
+    h = np.asarray(x, dtype)
+    n = len(w.weights)
+    for i, (W, b) in enumerate(zip(w.weights, w.biases)):
+        h = h @ np.asarray(W, dtype).T + np.asarray(b, dtype)
+        if i < n - 1:
+            h = np.maximum(h, 0)
+    sigma = h[:, 0]
+    rgb = 1.0 / (1.0 + np.exp(-h[:, 1:4]))
+    return sigma.astype(dtype), rgb.astype(dtype)
+
+
+# ---------------------------------------------------------------------------
+# Compositing (renderer.py:117-124, :193-218) and background (nsvf.py:94-101)
+
+def composite(sigma, rgb, depth, dists, idx, dtype=np.float64):
+    """Front-to-back compositing of padded [R,S] samples.
+
+    fe = relu(sigma) * dist * 7 (0 at pads); T_k = exp(-sum_{j<k} fe_j);
+    p_k = (1 - exp(-fe_k)) T_k; depth = sum p z; missed = 1 - sum p;
+    rgb = sum p c.  Returns (rgb [R,3], depth [R], missed [R], probs [R,S])."""
+    sigma = np.asarray(sigma, dtype)
+    valid = np.asarray(idx) != -1
+    fe = np.where(valid, np.maximum(sigma, 0) * np.asarray(dists, dtype) * FREE_ENERGY_SCALE, 0).astype(dtype)
+    shifted = np.concatenate([np.zeros_like(fe[:, :1]), fe[:, :-1]], 1)
+    a = 1 - np.exp(-fe)
+    T = np.exp(-np.cumsum(shifted, 1))
+    probs = a * T
+    z = np.asarray(depth, dtype)
+    out_depth = (z * probs).sum(-1)
+    missed = 1 - probs.sum(-1)
+    col = (np.where(valid[..., None], np.asarray(rgb, dtype), 0) * probs[..., None]).sum(1)
+    return col, out_depth, missed, probs
+
+
+def fill_background(hits, rgb, depth, missed, bg=(1.0, 1.0, 1.0), bg_depth=BG_DEPTH, dtype=np.float64):
+    """nsvf.py:89-101 with geometry.py:303-317: scatter hit rays back to the
+    full frame (missed=1, rgb=0, depth=0 elsewhere), then add the background."""
+    R = hits.shape[0]
+    full_rgb = np.zeros((R, 3), dtype)
+    full_depth = np.zeros(R, dtype)
+    full_missed = np.ones(R, dtype)
+    full_rgb[hits] = rgb
+    full_depth[hits] = depth
+    full_missed[hits] = missed
+    full_rgb = full_rgb + full_missed[:, None] * np.asarray(bg, dtype)[None]
+    full_depth = full_depth + full_missed * bg_depth
+    return full_rgb, full_depth, full_missed
+
+
+# ---------------------------------------------------------------------------
+# Pruning (encoder.py:605-654) and splitting (geometry.py:250-274)
+
+def prune_lattice(points, voxel_size, bits=16):
+    """encoder.py:629-631: bits^3 points per voxel at offsets
+    (2i+1-bits)/(bits-1) * vs/2 (corners included)."""
+    return offset_points(np.asarray(points, np.float32), np.float32(voxel_size) / np.float32(2), bits=bits)
+
+
+def split_voxels(points, feats, values, voxel_size):
+    """geometry.py:250-274: 8 children per voxel at +-vs/4, new unique sorted
+    corner keys, new values by trilinear interpolation of the parent corners.
+    The parent that supplies each new key is the lowest parent index among
+    those sharing it (the reference's scatter_ leaves this to the device)."""
+    half_voxel = np.float32(voxel_size) * np.float32(0.5)
+    quarter = half_voxel * np.float32(0.5)
+    pts = np.asarray(points, np.float32)
+    new_points = offset_points(pts, quarter).reshape(-1, 3).astype(np.float32)
+    old_coords, _ = discretize_points(pts, quarter)
+    new_coords = offset_points(old_coords, 1.0).reshape(-1, 3).astype(np.int64)
+    new_keys0 = offset_points(new_coords, 1.0).reshape(-1, 3).astype(np.int64)
+    new_keys, inv = np.unique(new_keys0, axis=0, return_inverse=True)
+    inv = inv.reshape(-1)
+    parent = np.arange(new_keys0.shape[0]) // 64
+    key_parent = np.full(new_keys.shape[0], np.iinfo(np.int64).max, np.int64)
+    np.minimum.at(key_parent, inv, parent)
+    new_feats = inv.reshape(-1, 8).astype(np.int64)
+    new_values = None
+    if values is not None:
+        p = (new_keys - old_coords[key_parent]).astype(np.float32)[:, None, :] * np.float32(0.25) + np.float32(0.5)
+        p = p[:, 0, :]
+        E = np.asarray(values, np.float64)[np.asarray(feats)[key_parent]]
+        out = np.zeros((new_keys.shape[0], E.shape[-1]), np.float64)
+        pp = p.astype(np.float64)
+        for corner in range(8):
+            bx, by, bz = (corner >> 2) & 1, (corner >> 1) & 1, corner & 1
+            w = (pp[:, 0] if bx else 1 - pp[:, 0]) * (pp[:, 1] if by else 1 - pp[:, 1]) * (pp[:, 2] if bz else 1 - pp[:, 2])
+            out += w[:, None] * E[:, corner]
+        new_values = out
+    return new_points, new_feats, new_values, new_keys
+
+
+# ---------------------------------------------------------------------------
+# Grazing rays: hit sets that an FP64 slab test would decide differently
+
+def aabb_intersect_f64(ray_start, ray_dir, points, voxelsize, n_max):
+    """FP64 brute-force slab test (same ternaries), for counting rays whose FP32
+    hit set differs because they graze a voxel face.  Vectorised over voxels,
+    looped over rays: for small cases only."""
+    rs = np.asarray(ray_start, np.float64).reshape(-1, 3)
+    rd = np.asarray(ray_dir, np.float64).reshape(-1, 3)
+    pts = np.asarray(points, np.float64).reshape(-1, 3)
+    h = float(voxelsize) * 0.5
+    R = rs.shape[0]
+    idx = np.full((R, n_max), -1, np.int32)
+    lo_out = np.zeros((R, n_max))
+    hi_out = np.zeros((R, n_max))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for r in range(R):
+            inv = 1.0 / rd[r]
+            f_low = np.zeros(pts.shape[0])
+            f_high = np.full(pts.shape[0], 1e5)
+            alive = np.ones(pts.shape[0], bool)
+            for a in range(3):
+                lo = (pts[:, a] - h - rs[r, a]) * inv[a]
+                hi = (pts[:, a] + h - rs[r, a]) * inv[a]
+                sw = hi < lo
+                lo, hi = np.where(sw, hi, lo), np.where(sw, lo, hi)
+                alive &= ~(hi < f_low)
+                alive &= ~(lo > f_high)
+                f_low = np.where(lo > f_low, lo, f_low)
+                f_high = np.where(hi < f_high, hi, f_high)
+                alive &= ~(f_low > f_high)
+            ks = np.nonzero(alive)[0][:n_max]
+            idx[r, :len(ks)] = ks
+            lo_out[r, :len(ks)] = f_low[ks]
+            hi_out[r, :len(ks)] = f_high[ks]
+    return idx, lo_out, hi_out

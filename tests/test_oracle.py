@@ -1,0 +1,447 @@
+"""Pins for the CPU oracle: closed forms, hand-computed cases, invariants and
+brute force.  The reference ships no tests or golden vectors, so these are
+what ties the oracle to the reference's algorithm; each targets a plausible
+transcription mistake (wrong octant bit order, BFS numbering, missing tail
+sample, wrong trailing-loop stop, swapped slab bounds, dropped x7, ...)."""
+import math
+
+import numpy as np
+import pytest
+
+import nsvf_oracle as O
+from _cases import axis_rays, random_rays, shell, shell_octree, shell_rays
+
+
+# ---------------------------------------------------------------------------
+# Slab test / intersection
+
+def dyadic_grid(res=8):
+    vs = np.float32(2.0 / res)
+    c = (np.float32(-1) + vs * (np.arange(res, dtype=np.float32) + np.float32(0.5)))
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    return np.stack([X, Y, Z], -1).reshape(-1, 3).astype(np.float32), vs
+
+
+def test_axis_ray_closed_form():
+    pts, vs = dyadic_grid(8)
+    keep = np.random.default_rng(0).random(len(pts)) < 0.6
+    pts = pts[keep]
+    y0, z0 = np.float32(-0.375 + 0.05), np.float32(0.125 + 0.03)
+    rs = np.array([[[-3.0, y0, z0]]], np.float32)
+    rd = np.array([[[1.0, 0.0, 0.0]]], np.float32)
+    idx, lo, hi = O.aabb_intersect(rs, rd, pts[None], vs, 60)
+    h = vs / 2
+    row = np.nonzero((np.abs(pts[:, 1] - y0) < h) & (np.abs(pts[:, 2] - z0) < h))[0]
+    n = len(row)
+    assert n > 0
+    assert list(idx[0, 0, :n]) == list(row)            # voxel-index order
+    assert (idx[0, 0, n:] == -1).all()
+    assert np.array_equal(lo[0, 0, :n], pts[row, 0] - h + np.float32(3))   # exact (dyadic)
+    assert np.array_equal(hi[0, 0, :n], pts[row, 0] + h + np.float32(3))
+    assert (lo[0, 0, n:] == 0).all() and (hi[0, 0, n:] == 0).all()
+
+
+def test_ray_on_face_plane_hits_both_rows():
+    """Direction 0 on an axis and origin exactly on a face plane: that axis
+    gives NaN bounds and is skipped, so both adjacent rows are hit."""
+    pts, vs = dyadic_grid(8)
+    y_plane = np.float32(0.0)   # the face between rows 3 and 4
+    rs = np.array([[[-3.0, y_plane, 0.1]]], np.float32)
+    rd = np.array([[[1.0, 0.0, 0.0]]], np.float32)
+    idx, lo, hi = O.aabb_intersect(rs, rd, pts[None], vs, 60)
+    h = vs / 2
+    rows = (np.abs(np.abs(pts[:, 1]) - h) < 1e-6) & (np.abs(pts[:, 2] - 0.125) < 1e-6)
+    assert (idx[0, 0] != -1).sum() == rows.sum() == 16
+
+
+def test_entry_clipped_at_origin_inside_box():
+    pts = np.array([[[0.0, 0.0, 0.0]]], np.float32)
+    rs = np.array([[[0.01, 0.02, 0.03]]], np.float32)
+    rd = np.array([[[0.0, 0.0, 1.0]]], np.float32)
+    idx, lo, hi = O.aabb_intersect(rs, rd, pts, 1.0, 4)
+    assert idx[0, 0, 0] == 0 and lo[0, 0, 0] == 0.0
+    assert hi[0, 0, 0] == np.float32(0.5) - np.float32(0.03)
+
+
+def test_fp32_hits_agree_with_fp64_except_grazing():
+    sc = shell()
+    rs, rd = shell_rays(width=32)
+    pts = O.shift_centers(sc.points, sc.voxel_size)
+    i32, lo32, hi32 = O.aabb_intersect(rs[None], rd[None], pts[None], sc.voxel_size, 60)
+    i64, lo64, hi64 = O.aabb_intersect_f64(rs, rd, pts, sc.voxel_size, 60)
+    diff_rays = [r for r in range(rs.shape[0]) if set(i32[0, r]) != set(i64[r])]
+    assert len(diff_rays) <= 3
+    for r in diff_rays:  # a grazing ray: the differing voxel is entered over < 1e-5
+        only = (set(i32[0, r]) ^ set(i64[r])) - {-1}
+        for k in only:
+            j = list(i64[r]).index(k) if k in i64[r] else None
+            if j is not None:
+                assert hi64[r, j] - lo64[r, j] < 1e-5
+    same = [r for r in range(rs.shape[0]) if r not in diff_rays]
+    m = i32[0, same] != -1
+    assert np.abs(lo32[0, same][m] - lo64[same][m]).max() < 1e-5
+    assert np.abs(hi32[0, same][m] - hi64[same][m]).max() < 1e-5
+
+
+@pytest.mark.parametrize("rays", ["camera", "random", "axis"])
+def test_svo_equals_brute_force_on_tree_leaves(rays):
+    """svo over the octree reports the same hit set and depths as brute force
+    over the octree's leaf centres (no truncation: hits < n_max)."""
+    sc = shell()
+    cen, ch = shell_octree()
+    leaves = cen[: sc.num_voxels]
+    if rays == "camera":
+        rs, rd = shell_rays(width=48)
+    elif rays == "random":
+        rs, rd = random_rays(1500, 1)
+    else:
+        rs, rd = axis_rays(sc, 1500, 2)
+    si, slo, shi = O.svo_intersect(rs[None], rd[None], cen[None], ch[None], sc.voxel_size, 60)
+    ai, alo, ahi = O.aabb_intersect(rs[None], rd[None], leaves[None], sc.voxel_size, 60)
+    full = (ai[0] != -1).sum(-1) == 60     # truncated: order-dependent, skip
+    assert full.mean() < 0.05
+    for r in np.nonzero(~full)[0]:
+        a = {k: (l, h) for k, l, h in zip(ai[0, r], alo[0, r], ahi[0, r]) if k != -1}
+        s = {k: (l, h) for k, l, h in zip(si[0, r], slo[0, r], shi[0, r]) if k != -1}
+        assert a == s, r
+
+
+def test_svo_truncation_follows_dfs_order():
+    """With n_max small the svo keeps the first leaves in traversal order
+    (child slot 7 first), not in depth or index order."""
+    sc = shell()
+    cen, ch = shell_octree()
+    rs, rd = shell_rays(width=16)
+    full = O.svo_intersect(rs[None], rd[None], cen[None], ch[None], sc.voxel_size, 60)[0][0]
+    cut = O.svo_intersect(rs[None], rd[None], cen[None], ch[None], sc.voxel_size, 3)[0][0]
+    for r in range(rs.shape[0]):
+        n = (full[r] != -1).sum()
+        assert list(cut[r][: min(n, 3)]) == list(full[r][: min(n, 3)])
+
+
+# ---------------------------------------------------------------------------
+# Octree
+
+def test_octree_two_points_hand_computed():
+    c, ch = O.build_octree([1, 1, 1], np.array([[0, 0, 0], [2, 2, 2]]), 0)
+    assert c.shape == (3, 3)
+    assert list(ch[2]) == [0, -1, -1, -1, -1, -1, -1, 1, 2]
+    assert list(c[2]) == [1, 1, 1]
+    assert list(ch[0]) == [-1] * 8 + [1] and list(ch[1]) == [-1] * 8 + [1]
+    assert list(c[0]) == [0, 0, 0] and list(c[1]) == [2, 2, 2]
+
+
+def test_octree_depth2_hand_computed():
+    c, ch = O.build_octree([3, 0, 0], np.array([[0, 0, 0], [6, 0, 0]]), 2)
+    assert c.shape[0] == 7
+    assert list(ch[6]) == [5, 4, -1, -1, -1, -1, -1, -1, 8]
+    assert list(c[6]) == [3, 0, 0]
+    assert list(c[5]) == [1, -2, -2] and list(ch[5]) == [-1] * 6 + [3, -1, 4]
+    assert list(c[4]) == [5, -2, -2] and list(ch[4]) == [-1] * 7 + [2, 4]
+    assert list(c[3]) == [0, -1, -1] and list(ch[3]) == [-1] * 6 + [0, -1, 2]
+    assert list(c[2]) == [6, -1, -1] and list(ch[2]) == [-1] * 6 + [1, -1, 2]
+
+
+def _pointer_octree(center, points, depth):
+    """Independent Python restatement of octree.cpp's insert + BFS finalize."""
+    root = {"c": list(center), "d": depth, "i": -1, "ch": [None] * 8}
+    for k, p in enumerate(points):
+        n = root
+        while True:
+            bits = [int(p[a] > n["c"][a]) for a in range(3)]
+            o = bits[0] + 2 * bits[1] + 4 * bits[2]
+            if n["d"] == 0:
+                n["ch"][o] = {"c": list(p), "d": -1, "i": k, "ch": [None] * 8}
+                break
+            if n["ch"][o] is None:
+                ln = 1 << (n["d"] - 1)
+                n["ch"][o] = {"c": [n["c"][a] + (2 * bits[a] - 1) * ln for a in range(3)],
+                              "d": n["d"] - 1, "i": -1, "ch": [None] * 8}
+            n = n["ch"][o]
+    stack, T = [root], 0
+    while stack:
+        n = stack.pop()
+        T += 1
+        stack += [c for c in n["ch"] if c is not None]
+    cen = np.zeros((T, 3), np.int32)
+    chl = -np.ones((T, 9), np.int32)
+    nid = T - 1
+    root["i"] = nid
+    q = [root]
+    while q:
+        n = q.pop(0)
+        for s, c in enumerate(n["ch"]):
+            if c is None:
+                continue
+            if c["d"] > -1:
+                nid -= 1
+                c["i"] = nid
+            q.append(c)
+            chl[n["i"], s] = c["i"]
+        chl[n["i"], 8] = 1 << (n["d"] + 1)
+        cen[n["i"]] = n["c"]
+    return cen, chl
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_octree_matches_pointer_restatement(seed):
+    rng = np.random.default_rng(seed)
+    g = int(rng.integers(2, 12))
+    ijk = np.argwhere(rng.random((g, g, g)) < 0.3)
+    if len(ijk) < 2:
+        ijk = np.array([[0, 0, 0], [1, 2, 1]])
+    co = (ijk[rng.permutation(len(ijk))] * 2).astype(np.int64)
+    center = (co.max(0) + co.min(0)) // 2
+    depth = max(int(math.ceil(math.log2((co.max(0) - co.min(0)).max()))) - 1, 0)
+    c1, h1 = O.build_octree(center, co, depth)
+    c2, h2 = _pointer_octree(center, co, depth)
+    assert np.array_equal(c1, c2) and np.array_equal(h1, h2)
+
+
+def test_octree_shell_known_size_and_invariants():
+    sc = shell()
+    cen_f, ch = shell_octree()
+    N, T = sc.num_voxels, ch.shape[0]
+    assert T == 14521
+    assert (ch[:N, 8] == 1).all() and (ch[:N, :8] == -1).all()
+    assert (ch[N:, 8] > 1).all()
+    # every node except the root has exactly one parent; scales halve
+    par = {}
+    for i in range(N, T):
+        for s in range(8):
+            c = ch[i, s]
+            if c >= 0:
+                assert c not in par
+                par[c] = i
+                assert ch[c, 8] * 2 == ch[i, 8]
+    assert len(par) == T - 1 and T - 1 not in par
+    # leaves' float centres are the voxel centres
+    assert np.abs(cen_f[:N] - sc.points).max() < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Inverse-CDF sampling
+
+def icdf_one(bins, probs, steps, noise=0.5, compat=True, P=None, extra_rows=None):
+    """Run the op on one ray (plus optional following rows) [1,R,P]."""
+    P = P or len(bins)
+    rows = [(bins, probs, steps)] + (extra_rows or [])
+    R = len(rows)
+    idx = -np.ones((1, R, P), np.int32)
+    lo = np.full((1, R, P), 1e4, np.float32)
+    hi = np.full((1, R, P), 1e4, np.float32)
+    pr = np.zeros((1, R, P), np.float32)
+    st = np.zeros((1, R), np.float32)
+    for r, (b, p, s) in enumerate(rows):
+        for k, (i, a, z) in enumerate(b):
+            idx[0, r, k], lo[0, r, k], hi[0, r, k] = i, a, z
+            pr[0, r, k] = p[k]
+        st[0, r] = s
+    S = int(np.ceil(st).max()) + P
+    nz = np.full((1, R, S), noise, np.float32)
+    oi, od, ox, fl = O.inverse_cdf_sampling_op(idx, lo, hi, nz, pr, st, -1.0, compat)
+    n = (oi[0, 0] != -1).sum()
+    return oi[0, 0], od[0, 0], ox[0, 0]
+
+
+def test_icdf_single_bin_closed_form():
+    oi, od, ox = icdf_one([(5, 1.0, 2.0)], [1.0], 4.0)
+    n = 5
+    assert list(oi[:n]) == [5] * 5 and (oi[n:] == -1).all()
+    assert list(od[:n]) == [1.0625, 1.25, 1.5, 1.75, 1.9375]
+    assert list(ox[:n]) == [0.125, 0.25, 0.25, 0.25, 0.125]
+
+
+def test_icdf_two_bins_with_gap():
+    oi, od, ox = icdf_one([(3, 1.0, 2.0), (9, 3.0, 4.0)], [0.5, 0.5], 4.0)
+    assert list(oi[:6]) == [3, 3, 3, 9, 9, 9] and (oi[6:] == -1).all()
+    assert list(od[:6]) == [1.125, 1.5, 1.875, 3.125, 3.5, 3.875]
+    assert list(ox[:6]) == [0.25, 0.5, 0.25, 0.25, 0.5, 0.25]
+
+
+def test_icdf_full_row_duplicates_next_rays_first_id_in_compat():
+    bins = [(3, 1.0, 2.0), (9, 3.0, 4.0)]
+    nxt = ([(77, 1.0, 2.0), (78, 3.0, 4.0)], [0.5, 0.5], 4.0)
+    oi, od, ox = icdf_one(bins, [0.25, 0.25], 4.0, compat=True, extra_rows=[nxt])
+    assert list(oi[:5]) == [3, 3, 9, 9, 77]
+    assert list(od[:5]) == [1.25, 1.75, 3.25, 3.75, 3.75]
+    oi2, _, _ = icdf_one(bins, [0.25, 0.25], 4.0, compat=False, extra_rows=[nxt])
+    assert list(oi2[:4]) == [3, 3, 9, 9] and oi2[4] == -1
+
+
+def test_icdf_trailing_loop_stops_on_row0_slot_in_compat():
+    """The trailing loop's stop test reads ray 0 of the batch row: a longer
+    ray behind a one-bin ray 0 loses its last bins in compat mode."""
+    ray0 = ([(1, 0.0, 1.0)], [1.0, 0.0, 0.0], 1.0)
+    ray1 = ([(4, 1.0, 2.0), (5, 3.0, 4.0), (6, 5.0, 6.0)], [1 / 3, 1 / 3, 1 / 3], 1.0)
+
+    def run(compat):
+        rows = [ray0, ray1]
+        P = 3
+        idx = -np.ones((1, 2, P), np.int32)
+        lo = np.full((1, 2, P), 1e4, np.float32)
+        hi = np.full((1, 2, P), 1e4, np.float32)
+        pr = np.zeros((1, 2, P), np.float32)
+        st = np.zeros((1, 2), np.float32)
+        for r, (b, p, s) in enumerate(rows):
+            for k, (i, a, z) in enumerate(b):
+                idx[0, r, k], lo[0, r, k], hi[0, r, k] = i, a, z
+            pr[0, r, :] = np.float32(p)
+            st[0, r] = s
+        nz = np.full((1, 2, 4), 0.5, np.float32)
+        return O.inverse_cdf_sampling_op(idx, lo, hi, nz, pr, st, -1.0, compat)
+
+    oi, od, _, _ = run(True)
+    assert list(oi[0, 1][:4]) == [4, 5, 5, -1]
+    assert list(od[0, 1][:3]) == [1.5, 3.25, 3.75]
+    oi, od, _, _ = run(False)
+    assert list(oi[0, 1][:4]) == [4, 5, 5, 6]
+    assert od[0, 1][3] == 5.5
+
+
+def test_icdf_invariants_on_camera_rays():
+    sc = shell()
+    from _cases import hits_for_sampling
+    idx, lo, hi, probs, steps, _ = hits_for_sampling(sc, *shell_rays(width=48))
+    si, sd, sx, S, fl = O.inverse_cdf_sampling(idx, lo, hi, probs, steps, -1.0, True, compat=False)
+    for r in range(idx.shape[0]):
+        nb = (idx[r] != -1).sum()
+        ns = (si[r] != -1).sum()
+        # one tail per bin, plus one per CDF step (k + 0.5) / steps that stays
+        # inside the total probability 1, i.e. round(steps) of them
+        frac = float(steps[r]) - math.floor(float(steps[r]))
+        if abs(frac - 0.5) > 1e-4:
+            assert ns == nb + int(math.floor(float(steps[r]) + 0.5))
+        seg = float((hi[r, :nb] - lo[r, :nb]).astype(np.float64).sum())
+        assert abs(float(sx[r, :ns].astype(np.float64).sum()) - seg) < 1e-5   # length conserved
+        assert (np.diff(sd[r, :ns]) > 0).all()             # depth-ordered
+        # every sample lies inside its voxel's interval
+        for k in range(ns):
+            j = list(idx[r, :nb]).index(si[r, k])
+            assert lo[r, j] - 1e-6 <= sd[r, k] <= hi[r, j] + 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Uniform sampling
+
+def uniform_one(bins, step, S=32, noise=0.5):
+    P = len(bins) + 1
+    idx = -np.ones((1, 1, P), np.int32)
+    lo = np.full((1, 1, P), 1e4, np.float32)
+    hi = np.full((1, 1, P), 1e4, np.float32)
+    for k, (i, a, z) in enumerate(bins):
+        idx[0, 0, k], lo[0, 0, k], hi[0, 0, k] = i, a, z
+    nz = np.full((1, 1, S), noise, np.float32)
+    oi, od, ox, fl = O.uniform_ray_sampling_op(idx, lo, hi, nz, step, S)
+    return oi[0, 0], od[0, 0], ox[0, 0]
+
+
+def test_uniform_single_bin_closed_form():
+    oi, od, ox = uniform_one([(5, 1.0, 2.0)], 0.25)
+    assert list(oi[:5]) == [5] * 5 and (oi[5:] == -1).all()
+    assert list(od[:5]) == [1.0625, 1.25, 1.5, 1.75, 1.9375]
+    assert list(ox[:5]) == [0.125, 0.25, 0.25, 0.25, 0.125]
+
+
+def test_uniform_skips_the_gap():
+    oi, od, ox = uniform_one([(3, 1.0, 2.0), (9, 3.0, 4.0)], 0.5)
+    assert list(oi[:6]) == [3, 3, 3, 9, 9, 9] and (oi[6:] == -1).all()
+    assert list(od[:6]) == [1.125, 1.5, 1.875, 3.125, 3.5, 3.875]
+    assert list(ox[:6]) == [0.25, 0.5, 0.25, 0.25, 0.5, 0.25]
+
+
+# ---------------------------------------------------------------------------
+# Glue, gather, compositing, prune, split
+
+def test_probs_and_steps():
+    idx = np.array([[1, 2, -1]], np.int32)
+    lo = np.array([[1.0, 3.0, 1e4]], np.float32)
+    hi = np.array([[2.0, 3.5, 1e4]], np.float32)
+    probs, steps = O.probs_steps(idx, lo, hi, 0.25)
+    assert list(probs[0]) == [np.float32(1 / 1.5), np.float32(0.5 / 1.5), 0.0]
+    assert steps[0] == 6.0
+
+
+def test_sort_hits_pads_last_and_stable():
+    idx = np.array([[4, -1, 2, 7]], np.int32)
+    lo = np.array([[2.0, 0.0, 1.0, 2.0]], np.float32)
+    hi = np.array([[3.0, 0.0, 2.0, 3.5]], np.float32)
+    i, a, b, h = O.sort_hits(idx, lo, hi)
+    assert list(i[0]) == [2, 4, 7, -1] and list(a[0]) == [1, 2, 2, 1e4] and h[0]
+
+
+def test_trilinear_reproduces_affine_fields():
+    sc = shell()
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((3, 5))
+    b0 = rng.standard_normal(5)
+    # corner key k sits at key * half_voxel + offset (same frame as centres)
+    coords, res = O.discretize_points(sc.points, np.float32(sc.voxel_size) / 2)
+    key_pos = sc.keys.astype(np.float64) * (sc.voxel_size / 2) + res.astype(np.float64)
+    values = key_pos @ A + b0
+    cen = coords.astype(np.float64) * (sc.voxel_size / 2) + res.astype(np.float64)
+    vi = rng.integers(0, sc.num_voxels, 500)
+    x = cen[vi] + rng.uniform(-0.5, 0.5, (500, 3)) * sc.voxel_size
+    emb = O.trilinear_gather(vi, x, cen, sc.feats, values, sc.voxel_size)
+    assert np.abs(emb - (x @ A + b0)).max() < 1e-10
+    W = O.trilinear_weights(vi, x, cen, sc.voxel_size)
+    assert np.abs(W.sum(1) - 1).max() < 1e-12
+
+
+def test_trilinear_corner_order_matches_feats():
+    """At a voxel's corner c (4x + 2y + z order) the gather returns exactly
+    that corner's embedding."""
+    sc = shell()
+    coords, res = O.discretize_points(sc.points, np.float32(sc.voxel_size) / 2)
+    cen = coords.astype(np.float64) * (sc.voxel_size / 2)
+    v = 123
+    for c in range(8):
+        off = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1]) - 0.5
+        x = cen[v] + off * sc.voxel_size
+        emb = O.trilinear_gather([v], x[None], cen, sc.feats, sc.values, sc.voxel_size)
+        assert np.abs(emb[0] - sc.values[sc.feats[v, c]]).max() < 1e-12
+
+
+def test_composite_constant_sigma_closed_form():
+    R, S = 3, 10
+    sigma = np.full((R, S), 2.0)
+    dists = np.full((R, S), 0.01)
+    idx = np.zeros((R, S), np.int32)
+    idx[1, 5:] = -1
+    depth = np.tile(np.arange(S, dtype=np.float64), (R, 1))
+    rgb = np.ones((R, S, 3)) * 0.25
+    col, d, missed, probs = O.composite(sigma, rgb, depth, dists, idx)
+    fe = 2.0 * 0.01 * 7.0
+    assert abs(missed[0] - math.exp(-fe * S)) < 1e-12
+    assert abs(missed[1] - math.exp(-fe * 5)) < 1e-12
+    assert abs(probs[0, 0] - (1 - math.exp(-fe))) < 1e-12
+    assert abs(probs[0, 3] - (1 - math.exp(-fe)) * math.exp(-3 * fe)) < 1e-12
+    assert np.abs(probs.sum(1) + missed - 1).max() < 1e-12
+    assert np.abs(col[0] - 0.25 * (1 - missed[0])).max() < 1e-12
+
+
+def test_prune_lattice_includes_corners():
+    pts = np.zeros((1, 3), np.float32)
+    lat = O.prune_lattice(pts, 0.5, bits=16)
+    assert lat.shape == (1, 4096, 3)
+    assert list(lat[0, 0]) == [-0.25, -0.25, -0.25] and list(lat[0, -1]) == [0.25, 0.25, 0.25]
+
+
+def test_split_reproduces_parent_corner_values():
+    sc = shell()
+    sub = np.arange(0, sc.num_voxels, 37)
+    pts = sc.points[sub]
+    keys, feats = O.voxel_keys(pts, sc.voxel_size)
+    rng = np.random.default_rng(0)
+    values = rng.standard_normal((len(keys), 4))
+    np_, nf, nv, nk = O.split_voxels(pts, feats, values, sc.voxel_size)
+    assert np_.shape == (8 * len(pts), 3) and nf.shape == (8 * len(pts), 8)
+    assert (np.diff(nk[:, 0]) >= 0).all()
+    # a child's outer corner coincides with its parent's corner: same value
+    for v in range(len(pts)):
+        for c in range(8):
+            child = 8 * v + c
+            assert np.abs(nv[nf[child, c]] - values[feats[v, c]]).max() < 1e-12
+    # children at parent +- vs/4
+    d = np_.reshape(-1, 8, 3) - pts[:, None, :]
+    assert np.allclose(np.abs(d), sc.voxel_size / 4)
