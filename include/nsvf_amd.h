@@ -66,7 +66,8 @@ int nsvf_aabb_intersect(const float* ray_start, const float* ray_dir,
  *   points   : [Bp,T,3] f32 node centres; children: [Bp,T,9] i32 (slots 0-7
  *              child ids or -1, slot 8 = box scale 1<<(depth+1), 1 = leaf)
  *   outputs as nsvf_aabb_intersect.
- * Returns NSVF_ESTACK if the tree is deeper than 30 levels.                 */
+ * Subtrees deeper than 30 internal levels are not descended (a valid tree
+ * over int32 coordinates has at most 31).                                  */
 int nsvf_svo_intersect(const float* ray_start, const float* ray_dir,
                        const float* points, const int32_t* children, int B,
                        int M, int T, int tree_shared, float voxelsize,
@@ -120,6 +121,125 @@ int nsvf_uniform_ray_sampling(const int32_t* pts_idx, const float* min_depth,
                               int B, int R, int P, int S, float step_size,
                               int32_t* out_idx, float* out_depth,
                               float* out_dists, int64_t* flags, void* stream);
+
+/* Inverse-CDF sampling of N rows [N,P] in one launch, with the results of the
+ * reference's Python wrapper `inverse_cdf_sampling` (fairnr/clib/__init__.py:
+ * 231-300), which pads the rows to a multiple of `group` (200) with copies of
+ * row 0, regroups them to [group, H/group, P] and calls the kernel on chunks
+ * of `chunk` (800) columns: each ray's "first ray of the batch row" and "next
+ * ray" (read by the compat-mode trailing loop) are derived from that layout.
+ *   noise [N,S] or NULL (then every noise value is 0.5, the wrapper's
+ *   deterministic mode); outputs [N,S] (idx -1, depth 0, dists 0 padded).    */
+int nsvf_inverse_cdf_sampling_rows(const int32_t* pts_idx, const float* min_depth,
+                                   const float* max_depth, const float* noise,
+                                   const float* probs, const float* steps, int N,
+                                   int P, int S, int group, int chunk,
+                                   float fixed_step_size, int compat,
+                                   int32_t* out_idx, float* out_depth,
+                                   float* out_dists, int64_t* flags, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Trilinear embedding gather (fairnr/modules/encoder.py:582-590 with
+ * fairnr/data/geometry.py:195-200; the reference runs it as torch ops).
+ *   s_idx [M] voxel ids; xyz [M,3] sample points; centers [N,3] (x-shifted
+ *   voxel centres); feats [N,8] corner key ids (corner order 4x+2y+z);
+ *   values [K,D]; out [M,D] = sum_c w_c values[feats[v][c]].
+ * ---------------------------------------------------------------------- */
+int nsvf_trilinear_gather(const int32_t* s_idx, const float* xyz, int64_t M,
+                          const float* centers, const int32_t* feats,
+                          const float* values, int D, float voxel_size,
+                          float* out, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Fused render pipeline.  No counterpart in the reference's native module:
+ * these replace the Python stages of one view (fairnr/models/
+ * fairnr_model.py:142-186 with encoder.py:498-592, nsvf.py:43-110,
+ * nerf.py:64-79 and renderer.py:77-252).  Sample streams are compacted per
+ * hit ray (offset, count) instead of padded [R,S] tensors.
+ * ---------------------------------------------------------------------- */
+
+/* Device-wide exclusive prefix sum of n int32 values (out may alias in);
+ * the grand total goes to *total (device).  workspace: device int32[
+ * nsvf_scan_workspace_ints(n)].                                            */
+int64_t nsvf_scan_workspace_ints(int64_t n);
+int nsvf_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n,
+                            int32_t* workspace, int32_t* total, void* stream);
+
+/* Octree traversal with the encoder's post-processing: per ray the hits
+ * sorted by entry depth (stable with respect to the reference's traversal
+ * order), their count, the sum of segment lengths and a hit flag.
+ *   hits: [R, n_max] records {float lo, float hi, int32 idx, int32 pad};
+ *   only the first count[r] records of a row are written.                  */
+int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
+                     const float* centers, const int32_t* children, int T,
+                     float voxelsize, int n_max, void* hits, int32_t* count,
+                     float* tot, int32_t* hit_flag, void* stream);
+
+/* list[rank[r]] = r for every ray with hit_flag[r] != 0.                  */
+int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,
+                        int64_t R, int32_t* list, void* stream);
+
+/* Coarse inverse-CDF sampling of the hit rays (deterministic noise 0.5,
+ * steps = sum of segment lengths / step_size, probs = length / sum).
+ * write = 0: counts_out[n] = valid samples of hit ray n, *max_count = max;
+ * write = 1: samples written at offsets[n] into the stream arrays.
+ * *n_hit_dev (device) is the number of hit rays, n_hit_max >= it.          */
+int nsvf_render_sample_coarse(int write, const int32_t* hit_list,
+                              const int32_t* n_hit_dev, int64_t n_hit_max,
+                              const void* hits, const int32_t* count,
+                              const float* tot, int n_max, float step_size,
+                              int compat, int32_t* counts_out,
+                              int32_t* max_count, const int32_t* offsets,
+                              int32_t* s_idx, float* s_depth, float* s_dist,
+                              int32_t* s_ray, void* stream);
+
+/* Fine (hierarchical) inverse-CDF sampling: bins are each ray's coarse
+ * samples [depth -+ dist/2] with probs (p + 1e-5) / column sum over the
+ * *c_max (device) columns; `fine_steps` CDF steps (fairnr/models/nerf.py:
+ * 64-79).  Same write/count protocol as the coarse pass.                   */
+int nsvf_render_sample_fine(int write, const int32_t* hit_list,
+                            const int32_t* n_hit_dev, int64_t n_hit_max,
+                            const int32_t* c_off, const int32_t* c_cnt,
+                            const int32_t* c_max, const int32_t* c_idx,
+                            const float* c_depth, const float* c_dist,
+                            const float* c_prob, float fine_steps, int compat,
+                            int32_t* counts_out, int32_t* max_count,
+                            const int32_t* offsets, int32_t* s_idx,
+                            float* s_depth, float* s_dist, int32_t* s_ray,
+                            void* stream);
+
+/* Field evaluation of a sample stream: trilinear gather fused with the MLP
+ * (L hidden ReLU layers of width H on a D-wide embedding, 4-wide head).
+ *   precision 0: BF16 MFMA with FP32 accumulation, wfrag = packed bf16;
+ *   precision 1: FP32 MFMA, wfrag = packed f32.
+ *   out [M,4] = (sigma, sigmoid(r), sigmoid(g), sigmoid(b)).
+ *   Supported shapes: nsvf_field_supported(D, H, L).                        */
+int nsvf_field_supported(int D, int H, int L);
+int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
+                    const int32_t* feats, const float* values, float voxel_size,
+                    const float* ray_start, const float* ray_dir,
+                    const int32_t* s_idx, const float* s_depth,
+                    const int32_t* s_ray, const int32_t* m_dev, int64_t m_max,
+                    const void* wfrag, const float* bias, float* out,
+                    void* stream);
+
+/* Front-to-back compositing of each hit ray's samples (one wavefront per
+ * ray): free energy relu(sigma) * dist * 7, probs, and the ray's rgb, depth
+ * and missed = 1 - sum(probs).  prob_out [M] optional.                      */
+int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev,
+                          int64_t n_hit_max, const int32_t* off,
+                          const int32_t* cnt, const float* depth,
+                          const float* dist, const float* field,
+                          float* prob_out, float* ray_rgb, float* ray_depth,
+                          float* ray_missed, void* stream);
+
+/* Frame outputs for all R rays: hit rays' composite plus missed * background
+ * (fairnr/models/nsvf.py:89-101); rays that hit nothing get the background. */
+int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const int32_t* rank,
+                       const float* ray_rgb, const float* ray_depth,
+                       const float* ray_missed, float bg_r, float bg_g,
+                       float bg_b, float bg_depth, float* rgb, float* depth,
+                       float* missed, void* stream);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@ _LIB = None
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _F = ctypes.c_float
+_L = ctypes.c_int64
 
 # name -> (argtypes, restype); every entry point declared in include/nsvf_amd.h
 SIGNATURES = {
@@ -31,6 +32,20 @@ SIGNATURES = {
     "nsvf_inverse_cdf_sampling": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P, _P], _I),
     "nsvf_inverse_cdf_sampling_rows": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P, _P], _I),
     "nsvf_uniform_ray_sampling": ([_P, _P, _P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P, _P], _I),
+    "nsvf_trilinear_gather": ([_P, _P, _L, _P, _P, _P, _I, _F, _P, _P], _I),
+    "nsvf_scan_workspace_ints": ([_L], _L),
+    "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
+    "nsvf_render_hits": ([_P, _P, _L, _P, _P, _I, _F, _I, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),
+    "nsvf_render_sample_coarse": ([_I, _P, _P, _L, _P, _P, _P, _I, _F, _I, _P, _P, _P,
+                                   _P, _P, _P, _P, _P], _I),
+    "nsvf_render_sample_fine": ([_I, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P,
+                                 _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_field_supported": ([_I, _I, _I], _I),
+    "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _L, _P, _P,
+                         _P, _P], _I),
+    "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_finish": ([_L, _P, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
 }
 
 

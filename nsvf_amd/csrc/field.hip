@@ -1,0 +1,507 @@
+// Field evaluation over a compacted sample stream: trilinear gather of the
+// voxel-corner embeddings fused with the field MLP on the matrix cores.
+//
+// Gather (encoder.py:582-590, geometry.py:195-200): x = o + d*z; p = (x - c)
+// / vs + 0.5; w_c = prod over axes of (bit ? p : 1 - p), corners c = 4x+2y+z;
+// emb = sum_c w_c * values[feats[v][c]], in corner order, each product and
+// sum rounded separately.
+//
+// MLP: L hidden layers (Linear + ReLU) of width H, then a 4-wide head giving
+// sigma and RGB logits (RGB through a sigmoid).  The layers run in the
+// transposed form Y^T = W X^T on 16x16 MFMA tiles: the weights are operand A
+// (rows = output features), the activations operand B (columns = samples).
+// Each lane's accumulator rows of one layer are then exactly the operand-B
+// elements it needs for the next layer, provided the next layer's weights are
+// stored with the input features in the matching permuted order -- which the
+// host packer does once per scene -- so activations never leave registers.
+//
+//   BF16: v_mfma_f32_16x16x32_bf16, FP32 accumulation.  K-tile of 32 input
+//         features; lane group g = lane/16 holds features
+//         {4g..4g+3, 16+4g..16+4g+3} of its K-tile.
+//   FP32: v_mfma_f32_16x16x4_f32.  K-tile of 4 features; lane group g holds
+//         feature 16t + 4g + i of K-tile (t, i).
+// Both layouts make a lane gather the same 8 embedding features per 32.
+//
+// Weight fragments: [layer][m-tile][k-tile][64 lanes][8 bf16 | 1 f32],
+// biases fp32 [layer][rows padded to 16].  BF16 fragments are staged in LDS.
+#include <algorithm>
+
+#include "render.h"
+
+namespace nsvf {
+namespace {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+constexpr int kFieldBlock = 256;  // 4 waves
+
+struct FieldScene {
+  const float* centers;   // [N,3] voxel centres (x-shifted)
+  const int32_t* feats;   // [N,8] corner key ids
+  const float* values;    // [K,D] corner embeddings
+  float voxel_size;
+};
+
+struct FieldRays {
+  const float* start;  // [R,3]
+  const float* dir;    // [R,3]
+};
+
+// Trilinear weights of one sample (corner order 4x + 2y + z).
+__device__ __forceinline__ void corner_weights(const FieldScene& sc, const FieldRays& rays,
+                                               int32_t v, float z, int32_t r, float w[8],
+                                               int32_t key[8]) {
+  float p[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float x = rays.start[int64_t(r) * 3 + a] + rays.dir[int64_t(r) * 3 + a] * z;
+    p[a] = (x - sc.centers[int64_t(v) * 3 + a]) / sc.voxel_size + 0.5f;
+  }
+  const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float wx = (c & 4) ? p[0] : q[0];
+    const float wy = (c & 2) ? p[1] : q[1];
+    const float wz = (c & 1) ? p[2] : q[2];
+    w[c] = (wx * wy) * wz;
+    key[c] = sc.feats[int64_t(v) * 8 + c];
+  }
+}
+
+// The 8 features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3} of one sample's
+// embedding (zero past D).
+template <int D>
+__device__ __forceinline__ void gather8(const FieldScene& sc, const float w[8],
+                                        const int32_t key[8], int kt, int g, float out[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = 0.0f;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int f0 = 32 * kt + 16 * half + 4 * g;
+    if (f0 < D) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float4 e = *reinterpret_cast<const float4*>(sc.values + int64_t(key[c]) * D + f0);
+        out[4 * half + 0] = out[4 * half + 0] + w[c] * e.x;
+        out[4 * half + 1] = out[4 * half + 1] + w[c] * e.y;
+        out[4 * half + 2] = out[4 * half + 2] + w[c] * e.z;
+        out[4 * half + 3] = out[4 * half + 3] + w[c] * e.w;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
+template <int D, int H, int L>
+struct Dims {
+  static constexpr int KT0 = (D + 31) / 32;   // k-tiles of the input (bf16)
+  static constexpr int KTH = H / 32;          // k-tiles of a hidden layer (bf16)
+  static constexpr int MTH = H / 16;          // m-tiles of a hidden layer
+  static constexpr int NT = H <= 128 ? 4 : 2; // 16-sample n-tiles per wave
+  static constexpr int SAMPLES = 16 * NT;     // samples per wave step
+  // bf16 fragment counts (1 KB each)
+  static constexpr int F0 = MTH * KT0;
+  static constexpr int FH = MTH * KTH;
+  static constexpr int FHEAD = KTH;
+  static constexpr int FRAGS = F0 + (L - 1) * FH + FHEAD;
+  static constexpr int BIASES = H * L + 16;
+};
+
+template <int D, int H, int L>
+constexpr bool weights_in_lds() {
+  return Dims<D, H, L>::FRAGS * 1024 <= 96 * 1024;
+}
+
+template <int D, int H, int L>
+__global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
+    FieldScene sc, FieldRays rays, const int32_t* __restrict__ s_idx,
+    const float* __restrict__ s_depth, const int32_t* __restrict__ s_ray, const int32_t* m_dev,
+    int64_t m_max, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
+    float4* __restrict__ out) {
+  using Dm = Dims<D, H, L>;
+  constexpr int NT = Dm::NT;
+  constexpr bool kLds = weights_in_lds<D, H, L>();
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const bf16x8* lw = wfrag;
+  float* lb = reinterpret_cast<float*>(smem_raw);
+  if constexpr (kLds) {
+    bf16x8* w = reinterpret_cast<bf16x8*>(smem_raw);
+    for (int i = threadIdx.x; i < Dm::FRAGS * 64; i += kFieldBlock) w[i] = wfrag[i];
+    lw = w;
+    lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 64 * sizeof(bf16x8));
+  }
+  for (int i = threadIdx.x; i < Dm::BIASES; i += kFieldBlock) lb[i] = bias[i];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  // Per-wave staging of the gathered layer-0 operands: each lane writes and
+  // reads back only its own slots, so no barrier is needed; staging keeps
+  // the gather loop rolled (one sample's loads in flight per lane).
+  bf16x8* stage = reinterpret_cast<bf16x8*>(
+                      reinterpret_cast<unsigned char*>(lb) +
+                      ((size_t(Dm::BIASES) * sizeof(float) + 15) & ~size_t(15))) +
+                  (threadIdx.x >> 6) * (NT * Dm::KT0 * 64);
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
+  for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
+       t += int64_t(gridDim.x) * 4) {
+    const int64_t base = t * Dm::SAMPLES;
+    // ---- gather -> operand B of layer 0
+#pragma unroll 1
+    for (int j = 0; j < NT; ++j) {
+      const int64_t s = base + 16 * j + col;
+      float w[8];
+      int32_t key[8];
+      const bool valid = s < M;
+      if (valid) corner_weights(sc, rays, s_idx[s], s_depth[s], s_ray[s], w, key);
+#pragma unroll
+      for (int kt = 0; kt < Dm::KT0; ++kt) {
+        float f[8];
+        if (valid) {
+          gather8<D>(sc, w, key, kt, g, f);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) f[i] = 0.0f;
+        }
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = (__bf16)f[i];
+        stage[(j * Dm::KT0 + kt) * 64 + lane] = v;
+      }
+    }
+    bf16x8 b0[Dm::KT0][NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = stage[(j * Dm::KT0 + kt) * 64 + lane];
+    // ---- layer 0
+    bf16x8 bh[Dm::KTH][NT];
+    {
+      const bf16x8* A = lw;
+#pragma unroll
+      for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          const int mt = 2 * kt2 + half;
+          f32x4 acc[NT];
+#pragma unroll
+          for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kt = 0; kt < Dm::KT0; ++kt) {
+            const bf16x8 a = A[(mt * Dm::KT0 + kt) * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+              acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0[kt][j], acc[j], 0, 0, 0);
+          }
+          const float4 bb = *reinterpret_cast<const float4*>(lb + 16 * mt + 4 * g);
+#pragma unroll
+          for (int j = 0; j < NT; ++j) {
+            bh[kt2][j][4 * half + 0] = (__bf16)fmaxf(acc[j][0] + bb.x, 0.0f);
+            bh[kt2][j][4 * half + 1] = (__bf16)fmaxf(acc[j][1] + bb.y, 0.0f);
+            bh[kt2][j][4 * half + 2] = (__bf16)fmaxf(acc[j][2] + bb.z, 0.0f);
+            bh[kt2][j][4 * half + 3] = (__bf16)fmaxf(acc[j][3] + bb.w, 0.0f);
+          }
+        }
+      }
+    }
+    // ---- hidden layers 1..L-1
+#pragma unroll 1
+    for (int l = 1; l < L; ++l) {
+      const bf16x8* A = lw + (Dm::F0 + (l - 1) * Dm::FH) * 64;
+      const float* B = lb + H * l;
+      bf16x8 nb[Dm::KTH][NT];
+#pragma unroll
+      for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          const int mt = 2 * kt2 + half;
+          f32x4 acc[NT];
+#pragma unroll
+          for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int kt = 0; kt < Dm::KTH; ++kt) {
+            const bf16x8 a = A[(mt * Dm::KTH + kt) * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+              acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], acc[j], 0, 0, 0);
+          }
+          const float4 bb = *reinterpret_cast<const float4*>(B + 16 * mt + 4 * g);
+#pragma unroll
+          for (int j = 0; j < NT; ++j) {
+            nb[kt2][j][4 * half + 0] = (__bf16)fmaxf(acc[j][0] + bb.x, 0.0f);
+            nb[kt2][j][4 * half + 1] = (__bf16)fmaxf(acc[j][1] + bb.y, 0.0f);
+            nb[kt2][j][4 * half + 2] = (__bf16)fmaxf(acc[j][2] + bb.z, 0.0f);
+            nb[kt2][j][4 * half + 3] = (__bf16)fmaxf(acc[j][3] + bb.w, 0.0f);
+          }
+        }
+      }
+#pragma unroll
+      for (int kt = 0; kt < Dm::KTH; ++kt)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
+    }
+    // ---- head (4 outputs in rows 0..3 of one m-tile)
+    {
+      const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
+      f32x4 acc[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < Dm::KTH; ++kt) {
+        const bf16x8 a = A[kt * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], acc[j], 0, 0, 0);
+      }
+      if (g == 0) {
+        const float* B = lb + H * L;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const int64_t s = base + 16 * j + col;
+          if (s < M)
+            out[s] = make_float4(acc[j][0] + B[0], sigmoidf(acc[j][1] + B[1]),
+                                 sigmoidf(acc[j][2] + B[2]), sigmoidf(acc[j][3] + B[3]));
+        }
+      }
+    }
+  }
+}
+
+// FP32 variant: same structure on v_mfma_f32_16x16x4_f32, weights read from
+// global memory (fragments of 64 floats).
+template <int D, int H, int L>
+__global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
+    FieldScene sc, FieldRays rays, const int32_t* __restrict__ s_idx,
+    const float* __restrict__ s_depth, const int32_t* __restrict__ s_ray, const int32_t* m_dev,
+    int64_t m_max, const float* __restrict__ wfrag, const float* __restrict__ bias,
+    float4* __restrict__ out) {
+  constexpr int NT = 2;
+  constexpr int KT0 = 4 * ((D + 15) / 16);  // k-tiles (t, i) of the input
+  constexpr int KTH = H / 4;
+  constexpr int MTH = H / 16;
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + 16 * NT - 1) / (16 * NT);
+  for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
+       t += int64_t(gridDim.x) * 4) {
+    const int64_t base = t * 16 * NT;
+    float x0[KT0][NT];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int64_t s = base + 16 * j + col;
+      float w[8];
+      int32_t key[8];
+      const bool valid = s < M;
+      if (valid) corner_weights(sc, rays, s_idx[s], s_depth[s], s_ray[s], w, key);
+#pragma unroll
+      for (int kt32 = 0; kt32 < (KT0 + 7) / 8; ++kt32) {
+        float f[8];
+        if (valid) {
+          gather8<D>(sc, w, key, kt32, g, f);
+        } else {
+#pragma unroll
+          for (int i = 0; i < 8; ++i) f[i] = 0.0f;
+        }
+        // features 32*kt32 + 4g + i -> k-tile (t = 2*kt32, i); 32*kt32 + 16 +
+        // 4g + i -> k-tile (t = 2*kt32 + 1, i)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if ((2 * kt32) * 4 + i < KT0) x0[(2 * kt32) * 4 + i][j] = f[i];
+          if ((2 * kt32 + 1) * 4 + i < KT0) x0[(2 * kt32 + 1) * 4 + i][j] = f[4 + i];
+        }
+      }
+    }
+    float xh[KTH][NT];
+    {
+      const float* A = wfrag;
+#pragma unroll
+      for (int mt = 0; mt < MTH; ++mt) {
+        f32x4 acc[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < KT0; ++kt) {
+          const float a = A[(mt * KT0 + kt) * 64 + lane];
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[kt][j], acc[j], 0, 0, 0);
+        }
+        const float4 bb = *reinterpret_cast<const float4*>(bias + 16 * mt + 4 * g);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          xh[4 * mt + 0][j] = fmaxf(acc[j][0] + bb.x, 0.0f);
+          xh[4 * mt + 1][j] = fmaxf(acc[j][1] + bb.y, 0.0f);
+          xh[4 * mt + 2][j] = fmaxf(acc[j][2] + bb.z, 0.0f);
+          xh[4 * mt + 3][j] = fmaxf(acc[j][3] + bb.w, 0.0f);
+        }
+      }
+    }
+    const float* A = wfrag + MTH * KT0 * 64;
+#pragma unroll 1
+    for (int l = 1; l < L; ++l) {
+      float nx[KTH][NT];
+#pragma unroll
+      for (int mt = 0; mt < MTH; ++mt) {
+        f32x4 acc[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < KTH; ++kt) {
+          const float a = A[(mt * KTH + kt) * 64 + lane];
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xh[kt][j], acc[j], 0, 0, 0);
+        }
+        const float4 bb = *reinterpret_cast<const float4*>(bias + H * l + 16 * mt + 4 * g);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          nx[4 * mt + 0][j] = fmaxf(acc[j][0] + bb.x, 0.0f);
+          nx[4 * mt + 1][j] = fmaxf(acc[j][1] + bb.y, 0.0f);
+          nx[4 * mt + 2][j] = fmaxf(acc[j][2] + bb.z, 0.0f);
+          nx[4 * mt + 3][j] = fmaxf(acc[j][3] + bb.w, 0.0f);
+        }
+      }
+#pragma unroll
+      for (int kt = 0; kt < KTH; ++kt)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) xh[kt][j] = nx[kt][j];
+      A += MTH * KTH * 64;
+    }
+    {
+      f32x4 acc[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kt = 0; kt < KTH; ++kt) {
+        const float a = A[kt * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xh[kt][j], acc[j], 0, 0, 0);
+      }
+      if (g == 0) {
+        const float* B = bias + H * L;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const int64_t s = base + 16 * j + col;
+          if (s < M)
+            out[s] = make_float4(acc[j][0] + B[0], sigmoidf(acc[j][1] + B[1]),
+                                 sigmoidf(acc[j][2] + B[2]), sigmoidf(acc[j][3] + B[3]));
+        }
+      }
+    }
+  }
+}
+
+// Op-level trilinear gather: one thread per (sample, feature); consecutive
+// threads cover consecutive features, so embedding reads and output writes
+// are coalesced.
+__global__ __launch_bounds__(256) void gather_kernel(const int32_t* __restrict__ s_idx,
+                                                     const float* __restrict__ xyz, int64_t M,
+                                                     int D, FieldScene sc,
+                                                     float* __restrict__ out) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= M * D) return;
+  const int64_t s = i / D;
+  const int f = int(i - s * D);
+  const int32_t v = s_idx[s];
+  float p[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    p[a] = (xyz[s * 3 + a] - sc.centers[int64_t(v) * 3 + a]) / sc.voxel_size + 0.5f;
+  const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+  float acc = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const float w = (((c & 4) ? p[0] : q[0]) * ((c & 2) ? p[1] : q[1])) * ((c & 1) ? p[2] : q[2]);
+    acc = acc + w * sc.values[int64_t(sc.feats[int64_t(v) * 8 + c]) * D + f];
+  }
+  out[i] = acc;
+}
+
+template <int D, int H, int L>
+int launch_bf16(FieldScene sc, FieldRays rays, const int32_t* s_idx, const float* s_depth,
+                const int32_t* s_ray, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                const float* bias, float* out, hipStream_t stream) {
+  using Dm = Dims<D, H, L>;
+  const size_t lds = (weights_in_lds<D, H, L>() ? size_t(Dm::FRAGS) * 1024 : 0) +
+                     ((size_t(Dm::BIASES) * sizeof(float) + 15) & ~size_t(15)) +
+                     size_t(4) * Dm::NT * Dm::KT0 * 64 * 16;
+  const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
+  const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
+  hipLaunchKernelGGL((field_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), lds,
+                     stream, sc, rays, s_idx, s_depth, s_ray, m_dev, m_max,
+                     static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out));
+  return check_launch("field_bf16");
+}
+
+template <int D, int H, int L>
+int launch_f32(FieldScene sc, FieldRays rays, const int32_t* s_idx, const float* s_depth,
+               const int32_t* s_ray, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+               const float* bias, float* out, hipStream_t stream) {
+  const int64_t tiles = ceil_div(m_max, 32);
+  const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
+  hipLaunchKernelGGL((field_f32_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), 0,
+                     stream, sc, rays, s_idx, s_depth, s_ray, m_dev, m_max,
+                     static_cast<const float*>(wfrag), bias, reinterpret_cast<float4*>(out));
+  return check_launch("field_f32");
+}
+
+}  // namespace
+}  // namespace nsvf
+
+using namespace nsvf;
+
+extern "C" {
+
+// Supported (D, H, L) shapes of the fused field kernels.
+int nsvf_field_supported(int D, int H, int L) {
+  return (D == 32 && H == 128 && L == 3) || (D == 48 && H == 256 && L == 4) ||
+         (D == 32 && H == 256 && L == 4);
+}
+
+int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
+                    const int32_t* feats, const float* values, float voxel_size,
+                    const float* ray_start, const float* ray_dir, const int32_t* s_idx,
+                    const float* s_depth, const int32_t* s_ray, const int32_t* m_dev,
+                    int64_t m_max, const void* wfrag, const float* bias, float* out,
+                    void* stream) {
+  clear_error();
+  if (m_max <= 0) return NSVF_OK;
+  if (!centers || !feats || !values || !ray_start || !ray_dir || !s_idx || !s_depth || !s_ray ||
+      !m_dev || !wfrag || !bias || !out)
+    return set_error(NSVF_EINVAL, "field_eval: null pointer");
+  FieldScene sc{centers, feats, values, voxel_size};
+  FieldRays rays{ray_start, ray_dir};
+  hipStream_t st = as_stream(stream);
+#define NSVF_FIELD_CASE(d, h, l)                                                               \
+  if (D == d && H == h && L == l) {                                                            \
+    if (precision == 0)                                                                        \
+      return launch_bf16<d, h, l>(sc, rays, s_idx, s_depth, s_ray, m_dev, m_max, wfrag, bias, \
+                                  out, st);                                                    \
+    return launch_f32<d, h, l>(sc, rays, s_idx, s_depth, s_ray, m_dev, m_max, wfrag, bias,    \
+                               out, st);                                                       \
+  }
+  NSVF_FIELD_CASE(32, 128, 3)
+  NSVF_FIELD_CASE(48, 256, 4)
+  NSVF_FIELD_CASE(32, 256, 4)
+#undef NSVF_FIELD_CASE
+  return set_error(NSVF_EINVAL, "field_eval: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_trilinear_gather(const int32_t* s_idx, const float* xyz, int64_t M, const float* centers,
+                          const int32_t* feats, const float* values, int D, float voxel_size,
+                          float* out, void* stream) {
+  clear_error();
+  if (M < 0 || D <= 0) return set_error(NSVF_EINVAL, "trilinear_gather: bad size");
+  if (M == 0) return NSVF_OK;
+  if (!s_idx || !xyz || !centers || !feats || !values || !out)
+    return set_error(NSVF_EINVAL, "trilinear_gather: null pointer");
+  FieldScene sc{centers, feats, values, voxel_size};
+  hipLaunchKernelGGL(gather_kernel, dim3(unsigned(ceil_div(M * D, 256))), dim3(256), 0,
+                     as_stream(stream), s_idx, xyz, M, D, sc, out);
+  return check_launch("trilinear_gather");
+}
+
+}  // extern "C"

@@ -39,9 +39,12 @@ class FieldMLP:
         return 2 * sum(w.shape[0] * w.shape[1] for w in self.weights)
 
 
-def make_mlp(in_dim, hidden, layers, seed):
+def make_mlp(in_dim, hidden, layers, seed, head_gain=20.0, sigma_bias=0.5):
     """Kaiming-normal weights (fan_in, ReLU gain, as implicit.py:73-75) and
-    nn.Linear-style uniform biases, from a fixed seed."""
+    nn.Linear-style uniform biases, from a fixed seed.  The head is scaled by
+    ``head_gain`` and the sigma bias set to ``sigma_bias`` so that a random
+    field over N(0, 0.1^2) embeddings renders a partly opaque, coloured image
+    (unscaled, sigma stays within +-0.15 and every view is nearly empty)."""
     rng = np.random.default_rng(seed)
     ws, bs = [], []
     d = in_dim
@@ -51,6 +54,8 @@ def make_mlp(in_dim, hidden, layers, seed):
         bound = 1.0 / math.sqrt(d)
         bs.append(rng.uniform(-bound, bound, out).astype(np.float32))
         d = out
+    ws[-1] = (ws[-1] * np.float32(head_gain)).astype(np.float32)
+    bs[-1][0] = np.float32(sigma_bias)
     return FieldMLP(ws, bs)
 
 

@@ -557,3 +557,85 @@ def aabb_intersect_f64(ray_start, ray_dir, points, voxelsize, n_max):
             lo_out[r, :len(ks)] = f_low[ks]
             hi_out[r, :len(ks)] = f_high[ks]
     return idx, lo_out, hi_out
+
+
+# ---------------------------------------------------------------------------
+# One view, end to end (fairnr_model.py:142-186 for the NSVF model with
+# hierarchical sampling; nsvf.py:43-110; nerf.py:64-79; renderer.py:77-252)
+
+def _field_eval(si, sd, sx, ray_start, ray_dir, centers_shifted, feats, values, voxel_size, mlp,
+                dtype):
+    """Field outputs at the valid samples of a padded [N,S] sample set, in
+    ``dtype``; sample points x = o + d * z in float32 (geometry.py:160-161)."""
+    valid = si != -1
+    rows, cols = np.nonzero(valid)
+    z = sd[rows, cols].astype(np.float32)
+    x = (ray_start[rows] + (ray_dir[rows] * z[:, None]).astype(np.float32)).astype(np.float32)
+    emb = trilinear_gather(si[rows, cols], x, centers_shifted, feats, values, voxel_size, dtype=dtype)
+    sigma, rgb = mlp_forward(emb, mlp, dtype=dtype)
+    S = np.zeros(si.shape, dtype)
+    C = np.zeros(si.shape + (3,), dtype)
+    S[rows, cols] = sigma
+    C[rows, cols] = rgb
+    return S, C
+
+
+def prepare_fine_bins(si, sd, sx, probs, fine_steps):
+    """nerf.py:64-79 (+ nsvf.py:83-87 with its ratio unset): coarse samples as
+    bins [z - d/2, z + d/2], probs (p + 1e-5) / sum over all columns (fp32,
+    left to right), steps = fine_steps."""
+    lo = (sd - (sx * np.float32(0.5)).astype(np.float32)).astype(np.float32)
+    hi = (sd + (sx * np.float32(0.5)).astype(np.float32)).astype(np.float32)
+    safe = (np.asarray(probs, np.float32) + np.float32(1e-5)).astype(np.float32)
+    den = np.zeros(safe.shape[0], np.float32)
+    for k in range(safe.shape[1]):
+        den = (den + safe[:, k]).astype(np.float32)
+    p = (safe / den[:, None]).astype(np.float32)
+    steps = np.full(safe.shape[0], np.float32(fine_steps), np.float32)
+    return si.astype(np.int32), lo, hi, p, steps
+
+
+def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, compat=True,
+                dtype=np.float64, coarse_probs=None, use_octree=True):
+    """Render rays [R,3] of ``scene`` (a VoxelScene).  Integer and sampling
+    decisions are the FP32 restatement's; gather, MLP and compositing run in
+    ``dtype``.  ``coarse_probs`` ([N_hit, S_c] float32) replaces the coarse
+    compositing probabilities that drive the fine sampler, so the fine
+    samples can be compared with another implementation's for the same
+    probabilities.  Returns a dict of the outputs and intermediates."""
+    rs = np.asarray(ray_start, np.float32)
+    rd = np.asarray(ray_dir, np.float32)
+    vs = np.float32(scene.voxel_size)
+    pts_shift = shift_centers(scene.points, vs)
+    if use_octree:
+        cen, ch = build_easy_octree(scene.points, vs * np.float32(0.5))
+        idx, lo, hi = svo_intersect(rs[None], rd[None], cen[None], ch[None], vs, scene.max_hits)
+    else:
+        idx, lo, hi = aabb_intersect(rs[None], rd[None], pts_shift[None], vs, scene.max_hits)
+    idx, lo, hi, hits = sort_hits(idx[0], lo[0], hi[0])
+    out = {"hits": hits, "sorted_idx": idx, "sorted_lo": lo, "sorted_hi": hi}
+    idx, lo, hi = idx[hits], lo[hits], hi[hits]
+    probs, steps = probs_steps(idx, lo, hi, scene.step_size)
+    out.update(probs=probs, steps=steps)
+    si, sd, sx, S, _ = inverse_cdf_sampling(idx, lo, hi, probs, steps, -1.0, True, compat=compat)
+    sd, sx = sample_tail(si, sd, sx)
+    out.update(c_idx=si, c_depth=sd, c_dist=sx)
+    rsh, rdh = rs[hits], rd[hits]
+    sig, col = _field_eval(si, sd, sx, rsh, rdh, pts_shift, scene.feats, scene.values, vs,
+                           scene.mlp, dtype)
+    rgb, depth, missed, p = composite(sig, col, sd, sx, si, dtype=dtype)
+    out.update(c_sigma=sig, c_rgb=col, c_probs=p, coarse_rgb=rgb, coarse_depth=depth,
+               coarse_missed=missed)
+    if hierarchical and si.shape[0] > 0:
+        p_use = np.asarray(p, np.float32) if coarse_probs is None else np.asarray(coarse_probs, np.float32)
+        fi, flo, fhi, fp, fst = prepare_fine_bins(si, sd, sx, p_use, fine_steps)
+        si2, sd2, sx2, S2, _ = inverse_cdf_sampling(fi, flo, fhi, fp, fst, -1.0, True, compat=compat)
+        sd2, sx2 = sample_tail(si2, sd2, sx2)
+        out.update(f_idx=si2, f_depth=sd2, f_dist=sx2)
+        sig2, col2 = _field_eval(si2, sd2, sx2, rsh, rdh, pts_shift, scene.feats, scene.values, vs,
+                                 scene.mlp, dtype)
+        rgb, depth, missed, p2 = composite(sig2, col2, sd2, sx2, si2, dtype=dtype)
+        out.update(f_sigma=sig2, f_probs=p2)
+    full_rgb, full_depth, full_missed = fill_background(hits, rgb, depth, missed, dtype=dtype)
+    out.update(rgb=full_rgb, depth=full_depth, missed=full_missed)
+    return out

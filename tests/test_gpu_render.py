@@ -1,0 +1,197 @@
+"""The fused render pipeline on the GPU against the oracle, stage by stage,
+on the spherical-shell scene (64x64 view).
+
+Bars: hit flags, sorted hit rows, sample counts and sample streams bit-exact
+(both sides use the same FP32 operation order); field outputs and images
+within the stated tolerances (FP32 MFMA: 1e-4 on sigma, 1e-5 on the image
+when both sides sample the same points; BF16 MFMA: error reported, PSNR
+against FP64 >= 40 dB)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import nsvf_oracle as O
+from _cases import shell, shell_rays
+from nsvf_amd import _lib
+from nsvf_amd.render import DeviceScene, Renderer
+
+pytestmark = pytest.mark.gpu
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+@pytest.fixture(scope="module")
+def setup(device):
+    sc = shell()
+    rs, rd = shell_rays(width=64)
+    dsc = DeviceScene(sc, device)
+    ref = O.render_view(sc, rs, rd)
+    return sc, rs, rd, dsc, ref
+
+
+def run(dsc, rs, rd, device, precision="bf16", hierarchical=True):
+    r = Renderer(device, precision=precision, hierarchical=hierarchical)
+    rgb, depth, missed = r.render(dsc, torch.from_numpy(rs).to(device),
+                                  torch.from_numpy(rd).to(device), keep=True)
+    torch.cuda.synchronize()
+    return r, N(rgb), N(depth), N(missed)
+
+
+def csr_to_padded(cnt, off, arr, width, fill):
+    out = np.full((len(cnt), width), fill, dtype=arr.dtype)
+    for n in range(len(cnt)):
+        out[n, : cnt[n]] = arr[off[n]: off[n] + cnt[n]]
+    return out
+
+
+def test_hits_and_coarse_samples_bit_exact(setup, device):
+    sc, rs, rd, dsc, ref = setup
+    r, *_ = run(dsc, rs, rd, device)
+    k = r.kept
+    hits = N(k["flag"]).astype(bool)
+    assert np.array_equal(hits, ref["hits"])
+    cnt = N(k["count"])
+    rows = N(k["hits"])
+    ridx, rlo, rhi = ref["sorted_idx"], ref["sorted_lo"], ref["sorted_hi"]
+    for ray in np.nonzero(hits)[0]:
+        n = cnt[ray]
+        assert n == (ridx[ray] != -1).sum()
+        assert np.array_equal(rows[ray, :n, 2], ridx[ray, :n])
+        assert np.array_equal(rows[ray, :n, 0].view(np.float32), rlo[ray, :n])
+        assert np.array_equal(rows[ray, :n, 1].view(np.float32), rhi[ray, :n])
+    # tot / steps: the same left-to-right sum
+    steps = N(k["tot"])[hits] / np.float32(sc.step_size)
+    assert np.array_equal(steps.astype(np.float32), ref["steps"])
+    # hit-ray order and coarse sample stream
+    assert np.array_equal(N(k["hit_list"]), np.nonzero(hits)[0])
+    c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
+    ci = ref["c_idx"]
+    assert np.array_equal(c_cnt, (ci != -1).sum(1))
+    W = ci.shape[1]
+    si = csr_to_padded(c_cnt, c_off, N(k["c_samples"][0]), W, -1)
+    sd = csr_to_padded(c_cnt, c_off, N(k["c_samples"][1]), W, np.float32(O.MAX_DEPTH))
+    sx = csr_to_padded(c_cnt, c_off, N(k["c_samples"][2]), W, np.float32(0))
+    assert np.array_equal(si, ci)
+    assert np.array_equal(sd, ref["c_depth"])
+    assert np.array_equal(sx, ref["c_dist"])
+
+
+@pytest.mark.parametrize("precision,tol_sigma,tol_rgb", [("fp32", 2e-4, 2e-5), ("bf16", 0.15, 0.03)])
+def test_field_matches_fp64(setup, device, precision, tol_sigma, tol_rgb):
+    sc, rs, rd, dsc, ref = setup
+    r, *_ = run(dsc, rs, rd, device, precision=precision)
+    k = r.kept
+    c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
+    W = ref["c_idx"].shape[1]
+    f = N(k["c_field"])
+    sig = csr_to_padded(c_cnt, c_off, f[:, 0], W, np.float32(0))
+    rgb = np.stack([csr_to_padded(c_cnt, c_off, f[:, c], W, np.float32(0)) for c in (1, 2, 3)], -1)
+    m = ref["c_idx"] != -1
+    es = np.abs(sig[m] - ref["c_sigma"][m]).max()
+    ec = np.abs(rgb[m] - ref["c_rgb"][m]).max()
+    print(f"{precision}: max |d sigma| {es:.3e}, max |d rgb| {ec:.3e}")
+    assert es < tol_sigma and ec < tol_rgb
+
+
+def test_fine_samples_bit_exact_given_coarse_probs(setup, device):
+    sc, rs, rd, dsc, ref = setup
+    r, *_ = run(dsc, rs, rd, device, precision="fp32")
+    k = r.kept
+    c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
+    W = ref["c_idx"].shape[1]
+    gp = csr_to_padded(c_cnt, c_off, N(k["c_prob"])[: c_off[-1] + c_cnt[-1]], W, np.float32(0))
+    ref2 = O.render_view(sc, rs, rd, coarse_probs=gp)
+    f_cnt, f_off = N(k["f_cnt"]), N(k["f_off"])
+    fi = ref2["f_idx"]
+    assert np.array_equal(f_cnt, (fi != -1).sum(1))
+    W2 = fi.shape[1]
+    assert np.array_equal(csr_to_padded(f_cnt, f_off, N(k["f_samples"][0]), W2, -1), fi)
+    assert np.array_equal(csr_to_padded(f_cnt, f_off, N(k["f_samples"][1]), W2,
+                                        np.float32(O.MAX_DEPTH)), ref2["f_depth"])
+    assert np.array_equal(csr_to_padded(f_cnt, f_off, N(k["f_samples"][2]), W2, np.float32(0)),
+                          ref2["f_dist"])
+
+
+def psnr(a, b):
+    mse = float(np.mean((np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2))
+    return float("inf") if mse == 0 else 10 * math.log10(1.0 / mse)
+
+
+def test_image_fp32_within_1e5_of_fp64(setup, device):
+    """Same sample points on both sides (the oracle's fine sampler is fed the
+    GPU's coarse probabilities): RGB, depth and alpha within 1e-5 (depth
+    within 1e-5 relative to the 5.0 background depth scale)."""
+    sc, rs, rd, dsc, ref = setup
+    r, rgb, depth, missed = run(dsc, rs, rd, device, precision="fp32")
+    k = r.kept
+    c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
+    W = ref["c_idx"].shape[1]
+    gp = csr_to_padded(c_cnt, c_off, N(k["c_prob"])[: c_off[-1] + c_cnt[-1]], W, np.float32(0))
+    ref2 = O.render_view(sc, rs, rd, coarse_probs=gp)
+    e_rgb = np.abs(rgb - ref2["rgb"]).max()
+    e_alpha = np.abs(missed - ref2["missed"]).max()
+    e_depth = np.abs(depth - ref2["depth"]).max()
+    print(f"fp32 vs fp64: rgb {e_rgb:.2e} alpha {e_alpha:.2e} depth {e_depth:.2e}")
+    assert e_rgb < 1e-5 and e_alpha < 1e-5 and e_depth < 5e-5
+
+
+def test_image_bf16_against_fp64(setup, device):
+    sc, rs, rd, dsc, ref = setup
+    r, rgb, depth, missed = run(dsc, rs, rd, device, precision="bf16")
+    p = psnr(rgb, ref["rgb"])
+    e = np.abs(rgb - ref["rgb"]).max()
+    print(f"bf16 vs fp64: PSNR {p:.2f} dB, max |d rgb| {e:.3e}, "
+          f"max |d alpha| {np.abs(missed - ref['missed']).max():.3e}")
+    assert p > 40.0
+
+
+def test_coarse_only_matches(setup, device):
+    sc, rs, rd, dsc, ref = setup
+    r, rgb, depth, missed = run(dsc, rs, rd, device, precision="fp32", hierarchical=False)
+    ref1 = O.render_view(sc, rs, rd, hierarchical=False)
+    assert np.abs(rgb - ref1["rgb"]).max() < 1e-5
+    assert np.abs(missed - ref1["missed"]).max() < 1e-5
+
+
+def test_trilinear_gather_op(setup, device):
+    sc, rs, rd, dsc, ref = setup
+    rng = np.random.default_rng(0)
+    M = 5000
+    v = rng.integers(0, sc.num_voxels, M).astype(np.int32)
+    cen = O.shift_centers(sc.points, sc.voxel_size)
+    x = (cen[v] + rng.uniform(-0.5, 0.5, (M, 3)).astype(np.float32) * np.float32(sc.voxel_size)).astype(np.float32)
+    out = torch.empty((M, sc.embed_dim), device=device)
+    import ctypes
+    L = _lib.lib()
+    vt, xt = torch.from_numpy(v).to(device), torch.from_numpy(x).to(device)
+    _lib.check(L.nsvf_trilinear_gather(ctypes.c_void_p(vt.data_ptr()), ctypes.c_void_p(xt.data_ptr()),
+                                       M, ctypes.c_void_p(dsc.centers.data_ptr()),
+                                       ctypes.c_void_p(dsc.feats.data_ptr()),
+                                       ctypes.c_void_p(dsc.values.data_ptr()), sc.embed_dim,
+                                       sc.voxel_size, ctypes.c_void_p(out.data_ptr()), None), "gather")
+    torch.cuda.synchronize()
+    g = N(out)
+    ref32 = O.trilinear_gather(v, x, cen, sc.feats, sc.values, sc.voxel_size, dtype=np.float32)
+    ref64 = O.trilinear_gather(v, x, cen, sc.feats, sc.values, sc.voxel_size, dtype=np.float64)
+    assert np.array_equal(g, ref32)
+    assert np.abs(g - ref64).max() < 1e-6
+
+
+def test_scan(device):
+    import ctypes
+    L = _lib.lib()
+    for n in (1, 7, 2048, 2049, 100_000, 5_000_000):
+        c = torch.randint(0, 50, (n,), dtype=torch.int32, device=device)
+        out = torch.empty_like(c)
+        ws = torch.empty(int(L.nsvf_scan_workspace_ints(n)), dtype=torch.int32, device=device)
+        tot = torch.zeros(1, dtype=torch.int32, device=device)
+        _lib.check(L.nsvf_scan_exclusive_i32(ctypes.c_void_p(c.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                             n, ctypes.c_void_p(ws.data_ptr()),
+                                             ctypes.c_void_p(tot.data_ptr()), None), "scan")
+        ref = torch.cumsum(c.long(), 0) - c.long()
+        assert torch.equal(out.long(), ref)
+        assert int(tot) == int(c.long().sum())
