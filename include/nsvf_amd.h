@@ -241,6 +241,33 @@ int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const int32_t* rank,
                        float bg_b, float bg_depth, float* rgb, float* depth,
                        float* missed, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Comparator kernels: the same ops in the reference's naive form (one thread
+ * per ray, the reference's launch geometry of one workgroup per batch row,
+ * no LDS, a 256-entry per-thread stack in scratch memory), used only to
+ * measure the plain FP32 baseline.  Inputs are per-batch copies as the
+ * reference's Python wrappers pass them; outputs as the fast ops
+ * (inverse-CDF: the caller pre-fills idx -1 and depths 0, as sample.cpp:81-90
+ * does).
+ * ---------------------------------------------------------------------- */
+int nsvf_plain_aabb_intersect(const float* ray_start, const float* ray_dir,
+                              const float* points, int B, int M, int N,
+                              float voxelsize, int n_max, int32_t* idx,
+                              float* min_depth, float* max_depth, void* stream);
+int nsvf_plain_svo_intersect(const float* ray_start, const float* ray_dir,
+                             const float* points, const int32_t* children,
+                             int B, int M, int T, float voxelsize, int n_max,
+                             int32_t* idx, float* min_depth, float* max_depth,
+                             void* stream);
+int nsvf_plain_inverse_cdf_sampling(const int32_t* pts_idx,
+                                    const float* min_depth,
+                                    const float* max_depth, const float* noise,
+                                    const float* probs, const float* steps,
+                                    int B, int R, int P, int S,
+                                    float fixed_step_size, int32_t* out_idx,
+                                    float* out_depth, float* out_dists,
+                                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -100,7 +100,14 @@ class Renderer:
         self.compat = int(bool(compat))
         self._ws = {}
         self.last = None
-        self.trace = None   # optional list of (stage, start_event, end_event)
+        self.trace = None   # optional list of (stage, event) marks
+        self._kernel_events = None  # optional list of (kernel, units, start, end)
+
+    def record_kernels(self, on=True):
+        """Start (or stop) recording events around the field-kernel launches;
+        returns the list the records go to."""
+        self._kernel_events = [] if on else None
+        return self._kernel_events
 
     def _buf(self, name, n, dtype):
         t = self._ws.get(name)
@@ -123,10 +130,18 @@ class Renderer:
         L = lib()
         prec = PRECISIONS[self.precision]
         w = sc.mlp["bf16"] if prec == 0 else sc.mlp["f32"]
+        ev = self._kernel_events is not None and m > 0
+        if ev:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record()
         check(L.nsvf_field_eval(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
                                 sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples[0]),
                                 _p(samples[1]), _p(samples[3]), _p(m_dev), m, _p(w),
                                 _p(sc.mlp["bias"]), _p(out), _stream(self.device)), "field_eval")
+        if ev:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            self._kernel_events.append(("field", m, e0, e1))
 
     def _scan(self, counts, offsets, n, total):
         ws = self._buf("scan_ws", int(lib().nsvf_scan_workspace_ints(n)), torch.int32)

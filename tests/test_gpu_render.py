@@ -195,3 +195,18 @@ def test_scan(device):
         ref = torch.cumsum(c.long(), 0) - c.long()
         assert torch.equal(out.long(), ref)
         assert int(tot) == int(c.long().sum())
+
+
+def test_plain_path_matches_fused(setup, device):
+    """The plain FP32 comparator (reference orchestration, naive kernels)
+    renders the same image as the fused FP32 path."""
+    from nsvf_amd.plain import PlainNSVF
+    sc, rs, rd, dsc, ref = setup
+    r, rgb, depth, missed = run(dsc, rs, rd, device, precision="fp32")
+    pl = PlainNSVF(sc, device)
+    prgb, pdepth, pmissed = pl.render(torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device))
+    prgb, pmissed = N(prgb), N(pmissed)
+    p = psnr(rgb, prgb)
+    print(f"plain vs fused fp32: PSNR {p:.1f} dB, max |d rgb| {np.abs(rgb - prgb).max():.2e}")
+    assert p > 60.0
+    assert np.abs(rgb - ref["rgb"]).max() < 1e-2
