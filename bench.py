@@ -94,14 +94,20 @@ def main():
     ms_per_step = wall * 1000.0 / args.steps
     value = R / (wall / args.steps)
     st = rnd.last
+    # exact sample counts of the view (identical for every repetition)
+    rnd.render(dsc, rs, rd, keep=True)
+    n_coarse, n_fine = rnd.count_samples()
+    rnd.kept = None
     log(f"fused bf16: {ms_per_step:.2f} ms/view, {value / 1e6:.2f} Mrays/s; hit rays {st.hit_rays}, "
-        f"coarse {st.coarse_samples}, fine {st.fine_samples} samples")
+        f"coarse {n_coarse} samples ({st.coarse_slots} slots), fine {n_fine} samples "
+        f"({st.fine_slots} slots allocated)")
 
-    # roofline of the dominant kernel: the fused gather + MLP (field) kernel
+    # roofline of the dominant kernel: the fused gather + MLP (field) kernel,
+    # algorithmic FLOPs of the samples it evaluates / its HIP-event time
     flops_per_sample = sc.mlp.flops_per_sample()
     f_ms = sum(a.elapsed_time(b) for _, _, a, b in kev)
-    f_samples = sum(m for _, m, _, _ in kev)
     f_launches = len(kev)
+    f_samples = (n_coarse + n_fine) * args.steps
     achieved = flops_per_sample * f_samples / (f_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "kernel": "field_bf16_kernel<32,128,3>",
                 "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
@@ -131,8 +137,8 @@ def main():
                    "parallelism": "single GPU"},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-        "samples_per_view": {"hit_rays": st.hit_rays, "coarse": st.coarse_samples,
-                             "fine": st.fine_samples},
+        "samples_per_view": {"hit_rays": st.hit_rays, "coarse": n_coarse, "fine": n_fine,
+                             "coarse_slots": st.coarse_slots},
         "stages_ms": stages,
         "roofline": roofline,
     }

@@ -167,79 +167,96 @@ int nsvf_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n,
 
 /* Octree traversal with the encoder's post-processing: per ray the hits
  * sorted by entry depth (stable with respect to the reference's traversal
- * order), their count, the sum of segment lengths and a hit flag.
+ * order), their count, the sum of segment lengths, a hit flag, and an upper
+ * bound on the ray's coarse sample count (ceil(sum / step_size) + count + 2,
+ * 0 for rays that hit nothing).
+ *   centers [T,3] / children [T,9]: the octree in the reference layout;
+ *   boxes [T] {float minx, miny, minz; int32 scale; float maxx, maxy, maxz;
+ *   float 0} and kids [T,8]: the same tree packed for a front-to-back
+ *   traversal (scale = children[k][8], kids = children[k][0..7]; a leaf's
+ *   box is fp32(c -+ h), an internal node's the exact bounding box of its
+ *   leaves' boxes), 16-byte aligned.  Rays whose result depends on traversal
+ *   order (n_max reached, equal entry depths) are redone over the reference
+ *   layout in the reference's order.
  *   hits: [R, n_max] records {float lo, float hi, int32 idx, int32 pad};
  *   only the first count[r] records of a row are written.                  */
 int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
-                     const float* centers, const int32_t* children, int T,
-                     float voxelsize, int n_max, void* hits, int32_t* count,
-                     float* tot, int32_t* hit_flag, void* stream);
+                     const float* centers, const int32_t* children,
+                     const void* boxes, const int32_t* kids, int T,
+                     float voxelsize, int n_max, float step_size, void* hits,
+                     int32_t* count, float* tot, int32_t* hit_flag,
+                     int32_t* bound, void* stream);
 
 /* list[rank[r]] = r for every ray with hit_flag[r] != 0.                  */
 int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,
                         int64_t R, int32_t* list, void* stream);
 
-/* Coarse inverse-CDF sampling of the hit rays (deterministic noise 0.5,
- * steps = sum of segment lengths / step_size, probs = length / sum).
- * write = 0: counts_out[n] = valid samples of hit ray n, *max_count = max;
- * write = 1: samples written at offsets[n] into the stream arrays.
- * *n_hit_dev (device) is the number of hit rays, n_hit_max >= it.          */
-int nsvf_render_sample_coarse(int write, const int32_t* hit_list,
-                              const int32_t* n_hit_dev, int64_t n_hit_max,
-                              const void* hits, const int32_t* count,
-                              const float* tot, int n_max, float step_size,
-                              int compat, int32_t* counts_out,
-                              int32_t* max_count, const int32_t* offsets,
-                              int32_t* s_idx, float* s_depth, float* s_dist,
-                              int32_t* s_ray, void* stream);
+/* Sample streams: records {int32 voxel id (-1 = unused slot), float depth,
+ * float length, int32 ray id}; ray r owns slots [off[r], off[r] + bound[r])
+ * (off = exclusive scan of the bounds), the first cnt[r] hold its samples.
+
+ * Coarse inverse-CDF sampling of the hit rays (deterministic noise 0.5,
+ * steps = sum of segment lengths / step_size, probs = length / sum).  Writes
+ * c_cnt[r], the fine-pass bound f_bound[r] = ceil(fine_steps) + c_cnt[r] + 2
+ * and *max_count = the longest row.  *n_hit_dev (device) is the number of
+ * hit rays, n_hit_max >= it.                                               */
+int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
+                              int64_t n_hit_max, const void* hits,
+                              const int32_t* count, const float* tot,
+                              const int32_t* bound, const int32_t* c_off,
+                              int n_max, float step_size, int compat,
+                              float fine_steps, void* samples, int32_t* c_cnt,
+                              int32_t* f_bound, int32_t* max_count,
+                              void* stream);
 
 /* Fine (hierarchical) inverse-CDF sampling: bins are each ray's coarse
- * samples [depth -+ dist/2] with probs (p + 1e-5) / column sum over the
+ * samples [depth -+ length/2] with probs (p + 1e-5) / column sum over the
  * *c_max (device) columns; `fine_steps` CDF steps (fairnr/models/nerf.py:
- * 64-79).  Same write/count protocol as the coarse pass.                   */
-int nsvf_render_sample_fine(int write, const int32_t* hit_list,
-                            const int32_t* n_hit_dev, int64_t n_hit_max,
-                            const int32_t* c_off, const int32_t* c_cnt,
-                            const int32_t* c_max, const int32_t* c_idx,
-                            const float* c_depth, const float* c_dist,
-                            const float* c_prob, float fine_steps, int compat,
-                            int32_t* counts_out, int32_t* max_count,
-                            const int32_t* offsets, int32_t* s_idx,
-                            float* s_depth, float* s_dist, int32_t* s_ray,
-                            void* stream);
+ * 64-79).  Writes the fine stream and f_cnt[r].                             */
+int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev,
+                            int64_t n_hit_max, const int32_t* c_off,
+                            const int32_t* c_cnt, const int32_t* c_max,
+                            const void* coarse, const float* c_prob,
+                            const int32_t* f_bound, const int32_t* f_off,
+                            float fine_steps, int compat, void* samples,
+                            int32_t* f_cnt, void* stream);
 
 /* Field evaluation of a sample stream: trilinear gather fused with the MLP
  * (L hidden ReLU layers of width H on a D-wide embedding, 4-wide head).
  *   precision 0: BF16 MFMA with FP32 accumulation, wfrag = packed bf16;
  *   precision 1: FP32 MFMA, wfrag = packed f32.
+ *   samples: *m_dev records (m_max >= *m_dev); unused slots are skipped.
  *   out [M,4] = (sigma, sigmoid(r), sigmoid(g), sigmoid(b)).
+ *   workspace (device, optional): with at least
+ *   nsvf_field_workspace_bytes(...) bytes, BF16 runs as a gather kernel
+ *   writing operand tiles plus an MLP kernel; with NULL, as one fused kernel.
  *   Supported shapes: nsvf_field_supported(D, H, L).                        */
 int nsvf_field_supported(int D, int H, int L);
+int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L,
+                                   int64_t m_max);
 int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
                     const int32_t* feats, const float* values, float voxel_size,
                     const float* ray_start, const float* ray_dir,
-                    const int32_t* s_idx, const float* s_depth,
-                    const int32_t* s_ray, const int32_t* m_dev, int64_t m_max,
+                    const void* samples, const int32_t* m_dev, int64_t m_max,
                     const void* wfrag, const float* bias, float* out,
-                    void* stream);
+                    void* workspace, int64_t workspace_bytes, void* stream);
 
 /* Front-to-back compositing of each hit ray's samples (one wavefront per
- * ray): free energy relu(sigma) * dist * 7, probs, and the ray's rgb, depth
- * and missed = 1 - sum(probs).  prob_out [M] optional.                      */
+ * ray): free energy relu(sigma) * length * 7, probs, and the ray's rgb,
+ * depth and missed = 1 - sum(probs), indexed by ray id.  prob_out [M]
+ * optional (slot-aligned with the stream).                                  */
 int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev,
                           int64_t n_hit_max, const int32_t* off,
-                          const int32_t* cnt, const float* depth,
-                          const float* dist, const float* field,
-                          float* prob_out, float* ray_rgb, float* ray_depth,
-                          float* ray_missed, void* stream);
+                          const int32_t* cnt, const void* samples,
+                          const float* field, float* prob_out, float* ray_rgb,
+                          float* ray_depth, float* ray_missed, void* stream);
 
 /* Frame outputs for all R rays: hit rays' composite plus missed * background
  * (fairnr/models/nsvf.py:89-101); rays that hit nothing get the background. */
-int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const int32_t* rank,
-                       const float* ray_rgb, const float* ray_depth,
-                       const float* ray_missed, float bg_r, float bg_g,
-                       float bg_b, float bg_depth, float* rgb, float* depth,
-                       float* missed, void* stream);
+int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const float* ray_rgb,
+                       const float* ray_depth, const float* ray_missed,
+                       float bg_r, float bg_g, float bg_b, float bg_depth,
+                       float* rgb, float* depth, float* missed, void* stream);
 
 /* ------------------------------------------------------------------------
  * Comparator kernels: the same ops in the reference's naive form (one thread

@@ -35,17 +35,17 @@ SIGNATURES = {
     "nsvf_trilinear_gather": ([_P, _P, _L, _P, _P, _P, _I, _F, _P, _P], _I),
     "nsvf_scan_workspace_ints": ([_L], _L),
     "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
-    "nsvf_render_hits": ([_P, _P, _L, _P, _P, _I, _F, _I, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_hits": ([_P, _P, _L, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),
-    "nsvf_render_sample_coarse": ([_I, _P, _P, _L, _P, _P, _P, _I, _F, _I, _P, _P, _P,
-                                   _P, _P, _P, _P, _P], _I),
-    "nsvf_render_sample_fine": ([_I, _P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P,
-                                 _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_sample_coarse": ([_P, _P, _L, _P, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P,
+                                   _P, _P], _I),
+    "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P, _P], _I),
     "nsvf_field_supported": ([_I, _I, _I], _I),
-    "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _P, _L, _P, _P,
-                         _P, _P], _I),
-    "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
-    "nsvf_render_finish": ([_L, _P, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
+    "nsvf_field_workspace_bytes": ([_I, _I, _I, _I, _L], _L),
+    "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _P, _P, _P, _L,
+                         _P], _I),
+    "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_finish": ([_L, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
     "nsvf_plain_aabb_intersect": ([_P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
     "nsvf_plain_svo_intersect": ([_P, _P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
     "nsvf_plain_inverse_cdf_sampling": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P, _P, _P,

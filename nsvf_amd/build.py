@@ -10,7 +10,10 @@ Every HIP source is compiled with:
   -fhip-fp32-correctly-rounded-divide-sqrt
                             IEEE division and square root;
   -Xclang -target-feature -Xclang -packed-fp32-ops
-                            no packed FP32 (v_pk_*_f32) instructions.
+                            no packed FP32 (v_pk_*_f32) instructions;
+  -mllvm -amdgpu-mfma-vgpr-form=1
+                            MFMA accumulators in VGPRs, so the layer
+                            epilogues read them without v_accvgpr_read.
 """
 from __future__ import annotations
 
@@ -33,6 +36,7 @@ HIP_FLAGS = [
     "-fno-fast-math",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wall",
     "-Wno-unused-function",
 ]

@@ -25,6 +25,7 @@
 // Weight fragments: [layer][m-tile][k-tile][64 lanes][8 bf16 | 1 f32],
 // biases fp32 [layer][rows padded to 16].  BF16 fragments are staged in LDS.
 #include <algorithm>
+#include <cmath>
 
 #include "render.h"
 
@@ -41,6 +42,8 @@ struct FieldScene {
   const int32_t* feats;   // [N,8] corner key ids
   const float* values;    // [K,D] corner embeddings
   float voxel_size;
+  float inv_voxel;        // 1 / voxel_size, used only when exact
+  int voxel_pow2;         // voxel_size is a power of two: x / vs == x * (1/vs)
 };
 
 struct FieldRays {
@@ -92,6 +95,74 @@ __device__ __forceinline__ void gather8(const FieldScene& sc, const float w[8],
   }
 }
 
+// Corner weights for the BF16 field kernel: the same formula, with the
+// division by the voxel size done as an exact multiply when the size is a
+// power of two.
+__device__ __forceinline__ void corner_weights_fast(const FieldScene& sc, const FieldRays& rays,
+                                                    int32_t v, float z, int32_t r, float w[8],
+                                                    int32_t key[8]) {
+  float p[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float x = rays.start[int64_t(r) * 3 + a] + rays.dir[int64_t(r) * 3 + a] * z;
+    const float d = x - sc.centers[int64_t(v) * 3 + a];
+    p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
+  }
+  const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+  const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
+    key[c] = sc.feats[int64_t(v) * 8 + c];
+  }
+}
+
+// gather8 with fused multiply-adds (the BF16 path rounds to bf16 afterwards).
+template <int D>
+__device__ __forceinline__ void gather8_fma(const FieldScene& sc, const float w[8],
+                                            const int32_t key[8], int kt, int g, float out[8]) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = 0.0f;
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    const int f0 = 32 * kt + 16 * half + 4 * g;
+    if (f0 < D) {
+      float4 e[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        e[c] = *reinterpret_cast<const float4*>(sc.values + int64_t(key[c]) * D + f0);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        out[4 * half + 0] = __fmaf_rn(w[c], e[c].x, out[4 * half + 0]);
+        out[4 * half + 1] = __fmaf_rn(w[c], e[c].y, out[4 * half + 1]);
+        out[4 * half + 2] = __fmaf_rn(w[c], e[c].z, out[4 * half + 2]);
+        out[4 * half + 3] = __fmaf_rn(w[c], e[c].w, out[4 * half + 3]);
+      }
+    }
+  }
+}
+
+typedef __attribute__((ext_vector_type(2))) short s16x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+// ReLU then round to bf16 == round then ReLU (rounding keeps the sign); the
+// ReLU runs on the packed bf16 pair as a signed 16-bit max with 0.
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+
+__device__ __forceinline__ unsigned int relu_pack(float a, float b) {
+  const bf16x2 v = __builtin_convertvector(f32x2{a, b}, bf16x2);
+  s16x2 t = __builtin_bit_cast(s16x2, v);
+  t = __builtin_elementwise_max(t, s16x2{0, 0});
+  return __builtin_bit_cast(unsigned int, t);
+}
+
+__device__ __forceinline__ bf16x8 relu_pack8(const f32x4& lo, const f32x4& hi) {
+  u32x4 u = {relu_pack(lo[0], lo[1]), relu_pack(lo[2], lo[3]), relu_pack(hi[0], hi[1]),
+             relu_pack(hi[2], hi[3])};
+  return __builtin_bit_cast(bf16x8, u);
+}
+
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
 
 template <int D, int H, int L>
@@ -114,10 +185,94 @@ constexpr bool weights_in_lds() {
   return Dims<D, H, L>::FRAGS * 1024 <= 96 * 1024;
 }
 
+// The MLP on one wave's tile of SAMPLES samples: operand B of layer 0 in
+// b0, weights lw / biases lb (LDS or global), outputs of samples < M to out.
+template <int D, int H, int L>
+__device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][Dims<D, H, L>::NT],
+                                         const bf16x8* lw, const float* lb, int lane, int g,
+                                         int col, int64_t base, int64_t M,
+                                         float4* __restrict__ out) {
+  using Dm = Dims<D, H, L>;
+  constexpr int NT = Dm::NT;
+  // ---- layer 0: bias enters as the first MFMA's C operand
+  bf16x8 bh[Dm::KTH][NT];
+#pragma unroll
+  for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int mt = 2 * kt2 + half;
+      const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + 16 * mt + 4 * g);
+#pragma unroll
+      for (int kt = 0; kt < Dm::KT0; ++kt) {
+        const bf16x8 a = lw[(mt * Dm::KT0 + kt) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              a, b0[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bh[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
+  }
+  // ---- hidden layers 1..L-1
+#pragma unroll 1
+  for (int l = 1; l < L; ++l) {
+    const bf16x8* A = lw + (Dm::F0 + (l - 1) * Dm::FH) * 64;
+    const float* B = lb + H * l;
+    bf16x8 nb[Dm::KTH][NT];
+#pragma unroll
+    for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
+      f32x4 acc[2][NT];
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int mt = 2 * kt2 + half;
+        const f32x4 bias4 = *reinterpret_cast<const f32x4*>(B + 16 * mt + 4 * g);
+#pragma unroll
+        for (int kt = 0; kt < Dm::KTH; ++kt) {
+          const bf16x8 a = A[(mt * Dm::KTH + kt) * 64 + lane];
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                a, bh[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NT; ++j) nb[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
+    }
+#pragma unroll
+    for (int kt = 0; kt < Dm::KTH; ++kt)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
+  }
+  // ---- head (4 outputs in rows 0..3 of one m-tile)
+  {
+    const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
+    const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + H * L + 4 * g);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int kt = 0; kt < Dm::KTH; ++kt) {
+      const bf16x8 a = A[kt * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], kt == 0 ? bias4 : acc[j],
+                                                         0, 0, 0);
+    }
+    if (g == 0) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int64_t s = base + 16 * j + col;
+        if (s < M)
+          out[s] = make_float4(acc[j][0], sigmoidf(acc[j][1]), sigmoidf(acc[j][2]),
+                               sigmoidf(acc[j][3]));
+      }
+    }
+  }
+}
+
 template <int D, int H, int L>
 __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
-    FieldScene sc, FieldRays rays, const int32_t* __restrict__ s_idx,
-    const float* __restrict__ s_depth, const int32_t* __restrict__ s_ray, const int32_t* m_dev,
+    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
     int64_t m_max, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
     float4* __restrict__ out) {
   using Dm = Dims<D, H, L>;
@@ -154,21 +309,26 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
       const int64_t s = base + 16 * j + col;
       float w[8];
       int32_t key[8];
-      const bool valid = s < M;
-      if (valid) corner_weights(sc, rays, s_idx[s], s_depth[s], s_ray[s], w, key);
+      SampleRec rec{-1, 0.0f, 0.0f, 0};
+      if (s < M) rec = samples[s];
+      const bool valid = rec.idx >= 0;  // -1: an unused slot of a ray's range
+      if (valid) corner_weights_fast(sc, rays, rec.idx, rec.depth, rec.ray, w, key);
 #pragma unroll
       for (int kt = 0; kt < Dm::KT0; ++kt) {
         float f[8];
         if (valid) {
-          gather8<D>(sc, w, key, kt, g, f);
+          gather8_fma<D>(sc, w, key, kt, g, f);
         } else {
 #pragma unroll
           for (int i = 0; i < 8; ++i) f[i] = 0.0f;
         }
-        bf16x8 v;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = (__bf16)f[i];
-        stage[(j * Dm::KT0 + kt) * 64 + lane] = v;
+        const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
+        const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
+        const bf16x2 p2 = __builtin_convertvector(f32x2{f[4], f[5]}, bf16x2);
+        const bf16x2 p3 = __builtin_convertvector(f32x2{f[6], f[7]}, bf16x2);
+        const u32x4 u = {__builtin_bit_cast(unsigned int, p0), __builtin_bit_cast(unsigned int, p1),
+                         __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
+        stage[(j * Dm::KT0 + kt) * 64 + lane] = __builtin_bit_cast(bf16x8, u);
       }
     }
     bf16x8 b0[Dm::KT0][NT];
@@ -176,96 +336,83 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
     for (int j = 0; j < NT; ++j)
 #pragma unroll
       for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = stage[(j * Dm::KT0 + kt) * 64 + lane];
-    // ---- layer 0
-    bf16x8 bh[Dm::KTH][NT];
-    {
-      const bf16x8* A = lw;
+    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, base, M, out);
+  }
+}
+
+// Gather kernel of the split BF16 path: one thread per (tile, n-tile, k-tile,
+// lane) writes the 8 bf16 operand values that lane of the MLP kernel loads, so
+// both the writes here and the MLP kernel's loads are contiguous 16 bytes per
+// lane.  Many waves per SIMD hide the gather's dependent loads.
+template <int D, int H, int L>
+__global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRays rays,
+                                                         const SampleRec* __restrict__ samples,
+                                                         const int32_t* m_dev, int64_t m_max,
+                                                         bf16x8* __restrict__ emb) {
+  using Dm = Dims<D, H, L>;
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t total = ((M + Dm::SAMPLES - 1) / Dm::SAMPLES) * Dm::NT * Dm::KT0 * 64;
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= total) return;
+  const int lane = int(i & 63);
+  const int64_t q = i >> 6;
+  const int kt = int(q % Dm::KT0);
+  const int64_t tj = q / Dm::KT0;           // t * NT + j
+  const int j = int(tj % Dm::NT);
+  const int64_t t = tj / Dm::NT;
+  const int64_t s = t * Dm::SAMPLES + 16 * j + (lane & 15);
+  float f[8];
 #pragma unroll
-      for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
-#pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          const int mt = 2 * kt2 + half;
-          f32x4 acc[NT];
-#pragma unroll
-          for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kt = 0; kt < Dm::KT0; ++kt) {
-            const bf16x8 a = A[(mt * Dm::KT0 + kt) * 64 + lane];
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
-              acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0[kt][j], acc[j], 0, 0, 0);
-          }
-          const float4 bb = *reinterpret_cast<const float4*>(lb + 16 * mt + 4 * g);
-#pragma unroll
-          for (int j = 0; j < NT; ++j) {
-            bh[kt2][j][4 * half + 0] = (__bf16)fmaxf(acc[j][0] + bb.x, 0.0f);
-            bh[kt2][j][4 * half + 1] = (__bf16)fmaxf(acc[j][1] + bb.y, 0.0f);
-            bh[kt2][j][4 * half + 2] = (__bf16)fmaxf(acc[j][2] + bb.z, 0.0f);
-            bh[kt2][j][4 * half + 3] = (__bf16)fmaxf(acc[j][3] + bb.w, 0.0f);
-          }
-        }
-      }
+  for (int k = 0; k < 8; ++k) f[k] = 0.0f;
+  if (s < M) {
+    const SampleRec rec = samples[s];
+    if (rec.idx >= 0) {
+      float w[8];
+      int32_t key[8];
+      corner_weights_fast(sc, rays, rec.idx, rec.depth, rec.ray, w, key);
+      gather8_fma<D>(sc, w, key, kt, lane >> 4, f);
     }
-    // ---- hidden layers 1..L-1
-#pragma unroll 1
-    for (int l = 1; l < L; ++l) {
-      const bf16x8* A = lw + (Dm::F0 + (l - 1) * Dm::FH) * 64;
-      const float* B = lb + H * l;
-      bf16x8 nb[Dm::KTH][NT];
+  }
+  const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
+  const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
+  const bf16x2 p2 = __builtin_convertvector(f32x2{f[4], f[5]}, bf16x2);
+  const bf16x2 p3 = __builtin_convertvector(f32x2{f[6], f[7]}, bf16x2);
+  const u32x4 u = {__builtin_bit_cast(unsigned int, p0), __builtin_bit_cast(unsigned int, p1),
+                   __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
+  emb[i] = __builtin_bit_cast(bf16x8, u);
+}
+
+// MLP kernel of the split BF16 path: operand B of layer 0 read from the
+// gather kernel's output.
+template <int D, int H, int L>
+__global__ __launch_bounds__(kFieldBlock) void mlp_bf16_kernel(
+    const bf16x8* __restrict__ emb, const int32_t* m_dev, int64_t m_max,
+    const bf16x8* __restrict__ wfrag, const float* __restrict__ bias, float4* __restrict__ out) {
+  using Dm = Dims<D, H, L>;
+  constexpr int NT = Dm::NT;
+  constexpr bool kLds = weights_in_lds<D, H, L>();
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const bf16x8* lw = wfrag;
+  float* lb = reinterpret_cast<float*>(smem_raw);
+  if constexpr (kLds) {
+    bf16x8* w = reinterpret_cast<bf16x8*>(smem_raw);
+    for (int i = threadIdx.x; i < Dm::FRAGS * 64; i += kFieldBlock) w[i] = wfrag[i];
+    lw = w;
+    lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 64 * sizeof(bf16x8));
+  }
+  for (int i = threadIdx.x; i < Dm::BIASES; i += kFieldBlock) lb[i] = bias[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
+  for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
+       t += int64_t(gridDim.x) * 4) {
+    bf16x8 b0[Dm::KT0][NT];
 #pragma unroll
-      for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
+    for (int j = 0; j < NT; ++j)
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          const int mt = 2 * kt2 + half;
-          f32x4 acc[NT];
-#pragma unroll
-          for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kt = 0; kt < Dm::KTH; ++kt) {
-            const bf16x8 a = A[(mt * Dm::KTH + kt) * 64 + lane];
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
-              acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], acc[j], 0, 0, 0);
-          }
-          const float4 bb = *reinterpret_cast<const float4*>(B + 16 * mt + 4 * g);
-#pragma unroll
-          for (int j = 0; j < NT; ++j) {
-            nb[kt2][j][4 * half + 0] = (__bf16)fmaxf(acc[j][0] + bb.x, 0.0f);
-            nb[kt2][j][4 * half + 1] = (__bf16)fmaxf(acc[j][1] + bb.y, 0.0f);
-            nb[kt2][j][4 * half + 2] = (__bf16)fmaxf(acc[j][2] + bb.z, 0.0f);
-            nb[kt2][j][4 * half + 3] = (__bf16)fmaxf(acc[j][3] + bb.w, 0.0f);
-          }
-        }
-      }
-#pragma unroll
-      for (int kt = 0; kt < Dm::KTH; ++kt)
-#pragma unroll
-        for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
-    }
-    // ---- head (4 outputs in rows 0..3 of one m-tile)
-    {
-      const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
-      f32x4 acc[NT];
-#pragma unroll
-      for (int j = 0; j < NT; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kt = 0; kt < Dm::KTH; ++kt) {
-        const bf16x8 a = A[kt * 64 + lane];
-#pragma unroll
-        for (int j = 0; j < NT; ++j)
-          acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], acc[j], 0, 0, 0);
-      }
-      if (g == 0) {
-        const float* B = lb + H * L;
-#pragma unroll
-        for (int j = 0; j < NT; ++j) {
-          const int64_t s = base + 16 * j + col;
-          if (s < M)
-            out[s] = make_float4(acc[j][0] + B[0], sigmoidf(acc[j][1] + B[1]),
-                                 sigmoidf(acc[j][2] + B[2]), sigmoidf(acc[j][3] + B[3]));
-        }
-      }
-    }
+      for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = emb[((t * NT + j) * Dm::KT0 + kt) * 64 + lane];
+    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, out);
   }
 }
 
@@ -273,8 +420,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
 // global memory (fragments of 64 floats).
 template <int D, int H, int L>
 __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
-    FieldScene sc, FieldRays rays, const int32_t* __restrict__ s_idx,
-    const float* __restrict__ s_depth, const int32_t* __restrict__ s_ray, const int32_t* m_dev,
+    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
     int64_t m_max, const float* __restrict__ wfrag, const float* __restrict__ bias,
     float4* __restrict__ out) {
   constexpr int NT = 2;
@@ -293,8 +439,10 @@ __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
       const int64_t s = base + 16 * j + col;
       float w[8];
       int32_t key[8];
-      const bool valid = s < M;
-      if (valid) corner_weights(sc, rays, s_idx[s], s_depth[s], s_ray[s], w, key);
+      SampleRec rec{-1, 0.0f, 0.0f, 0};
+      if (s < M) rec = samples[s];
+      const bool valid = rec.idx >= 0;  // -1: an unused slot of a ray's range
+      if (valid) corner_weights(sc, rays, rec.idx, rec.depth, rec.ray, w, key);
 #pragma unroll
       for (int kt32 = 0; kt32 < (KT0 + 7) / 8; ++kt32) {
         float f[8];
@@ -420,10 +568,45 @@ __global__ __launch_bounds__(256) void gather_kernel(const int32_t* __restrict__
   out[i] = acc;
 }
 
+// True when v is a (normal) power of two, so that x / v == x * (1 / v)
+// exactly for every x.
+int is_pow2(float v) {
+  int e;
+  const float m = std::frexp(v, &e);
+  return (m == 0.5f && v > 0.0f && std::isnormal(v) && std::isnormal(1.0f / v)) ? 1 : 0;
+}
+
 template <int D, int H, int L>
-int launch_bf16(FieldScene sc, FieldRays rays, const int32_t* s_idx, const float* s_depth,
-                const int32_t* s_ray, const int32_t* m_dev, int64_t m_max, const void* wfrag,
-                const float* bias, float* out, hipStream_t stream) {
+int64_t bf16_workspace_bytes(int64_t m_max) {
+  using Dm = Dims<D, H, L>;
+  return ceil_div(m_max, Dm::SAMPLES) * Dm::NT * Dm::KT0 * 64 * 16;
+}
+
+template <int D, int H, int L>
+int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
+                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                      float* out, void* workspace, hipStream_t stream) {
+  using Dm = Dims<D, H, L>;
+  bf16x8* emb = static_cast<bf16x8*>(workspace);
+  const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
+  const int64_t threads = tiles * Dm::NT * Dm::KT0 * 64;
+  hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(ceil_div(threads, 256))),
+                     dim3(256), 0, stream, sc, rays, samples, m_dev, m_max, emb);
+  int rc = check_launch("field_embed_bf16");
+  if (rc) return rc;
+  const size_t lds = (weights_in_lds<D, H, L>() ? size_t(Dm::FRAGS) * 1024 : 0) +
+                     size_t(Dm::BIASES) * sizeof(float);
+  const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
+  hipLaunchKernelGGL((mlp_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), lds,
+                     stream, emb, m_dev, m_max, static_cast<const bf16x8*>(wfrag), bias,
+                     reinterpret_cast<float4*>(out));
+  return check_launch("field_mlp_bf16");
+}
+
+template <int D, int H, int L>
+int launch_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples, const int32_t* m_dev,
+                int64_t m_max, const void* wfrag, const float* bias, float* out,
+                hipStream_t stream) {
   using Dm = Dims<D, H, L>;
   const size_t lds = (weights_in_lds<D, H, L>() ? size_t(Dm::FRAGS) * 1024 : 0) +
                      ((size_t(Dm::BIASES) * sizeof(float) + 15) & ~size_t(15)) +
@@ -431,20 +614,20 @@ int launch_bf16(FieldScene sc, FieldRays rays, const int32_t* s_idx, const float
   const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
   const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
   hipLaunchKernelGGL((field_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), lds,
-                     stream, sc, rays, s_idx, s_depth, s_ray, m_dev, m_max,
-                     static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out));
+                     stream, sc, rays, samples, m_dev, m_max, static_cast<const bf16x8*>(wfrag), bias,
+                     reinterpret_cast<float4*>(out));
   return check_launch("field_bf16");
 }
 
 template <int D, int H, int L>
-int launch_f32(FieldScene sc, FieldRays rays, const int32_t* s_idx, const float* s_depth,
-               const int32_t* s_ray, const int32_t* m_dev, int64_t m_max, const void* wfrag,
-               const float* bias, float* out, hipStream_t stream) {
+int launch_f32(FieldScene sc, FieldRays rays, const SampleRec* samples, const int32_t* m_dev,
+               int64_t m_max, const void* wfrag, const float* bias, float* out,
+               hipStream_t stream) {
   const int64_t tiles = ceil_div(m_max, 32);
   const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
   hipLaunchKernelGGL((field_f32_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), 0,
-                     stream, sc, rays, s_idx, s_depth, s_ray, m_dev, m_max,
-                     static_cast<const float*>(wfrag), bias, reinterpret_cast<float4*>(out));
+                     stream, sc, rays, samples, m_dev, m_max, static_cast<const float*>(wfrag), bias,
+                     reinterpret_cast<float4*>(out));
   return check_launch("field_f32");
 }
 
@@ -461,27 +644,41 @@ int nsvf_field_supported(int D, int H, int L) {
          (D == 32 && H == 256 && L == 4);
 }
 
+int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L, int64_t m_max) {
+  if (precision != 0 || m_max <= 0) return 0;
+#define NSVF_WS_CASE(d, h, l) \
+  if (D == d && H == h && L == l) return bf16_workspace_bytes<d, h, l>(m_max);
+  NSVF_WS_CASE(32, 128, 3)
+  NSVF_WS_CASE(48, 256, 4)
+  NSVF_WS_CASE(32, 256, 4)
+#undef NSVF_WS_CASE
+  return 0;
+}
+
 int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
                     const int32_t* feats, const float* values, float voxel_size,
-                    const float* ray_start, const float* ray_dir, const int32_t* s_idx,
-                    const float* s_depth, const int32_t* s_ray, const int32_t* m_dev,
-                    int64_t m_max, const void* wfrag, const float* bias, float* out,
-                    void* stream) {
+                    const float* ray_start, const float* ray_dir, const void* samples,
+                    const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                    float* out, void* workspace, int64_t workspace_bytes, void* stream) {
   clear_error();
   if (m_max <= 0) return NSVF_OK;
-  if (!centers || !feats || !values || !ray_start || !ray_dir || !s_idx || !s_depth || !s_ray ||
-      !m_dev || !wfrag || !bias || !out)
+  if (!centers || !feats || !values || !ray_start || !ray_dir || !samples || !m_dev || !wfrag ||
+      !bias || !out)
     return set_error(NSVF_EINVAL, "field_eval: null pointer");
-  FieldScene sc{centers, feats, values, voxel_size};
+  FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
   FieldRays rays{ray_start, ray_dir};
   hipStream_t st = as_stream(stream);
 #define NSVF_FIELD_CASE(d, h, l)                                                               \
   if (D == d && H == h && L == l) {                                                            \
+    if (precision == 0 && workspace &&                                                        \
+        workspace_bytes >= bf16_workspace_bytes<d, h, l>(m_max))                               \
+      return launch_bf16_split<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
+                                        m_dev, m_max, wfrag, bias, out, workspace, st);        \
     if (precision == 0)                                                                        \
-      return launch_bf16<d, h, l>(sc, rays, s_idx, s_depth, s_ray, m_dev, m_max, wfrag, bias, \
-                                  out, st);                                                    \
-    return launch_f32<d, h, l>(sc, rays, s_idx, s_depth, s_ray, m_dev, m_max, wfrag, bias,    \
-                               out, st);                                                       \
+      return launch_bf16<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), m_dev,     \
+                                  m_max, wfrag, bias, out, st);                                \
+    return launch_f32<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), m_dev, m_max, \
+                               wfrag, bias, out, st);                                          \
   }
   NSVF_FIELD_CASE(32, 128, 3)
   NSVF_FIELD_CASE(48, 256, 4)
@@ -498,7 +695,7 @@ int nsvf_trilinear_gather(const int32_t* s_idx, const float* xyz, int64_t M, con
   if (M == 0) return NSVF_OK;
   if (!s_idx || !xyz || !centers || !feats || !values || !out)
     return set_error(NSVF_EINVAL, "trilinear_gather: null pointer");
-  FieldScene sc{centers, feats, values, voxel_size};
+  FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
   hipLaunchKernelGGL(gather_kernel, dim3(unsigned(ceil_div(M * D, 256))), dim3(256), 0,
                      as_stream(stream), s_idx, xyz, M, D, sc, out);
   return check_launch("trilinear_gather");

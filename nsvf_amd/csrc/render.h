@@ -15,12 +15,15 @@ struct HitRec {
 };
 static_assert(sizeof(HitRec) == 16, "HitRec must be 16 bytes");
 
-// A compacted sample stream (structure of arrays, one entry per sample).
-struct SampleStream {
-  int32_t* idx;    // voxel id
-  float* depth;    // midpoint depth along the ray
-  float* dist;     // interval length
-  int32_t* ray;    // ray id in the frame
+// One sample of a sample stream: voxel id (-1 marks an unused slot), midpoint
+// depth along the ray, interval length, and the ray id in the frame.  16
+// bytes, written and read with one 128-bit access.
+struct SampleRec {
+  int32_t idx;
+  float depth;
+  float dist;
+  int32_t ray;
 };
+static_assert(sizeof(SampleRec) == 16, "SampleRec must be 16 bytes");
 
 }  // namespace nsvf

@@ -4,19 +4,22 @@
 //   hits      octree traversal + depth sort + segment-length sum per ray
 //             (encoder.py:498-535 with nsvf.py:65-74 folded in)
 //   compact   ray ranks -> list of hit rays (fairnr_model.py:157-162)
-//   sample    inverse-CDF sampling of a ray's bins into a compacted sample
-//             stream; counting pass + writing pass around a device scan
+//   sample    inverse-CDF sampling of a ray's bins into a sample stream
 //             (encoder.py:538-556, clib/__init__.py:231-300); coarse bins are
 //             the sorted hits, fine bins are the coarse samples
-//             (nerf.py:64-79)
+//             (nerf.py:64-79).  Each ray owns a slot range sized by an upper
+//             bound on its sample count, known before sampling (ceil(steps) +
+//             bins + 2), so one pass writes the samples and unused slots are
+//             marked with voxel id -1; no counting pass, no compaction.
 //   composite front-to-back compositing of one ray's samples by one
 //             wavefront with a prefix scan (renderer.py:193-218)
 //   finish    background fill of every ray of the frame (nsvf.py:89-101)
 //
-// Sample streams are compacted (CSR by hit ray): per-ray offset and count,
-// per-sample voxel id, midpoint depth, interval length and ray id.  No
-// padded [R, S] sample tensor exists on this path; padding only appears in
-// the op-level kernels that keep the reference's layout.
+// Sample streams are per-ray slot ranges (offset from a device scan of the
+// bounds, count of written samples) of 16-byte records {voxel id, midpoint
+// depth, interval length, ray id}.  No padded [R, S] sample tensor exists on
+// this path; padding only appears in the op-level kernels that keep the
+// reference's layout.
 #include "render.h"
 #include "sample_core.h"
 
@@ -25,7 +28,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kHitsBlock = 128;
-constexpr int kMaxLevels = 30;
+constexpr int kMaxLevels = 24;
 
 // ---------------------------------------------------------------------------
 // hits
@@ -83,6 +86,155 @@ __device__ int traverse(const float o[3], const float inv[3], int mask,
   return cnt;
 }
 
+// Slab test against a box given by its corners; with mn = fp32(c - h) and
+// mx = fp32(c + h) this is the reference's test (intersect_gpu.cu:93-94
+// evaluates (c - h) - o and (c + h) - o) bit for bit.
+__device__ __forceinline__ bool ray_box(const float o[3], const float inv[3], const float4& mn,
+                                        const float4& mx, float* out_lo, float* out_hi) {
+  float f_low = 0.0f;
+  float f_high = 100000.0f;
+  const float a0[3] = {mn.x, mn.y, mn.z};
+  const float a1[3] = {mx.x, mx.y, mx.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float lo = (a0[a] - o[a]) * inv[a];
+    float hi = (a1[a] - o[a]) * inv[a];
+    if (hi < lo) {
+      float t = lo;
+      lo = hi;
+      hi = t;
+    }
+    if (hi < f_low) return false;
+    if (lo > f_high) return false;
+    f_low = (lo > f_low) ? lo : f_low;
+    f_high = (hi < f_high) ? hi : f_high;
+    if (f_low > f_high) return false;
+  }
+  *out_lo = f_low;
+  *out_hi = f_high;
+  return true;
+}
+
+// Front-to-back traversal over the packed tree: boxes[2k] = (min xyz, scale
+// bits), boxes[2k+1] = (max xyz, 0), kids[k] = the 8 child ids.  Leaf boxes
+// are the voxel boxes; an internal node's box is the exact bounding box of
+// its leaves' boxes (computed with the same fp32 values), so it prunes every
+// subtree the ray misses while keeping every leaf the ray hits -- the slab
+// test is monotone under box containment -- and the hit set equals the
+// reference's, whose loose boxes (twice the octant) only prune less.
+// Expanding a node tests all its children at once in visit order (slot =
+// i ^ dmask, dmask = the ray's direction-sign octant).  If every hit child is
+// a leaf they are emitted right away in that order; otherwise the hit mask
+// goes on the level stack and the children are visited from it in order (a
+// leaf among them is re-tested when visited).  Returns the hit count.
+__device__ __forceinline__ unsigned expand_node(int node, int psc, const float o[3],
+                                                const float inv[3], int dmask,
+                                                const float4* __restrict__ boxes,
+                                                const int4* __restrict__ kids, int T, int n_max,
+                                                HitRec* __restrict__ row, int* cnt) {
+  const int4 k0 = kids[int64_t(node) * 2], k1 = kids[int64_t(node) * 2 + 1];
+  int c[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+  // c[i] <- c[i ^ dmask] with static indices only
+  if (dmask & 1) {
+#pragma unroll
+    for (int i = 0; i < 8; i += 2) {
+      const int t = c[i];
+      c[i] = c[i + 1];
+      c[i + 1] = t;
+    }
+  }
+  if (dmask & 2) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (!(i & 2)) {
+        const int t = c[i];
+        c[i] = c[i + 2];
+        c[i + 2] = t;
+      }
+  }
+  if (dmask & 4) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = c[i];
+      c[i] = c[i + 4];
+      c[i + 4] = t;
+    }
+  }
+  unsigned hit = 0, leaf = 0;
+  float los[8], his[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    los[i] = 0.0f;
+    his[i] = 0.0f;
+    const int id = c[i];
+    if (id >= 0 && id < T) {
+      const float4 mn = boxes[int64_t(id) * 2];
+      const int sc = __float_as_int(mn.w);
+      if (sc < psc) {
+        const float4 mx = boxes[int64_t(id) * 2 + 1];
+        float lo, hi;
+        if (ray_box(o, inv, mn, mx, &lo, &hi)) {
+          hit |= 1u << i;
+          if (sc == 1) leaf |= 1u << i;
+          los[i] = lo;
+          his[i] = hi;
+        }
+      }
+    }
+  }
+  if ((hit & ~leaf) == 0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (((hit >> i) & 1u) && *cnt < n_max) row[(*cnt)++] = HitRec{los[i], his[i], c[i], 0};
+    return 0u;
+  }
+  return hit;
+}
+
+__device__ int traverse_packed(const float o[3], const float inv[3], int dmask,
+                               const float4* __restrict__ boxes, const int4* __restrict__ kids,
+                               int T, int n_max, HitRec* __restrict__ row, int32_t* st_node,
+                               uint8_t* st_mask) {
+  int cnt = 0;
+  const int root = T - 1;
+  const float4 rmn = boxes[int64_t(root) * 2], rmx = boxes[int64_t(root) * 2 + 1];
+  const int rsc = __float_as_int(rmn.w);
+  float lo, hi;
+  if (n_max <= 0 || !ray_box(o, inv, rmn, rmx, &lo, &hi)) return 0;
+  if (rsc == 1) {
+    row[0] = HitRec{lo, hi, root, 0};
+    return 1;
+  }
+  int level = 0;
+  st_node[0] = root;
+  st_mask[0] = uint8_t(expand_node(root, rsc, o, inv, dmask, boxes, kids, T, n_max, row, &cnt));
+  while (level >= 0 && cnt < n_max) {
+    const unsigned m = st_mask[level * kHitsBlock];
+    if (m == 0) {
+      --level;
+      continue;
+    }
+    const int i = __builtin_ctz(m);
+    st_mask[level * kHitsBlock] = uint8_t(m & (m - 1));
+    const int node = st_node[level * kHitsBlock];
+    const int id = reinterpret_cast<const int*>(kids)[int64_t(node) * 8 + (i ^ dmask)];
+    const float4 mn = boxes[int64_t(id) * 2];
+    const int sc = __float_as_int(mn.w);
+    if (sc == 1) {
+      const float4 mx = boxes[int64_t(id) * 2 + 1];
+      if (ray_box(o, inv, mn, mx, &lo, &hi)) row[cnt++] = HitRec{lo, hi, id, 0};
+      continue;
+    }
+    if (level + 1 < kMaxLevels) {
+      ++level;
+      st_node[level * kHitsBlock] = id;
+      st_mask[level * kHitsBlock] =
+          uint8_t(expand_node(id, sc, o, inv, dmask, boxes, kids, T, n_max, row, &cnt));
+    }
+  }
+  return cnt;
+}
+
 // Per ray: hits sorted by entry depth, count, and the left-to-right sum of
 // the segment lengths.  The front-to-back traversal yields the hits already
 // sorted (leaves are disjoint voxels and a ray crosses octants in increasing
@@ -93,10 +245,12 @@ __device__ int traverse(const float o[3], const float inv[3], int mask,
 // insertion-sorted, which is exactly the reference pipeline's result.
 __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
     const float* __restrict__ ray_start, const float* __restrict__ ray_dir, int64_t R,
-    const float* __restrict__ cen, const int32_t* __restrict__ ch, int T, float half_voxel,
-    int n_max, HitRec* __restrict__ hits, int32_t* __restrict__ count, float* __restrict__ tot_out,
-    int32_t* __restrict__ hit_flag) {
+    const float* __restrict__ cen, const int32_t* __restrict__ ch, const float4* __restrict__ boxes,
+    const int4* __restrict__ kids, int T, float half_voxel, int n_max, float step_size,
+    HitRec* __restrict__ hits, int32_t* __restrict__ count, float* __restrict__ tot_out,
+    int32_t* __restrict__ hit_flag, int32_t* __restrict__ bound) {
   __shared__ int32_t stack[kMaxLevels * kHitsBlock];
+  __shared__ uint8_t stack_mask[kMaxLevels * kHitsBlock];
   const int64_t r = int64_t(blockIdx.x) * kHitsBlock + threadIdx.x;
   if (r >= R) return;
   float o[3], inv[3];
@@ -110,7 +264,8 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
   }
   HitRec* row = hits + r * n_max;
   int32_t* st = stack + threadIdx.x;
-  int cnt = traverse<true>(o, inv, mask, cen, ch, T, half_voxel, n_max, row, st);
+  int cnt = traverse_packed(o, inv, mask, boxes, kids, T, n_max, row, st,
+                            stack_mask + threadIdx.x);
   bool ok = cnt < n_max;
   for (int k = 1; ok && k < cnt; ++k) ok = row[k - 1].lo < row[k].lo;
   if (!ok) {
@@ -130,6 +285,9 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
   count[r] = cnt;
   tot_out[r] = tot;
   hit_flag[r] = cnt > 0 ? 1 : 0;
+  // at most ceil(steps) + cnt samples carry a voxel id (one per CDF step and
+  // one tail per bin; the compat duplicate replaces a step sample)
+  bound[r] = cnt > 0 ? int(ceilf(tot / step_size)) + cnt + 2 : 0;
 }
 
 __global__ __launch_bounds__(kBlock) void compact_kernel(const int32_t* __restrict__ flag,
@@ -165,52 +323,42 @@ struct HitBins {
 // Bins of a hit ray taken from its coarse samples (fine pass, nerf.py:64-79):
 // [depth - dist/2, depth + dist/2], probs (p + 1e-5) / sum over all P columns.
 struct SampleBins {
-  const int32_t* idx;
-  const float* depth;
-  const float* dist;
+  const SampleRec* rec;
   const float* prob_c;
   int cnt;
   float den;  // column sum of (p + 1e-5)
-  const int32_t* idx0;
+  const SampleRec* rec0;
   int cnt0;
   int32_t next;
-  __device__ __forceinline__ int32_t id(int k) const { return k < cnt ? idx[k] : -1; }
+  __device__ __forceinline__ int32_t id(int k) const { return k < cnt ? rec[k].idx : -1; }
   __device__ __forceinline__ float lo(int k) const {
-    return k < cnt ? depth[k] - dist[k] * 0.5f : kMaxDepth;
+    return k < cnt ? rec[k].depth - rec[k].dist * 0.5f : kMaxDepth;
   }
   __device__ __forceinline__ float hi(int k) const {
-    return k < cnt ? depth[k] + dist[k] * 0.5f : kMaxDepth;
+    return k < cnt ? rec[k].depth + rec[k].dist * 0.5f : kMaxDepth;
   }
   __device__ __forceinline__ float prob(int k) const {
     return (k < cnt ? prob_c[k] + 1e-5f : 1e-5f) / den;
   }
   __device__ __forceinline__ int32_t next_first() const { return next; }
-  __device__ __forceinline__ int32_t ray0_id(int k) const { return k < cnt0 ? idx0[k] : -1; }
+  __device__ __forceinline__ int32_t ray0_id(int k) const { return k < cnt0 ? rec0[k].idx : -1; }
 };
 
-// Counts the samples with a voxel id (pads are dropped from the stream).
-struct ValidCountSink {
-  int n = 0;
-  __device__ __forceinline__ void operator()(int, int32_t id, float, float) {
-    if (id != -1) ++n;
-  }
-};
-
-// Writes valid samples to the stream (dists clamped at 0: encoder.py:547).
+// Writes the samples that carry a voxel id into the ray's slot range (dists
+// clamped at 0: encoder.py:547).
 struct StreamSink {
-  int32_t* idx;
-  float* depth;
-  float* dist;
-  int32_t* ray;
+  SampleRec* out;
   int32_t ray_id;
+  int cap;  // the ray's slot count (never exceeded for valid bins; a guard)
   int n = 0;
   __device__ __forceinline__ void operator()(int, int32_t id, float d, float mid) {
-    if (id == -1) return;
-    idx[n] = id;
-    depth[n] = mid;
-    dist[n] = d > 0.0f ? d : 0.0f;
-    ray[n] = ray_id;
+    if (id == -1 || n >= cap) return;
+    out[n] = SampleRec{id, mid, d > 0.0f ? d : 0.0f, ray_id};
     ++n;
+  }
+  // Marks the slots past the written samples as unused.
+  __device__ __forceinline__ void close(int bound) {
+    for (int k = n; k < bound; ++k) out[k] = SampleRec{-1, 0.0f, 0.0f, ray_id};
   }
 };
 
@@ -233,21 +381,19 @@ __device__ __forceinline__ void grouped_neighbours(int64_t n, int64_t N, int64_t
   *next = nn >= N ? 0 : nn;
 }
 
-template <bool kWrite>
+// Coarse pass for hit ray n (ray r = hit_list[n]): samples into the slot
+// range [c_off[r], c_off[r] + bound[r]); count, fine-pass bound and the
+// longest row are recorded for the fine pass.
 __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const HitRec* __restrict__ hits, const int32_t* __restrict__ count,
-    const float* __restrict__ tot, int n_max, float step_size, int compat,
-    int32_t* __restrict__ counts_out, int32_t* __restrict__ max_count,
-    const int32_t* __restrict__ offsets, SampleStream out, int64_t n_hit_max) {
+    const float* __restrict__ tot, const int32_t* __restrict__ bound,
+    const int32_t* __restrict__ c_off, int n_max, float step_size, int compat, float fine_steps,
+    SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
+    int32_t* __restrict__ max_count) {
   const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t N = *n_hit_dev;
-  if (n >= N) {
-    // counts past the hit rays are zero, so a scan over n_hit_max entries
-    // gives the right offsets and total
-    if (!kWrite && n < n_hit_max) counts_out[n] = 0;
-    return;
-  }
+  if (n >= N) return;
   const int32_t r = hit_list[n];
   int64_t n0, nn;
   grouped_neighbours(n, N, &n0, &nn);
@@ -256,60 +402,51 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
                nn < 0 ? -1 : hits[int64_t(hit_list[nn]) * n_max].idx};
   const float steps = tot[r] / step_size;
   ConstNoise noise{0.5f};
-  const int limit = int(ceilf(steps)) + n_max + 2;
-  if (!kWrite) {
-    ValidCountSink sink;
-    icdf_ray(bins, n_max, steps, -1.0f, noise, compat, limit, sink);
-    counts_out[n] = sink.n;
-    atomicMax(max_count, sink.n);
-  } else {
-    const int64_t off = offsets[n];
-    StreamSink sink{out.idx + off, out.depth + off, out.dist + off, out.ray + off, r};
-    icdf_ray(bins, n_max, steps, -1.0f, noise, compat, limit, sink);
-  }
+  const int b = bound[r];
+  StreamSink sink{out + c_off[r], r, b};
+  icdf_ray(bins, n_max, steps, -1.0f, noise, compat, b, sink);
+  sink.close(b);
+  c_cnt[r] = sink.n;
+  f_bound[r] = int(ceilf(fine_steps)) + sink.n + 2;
+  atomicMax(max_count, sink.n);
 }
 
-template <bool kWrite>
+// Fine pass for hit ray n: bins are the ray's coarse samples and the *c_max
+// columns of the longest coarse row (nerf.py:64-79).
 __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_cnt,
-    const int32_t* __restrict__ c_max_dev, SampleStream coarse, const float* __restrict__ c_prob,
-    float fine_steps, int compat, int32_t* __restrict__ counts_out, int32_t* __restrict__ max_count,
-    const int32_t* __restrict__ offsets, SampleStream out, int64_t n_hit_max) {
+    const int32_t* __restrict__ c_max_dev, const SampleRec* __restrict__ coarse,
+    const float* __restrict__ c_prob, const int32_t* __restrict__ f_bound,
+    const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
+    int32_t* __restrict__ f_cnt) {
   const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   const int64_t N = *n_hit_dev;
-  if (n >= N) {
-    if (!kWrite && n < n_hit_max) counts_out[n] = 0;
-    return;
-  }
-  const int P = *c_max_dev;  // bin columns = longest coarse row
-  if (P <= 0) {
-    if (!kWrite) counts_out[n] = 0;
-    return;
-  }
-  int64_t n0, nn;
-  grouped_neighbours(n, N, &n0, &nn);
-  const int64_t off = c_off[n];
-  const int cnt = c_cnt[n];
-  // Column sum of (p + 1e-5) over all P columns, left to right.
-  float den = 0.0f;
-  for (int k = 0; k < P; ++k) den = den + ((k < cnt ? c_prob[off + k] : 0.0f) + 1e-5f);
-  SampleBins bins{coarse.idx + off, coarse.depth + off, coarse.dist + off, c_prob + off, cnt, den,
-                  coarse.idx + c_off[n0], c_cnt[n0],
-                  nn < 0 ? -1 : (c_cnt[nn] > 0 ? coarse.idx[c_off[nn]] : -1)};
-  ConstNoise noise{0.5f};
-  const int limit = int(ceilf(fine_steps)) + P + 2;
+  if (n >= N) return;
   const int32_t r = hit_list[n];
-  if (!kWrite) {
-    ValidCountSink sink;
-    icdf_ray(bins, P, fine_steps, -1.0f, noise, compat, limit, sink);
-    counts_out[n] = sink.n;
-    atomicMax(max_count, sink.n);
-  } else {
-    const int64_t o = offsets[n];
-    StreamSink sink{out.idx + o, out.depth + o, out.dist + o, out.ray + o, r};
-    icdf_ray(bins, P, fine_steps, -1.0f, noise, compat, limit, sink);
+  const int P = *c_max_dev;  // bin columns = longest coarse row
+  const int b = f_bound[r];
+  StreamSink sink{out + f_off[r], r, b};
+  if (P > 0) {
+    int64_t n0, nn;
+    grouped_neighbours(n, N, &n0, &nn);
+    const int32_t r0 = hit_list[n0];
+    const int64_t off = c_off[r];
+    const int cnt = c_cnt[r];
+    // Column sum of (p + 1e-5) over all P columns, left to right.
+    float den = 0.0f;
+    for (int k = 0; k < P; ++k) den = den + ((k < cnt ? c_prob[off + k] : 0.0f) + 1e-5f);
+    int32_t next = -1;
+    if (nn >= 0) {
+      const int32_t rn = hit_list[nn];
+      next = c_cnt[rn] > 0 ? coarse[c_off[rn]].idx : -1;
+    }
+    SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
+    ConstNoise noise{0.5f};
+    icdf_ray(bins, P, fine_steps, -1.0f, noise, compat, b, sink);
   }
+  sink.close(b);
+  f_cnt[r] = sink.n;
 }
 
 // ---------------------------------------------------------------------------
@@ -328,14 +465,15 @@ __device__ __forceinline__ float wave_sum(float v) {
 __global__ __launch_bounds__(kBlock) void composite_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ off_arr, const int32_t* __restrict__ cnt_arr,
-    const float* __restrict__ depth, const float* __restrict__ dist,
-    const float4* __restrict__ field, float* __restrict__ prob_out, float* __restrict__ ray_rgb,
-    float* __restrict__ ray_depth, float* __restrict__ ray_missed) {
+    const SampleRec* __restrict__ samples, const float4* __restrict__ field,
+    float* __restrict__ prob_out, float* __restrict__ ray_rgb, float* __restrict__ ray_depth,
+    float* __restrict__ ray_missed) {
   const int64_t n = int64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (n >= *n_hit_dev) return;
-  const int64_t off = off_arr[n];
-  const int cnt = cnt_arr[n];
+  const int32_t r = hit_list[n];
+  const int64_t off = off_arr[r];
+  const int cnt = cnt_arr[r];
   float carry = 0.0f;
   float acc_p = 0.0f, acc_z = 0.0f, acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f;
   for (int base = 0; base < cnt; base += 64) {
@@ -345,9 +483,10 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(
     float z = 0.0f;
     if (k < cnt) {
       f = field[off + k];
+      const SampleRec sr = samples[off + k];
       const float sg = f.x > 0.0f ? f.x : 0.0f;
-      fe = (sg * dist[off + k]) * 7.0f;
-      z = depth[off + k];
+      fe = (sg * sr.dist) * 7.0f;
+      z = sr.depth;
     }
     // inclusive scan of fe over the wave
     float incl = fe;
@@ -374,31 +513,30 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(
   acc_g = wave_sum(acc_g);
   acc_b = wave_sum(acc_b);
   if (lane == 0) {
-    ray_rgb[n * 3 + 0] = acc_r;
-    ray_rgb[n * 3 + 1] = acc_g;
-    ray_rgb[n * 3 + 2] = acc_b;
-    ray_depth[n] = acc_z;
-    ray_missed[n] = 1.0f - acc_p;
+    ray_rgb[int64_t(r) * 3 + 0] = acc_r;
+    ray_rgb[int64_t(r) * 3 + 1] = acc_g;
+    ray_rgb[int64_t(r) * 3 + 2] = acc_b;
+    ray_depth[r] = acc_z;
+    ray_missed[r] = 1.0f - acc_p;
   }
 }
 
 // Frame outputs for every ray: the hit rays' composite plus the background
 // weighted by `missed`; rays that hit nothing are pure background.
 __global__ __launch_bounds__(kBlock) void finish_kernel(
-    int64_t R, const int32_t* __restrict__ hit_flag, const int32_t* __restrict__ rank,
-    const float* __restrict__ ray_rgb, const float* __restrict__ ray_depth,
-    const float* __restrict__ ray_missed, float bg_r, float bg_g, float bg_b, float bg_depth,
-    float* __restrict__ rgb, float* __restrict__ depth, float* __restrict__ missed) {
+    int64_t R, const int32_t* __restrict__ hit_flag, const float* __restrict__ ray_rgb,
+    const float* __restrict__ ray_depth, const float* __restrict__ ray_missed, float bg_r,
+    float bg_g, float bg_b, float bg_depth, float* __restrict__ rgb, float* __restrict__ depth,
+    float* __restrict__ missed) {
   const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (r >= R) return;
   float cr = 0.f, cg = 0.f, cb = 0.f, d = 0.f, m = 1.f;
   if (hit_flag[r]) {
-    const int64_t n = rank[r];
-    cr = ray_rgb[n * 3];
-    cg = ray_rgb[n * 3 + 1];
-    cb = ray_rgb[n * 3 + 2];
-    d = ray_depth[n];
-    m = ray_missed[n];
+    cr = ray_rgb[r * 3];
+    cg = ray_rgb[r * 3 + 1];
+    cb = ray_rgb[r * 3 + 2];
+    d = ray_depth[r];
+    m = ray_missed[r];
   }
   rgb[r * 3 + 0] = cr + m * bg_r;
   rgb[r * 3 + 1] = cg + m * bg_g;
@@ -415,18 +553,22 @@ using namespace nsvf;
 extern "C" {
 
 int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
-                     const float* centers, const int32_t* children, int T, float voxelsize,
-                     int n_max, void* hits, int32_t* count, float* tot, int32_t* hit_flag,
+                     const float* centers, const int32_t* children, const void* boxes,
+                     const int32_t* kids, int T, float voxelsize, int n_max, float step_size,
+                     void* hits, int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
                      void* stream) {
   clear_error();
   if (R < 0 || T <= 0 || n_max <= 0) return set_error(NSVF_EINVAL, "render_hits: bad size");
   if (R == 0) return NSVF_OK;
-  if (!ray_start || !ray_dir || !centers || !children || !hits || !count || !tot || !hit_flag)
+  if (!ray_start || !ray_dir || !centers || !children || !boxes || !kids || !hits || !count ||
+      !tot || !hit_flag || !bound)
     return set_error(NSVF_EINVAL, "render_hits: null pointer");
   if (T >= (1 << 27)) return set_error(NSVF_EINVAL, "render_hits: more than 2^27 nodes");
   hipLaunchKernelGGL(hits_kernel, dim3(unsigned(ceil_div(R, kHitsBlock))), dim3(kHitsBlock), 0,
-                     as_stream(stream), ray_start, ray_dir, R, centers, children, T,
-                     voxelsize * 0.5f, n_max, static_cast<HitRec*>(hits), count, tot, hit_flag);
+                     as_stream(stream), ray_start, ray_dir, R, centers, children,
+                     static_cast<const float4*>(boxes), reinterpret_cast<const int4*>(kids), T,
+                     voxelsize * 0.5f, n_max, step_size, static_cast<HitRec*>(hits), count, tot,
+                     hit_flag, bound);
   return check_launch("render_hits");
 }
 
@@ -439,75 +581,59 @@ int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank, int64_t R,
   return check_launch("render_compact");
 }
 
-int nsvf_render_sample_coarse(int write, const int32_t* hit_list, const int32_t* n_hit_dev,
-                              int64_t n_hit_max, const void* hits, const int32_t* count,
-                              const float* tot, int n_max, float step_size, int compat,
-                              int32_t* counts_out, int32_t* max_count, const int32_t* offsets,
-                              int32_t* s_idx, float* s_depth, float* s_dist, int32_t* s_ray,
+int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
+                              const void* hits, const int32_t* count, const float* tot,
+                              const int32_t* bound, const int32_t* c_off, int n_max,
+                              float step_size, int compat, float fine_steps, void* samples,
+                              int32_t* c_cnt, int32_t* f_bound, int32_t* max_count,
                               void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
-  const dim3 grid(unsigned(ceil_div(n_hit_max, kBlock)));
-  SampleStream out{s_idx, s_depth, s_dist, s_ray};
-  const HitRec* h = static_cast<const HitRec*>(hits);
-  if (write)
-    hipLaunchKernelGGL(coarse_sample_kernel<true>, grid, dim3(kBlock), 0, as_stream(stream),
-                       hit_list, n_hit_dev, h, count, tot, n_max, step_size, compat, counts_out,
-                       max_count, offsets, out, n_hit_max);
-  else
-    hipLaunchKernelGGL(coarse_sample_kernel<false>, grid, dim3(kBlock), 0, as_stream(stream),
-                       hit_list, n_hit_dev, h, count, tot, n_max, step_size, compat, counts_out,
-                       max_count, offsets, out, n_hit_max);
+  hipLaunchKernelGGL(coarse_sample_kernel, dim3(unsigned(ceil_div(n_hit_max, kBlock))),
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev,
+                     static_cast<const HitRec*>(hits), count, tot, bound, c_off, n_max, step_size,
+                     compat, fine_steps, static_cast<SampleRec*>(samples), c_cnt, f_bound,
+                     max_count);
   return check_launch("render_sample_coarse");
 }
 
-int nsvf_render_sample_fine(int write, const int32_t* hit_list, const int32_t* n_hit_dev,
-                            int64_t n_hit_max, const int32_t* c_off, const int32_t* c_cnt,
-                            const int32_t* c_max, const int32_t* c_idx, const float* c_depth,
-                            const float* c_dist, const float* c_prob, float fine_steps, int compat,
-                            int32_t* counts_out, int32_t* max_count, const int32_t* offsets,
-                            int32_t* s_idx, float* s_depth, float* s_dist, int32_t* s_ray,
-                            void* stream) {
+int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
+                            const int32_t* c_off, const int32_t* c_cnt, const int32_t* c_max,
+                            const void* coarse, const float* c_prob, const int32_t* f_bound,
+                            const int32_t* f_off, float fine_steps, int compat, void* samples,
+                            int32_t* f_cnt, void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
-  const dim3 grid(unsigned(ceil_div(n_hit_max, kBlock)));
-  SampleStream coarse{const_cast<int32_t*>(c_idx), const_cast<float*>(c_depth),
-                      const_cast<float*>(c_dist), nullptr};
-  SampleStream out{s_idx, s_depth, s_dist, s_ray};
-  if (write)
-    hipLaunchKernelGGL(fine_sample_kernel<true>, grid, dim3(kBlock), 0, as_stream(stream),
-                       hit_list, n_hit_dev, c_off, c_cnt, c_max, coarse, c_prob, fine_steps, compat,
-                       counts_out, max_count, offsets, out, n_hit_max);
-  else
-    hipLaunchKernelGGL(fine_sample_kernel<false>, grid, dim3(kBlock), 0, as_stream(stream),
-                       hit_list, n_hit_dev, c_off, c_cnt, c_max, coarse, c_prob, fine_steps, compat,
-                       counts_out, max_count, offsets, out, n_hit_max);
+  hipLaunchKernelGGL(fine_sample_kernel, dim3(unsigned(ceil_div(n_hit_max, kBlock))),
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, c_off, c_cnt, c_max,
+                     static_cast<const SampleRec*>(coarse), c_prob, f_bound, f_off, fine_steps,
+                     compat, static_cast<SampleRec*>(samples), f_cnt);
   return check_launch("render_sample_fine");
 }
 
 int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
-                          const int32_t* off, const int32_t* cnt, const float* depth,
-                          const float* dist, const float* field, float* prob_out,
-                          float* ray_rgb, float* ray_depth, float* ray_missed, void* stream) {
+                          const int32_t* off, const int32_t* cnt, const void* samples,
+                          const float* field, float* prob_out, float* ray_rgb, float* ray_depth,
+                          float* ray_missed, void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
   const int per_block = kBlock / 64;
   hipLaunchKernelGGL(composite_kernel, dim3(unsigned(ceil_div(n_hit_max, per_block))),
-                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, off, cnt, depth, dist,
-                     reinterpret_cast<const float4*>(field), prob_out, ray_rgb, ray_depth,
-                     ray_missed);
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, off, cnt,
+                     static_cast<const SampleRec*>(samples), reinterpret_cast<const float4*>(field),
+                     prob_out, ray_rgb, ray_depth, ray_missed);
   return check_launch("render_composite");
 }
 
-int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const int32_t* rank,
-                       const float* ray_rgb, const float* ray_depth, const float* ray_missed,
-                       float bg_r, float bg_g, float bg_b, float bg_depth, float* rgb,
-                       float* depth, float* missed, void* stream) {
+int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const float* ray_rgb,
+                       const float* ray_depth, const float* ray_missed, float bg_r, float bg_g,
+                       float bg_b, float bg_depth, float* rgb, float* depth, float* missed,
+                       void* stream) {
   clear_error();
   if (R <= 0) return NSVF_OK;
   hipLaunchKernelGGL(finish_kernel, dim3(unsigned(ceil_div(R, kBlock))), dim3(kBlock), 0,
-                     as_stream(stream), R, hit_flag, rank, ray_rgb, ray_depth, ray_missed, bg_r,
-                     bg_g, bg_b, bg_depth, rgb, depth, missed);
+                     as_stream(stream), R, hit_flag, ray_rgb, ray_depth, ray_missed, bg_r, bg_g,
+                     bg_b, bg_depth, rgb, depth, missed);
   return check_launch("render_finish");
 }
 

@@ -9,14 +9,15 @@ hierarchical sampling:
   field (gather + MLP) -> composite -> fine inverse-CDF samples over the
   coarse samples -> field -> composite -> background fill.
 
-All stages are kernels of libnsvf_amd.so on the current stream.  Two device
-to host reads per view size the sample streams (the coarse and fine sample
-totals); the reference reads back far more (hit compaction, max lengths, a
+All stages are kernels of libnsvf_amd.so on the current stream.  One device
+to host read per view (hit-ray count and coarse slot total) sizes the sample
+streams; the reference reads back far more (hit compaction, max lengths, a
 per-column sum in its renderer loop).
 """
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -40,6 +41,37 @@ def _stream(dev):
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
+def pack_tree(centers, children, half_voxel):
+    """Packed octree for the front-to-back traversal: per node two float4
+    (min xyz + scale bits, max xyz + 0) and the 8 child ids.  Leaf boxes are
+    fp32(c - h), fp32(c + h); an internal node's box is the bounding box of
+    its children's boxes (children have smaller ids than their parent, so one
+    pass per depth from the leaves up fills it)."""
+    T = children.shape[0]
+    h = np.float32(half_voxel)
+    scale = children[:, 8].astype(np.int32)
+    mn = np.zeros((T, 3), np.float32)
+    mx = np.zeros((T, 3), np.float32)
+    leaf = scale == 1
+    mn[leaf] = (centers[leaf] - h).astype(np.float32)
+    mx[leaf] = (centers[leaf] + h).astype(np.float32)
+    kids = children[:, :8].astype(np.int32)
+    for sc in np.unique(scale[~leaf]):          # increasing depth order from the leaves up
+        nodes = np.nonzero(scale == sc)[0]
+        k = kids[nodes]                           # [n,8]
+        valid = k >= 0
+        kk = np.where(valid, k, 0)
+        cmn = np.where(valid[..., None], mn[kk], np.float32(np.inf))
+        cmx = np.where(valid[..., None], mx[kk], np.float32(-np.inf))
+        mn[nodes] = cmn.min(1)
+        mx[nodes] = cmx.max(1)
+    boxes = np.zeros((T, 2, 4), np.float32)
+    boxes[:, 0, :3] = mn
+    boxes[:, 0, 3] = scale.view(np.float32)
+    boxes[:, 1, :3] = mx
+    return boxes.reshape(T * 2, 4), np.ascontiguousarray(kids)
+
+
 class DeviceScene:
     """A voxel scene resident on one GPU: shifted voxel centres (gather),
     octree over the unshifted centres (intersection), corner ids, embeddings
@@ -58,6 +90,10 @@ class DeviceScene:
         self.centers = shifted.contiguous().to(self.device)
         self.tree_centers = cen.float().contiguous().to(self.device)
         self.tree_children = ch.int().contiguous().to(self.device)
+        # packed copy for the front-to-back traversal
+        boxes, kids = pack_tree(cen.float().numpy(), ch.int().numpy(), np.float32(vs) * np.float32(0.5))
+        self.tree_boxes = torch.from_numpy(boxes).to(self.device)
+        self.tree_kids = torch.from_numpy(kids).to(self.device)
         self.feats = torch.from_numpy(scene.feats.astype(np.int32)).to(self.device)
         self.values = torch.from_numpy(scene.values).to(self.device)
         self.voxel_size = float(vs)
@@ -73,7 +109,8 @@ class DeviceScene:
         return self.tree_children.shape[0]
 
     def nbytes(self):
-        ts = [self.centers, self.tree_centers, self.tree_children, self.feats, self.values,
+        ts = [self.centers, self.tree_centers, self.tree_children, self.tree_boxes, self.tree_kids,
+              self.feats, self.values,
               self.mlp["bf16"], self.mlp["f32"], self.mlp["bias"]]
         return sum(t.numel() * t.element_size() for t in ts)
 
@@ -82,8 +119,8 @@ class DeviceScene:
 class RenderStats:
     rays: int
     hit_rays: int
-    coarse_samples: int
-    fine_samples: int
+    coarse_slots: int     # slots of the coarse stream (samples + unused)
+    fine_slots: int       # upper bound of the fine stream's slots
     coarse_width: int
 
 
@@ -92,8 +129,11 @@ class Renderer:
     grown as needed, so repeated views of the same size allocate nothing."""
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
-                 compat=True):
+                 compat=True, split_field=True):
         self.device = torch.device(device)
+        # BF16: gather kernel + MLP kernel through a workspace (True) or one
+        # fused kernel (False)
+        self.split_field = split_field
         self.precision = precision
         self.hierarchical = hierarchical
         self.fine_steps = float(fine_steps)
@@ -122,9 +162,9 @@ class Renderer:
             e.record()
             self.trace.append((stage, e))
 
-    def _samples(self, prefix, m):
-        return (self._buf(prefix + "idx", m, torch.int32), self._buf(prefix + "depth", m, torch.float32),
-                self._buf(prefix + "dist", m, torch.float32), self._buf(prefix + "ray", m, torch.int32))
+    def _stream(self, name, m):
+        """m sample records {idx, depth, dist, ray} as an int32 [m, 4] buffer."""
+        return self._buf(name, m * 4, torch.int32).view(m, 4)
 
     def _field(self, sc, ray_start, ray_dir, samples, m_dev, m, out):
         L = lib()
@@ -134,10 +174,12 @@ class Renderer:
         if ev:
             e0 = torch.cuda.Event(enable_timing=True)
             e0.record()
+        ws_bytes = int(L.nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, m)) if self.split_field else 0
+        ws = self._buf("field_ws", (ws_bytes + 15) // 16 * 4, torch.int32) if ws_bytes else None
         check(L.nsvf_field_eval(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
-                                sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples[0]),
-                                _p(samples[1]), _p(samples[3]), _p(m_dev), m, _p(w),
-                                _p(sc.mlp["bias"]), _p(out), _stream(self.device)), "field_eval")
+                                sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples), _p(m_dev), m,
+                                _p(w), _p(sc.mlp["bias"]), _p(out), _p(ws), ws_bytes,
+                                _stream(self.device)), "field_eval")
         if ev:
             e1 = torch.cuda.Event(enable_timing=True)
             e1.record()
@@ -160,90 +202,96 @@ class Renderer:
         ray_dir = ray_dir.contiguous()
         R = ray_start.shape[0]
         P = sc.max_hits
+        fs = self.fine_steps
         self._mark("start")
         hits = self._buf("hits", R * P * 4, torch.int32)
         count = self._buf("count", R, torch.int32)
         tot = self._buf("tot", R, torch.float32)
         flag = self._buf("flag", R, torch.int32)
+        c_bound = self._buf("c_bound", R, torch.int32)
         check(L.nsvf_render_hits(_p(ray_start), _p(ray_dir), R, _p(sc.tree_centers),
-                                 _p(sc.tree_children), sc.num_nodes, sc.voxel_size, P, _p(hits),
-                                 _p(count), _p(tot), _p(flag), st), "render_hits")
+                                 _p(sc.tree_children), _p(sc.tree_boxes), _p(sc.tree_kids),
+                                 sc.num_nodes, sc.voxel_size, P,
+                                 sc.step_size, _p(hits), _p(count), _p(tot), _p(flag),
+                                 _p(c_bound), st), "render_hits")
         self._mark("intersect")
-        scal = self._buf("scalars", 8, torch.int32)   # [n_hit, m_c, max_c, m_f, max_f]
+        scal = self._buf("scalars", 8, torch.int32)   # [n_hit, slots_c, max_c, slots_f]
         scal.zero_()
+        f_bound = self._buf("f_bound", R, torch.int32)
+        f_bound.zero_()
         rank = self._buf("rank", R, torch.int32)
         self._scan(flag, rank, R, scal[0:1])
         hit_list = self._buf("hit_list", R, torch.int32)
         check(L.nsvf_render_compact(_p(flag), _p(rank), R, _p(hit_list), st), "render_compact")
-        # coarse samples
-        c_cnt = self._buf("c_cnt", R, torch.int32)
         c_off = self._buf("c_off", R, torch.int32)
-        check(L.nsvf_render_sample_coarse(0, _p(hit_list), _p(scal[0:1]), R, _p(hits), _p(count),
-                                          _p(tot), P, sc.step_size, self.compat, _p(c_cnt),
-                                          _p(scal[2:3]), None, None, None, None, None, st),
-              "sample_coarse count")
-        self._scan(c_cnt, c_off, R, scal[1:2])
-        self._mark("coarse_count")
-        host = scal[:3].cpu()
-        n_hit, m_c, max_c = int(host[0]), int(host[1]), int(host[2])
-        cs = self._samples("c_", m_c)
-        check(L.nsvf_render_sample_coarse(1, _p(hit_list), _p(scal[0:1]), R, _p(hits), _p(count),
-                                          _p(tot), P, sc.step_size, self.compat, None, None,
-                                          _p(c_off), _p(cs[0]), _p(cs[1]), _p(cs[2]), _p(cs[3]), st),
-              "sample_coarse write")
+        self._scan(c_bound, c_off, R, scal[1:2])
+        # the one device-to-host read of the view: hit rays and slot total
+        host = scal[:2].cpu()
+        n_hit, slots_c = int(host[0]), int(host[1])
+        slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)   # bound on the fine slots
+        self._mark("compact")
+        cs = self._stream("c_samples", slots_c)
+        c_cnt = self._buf("c_cnt", R, torch.int32)
+        check(L.nsvf_render_sample_coarse(_p(hit_list), _p(scal[0:1]), R, _p(hits), _p(count),
+                                          _p(tot), _p(c_bound), _p(c_off), P, sc.step_size,
+                                          self.compat, fs, _p(cs), _p(c_cnt), _p(f_bound),
+                                          _p(scal[2:3]), st), "sample_coarse")
         self._mark("coarse_sample")
-        c_field = self._buf("c_field", m_c * 4, torch.float32)
-        self._field(sc, ray_start, ray_dir, cs, scal[1:2], m_c, c_field)
+        c_field = self._buf("c_field", slots_c * 4, torch.float32)
+        self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
         self._mark("coarse_field")
-        c_prob = self._buf("c_prob", m_c, torch.float32)
+        c_prob = self._buf("c_prob", slots_c, torch.float32)
         ray_rgb = self._buf("ray_rgb", R * 3, torch.float32)
         ray_depth = self._buf("ray_depth", R, torch.float32)
         ray_missed = self._buf("ray_missed", R, torch.float32)
         check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), max(n_hit, 1), _p(c_off),
-                                      _p(c_cnt), _p(cs[1]), _p(cs[2]), _p(c_field), _p(c_prob),
-                                      _p(ray_rgb), _p(ray_depth), _p(ray_missed), st),
-              "composite coarse")
+                                      _p(c_cnt), _p(cs), _p(c_field), _p(c_prob), _p(ray_rgb),
+                                      _p(ray_depth), _p(ray_missed), st), "composite coarse")
         self._mark("coarse_composite")
-        m_f = 0
         if self.hierarchical and n_hit > 0:
-            f_cnt = self._buf("f_cnt", R, torch.int32)
             f_off = self._buf("f_off", R, torch.int32)
-            check(L.nsvf_render_sample_fine(0, _p(hit_list), _p(scal[0:1]), R, _p(c_off), _p(c_cnt),
-                                            _p(scal[2:3]), _p(cs[0]), _p(cs[1]), _p(cs[2]),
-                                            _p(c_prob), self.fine_steps, self.compat, _p(f_cnt),
-                                            _p(scal[4:5]), None, None, None, None, None, st),
-                  "sample_fine count")
-            self._scan(f_cnt, f_off, R, scal[3:4])
-            m_f = int(scal[3:4].cpu())
-            fs = self._samples("f_", m_f)
-            check(L.nsvf_render_sample_fine(1, _p(hit_list), _p(scal[0:1]), R, _p(c_off), _p(c_cnt),
-                                            _p(scal[2:3]), _p(cs[0]), _p(cs[1]), _p(cs[2]),
-                                            _p(c_prob), self.fine_steps, self.compat, None, None,
-                                            _p(f_off), _p(fs[0]), _p(fs[1]), _p(fs[2]), _p(fs[3]),
-                                            st), "sample_fine write")
+            self._scan(f_bound, f_off, R, scal[3:4])
+            fsm = self._stream("f_samples", slots_f)
+            f_cnt = self._buf("f_cnt", R, torch.int32)
+            check(L.nsvf_render_sample_fine(_p(hit_list), _p(scal[0:1]), R, _p(c_off), _p(c_cnt),
+                                            _p(scal[2:3]), _p(cs), _p(c_prob), _p(f_bound),
+                                            _p(f_off), fs, self.compat, _p(fsm), _p(f_cnt), st),
+                  "sample_fine")
             self._mark("fine_sample")
-            f_field = self._buf("f_field", m_f * 4, torch.float32)
-            self._field(sc, ray_start, ray_dir, fs, scal[3:4], m_f, f_field)
+            f_field = self._buf("f_field", slots_f * 4, torch.float32)
+            self._field(sc, ray_start, ray_dir, fsm, scal[3:4], slots_f, f_field)
             self._mark("fine_field")
             check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), n_hit, _p(f_off), _p(f_cnt),
-                                          _p(fs[1]), _p(fs[2]), _p(f_field), None, _p(ray_rgb),
-                                          _p(ray_depth), _p(ray_missed), st), "composite fine")
+                                          _p(fsm), _p(f_field), None, _p(ray_rgb), _p(ray_depth),
+                                          _p(ray_missed), st), "composite fine")
             self._mark("fine_composite")
         rgb = torch.empty((R, 3), dtype=torch.float32, device=dev)
         depth = torch.empty(R, dtype=torch.float32, device=dev)
         missed = torch.empty(R, dtype=torch.float32, device=dev)
-        check(L.nsvf_render_finish(R, _p(flag), _p(rank), _p(ray_rgb), _p(ray_depth), _p(ray_missed),
+        check(L.nsvf_render_finish(R, _p(flag), _p(ray_rgb), _p(ray_depth), _p(ray_missed),
                                    BG_COLOR[0], BG_COLOR[1], BG_COLOR[2], BG_DEPTH, _p(rgb),
                                    _p(depth), _p(missed), st), "render_finish")
         self._mark("finish")
-        self.last = RenderStats(R, n_hit, m_c, m_f, max_c)
+        self.last = RenderStats(R, n_hit, slots_c, slots_f if self.hierarchical else 0, 0)
         if keep:
+            hl = hit_list[:n_hit].long()
             self.kept = {
                 "hits": hits.view(R, P, 4), "count": count, "tot": tot, "flag": flag, "rank": rank,
-                "hit_list": hit_list[:n_hit], "c_cnt": c_cnt[:n_hit], "c_off": c_off[:n_hit],
-                "c_samples": cs, "c_field": c_field.view(-1, 4), "c_prob": c_prob,
+                "hit_list": hit_list[:n_hit], "c_cnt": c_cnt[hl], "c_off": c_off[hl],
+                "c_samples": _columns(cs), "c_field": c_field.view(-1, 4), "c_prob": c_prob,
+                "c_max": int(scal[2]),
             }
-            if m_f:
-                self.kept.update({"f_cnt": f_cnt[:n_hit], "f_off": f_off[:n_hit],
-                                  "f_samples": fs, "f_field": f_field.view(-1, 4)})
+            if self.hierarchical and n_hit > 0:
+                self.kept.update({"f_cnt": f_cnt[hl], "f_off": f_off[hl],
+                                  "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4)})
         return rgb, depth, missed
+
+    def count_samples(self):
+        """Exact sample counts of the last view (reads the device)."""
+        k = getattr(self, "kept", None)
+        return None if k is None else (int(k["c_cnt"].sum()), int(k.get("f_cnt", k["c_cnt"]).sum()))
+
+
+def _columns(rec):
+    """(idx, depth, dist, ray) views of an int32 [m, 4] record buffer."""
+    return (rec[:, 0], rec[:, 1].view(torch.float32), rec[:, 2].view(torch.float32), rec[:, 3])
