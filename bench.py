@@ -41,7 +41,22 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--camera-seed", type=int, default=11)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first copy a buffer of this many MiB (a known-size kernel for "
+                         "calibrating PMC byte counters)")
     return ap.parse_args()
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the PMC runs (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated), recorded in profiles/traffic.json; None
+    when that measurement is absent."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
 
 
 def log(*a):
@@ -60,6 +75,13 @@ def main():
     from nsvf_amd.render import DeviceScene, Renderer
 
     dev = torch.device("cuda:0")
+    if args.calib_copy_mib > 0:
+        src = torch.ones(args.calib_copy_mib * (1 << 20) // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     sc = S.noise_scene(res=128, num_voxels=args.voxels)
     cam = S.sphere_camera(2.5, args.camera_seed)
@@ -102,19 +124,30 @@ def main():
         f"coarse {n_coarse} samples ({st.coarse_slots} slots), fine {n_fine} samples "
         f"({st.fine_slots} slots allocated)")
 
-    # roofline of the dominant kernel: the fused gather + MLP (field) kernel,
-    # algorithmic FLOPs of the samples it evaluates / its HIP-event time
+    # roofline of the dominant kernel: the BF16 MLP kernel (MFMA-bound), with
+    # algorithmic FLOPs of the samples it evaluates / its HIP-event time; the
+    # gather kernel feeding it reported beside it (HBM/L2-bound)
     flops_per_sample = sc.mlp.flops_per_sample()
-    f_ms = sum(a.elapsed_time(b) for _, _, a, b in kev)
-    f_launches = len(kev)
-    f_samples = (n_coarse + n_fine) * args.steps
-    achieved = flops_per_sample * f_samples / (f_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "field_bf16_kernel<32,128,3>",
+    n_views = args.steps
+    mlp_ev = [(m, a, b) for k, m, a, b in kev if k == "mlp"]
+    emb_ev = [(m, a, b) for k, m, a, b in kev if k == "embed"]
+    mlp_ms = sum(a.elapsed_time(b) for _, a, b in mlp_ev)
+    emb_ms = sum(a.elapsed_time(b) for _, a, b in emb_ev)
+    samples = (n_coarse + n_fine) * n_views
+    achieved = flops_per_sample * samples / (mlp_ms * 1e-3) / 1e12
+    D = dsc.D
+    emb_bytes = samples * (16 + 2 * D) + n_views * 2 * (
+        sc.num_voxels * 44 + sc.num_keys * 4 * D)
+    roofline = {"bound": "mfma", "kernel": "mlp_bf16_kernel<32,128,3>",
                 "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_BF16_TFLOPS, 4), "traffic": None,
-                "avg_launch_ms": round(f_ms / max(f_launches, 1), 4),
+                "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
+                "traffic": measured_traffic("mlp_bf16_kernel"),
+                "avg_launch_ms": round(mlp_ms / max(len(mlp_ev), 1), 4),
                 "flops_per_sample": flops_per_sample,
-                "samples_per_launch": round(f_samples / max(f_launches, 1))}
+                "samples_per_launch": round(samples / max(len(mlp_ev), 1))}
+    kernels = {"embed_bf16_kernel": {"avg_launch_ms": round(emb_ms / max(len(emb_ev), 1), 4),
+                                     "achieved_GBs": round(emb_bytes / (emb_ms * 1e-3) / 1e9, 1),
+                                     "bound": "hbm/l2"}}
 
     # per-stage breakdown of one traced view (outside the timed region)
     rnd.trace = []
@@ -141,6 +174,7 @@ def main():
                              "coarse_slots": st.coarse_slots},
         "stages_ms": stages,
         "roofline": roofline,
+        "kernels": kernels,
     }
 
     # ---- plain FP32 path (the comparator)

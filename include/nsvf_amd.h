@@ -241,6 +241,21 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
                     const void* wfrag, const float* bias, float* out,
                     void* workspace, int64_t workspace_bytes, void* stream);
 
+/* The two kernels of the workspace BF16 path, callable one by one (so a
+ * caller can time or overlap them): the gather writes the MLP's operand
+ * tiles into the workspace, the MLP reads them.  Same arguments as
+ * nsvf_field_eval.                                                          */
+int nsvf_field_embed_bf16(int D, int H, int L, const float* centers,
+                          const int32_t* feats, const float* values,
+                          float voxel_size, const float* ray_start,
+                          const float* ray_dir, const void* samples,
+                          const int32_t* m_dev, int64_t m_max, void* workspace,
+                          int64_t workspace_bytes, void* stream);
+int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace,
+                        int64_t workspace_bytes, const int32_t* m_dev,
+                        int64_t m_max, const void* wfrag, const float* bias,
+                        float* out, void* stream);
+
 /* Front-to-back compositing of each hit ray's samples (one wavefront per
  * ray): free energy relu(sigma) * length * 7, probs, and the ray's rgb,
  * depth and missed = 1 - sum(probs), indexed by ray id.  prob_out [M]

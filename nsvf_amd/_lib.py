@@ -44,6 +44,8 @@ SIGNATURES = {
     "nsvf_field_workspace_bytes": ([_I, _I, _I, _I, _L], _L),
     "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _P, _P, _P, _L,
                          _P], _I),
+    "nsvf_field_embed_bf16": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _L, _P], _I),
+    "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P], _I),
     "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_finish": ([_L, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
     "nsvf_plain_aabb_intersect": ([_P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),

@@ -583,24 +583,38 @@ int64_t bf16_workspace_bytes(int64_t m_max) {
 }
 
 template <int D, int H, int L>
-int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
-                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                      float* out, void* workspace, hipStream_t stream) {
+int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
+                      const int32_t* m_dev, int64_t m_max, void* workspace, hipStream_t stream) {
   using Dm = Dims<D, H, L>;
-  bf16x8* emb = static_cast<bf16x8*>(workspace);
   const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
   const int64_t threads = tiles * Dm::NT * Dm::KT0 * 64;
   hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(ceil_div(threads, 256))),
-                     dim3(256), 0, stream, sc, rays, samples, m_dev, m_max, emb);
-  int rc = check_launch("field_embed_bf16");
-  if (rc) return rc;
+                     dim3(256), 0, stream, sc, rays, samples, m_dev, m_max,
+                     static_cast<bf16x8*>(workspace));
+  return check_launch("field_embed_bf16");
+}
+
+template <int D, int H, int L>
+int launch_mlp_bf16(const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                    float* out, const void* workspace, hipStream_t stream) {
+  using Dm = Dims<D, H, L>;
+  const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
   const size_t lds = (weights_in_lds<D, H, L>() ? size_t(Dm::FRAGS) * 1024 : 0) +
                      size_t(Dm::BIASES) * sizeof(float);
   const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
   hipLaunchKernelGGL((mlp_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), lds,
-                     stream, emb, m_dev, m_max, static_cast<const bf16x8*>(wfrag), bias,
-                     reinterpret_cast<float4*>(out));
+                     stream, static_cast<const bf16x8*>(workspace), m_dev, m_max,
+                     static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out));
   return check_launch("field_mlp_bf16");
+}
+
+template <int D, int H, int L>
+int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
+                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                      float* out, void* workspace, hipStream_t stream) {
+  const int rc = launch_embed_bf16<D, H, L>(sc, rays, samples, m_dev, m_max, workspace, stream);
+  if (rc) return rc;
+  return launch_mlp_bf16<D, H, L>(m_dev, m_max, wfrag, bias, out, workspace, stream);
 }
 
 template <int D, int H, int L>
@@ -685,6 +699,48 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
   NSVF_FIELD_CASE(32, 256, 4)
 #undef NSVF_FIELD_CASE
   return set_error(NSVF_EINVAL, "field_eval: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_field_embed_bf16(int D, int H, int L, const float* centers, const int32_t* feats,
+                          const float* values, float voxel_size, const float* ray_start,
+                          const float* ray_dir, const void* samples, const int32_t* m_dev,
+                          int64_t m_max, void* workspace, int64_t workspace_bytes, void* stream) {
+  clear_error();
+  if (m_max <= 0) return NSVF_OK;
+  if (!centers || !feats || !values || !ray_start || !ray_dir || !samples || !m_dev || !workspace)
+    return set_error(NSVF_EINVAL, "field_embed_bf16: null pointer");
+  if (workspace_bytes < nsvf_field_workspace_bytes(0, D, H, L, m_max))
+    return set_error(NSVF_EINVAL, "field_embed_bf16: workspace too small");
+  FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
+  FieldRays rays{ray_start, ray_dir};
+#define NSVF_EMB_CASE(d, h, l)                                                                \
+  if (D == d && H == h && L == l)                                                             \
+    return launch_embed_bf16<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), m_dev, \
+                                      m_max, workspace, as_stream(stream));
+  NSVF_EMB_CASE(32, 128, 3)
+  NSVF_EMB_CASE(48, 256, 4)
+  NSVF_EMB_CASE(32, 256, 4)
+#undef NSVF_EMB_CASE
+  return set_error(NSVF_EINVAL, "field_embed_bf16: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace, int64_t workspace_bytes,
+                        const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                        float* out, void* stream) {
+  clear_error();
+  if (m_max <= 0) return NSVF_OK;
+  if (!workspace || !m_dev || !wfrag || !bias || !out)
+    return set_error(NSVF_EINVAL, "field_mlp_bf16: null pointer");
+  if (workspace_bytes < nsvf_field_workspace_bytes(0, D, H, L, m_max))
+    return set_error(NSVF_EINVAL, "field_mlp_bf16: workspace too small");
+#define NSVF_MLP_CASE(d, h, l) \
+  if (D == d && H == h && L == l) \
+    return launch_mlp_bf16<d, h, l>(m_dev, m_max, wfrag, bias, out, workspace, as_stream(stream));
+  NSVF_MLP_CASE(32, 128, 3)
+  NSVF_MLP_CASE(48, 256, 4)
+  NSVF_MLP_CASE(32, 256, 4)
+#undef NSVF_MLP_CASE
+  return set_error(NSVF_EINVAL, "field_mlp_bf16: unsupported shape D=%d H=%d L=%d", D, H, L);
 }
 
 int nsvf_trilinear_gather(const int32_t* s_idx, const float* xyz, int64_t M, const float* centers,

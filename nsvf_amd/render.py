@@ -170,20 +170,36 @@ class Renderer:
         L = lib()
         prec = PRECISIONS[self.precision]
         w = sc.mlp["bf16"] if prec == 0 else sc.mlp["f32"]
+        st = _stream(self.device)
         ev = self._kernel_events is not None and m > 0
-        if ev:
-            e0 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-        ws_bytes = int(L.nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, m)) if self.split_field else 0
-        ws = self._buf("field_ws", (ws_bytes + 15) // 16 * 4, torch.int32) if ws_bytes else None
+
+        def event():
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            return e
+
+        if prec == 0 and self.split_field:
+            ws_bytes = int(L.nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
+            ws = self._buf("field_ws", (ws_bytes + 15) // 16 * 4, torch.int32)
+            e0 = event() if ev else None
+            check(L.nsvf_field_embed_bf16(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats),
+                                          _p(sc.values), sc.voxel_size, _p(ray_start), _p(ray_dir),
+                                          _p(samples), _p(m_dev), m, _p(ws), ws_bytes, st),
+                  "field_embed_bf16")
+            e1 = event() if ev else None
+            check(L.nsvf_field_mlp_bf16(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(m_dev), m, _p(w),
+                                        _p(sc.mlp["bias"]), _p(out), st), "field_mlp_bf16")
+            if ev:
+                e2 = event()
+                self._kernel_events.append(("embed", m, e0, e1))
+                self._kernel_events.append(("mlp", m, e1, e2))
+            return
+        e0 = event() if ev else None
         check(L.nsvf_field_eval(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
                                 sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples), _p(m_dev), m,
-                                _p(w), _p(sc.mlp["bias"]), _p(out), _p(ws), ws_bytes,
-                                _stream(self.device)), "field_eval")
+                                _p(w), _p(sc.mlp["bias"]), _p(out), None, 0, st), "field_eval")
         if ev:
-            e1 = torch.cuda.Event(enable_timing=True)
-            e1.record()
-            self._kernel_events.append(("field", m, e0, e1))
+            self._kernel_events.append(("field", m, e0, event()))
 
     def _scan(self, counts, offsets, n, total):
         ws = self._buf("scan_ws", int(lib().nsvf_scan_workspace_ints(n)), torch.int32)
