@@ -143,12 +143,32 @@ int nsvf_inverse_cdf_sampling_rows(const int32_t* pts_idx, const float* min_dept
  * fairnr/data/geometry.py:195-200; the reference runs it as torch ops).
  *   s_idx [M] voxel ids; xyz [M,3] sample points; centers [N,3] (x-shifted
  *   voxel centres); feats [N,8] corner key ids (corner order 4x+2y+z);
- *   values [K,D]; out [M,D] = sum_c w_c values[feats[v][c]].
+ *   values [K,D]; out [M,D] = sum_c w_c values[feats[v][c]]; a sample
+ *   with s_idx -1 (an unused slot) gives a zero row.
  * ---------------------------------------------------------------------- */
 int nsvf_trilinear_gather(const int32_t* s_idx, const float* xyz, int64_t M,
                           const float* centers, const int32_t* feats,
                           const float* values, int D, float voxel_size,
                           float* out, void* stream);
+
+/* Backward of nsvf_trilinear_gather: grad_values [K,D] (fully written) =
+ * sum over samples and corners of w_c * grad_out[m] into key feats[v][c] --
+ * what torch's F.embedding backward forms for the reference's gather
+ * (encoder.py:588-590), here deterministic and atomic-free (samples sorted by
+ * voxel, per-(voxel, corner) partials, per-key sums in a fixed order).
+ * Samples with s_idx outside [0, N) contribute nothing.  D a multiple of 4,
+ * at most 128; grad_out and grad_values 16-byte aligned.  workspace: device
+ * memory of nsvf_trilinear_gather_backward_workspace(M, N, K, D, stream)
+ * bytes (that query returns -1 on error).                                    */
+int64_t nsvf_trilinear_gather_backward_workspace(int64_t M, int64_t N, int64_t K,
+                                                 int D, void* stream);
+int nsvf_trilinear_gather_backward(const int32_t* s_idx, const float* xyz,
+                                   int64_t M, const float* centers,
+                                   const int32_t* feats, int64_t N, int64_t K,
+                                   int D, float voxel_size,
+                                   const float* grad_out, float* grad_values,
+                                   void* workspace, int64_t workspace_bytes,
+                                   void* stream);
 
 /* ------------------------------------------------------------------------
  * Fused render pipeline.  No counterpart in the reference's native module:
@@ -265,6 +285,19 @@ int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev,
                           const int32_t* cnt, const void* samples,
                           const float* field, float* prob_out, float* ray_rgb,
                           float* ray_depth, float* ray_missed, void* stream);
+
+/* Backward of nsvf_render_composite (the autograd of renderer.py:119-121,
+ * 195-218): from the gradients of the ray outputs (grad_rgb [R,3],
+ * grad_depth [R], grad_missed [R], indexed by ray id) to grad_field [M,4]
+ * = (d/dsigma, d/dr, d/dg, d/db) of each used slot.  Transmittance is
+ * recomputed, not stored; slots past a ray's count are not written.         */
+int nsvf_render_composite_backward(const int32_t* hit_list,
+                                   const int32_t* n_hit_dev, int64_t n_hit_max,
+                                   const int32_t* off, const int32_t* cnt,
+                                   const void* samples, const float* field,
+                                   const float* grad_rgb, const float* grad_depth,
+                                   const float* grad_missed, float* grad_field,
+                                   void* stream);
 
 /* Frame outputs for all R rays: hit rays' composite plus missed * background
  * (fairnr/models/nsvf.py:89-101); rays that hit nothing get the background. */

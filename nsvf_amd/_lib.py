@@ -33,6 +33,9 @@ SIGNATURES = {
     "nsvf_inverse_cdf_sampling_rows": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _F, _I, _P, _P, _P, _P, _P], _I),
     "nsvf_uniform_ray_sampling": ([_P, _P, _P, _P, _I, _I, _I, _I, _F, _P, _P, _P, _P, _P], _I),
     "nsvf_trilinear_gather": ([_P, _P, _L, _P, _P, _P, _I, _F, _P, _P], _I),
+    "nsvf_trilinear_gather_backward_workspace": ([_L, _L, _L, _I, _P], _L),
+    "nsvf_trilinear_gather_backward": ([_P, _P, _L, _P, _P, _L, _L, _I, _F, _P, _P, _P, _L, _P],
+                                       _I),
     "nsvf_scan_workspace_ints": ([_L], _L),
     "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_render_hits": ([_P, _P, _L, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P, _P, _P, _P], _I),
@@ -47,6 +50,7 @@ SIGNATURES = {
     "nsvf_field_embed_bf16": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _L, _P], _I),
     "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P], _I),
     "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_composite_backward": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_finish": ([_L, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
     "nsvf_plain_aabb_intersect": ([_P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
     "nsvf_plain_svo_intersect": ([_P, _P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),

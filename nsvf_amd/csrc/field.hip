@@ -554,6 +554,10 @@ __global__ __launch_bounds__(256) void gather_kernel(const int32_t* __restrict__
   const int64_t s = i / D;
   const int f = int(i - s * D);
   const int32_t v = s_idx[s];
+  if (v < 0) {  // unused sample slot
+    out[i] = 0.0f;
+    return;
+  }
   float p[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a)

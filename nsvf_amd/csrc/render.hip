@@ -521,6 +521,85 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(
   }
 }
 
+// Gradient of composite_kernel's outputs with respect to the field values of
+// each sample (the autograd of renderer.py:119-121,195-218).  With
+// g_p(k) = <g_rgb, c_k> + g_depth z_k - g_missed and u_k = g_p(k) p_k:
+//   d/dfe_k    = g_p(k) T_k exp(-fe_k) - sum_{j>k} u_j
+//   d/dsigma_k = d/dfe_k * 7 dist_k  if sigma_k > 0, else 0
+//   d/dc_k     = p_k g_rgb
+// One wavefront per hit ray.  Pass 1 runs the forward scan again and parks
+// each sample's transmittance exponent in grad_field[k].x; pass 2 walks the
+// ray back to front, so the suffix sums of u are accumulated directly (no
+// difference of two large prefix sums), and overwrites each record with its
+// gradient.  T_k and p_k are recomputed with the forward's exact operations.
+__global__ __launch_bounds__(kBlock) void composite_backward_kernel(
+    const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
+    const int32_t* __restrict__ off_arr, const int32_t* __restrict__ cnt_arr,
+    const SampleRec* __restrict__ samples, const float4* __restrict__ field,
+    const float* __restrict__ g_rgb, const float* __restrict__ g_depth,
+    const float* __restrict__ g_missed, float4* __restrict__ grad_field) {
+  const int64_t n = int64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (n >= *n_hit_dev) return;
+  const int32_t r = hit_list[n];
+  const int64_t off = off_arr[r];
+  const int cnt = cnt_arr[r];
+  float carry = 0.0f;
+  for (int base = 0; base < cnt; base += 64) {
+    const int k = base + lane;
+    float fe = 0.0f;
+    if (k < cnt) {
+      const float sg = field[off + k].x > 0.0f ? field[off + k].x : 0.0f;
+      fe = (sg * samples[off + k].dist) * 7.0f;
+    }
+    float incl = fe;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const float u = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += u;
+    }
+    if (k < cnt) grad_field[off + k].x = carry + (incl - fe);
+    carry += __shfl(incl, 63, 64);
+  }
+  const float gr = g_rgb[int64_t(r) * 3 + 0], gg = g_rgb[int64_t(r) * 3 + 1],
+              gb = g_rgb[int64_t(r) * 3 + 2];
+  const float gd = g_depth[r], gm = g_missed[r];
+  float suffix = 0.0f;  // sum of u over the samples after the current chunk
+  for (int base = (cnt - 1) & ~63; base >= 0; base -= 64) {
+    const int k = base + lane;
+    float u = 0.0f, gp = 0.0f, p = 0.0f, tnext = 0.0f, sig = 0.0f, dist = 0.0f;
+    if (k < cnt) {
+      const float4 f = field[off + k];
+      const SampleRec sr = samples[off + k];
+      const float excl = grad_field[off + k].x;
+      sig = f.x;
+      dist = sr.dist;
+      const float sg = f.x > 0.0f ? f.x : 0.0f;
+      const float fe = (sg * sr.dist) * 7.0f;
+      const float e = expf(-fe);
+      const float tr = expf(-excl);
+      p = (1.0f - e) * tr;
+      tnext = tr * e;
+      gp = gr * f.y + gg * f.z + gb * f.w + gd * sr.depth - gm;
+      u = gp * p;
+    }
+    // inclusive suffix scan of u within the wave (lanes >= this one)
+    float incl = u;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const float v = __shfl_down(incl, d, 64);
+      if (lane + d < 64) incl += v;
+    }
+    if (k < cnt) {
+      const float later = suffix + (incl - u);
+      const float dfe = gp * tnext - later;
+      const float ds = sig > 0.0f ? (dfe * dist) * 7.0f : 0.0f;
+      grad_field[off + k] = make_float4(ds, p * gr, p * gg, p * gb);
+    }
+    suffix += __shfl(incl, 0, 64);
+  }
+}
+
 // Frame outputs for every ray: the hit rays' composite plus the background
 // weighted by `missed`; rays that hit nothing are pure background.
 __global__ __launch_bounds__(kBlock) void finish_kernel(
@@ -623,6 +702,24 @@ int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev, int
                      static_cast<const SampleRec*>(samples), reinterpret_cast<const float4*>(field),
                      prob_out, ray_rgb, ray_depth, ray_missed);
   return check_launch("render_composite");
+}
+
+int nsvf_render_composite_backward(const int32_t* hit_list, const int32_t* n_hit_dev,
+                                   int64_t n_hit_max, const int32_t* off, const int32_t* cnt,
+                                   const void* samples, const float* field, const float* grad_rgb,
+                                   const float* grad_depth, const float* grad_missed,
+                                   float* grad_field, void* stream) {
+  clear_error();
+  if (n_hit_max <= 0) return NSVF_OK;
+  if (!hit_list || !n_hit_dev || !off || !cnt || !samples || !field || !grad_rgb ||
+      !grad_depth || !grad_missed || !grad_field)
+    return set_error(NSVF_EINVAL, "render_composite_backward: null pointer");
+  const int per_block = kBlock / 64;
+  hipLaunchKernelGGL(composite_backward_kernel, dim3(unsigned(ceil_div(n_hit_max, per_block))),
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, off, cnt,
+                     static_cast<const SampleRec*>(samples), reinterpret_cast<const float4*>(field),
+                     grad_rgb, grad_depth, grad_missed, reinterpret_cast<float4*>(grad_field));
+  return check_launch("render_composite_backward");
 }
 
 int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const float* ray_rgb,

@@ -266,6 +266,13 @@ class PlainNSVF:
     @torch.no_grad()
     def render(self, ray_start, ray_dir):
         """ray_start, ray_dir: [R,3].  Returns (rgb [R,3], depth [R], missed [R])."""
+        return self.forward(ray_start, ray_dir)
+
+    def forward(self, ray_start, ray_dir):
+        """render() with autograd: differentiable in self.values and the
+        tensors of self.layers when they require grad (the training forward,
+        rendering_loss.py:37 -> fairnr_model.py:142-186, with eval-mode
+        sampling noise 0.5)."""
         st = self.precompute()
         R = ray_dir.shape[0]
         io, hits = self.ray_intersect(ray_start, ray_dir, st)

@@ -116,6 +116,46 @@ class DeviceScene:
 
 
 @dataclass
+class CoarseStream:
+    """Output of the geometry stages for R rays: sorted hit rows, the list of
+    the n_hit rays that hit a voxel, and the coarse sample stream -- `slots`
+    16-byte records {voxel id (-1 = unused), depth, interval, ray id}; ray r
+    owns slots [off[r], off[r] + bound) of which the first cnt[r] are
+    samples.  `scal` holds the device copies of [n_hit, slots, longest row,
+    fine slots]."""
+    rays: int
+    n_hit: int
+    slots: int
+    hits: torch.Tensor
+    count: torch.Tensor
+    tot: torch.Tensor
+    flag: torch.Tensor
+    rank: torch.Tensor
+    hit_list: torch.Tensor
+    off: torch.Tensor
+    cnt: torch.Tensor
+    samples: torch.Tensor
+    f_bound: torch.Tensor
+    scal: torch.Tensor
+
+    def owned(self):
+        """A copy whose tensors no later coarse_stream call overwrites (what
+        an autograd graph keeps until its backward)."""
+        c = lambda t: t.clone()
+        return CoarseStream(self.rays, self.n_hit, self.slots, None, None, None, c(self.flag), None,
+                            c(self.hit_list), c(self.off), c(self.cnt), c(self.samples), None,
+                            c(self.scal))
+
+    @property
+    def c_off(self):
+        return self.off
+
+    @property
+    def c_cnt(self):
+        return self.cnt
+
+
+@dataclass
 class RenderStats:
     rays: int
     hit_rays: int
@@ -207,19 +247,16 @@ class Renderer:
                                             _stream(self.device)), "scan")
 
     @torch.no_grad()
-    def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False):
-        """ray_start, ray_dir: [R,3] float32 on the device.  Returns
-        (rgb [R,3], depth [R], missed [R]) -- the fine pass when hierarchical,
-        else the coarse pass -- after background fill."""
+    def coarse_stream(self, sc: DeviceScene, ray_start, ray_dir):
+        """The geometry stages of a view: intersection, hit compaction and the
+        coarse sample stream (encoder.py:498-556).  Returns a CoarseStream
+        whose tensors are workspace views, valid until the next call."""
         L = lib()
         dev = self.device
         st = _stream(dev)
-        ray_start = ray_start.contiguous()
-        ray_dir = ray_dir.contiguous()
         R = ray_start.shape[0]
         P = sc.max_hits
         fs = self.fine_steps
-        self._mark("start")
         hits = self._buf("hits", R * P * 4, torch.int32)
         count = self._buf("count", R, torch.int32)
         tot = self._buf("tot", R, torch.float32)
@@ -244,7 +281,6 @@ class Renderer:
         # the one device-to-host read of the view: hit rays and slot total
         host = scal[:2].cpu()
         n_hit, slots_c = int(host[0]), int(host[1])
-        slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)   # bound on the fine slots
         self._mark("compact")
         cs = self._stream("c_samples", slots_c)
         c_cnt = self._buf("c_cnt", R, torch.int32)
@@ -253,6 +289,28 @@ class Renderer:
                                           self.compat, fs, _p(cs), _p(c_cnt), _p(f_bound),
                                           _p(scal[2:3]), st), "sample_coarse")
         self._mark("coarse_sample")
+        return CoarseStream(R, n_hit, slots_c, hits, count, tot, flag, rank, hit_list, c_off,
+                            c_cnt, cs, f_bound, scal)
+
+    @torch.no_grad()
+    def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False):
+        """ray_start, ray_dir: [R,3] float32 on the device.  Returns
+        (rgb [R,3], depth [R], missed [R]) -- the fine pass when hierarchical,
+        else the coarse pass -- after background fill."""
+        L = lib()
+        dev = self.device
+        st = _stream(dev)
+        ray_start = ray_start.contiguous()
+        ray_dir = ray_dir.contiguous()
+        R = ray_start.shape[0]
+        P = sc.max_hits
+        fs = self.fine_steps
+        self._mark("start")
+        g = self.coarse_stream(sc, ray_start, ray_dir)
+        n_hit, slots_c, scal, hit_list = g.n_hit, g.slots, g.scal, g.hit_list
+        hits, count, tot, flag, rank = g.hits, g.count, g.tot, g.flag, g.rank
+        c_off, c_cnt, cs, f_bound = g.c_off, g.c_cnt, g.samples, g.f_bound
+        slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)   # bound on the fine slots
         c_field = self._buf("c_field", slots_c * 4, torch.float32)
         self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
         self._mark("coarse_field")

@@ -1,0 +1,214 @@
+"""Differentiable rendering for training: the gradient of an image loss with
+respect to the corner embeddings and the field MLP.
+
+The reference trains through torch autograd over its forward
+(fairnr/criterions/rendering_loss.py:37-139 -> NSVFModel._forward): the
+intersection and sampling ops are non-differentiable (their autograd
+Functions mark every output so, clib/__init__.py:58-300), and gradients flow
+through three parts:
+
+  trilinear gather   encoder.py:582-590 -- F.embedding backward, an index-add
+                     into the embedding table;
+  field MLP          field.py:217-279 -- nn.Linear layers;
+  compositing        renderer.py:119-121,195-218 -- relu, exp, cumsum.
+
+Here the geometry stages are the fused renderer's kernels (one coarse sample
+stream per batch of rays), the gather and the compositing are autograd
+Functions over hand-written HIP forward and backward kernels (the gather's
+backward deterministic and atomic-free), and the MLP is FP32 torch.nn.Linear
+on the GPU's BLAS, as in the reference.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+import torch.nn as nn
+import torch.nn.functional as F
+
+from ._lib import check, lib
+from .render import BG_COLOR, BG_DEPTH, CoarseStream, DeviceScene, Renderer
+from .scene import VoxelScene
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(dev):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def _need_cuda(t, name):
+    if not t.is_cuda:
+        raise RuntimeError(f"{name} must be a CUDA tensor")
+
+
+class TrilinearGather(torch.autograd.Function):
+    """emb [M,D] = sum_c w_c(x_m) values[feats[s_idx[m]][c]] (encoder.py:582-590);
+    rows with s_idx -1 are zero.  Differentiable in `values` only, as in the
+    reference (sample points come from non-differentiable sampling)."""
+
+    @staticmethod
+    def forward(ctx, values, s_idx, xyz, centers, feats, voxel_size):
+        for t, n in ((values, "values"), (s_idx, "s_idx"), (xyz, "xyz"), (centers, "centers"),
+                     (feats, "feats")):
+            _need_cuda(t, n)
+        if values.dtype != torch.float32 or xyz.dtype != torch.float32:
+            raise RuntimeError("values and xyz must be float32")
+        if s_idx.dtype != torch.int32 or feats.dtype != torch.int32:
+            raise RuntimeError("s_idx and feats must be int32")
+        values = values.contiguous()
+        s_idx, xyz = s_idx.contiguous(), xyz.contiguous()
+        centers, feats = centers.contiguous(), feats.contiguous()
+        M, (K, D) = s_idx.shape[0], values.shape
+        out = torch.empty((M, D), dtype=torch.float32, device=values.device)
+        check(lib().nsvf_trilinear_gather(_p(s_idx), _p(xyz), M, _p(centers), _p(feats),
+                                          _p(values), D, float(voxel_size), _p(out),
+                                          _stream(values.device)), "trilinear_gather")
+        ctx.save_for_backward(s_idx, xyz, centers, feats)
+        ctx.shape = (K, D)
+        ctx.voxel_size = float(voxel_size)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        s_idx, xyz, centers, feats = ctx.saved_tensors
+        K, D = ctx.shape
+        grad_values = None
+        if ctx.needs_input_grad[0]:
+            grad_values = trilinear_gather_backward(s_idx, xyz, centers, feats, K, grad_out,
+                                                    ctx.voxel_size)
+        return grad_values, None, None, None, None, None
+
+
+def trilinear_gather(values, s_idx, xyz, centers, feats, voxel_size):
+    return TrilinearGather.apply(values, s_idx, xyz, centers, feats, voxel_size)
+
+
+def trilinear_gather_backward(s_idx, xyz, centers, feats, num_keys, grad_out, voxel_size):
+    """grad_values [K,D] of the gather (deterministic; see the HIP source)."""
+    L = lib()
+    dev = grad_out.device
+    grad_out = grad_out.contiguous().float()
+    M, D = grad_out.shape
+    N = centers.shape[0]
+    st = _stream(dev)
+    nbytes = int(L.nsvf_trilinear_gather_backward_workspace(M, N, num_keys, D, st))
+    if nbytes < 0:
+        check(-1, "trilinear_gather_backward_workspace")
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
+    grad_values = torch.empty((num_keys, D), dtype=torch.float32, device=dev)
+    check(L.nsvf_trilinear_gather_backward(_p(s_idx), _p(xyz), M, _p(centers), _p(feats), N,
+                                           num_keys, D, float(voxel_size), _p(grad_out),
+                                           _p(grad_values), _p(ws), nbytes, st),
+          "trilinear_gather_backward")
+    return grad_values
+
+
+class StreamComposite(torch.autograd.Function):
+    """Front-to-back compositing of a sample stream (renderer.py:119-121,
+    195-218).  field [slots,4] = (sigma, r, g, b) per slot; returns per-ray
+    (rgb [R,3], depth [R], missed [R]) with rays that hit nothing at
+    (0, 0, 1).  Differentiable in `field`."""
+
+    @staticmethod
+    def forward(ctx, field, g: CoarseStream):
+        field = field.contiguous()
+        if field.dtype != torch.float32 or field.shape != (g.slots, 4):
+            raise RuntimeError(f"field must be float32 [{g.slots}, 4]")
+        dev = field.device
+        R = g.rays
+        rgb = torch.zeros((R, 3), dtype=torch.float32, device=dev)
+        depth = torch.zeros(R, dtype=torch.float32, device=dev)
+        missed = torch.ones(R, dtype=torch.float32, device=dev)
+        if g.n_hit > 0:
+            check(lib().nsvf_render_composite(_p(g.hit_list), _p(g.scal[0:1]), g.n_hit, _p(g.off),
+                                              _p(g.cnt), _p(g.samples), _p(field), None, _p(rgb),
+                                              _p(depth), _p(missed), _stream(dev)), "composite")
+        ctx.save_for_backward(field)
+        ctx.stream = g
+        return rgb, depth, missed
+
+    @staticmethod
+    def backward(ctx, g_rgb, g_depth, g_missed):
+        (field,) = ctx.saved_tensors
+        g = ctx.stream
+        dev = field.device
+        R = g.rays
+        g_rgb = torch.zeros((R, 3), device=dev) if g_rgb is None else g_rgb.contiguous().float()
+        g_depth = torch.zeros(R, device=dev) if g_depth is None else g_depth.contiguous().float()
+        g_missed = torch.zeros(R, device=dev) if g_missed is None else g_missed.contiguous().float()
+        grad_field = torch.zeros_like(field)
+        if g.n_hit > 0:
+            check(lib().nsvf_render_composite_backward(
+                _p(g.hit_list), _p(g.scal[0:1]), g.n_hit, _p(g.off), _p(g.cnt), _p(g.samples),
+                _p(field), _p(g_rgb), _p(g_depth), _p(g_missed), _p(grad_field), _stream(dev)),
+                "composite_backward")
+        return grad_field, None
+
+
+class TrainableField(nn.Module):
+    """The trainable state of a voxel scene: corner embeddings [K,D] and the
+    field MLP (L hidden ReLU layers, a 4-wide head: sigma, sigmoid RGB)."""
+
+    def __init__(self, scene: VoxelScene, device):
+        super().__init__()
+        dev = torch.device(device)
+        self.values = nn.Parameter(torch.from_numpy(np.array(scene.values, np.float32)).to(dev))
+        self.weights = nn.ParameterList(
+            [nn.Parameter(torch.from_numpy(np.array(w, np.float32)).to(dev))
+             for w in scene.mlp.weights])
+        self.biases = nn.ParameterList(
+            [nn.Parameter(torch.from_numpy(np.array(b, np.float32)).to(dev))
+             for b in scene.mlp.biases])
+
+    def mlp(self, emb):
+        """[M,D] -> [M,4] = (sigma, r, g, b)."""
+        h = emb
+        n = len(self.weights)
+        for i in range(n):
+            h = F.linear(h, self.weights[i], self.biases[i])
+            if i < n - 1:
+                h = torch.relu(h)
+        return torch.cat([h[:, :1], torch.sigmoid(h[:, 1:4])], 1)
+
+
+class TrainRenderer:
+    """Differentiable view rendering against a DeviceScene's geometry (its
+    octree, shifted centres and corner ids) with a TrainableField's
+    parameters: intersect + coarse samples (non-differentiable kernels), then
+    gather, MLP and compositing with autograd, then the background fill
+    (nsvf.py:94-101)."""
+
+    def __init__(self, device, compat=True):
+        self.device = torch.device(device)
+        self.geometry = Renderer(device, precision="fp32", hierarchical=False, compat=compat)
+        self.last = None
+
+    def forward(self, sc: DeviceScene, field: TrainableField, ray_start, ray_dir):
+        """ray_start, ray_dir [R,3] float32 on the device.  Returns (rgb [R,3],
+        depth [R], missed [R]) connected to `field`'s parameters."""
+        _need_cuda(ray_start, "ray_start")
+        _need_cuda(ray_dir, "ray_dir")
+        rs, rd = ray_start.contiguous().float(), ray_dir.contiguous().float()
+        g = self.geometry.coarse_stream(sc, rs, rd).owned()
+        self.last = g
+        rec = g.samples
+        s_idx = rec[:, 0]
+        depth = rec[:, 1].view(torch.float32)
+        ray = rec[:, 3].long()
+        # sample points x = o + d * z, each operation rounded (geometry.py:160-161)
+        xyz = rs.index_select(0, ray) + rd.index_select(0, ray) * depth[:, None]
+        emb = trilinear_gather(field.values, s_idx, xyz, sc.centers, sc.feats, sc.voxel_size)
+        out = field.mlp(emb)
+        rgb, dep, missed = StreamComposite.apply(out, g)
+        bg = torch.tensor(BG_COLOR, dtype=torch.float32, device=rgb.device)
+        return rgb + missed[:, None] * bg, dep + missed * BG_DEPTH, missed
+
+
+def mse_loss(rgb, target):
+    """Mean squared RGB error over rays and channels (rendering_loss.py:139
+    with the RGB term only)."""
+    return torch.mean((rgb - target) ** 2)

@@ -464,6 +464,54 @@ def composite(sigma, rgb, depth, dists, idx, dtype=np.float64):
     return col, out_depth, missed, probs
 
 
+def composite_backward(sigma, rgb, depth, dists, idx, g_rgb, g_depth, g_missed, dtype=np.float64):
+    """Gradients of composite()'s (rgb, depth, missed) with respect to sigma
+    [R,S] and rgb [R,S,3], as autograd forms them through renderer.py:
+    119-121,195-218.  With g_p(k) = <g_rgb, c_k> + g_depth z_k - g_missed:
+      d/dfe_k = g_p(k) T_k exp(-fe_k) - sum_{j>k} g_p(j) p_j
+      d/dsigma_k = d/dfe_k * 7 dist_k where sigma_k > 0 (relu), else 0
+      d/dc_k = p_k g_rgb.
+    Returns (g_sigma [R,S], g_rgb_samples [R,S,3]); zero at pads."""
+    sigma = np.asarray(sigma, dtype)
+    dists = np.asarray(dists, dtype)
+    z = np.asarray(depth, dtype)
+    c = np.asarray(rgb, dtype)
+    valid = np.asarray(idx) != -1
+    fe = np.where(valid, np.maximum(sigma, 0) * dists * FREE_ENERGY_SCALE, 0).astype(dtype)
+    excl = np.cumsum(np.concatenate([np.zeros_like(fe[:, :1]), fe[:, :-1]], 1), 1)
+    T = np.exp(-excl)
+    e = np.exp(-fe)
+    p = (1 - e) * T
+    gr = np.asarray(g_rgb, dtype)
+    gp = np.where(valid, (c * gr[:, None, :]).sum(-1) + z * np.asarray(g_depth, dtype)[:, None], 0) \
+        - np.asarray(g_missed, dtype)[:, None]
+    u = gp * p
+    later = np.cumsum(u[:, ::-1], 1)[:, ::-1] - u          # sum over j > k
+    dfe = gp * T * e - later
+    g_sigma = np.where(valid & (sigma > 0), dfe * dists * FREE_ENERGY_SCALE, 0)
+    g_c = np.where(valid[..., None], p[..., None] * gr[:, None, :], 0)
+    return g_sigma, g_c
+
+
+def trilinear_backward(sample_idx, xyz, centers, feats, num_keys, grad_out, voxel_size,
+                       dtype=np.float64):
+    """Gradient of trilinear_gather with respect to the embedding table: the
+    adjoint scatter-add grad[feats[v][c]] += w_c * grad_out[m] that torch's
+    F.embedding backward performs for encoder.py:588-590.  Samples with id -1
+    contribute nothing.  Returns [num_keys, D]."""
+    idx = np.asarray(sample_idx, np.int64)
+    g = np.asarray(grad_out, dtype)
+    D = g.shape[1]
+    out = np.zeros((num_keys, D), dtype)
+    ok = idx >= 0
+    if not ok.any():
+        return out
+    W = trilinear_weights(idx[ok], np.asarray(xyz)[ok], centers, voxel_size, dtype=dtype)
+    keys = np.asarray(feats, np.int64)[idx[ok]]                   # [M,8]
+    np.add.at(out, keys.reshape(-1), (W[:, :, None] * g[ok][:, None, :]).reshape(-1, D))
+    return out
+
+
 def fill_background(hits, rgb, depth, missed, bg=(1.0, 1.0, 1.0), bg_depth=BG_DEPTH, dtype=np.float64):
     """nsvf.py:89-101 with geometry.py:303-317: scatter hit rays back to the
     full frame (missed=1, rgb=0, depth=0 elsewhere), then add the background."""
@@ -639,3 +687,113 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
     full_rgb, full_depth, full_missed = fill_background(hits, rgb, depth, missed, dtype=dtype)
     out.update(rgb=full_rgb, depth=full_depth, missed=full_missed)
     return out
+
+
+# ---------------------------------------------------------------------------
+# Training objective with the samples held fixed (finite-difference checks)
+
+class FixedSampleLoss:
+    """Sum of squared RGB errors of the hit rays for given embeddings and MLP,
+    with the samples held fixed -- the part of the training loss
+    (rendering_loss.py:139, RGB term) that depends on the parameters; rays
+    that hit nothing contribute a constant and are left out.  Everything
+    after the sample points (gather weights, MLP, compositing, background)
+    is evaluated in float64; sample points are the float32 o + d * z.
+
+    si, sd, sx: padded [N_hit, S] coarse samples of the hit rays (as
+    render_view returns them); hits: boolean mask over the frame's rays."""
+
+    def __init__(self, si, sd, sx, hits, ray_start, ray_dir, centers_shifted, feats,
+                 voxel_size, target, bg=(1.0, 1.0, 1.0)):
+        self.si, self.sd, self.sx = si, sd, sx
+        self.frame_rows = np.nonzero(np.asarray(hits))[0]
+        valid = si != -1
+        self.rows, self.cols = np.nonzero(valid)
+        rs = np.asarray(ray_start, np.float32)[self.frame_rows[self.rows]]
+        rd = np.asarray(ray_dir, np.float32)[self.frame_rows[self.rows]]
+        z = sd[self.rows, self.cols].astype(np.float32)
+        x = (rs + (rd * z[:, None]).astype(np.float32)).astype(np.float32)
+        vox = si[self.rows, self.cols].astype(np.int64)
+        self.W = trilinear_weights(vox, x, centers_shifted, voxel_size)      # [M,8] float64
+        self.keys = np.asarray(feats, np.int64)[vox]                        # [M,8]
+        self.target = np.asarray(target, np.float64)[self.frame_rows]       # [N_hit,3]
+        self.bg = np.asarray(bg, np.float64)
+
+    def _select(self, rows):
+        if rows is None:
+            return np.arange(len(self.rows)), np.arange(self.si.shape[0])
+        rows = np.unique(np.asarray(rows, np.int64))
+        return np.nonzero(np.isin(self.rows, rows))[0], rows
+
+    def embed(self, values, rows=None):
+        """float64 embeddings [M_sel, D] of the samples of `rows` (all when None)."""
+        m, _ = self._select(rows)
+        V = np.asarray(values, np.float64)
+        emb = np.zeros((len(m), V.shape[1]))
+        for c in range(8):
+            emb += self.W[m, c, None] * V[self.keys[m, c]]
+        return emb
+
+    def loss_from_embedding(self, emb, mlp, rows=None):
+        sigma, rgb = mlp_forward(emb, mlp, dtype=np.float64)
+        return self._composite_loss(sigma, rgb, rows)
+
+    def loss(self, values, mlp, rows=None):
+        return self.loss_from_embedding(self.embed(values, rows), mlp, rows)
+
+    def _composite_loss(self, sigma, rgb, rows=None):
+        m, r = self._select(rows)
+        pos = np.searchsorted(r, self.rows[m])
+        S = np.zeros((len(r), self.si.shape[1]))
+        C = np.zeros((len(r), self.si.shape[1], 3))
+        S[pos, self.cols[m]] = sigma
+        C[pos, self.cols[m]] = rgb
+        col, _, missed, _ = composite(S, C, self.sd[r], self.sx[r], self.si[r])
+        out = col + missed[:, None] * self.bg
+        return float(((out - self.target[r]) ** 2).sum())
+
+    def mlp_central_difference(self, emb, mlp, layer, kind, pos, eps):
+        """(loss(theta + eps) - loss(theta - eps)) / (2 eps) for one MLP entry
+        (weights[layer][pos] if kind == "w", biases[layer][pos] if "b"),
+        over all hit rays.  Exact reformulation of two loss() calls: the
+        perturbation changes one output unit j of `layer`, so that layer's
+        pre-activations change in column j only, and the next layer's by the
+        rank-1 term outer(dh_j, W_next[:, j]); later layers are recomputed."""
+        Ws = [np.asarray(w, np.float64) for w in mlp.weights]
+        bs = [np.asarray(b, np.float64) for b in mlp.biases]
+        n = len(Ws)
+        key = (id(emb), id(mlp))
+        if getattr(self, "_base_key", None) != key:     # unperturbed layers, kept between calls
+            pre, act = [], []
+            h = emb
+            for i in range(n):
+                a = h @ Ws[i].T + bs[i]
+                pre.append(a)
+                h = np.maximum(a, 0) if i < n - 1 else a
+                act.append(h)
+            self._base_key, self._base = key, (pre, act)
+        pre, act = self._base
+        j = pos[0]
+        col_in = emb if layer == 0 else act[layer - 1]
+        du = col_in[:, pos[1]] if kind == "w" else np.ones(emb.shape[0])
+        vals = []
+        for delta in (eps, -eps):
+            aj = pre[layer][:, j] + delta * du
+            if layer == n - 1:
+                out = pre[layer].copy()
+                out[:, j] = aj
+            else:
+                dh = np.maximum(aj, 0) - act[layer][:, j]
+                a = pre[layer + 1] + np.outer(dh, Ws[layer + 1][:, j])
+                for i in range(layer + 1, n):
+                    if i > layer + 1:
+                        a = h @ Ws[i].T + bs[i]
+                    h = np.maximum(a, 0) if i < n - 1 else a
+                out = h
+            vals.append(self._composite_loss(out[:, 0], 1.0 / (1.0 + np.exp(-out[:, 1:4]))))
+        return (vals[0] - vals[1]) / (2 * eps)
+
+    def rows_touching_key(self, key):
+        """Hit rays with a sample in a voxel that has corner `key`."""
+        m = np.nonzero((self.keys == key).any(1))[0]
+        return np.unique(self.rows[m])

@@ -420,6 +420,132 @@ def test_composite_constant_sigma_closed_form():
     assert np.abs(col[0] - 0.25 * (1 - missed[0])).max() < 1e-12
 
 
+def _composite_objective(sigma, rgb, depth, dists, idx, g_rgb, g_depth, g_missed):
+    col, d, missed, _ = O.composite(sigma, rgb, depth, dists, idx)
+    return (col * g_rgb).sum() + (d * g_depth).sum() + (missed * g_missed).sum()
+
+
+def test_composite_backward_matches_central_differences():
+    """Every sigma and colour gradient of the FP64 restatement against central
+    differences of composite() (pads, negative sigmas and zero-length
+    intervals included)."""
+    rng = np.random.default_rng(11)
+    R, S = 5, 9
+    sigma = rng.normal(1.0, 2.0, (R, S))
+    dists = rng.uniform(0.0, 0.2, (R, S))
+    dists[0, 2] = 0.0
+    idx = np.zeros((R, S), np.int32)
+    idx[1, 6:] = -1
+    idx[3, 1:] = -1
+    depth = np.cumsum(rng.uniform(0.1, 0.3, (R, S)), 1)
+    rgb = rng.uniform(0, 1, (R, S, 3))
+    g_rgb, g_depth, g_missed = rng.normal(size=(R, 3)), rng.normal(size=R), rng.normal(size=R)
+    gs, gc = O.composite_backward(sigma, rgb, depth, dists, idx, g_rgb, g_depth, g_missed)
+    f = lambda sg, c: _composite_objective(sg, c, depth, dists, idx, g_rgb, g_depth, g_missed)
+    eps = 1e-6
+    for r in range(R):
+        for k in range(S):
+            if abs(sigma[r, k]) < 10 * eps:
+                continue
+            a, b = sigma.copy(), sigma.copy()
+            a[r, k] += eps
+            b[r, k] -= eps
+            fd = (f(a, rgb) - f(b, rgb)) / (2 * eps)
+            assert abs(fd - gs[r, k]) < 1e-7 * max(1.0, abs(fd)), (r, k, fd, gs[r, k])
+            for ch in range(3):
+                a, b = rgb.copy(), rgb.copy()
+                a[r, k, ch] += eps
+                b[r, k, ch] -= eps
+                fd = (f(sigma, a) - f(sigma, b)) / (2 * eps)
+                assert abs(fd - gc[r, k, ch]) < 1e-7 * max(1.0, abs(fd))
+    assert (gs[idx == -1] == 0).all() and (gs[(sigma <= 0)] == 0).all()
+
+
+def test_composite_backward_single_sample_closed_form():
+    """One sample: rgb = (1 - exp(-7 s d)) c, so d rgb.g / ds = 7 d exp(-7 s d) <c, g>."""
+    s, d = 1.7, 0.05
+    c = np.array([[[0.2, 0.5, 0.9]]])
+    g = np.array([[0.3, -1.0, 2.0]])
+    gs, gc = O.composite_backward(np.array([[s]]), c, np.array([[1.0]]), np.array([[d]]),
+                                  np.zeros((1, 1), np.int32), g, np.zeros(1), np.zeros(1))
+    assert abs(gs[0, 0] - 7 * d * math.exp(-7 * s * d) * float(c[0, 0] @ g[0])) < 1e-14
+    assert np.abs(gc[0, 0] - (1 - math.exp(-7 * s * d)) * g[0]).max() < 1e-15
+
+
+def test_trilinear_backward_is_the_adjoint_of_the_gather():
+    """<gather(V), G> == <V, gather_backward(G)> for random V and G (the
+    backward is exactly the transpose of the forward's linear map), and a
+    sample at a voxel centre spreads 1/8 of its gradient to each corner."""
+    sc = shell()
+    rng = np.random.default_rng(5)
+    cen = O.shift_centers(sc.points, sc.voxel_size).astype(np.float64)
+    M, D = 700, 6
+    vi = rng.integers(0, sc.num_voxels, M)
+    vi[::17] = -1
+    x = cen[np.maximum(vi, 0)] + rng.uniform(-0.5, 0.5, (M, 3)) * sc.voxel_size
+    V = rng.standard_normal((sc.num_keys, D))
+    G = rng.standard_normal((M, D))
+    ok = vi >= 0
+    emb = O.trilinear_gather(vi[ok], x[ok], cen, sc.feats, V, sc.voxel_size)
+    gv = O.trilinear_backward(vi, x, cen, sc.feats, sc.num_keys, G, sc.voxel_size)
+    lhs = (emb * G[ok]).sum()
+    rhs = (V * gv).sum()
+    assert abs(lhs - rhs) < 1e-10 * max(1.0, abs(lhs))
+    v = 42
+    g1 = O.trilinear_backward([v], cen[v][None], cen, sc.feats, sc.num_keys, np.ones((1, 2)),
+                              sc.voxel_size)
+    assert np.allclose(g1[sc.feats[v]], 0.125, atol=1e-15)
+    assert abs(g1.sum() - 2.0) < 1e-14
+
+
+def test_fixed_sample_loss_restates_render_view():
+    """The FP64 training objective with samples held fixed equals the squared
+    error of render_view's image (plus the constant of the rays that miss),
+    in full and over a subset of rays."""
+    sc = shell()
+    rs, rd = shell_rays(width=24, seed=3)
+    ref = O.render_view(sc, rs, rd, hierarchical=False)
+    tgt = np.random.default_rng(1).uniform(0, 1, (rs.shape[0], 3))
+    obj = O.FixedSampleLoss(ref["c_idx"], ref["c_depth"], ref["c_dist"], ref["hits"], rs, rd,
+                            O.shift_centers(sc.points, sc.voxel_size), sc.feats, sc.voxel_size,
+                            tgt)
+    miss = ~ref["hits"]
+    err = ((ref["rgb"] - tgt) ** 2).sum(1)
+    assert abs(obj.loss(sc.values, sc.mlp) + err[miss].sum() - err.sum()) < 1e-9 * err.sum()
+    rows = [0, 3, 5]
+    frame = np.nonzero(ref["hits"])[0][rows]
+    assert abs(obj.loss(sc.values, sc.mlp, rows) - err[frame].sum()) < 1e-12 * err.sum()
+    k = int(obj.keys[obj.rows == 3][0, 0])
+    assert 3 in obj.rows_touching_key(k)
+
+
+def test_mlp_central_difference_equals_two_loss_evaluations():
+    """The rank-1 shortcut for MLP-entry central differences gives the same
+    number as perturbing the MLP and evaluating the loss twice."""
+    sc = shell()
+    rs, rd = shell_rays(width=16, seed=4)
+    ref = O.render_view(sc, rs, rd, hierarchical=False)
+    tgt = np.random.default_rng(2).uniform(0, 1, (rs.shape[0], 3))
+    obj = O.FixedSampleLoss(ref["c_idx"], ref["c_depth"], ref["c_dist"], ref["hits"], rs, rd,
+                            O.shift_centers(sc.points, sc.voxel_size), sc.feats, sc.voxel_size,
+                            tgt)
+    emb = obj.embed(sc.values)
+    eps = 1e-4
+    rng = np.random.default_rng(0)
+    for layer in range(len(sc.mlp.weights)):
+        W = sc.mlp.weights[layer]
+        for kind, pos in (("w", (int(rng.integers(W.shape[0])), int(rng.integers(W.shape[1])))),
+                          ("b", (int(rng.integers(W.shape[0])),))):
+            def at(delta):
+                ws = [w.astype(np.float64) for w in sc.mlp.weights]
+                bs = [b.astype(np.float64) for b in sc.mlp.biases]
+                (ws if kind == "w" else bs)[layer][pos] += delta
+                return obj.loss_from_embedding(emb, O.MLPWeights(ws, bs))
+            naive = (at(eps) - at(-eps)) / (2 * eps)
+            fast = obj.mlp_central_difference(emb, sc.mlp, layer, kind, pos, eps)
+            assert abs(fast - naive) <= 1e-9 * max(1e-3, abs(naive)), (layer, kind, fast, naive)
+
+
 def test_prune_lattice_includes_corners():
     pts = np.zeros((1, 3), np.float32)
     lat = O.prune_lattice(pts, 0.5, bits=16)
