@@ -7,8 +7,9 @@ differences of the oracle's loss.
 Tolerances: kernel gradients within 1e-5 of the FP64 oracle (relative to the
 largest entry); the training gradients within 1e-4 (relative Frobenius
 norm) of the plain path's FP32 autograd, and within 1e-4 relative of the
-central differences (eps 1e-4), with an absolute floor of 1e-6 times the
-largest difference for entries whose gradient nearly cancels."""
+central differences (eps 1e-4; eps 1e-6 for parameters whose 1e-4 interval
+straddles a ReLU kink), with an absolute floor of 1e-6 times the largest
+difference for entries whose gradient nearly cancels."""
 import numpy as np
 import pytest
 import torch
@@ -178,7 +179,7 @@ def c3(scene, device):
     for i, (w, b) in enumerate(zip(field.weights, field.biases)):
         grads[f"w{i}"] = N(w.grad)
         grads[f"b{i}"] = N(b.grad)
-    return {"rs": rs, "rd": rd, "target": target, "loss": float(loss), "grads": grads,
+    return {"rs": rs, "rd": rd, "target": target, "loss": float(loss.detach()), "grads": grads,
             "rgb": N(rgb), "n_hit": tr.last.n_hit}
 
 
@@ -220,7 +221,11 @@ def test_training_gradients_match_fp64_central_differences(scene, c3):
     const = float(((1.0 - target[miss].astype(np.float64)) ** 2).sum())
     full = (obj.loss(sc.values, sc.mlp) + const) / R3
     assert abs(full - c3["loss"]) < 1e-6 * full
-    eps = 1e-4
+    # Central differences at eps 1e-4 and 1e-6.  The loss is piecewise smooth
+    # (ReLU in the MLP and on sigma): where the eps 1e-4 interval straddles a
+    # kink the two differences disagree, and the parameter is checked at 1e-6
+    # instead, where FP64 rounding still leaves ~1e-10 relative accuracy.
+    EPS, FINE = 1e-4, 1e-6
     rng = np.random.default_rng(4)
     g = c3["grads"]
     checks = []
@@ -228,11 +233,13 @@ def test_training_gradients_match_fp64_central_differences(scene, c3):
     touched = np.unique(obj.keys)
     for k, d in zip(rng.choice(touched, 48, replace=False), rng.integers(0, sc.values.shape[1], 48)):
         rows = obj.rows_touching_key(k)
-        vp, vm = sc.values.astype(np.float64), sc.values.astype(np.float64)
-        vp[k, d] += eps
-        vm[k, d] -= eps
-        fd = (obj.loss(vp, sc.mlp, rows) - obj.loss(vm, sc.mlp, rows)) / (2 * eps) / R3
-        checks.append(("values", (k, d), fd, g["values"][k, d]))
+
+        def fd(eps):
+            vp, vm = sc.values.astype(np.float64), sc.values.astype(np.float64)
+            vp[k, d] += eps
+            vm[k, d] -= eps
+            return (obj.loss(vp, sc.mlp, rows) - obj.loss(vm, sc.mlp, rows)) / (2 * eps) / R3
+        checks.append(("values", (k, d), fd(EPS), fd(FINE), g["values"][k, d]))
     # 16 MLP entries: 3 weights and 1 bias of each layer
     emb = obj.embed(sc.values)
     n_layers = len(sc.mlp.weights)
@@ -241,10 +248,14 @@ def test_training_gradients_match_fp64_central_differences(scene, c3):
         picks = [("w", (int(rng.integers(W.shape[0])), int(rng.integers(W.shape[1])))) for _ in range(3)]
         picks.append(("b", (int(rng.integers(W.shape[0])),)))
         for kind, pos in picks:
-            fd = obj.mlp_central_difference(emb, sc.mlp, i, kind, pos, eps) / R3
-            checks.append((f"{kind}{i}", pos, fd, g[f"{kind}{i}"][pos]))
-    fds = np.array([c[2] for c in checks])
-    floor = 1e-6 * np.abs(fds).max()
-    bad = [c for c in checks if abs(c[3] - c[2]) > 1e-4 * abs(c[2]) + floor]
+            a, b = (obj.mlp_central_difference(emb, sc.mlp, i, kind, pos, e) / R3 for e in (EPS, FINE))
+            checks.append((f"{kind}{i}", pos, a, b, g[f"{kind}{i}"][pos]))
     assert len(checks) == 64
+    floor = 1e-6 * max(abs(c[3]) for c in checks)
+    tol = lambda ref: 1e-4 * abs(ref) + floor
+    kink = [abs(c[2] - c[3]) > tol(c[3]) for c in checks]
+    kinked = [c for c, k in zip(checks, kink) if k]
+    bad = [c for c, k in zip(checks, kink) if abs(c[4] - (c[3] if k else c[2])) > tol(c[3])]
+    print(f"{len(kinked)} of 64 parameters sit within 1e-4 of a ReLU kink")
     assert not bad, bad
+    assert len(kinked) <= 12, kinked
