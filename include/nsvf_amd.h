@@ -139,6 +139,48 @@ int nsvf_inverse_cdf_sampling_rows(const int32_t* pts_idx, const float* min_dept
                                    float* out_dists, int64_t* flags, void* stream);
 
 /* ------------------------------------------------------------------------
+ * Pruning scores (fairnr/modules/encoder.py:605-654, nsvf.py:143-146): for
+ * voxels [v0, v0 + n), the max of relu(sigma) over a bits^3 lattice inside
+ * each voxel (offsets (2i+1-bits)/(bits-1) * vs/2 per axis, corners
+ * included) evaluated with the field MLP, into vmax [n].  The reference keeps
+ * a voxel when 1 - min exp(-relu sigma) > th, i.e. 1 - exp(-vmax) > th.
+ * precision 0 = BF16 MFMA, 1 = FP32 MFMA (the reference computes in FP32);
+ * wfrag/bias as for nsvf_field_eval; centers are the x-shifted centres.
+ * bits a multiple of 4 (the reference uses 16).
+ * ---------------------------------------------------------------------- */
+int nsvf_field_prune(int precision, int D, int H, int L, const float* centers,
+                     const int32_t* feats, const float* values, float voxel_size,
+                     int64_t v0, int64_t n, int bits, const void* wfrag,
+                     const float* bias, float* vmax, void* stream);
+
+/* ------------------------------------------------------------------------
+ * Splitting (fairnr/data/geometry.py:250-274, encoder.py:656-676): each of N
+ * voxels of size vs becomes 8 children of size vs/2 (new_points [8N,3] =
+ * p +- vs/4, child order 4x+2y+z), the corner keys are rebuilt (unique,
+ * sorted) from the children's corners (new_feats [8N,8] = key ids, as the
+ * reference's unique inverse), and the new embeddings are the trilinear
+ * interpolation of a parent's corner embeddings at each new corner.  Each
+ * new key takes the lowest-index parent among the voxels sharing it (the
+ * reference's scatter_ leaves that choice to the device).
+ *   nsvf_split_voxels writes new_points, new_feats and the new key count K'
+ *   (device int32 *num_keys), and keeps the sorted keys in the workspace;
+ *   nsvf_split_values then writes new_keys [K',3] (int64, quarter-voxel
+ *   integer coordinates relative to the minimum point) and new_values
+ *   [K',D].  Both take the same workspace of nsvf_split_workspace_bytes(N)
+ *   bytes (-1 on error).  nsvf_split_voxels reads the points' extent back to
+ *   the host once (the packed keys hold 21 bits per axis).
+ * ---------------------------------------------------------------------- */
+int64_t nsvf_split_workspace_bytes(int64_t N, void* stream);
+int nsvf_split_voxels(const float* points, int64_t N, float voxel_size,
+                      void* workspace, int64_t workspace_bytes, float* new_points,
+                      int32_t* new_feats, int32_t* num_keys, void* stream);
+int nsvf_split_values(const float* points, const int32_t* feats,
+                      const float* values, int64_t N, int D, float voxel_size,
+                      const void* workspace, int64_t workspace_bytes,
+                      int64_t K_new, int64_t* new_keys, float* new_values,
+                      void* stream);
+
+/* ------------------------------------------------------------------------
  * Trilinear embedding gather (fairnr/modules/encoder.py:582-590 with
  * fairnr/data/geometry.py:195-200; the reference runs it as torch ops).
  *   s_idx [M] voxel ids; xyz [M,3] sample points; centers [N,3] (x-shifted

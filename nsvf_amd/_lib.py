@@ -36,6 +36,10 @@ SIGNATURES = {
     "nsvf_trilinear_gather_backward_workspace": ([_L, _L, _L, _I, _P], _L),
     "nsvf_trilinear_gather_backward": ([_P, _P, _L, _P, _P, _L, _L, _I, _F, _P, _P, _P, _L, _P],
                                        _I),
+    "nsvf_field_prune": ([_I, _I, _I, _I, _P, _P, _P, _F, _L, _L, _I, _P, _P, _P, _P], _I),
+    "nsvf_split_workspace_bytes": ([_L, _P], _L),
+    "nsvf_split_voxels": ([_P, _L, _F, _P, _L, _P, _P, _P, _P], _I),
+    "nsvf_split_values": ([_P, _P, _P, _L, _I, _F, _P, _L, _L, _P, _P, _P], _I),
     "nsvf_scan_workspace_ints": ([_L], _L),
     "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_render_hits": ([_P, _P, _L, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P, _P, _P, _P], _I),

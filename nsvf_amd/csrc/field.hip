@@ -51,16 +51,12 @@ struct FieldRays {
   const float* dir;    // [R,3]
 };
 
-// Trilinear weights of one sample (corner order 4x + 2y + z).
-__device__ __forceinline__ void corner_weights(const FieldScene& sc, const FieldRays& rays,
-                                               int32_t v, float z, int32_t r, float w[8],
-                                               int32_t key[8]) {
+// Trilinear weights of a point x in voxel v (corner order 4x + 2y + z).
+__device__ __forceinline__ void corner_weights(const FieldScene& sc, int32_t v, const float x[3],
+                                               float w[8], int32_t key[8]) {
   float p[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const float x = rays.start[int64_t(r) * 3 + a] + rays.dir[int64_t(r) * 3 + a] * z;
-    p[a] = (x - sc.centers[int64_t(v) * 3 + a]) / sc.voxel_size + 0.5f;
-  }
+  for (int a = 0; a < 3; ++a) p[a] = (x[a] - sc.centers[int64_t(v) * 3 + a]) / sc.voxel_size + 0.5f;
   const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -98,14 +94,13 @@ __device__ __forceinline__ void gather8(const FieldScene& sc, const float w[8],
 // Corner weights for the BF16 field kernel: the same formula, with the
 // division by the voxel size done as an exact multiply when the size is a
 // power of two.
-__device__ __forceinline__ void corner_weights_fast(const FieldScene& sc, const FieldRays& rays,
-                                                    int32_t v, float z, int32_t r, float w[8],
+__device__ __forceinline__ void corner_weights_fast(const FieldScene& sc, int32_t v,
+                                                    const float x[3], float w[8],
                                                     int32_t key[8]) {
   float p[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float x = rays.start[int64_t(r) * 3 + a] + rays.dir[int64_t(r) * 3 + a] * z;
-    const float d = x - sc.centers[int64_t(v) * 3 + a];
+    const float d = x[a] - sc.centers[int64_t(v) * 3 + a];
     p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
   }
   const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
@@ -165,6 +160,95 @@ __device__ __forceinline__ bf16x8 relu_pack8(const f32x4& lo, const f32x4& hi) {
 
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
 
+// ---------------------------------------------------------------------------
+// Sample sources: sample s -> (voxel, corner weights and keys), false for a
+// slot without a sample.
+
+// Samples of a sample stream: x = o + d * z of the record's ray
+// (geometry.py:160-161), each operation rounded.
+struct StreamSource {
+  FieldScene sc;
+  FieldRays rays;
+  const SampleRec* samples;
+  template <bool kFast>
+  __device__ __forceinline__ bool weights(int64_t s, float w[8], int32_t key[8]) const {
+    const SampleRec rec = samples[s];
+    if (rec.idx < 0) return false;  // an unused slot of a ray's range
+    float x[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      x[a] = rays.start[int64_t(rec.ray) * 3 + a] + rays.dir[int64_t(rec.ray) * 3 + a] * rec.depth;
+    if (kFast)
+      corner_weights_fast(sc, rec.idx, x, w, key);
+    else
+      corner_weights(sc, rec.idx, x, w, key);
+    return true;
+  }
+};
+
+// The pruning lattice (encoder.py:626-631, geometry.py:229-238): bits^3
+// points per voxel v0 + s / bits^3, point (i, j, k) (i slowest) at
+// c + fp32((2i + 1 - bits) / (bits - 1)) * (vs / 2) per axis, corners
+// included.
+struct LatticeSource {
+  FieldScene sc;
+  int64_t voxel0;
+  int bits;
+  float half_voxel;
+  template <bool kFast>
+  __device__ __forceinline__ bool weights(int64_t s, float w[8], int32_t key[8]) const {
+    const int64_t per = int64_t(bits) * bits * bits;
+    const int64_t v = voxel0 + s / per;
+    const int l = int(s - (s / per) * per);
+    const int ijk[3] = {l / (bits * bits), (l / bits) % bits, l % bits};
+    float x[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float off = float(2 * ijk[a] + 1 - bits) / float(bits - 1);
+      x[a] = sc.centers[v * 3 + a] + off * half_voxel;
+    }
+    if (kFast)
+      corner_weights_fast(sc, int32_t(v), x, w, key);
+    else
+      corner_weights(sc, int32_t(v), x, w, key);
+    return true;
+  }
+};
+
+// Output sinks: put() receives each head result (sigma and the three RGB
+// logits, biases added) for lanes with ok; end_tile() runs on every lane of
+// the wave after each tile.
+
+// Writes (sigma, sigmoid(rgb)) per sample.
+struct StoreSink {
+  float4* out;
+  __device__ __forceinline__ void put(int64_t s, bool ok, float sg, float r, float g, float b) {
+    if (ok) out[s] = make_float4(sg, sigmoidf(r), sigmoidf(g), sigmoidf(b));
+  }
+  __device__ __forceinline__ void end_tile(int64_t) {}
+};
+
+// Per-voxel max of relu(sigma) over the lattice (encoder.py:633-638 keeps a
+// voxel when 1 - min exp(-relu sigma) > th, i.e. from this max).  A tile's
+// samples lie in one voxel (bits^3 is a multiple of the tile size), so the
+// wave reduces the tile's max and one lane folds it into the voxel's value
+// with an integer atomicMax (non-negative floats order as their bits).
+struct MaxSigmaSink {
+  float* vmax;   // [voxels of the launch]
+  int64_t per;   // samples per voxel
+  float m;
+  __device__ __forceinline__ void put(int64_t, bool ok, float sg, float, float, float) {
+    if (ok && sg > m) m = sg;
+  }
+  __device__ __forceinline__ void end_tile(int64_t base) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d, 64));
+    if ((threadIdx.x & 63) == 0)
+      atomicMax(reinterpret_cast<int*>(vmax + base / per), __float_as_int(m));
+    m = 0.0f;
+  }
+};
+
 template <int D, int H, int L>
 struct Dims {
   static constexpr int KT0 = (D + 31) / 32;   // k-tiles of the input (bf16)
@@ -186,12 +270,12 @@ constexpr bool weights_in_lds() {
 }
 
 // The MLP on one wave's tile of SAMPLES samples: operand B of layer 0 in
-// b0, weights lw / biases lb (LDS or global), outputs of samples < M to out.
-template <int D, int H, int L>
+// b0, weights lw / biases lb (LDS or global), head results of samples < M
+// to the sink.
+template <int D, int H, int L, class Sink>
 __device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][Dims<D, H, L>::NT],
                                          const bf16x8* lw, const float* lb, int lane, int g,
-                                         int col, int64_t base, int64_t M,
-                                         float4* __restrict__ out) {
+                                         int col, int64_t base, int64_t M, Sink& sink) {
   using Dm = Dims<D, H, L>;
   constexpr int NT = Dm::NT;
   // ---- layer 0: bias enters as the first MFMA's C operand
@@ -258,23 +342,19 @@ __device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], kt == 0 ? bias4 : acc[j],
                                                          0, 0, 0);
     }
-    if (g == 0) {
 #pragma unroll
-      for (int j = 0; j < NT; ++j) {
-        const int64_t s = base + 16 * j + col;
-        if (s < M)
-          out[s] = make_float4(acc[j][0], sigmoidf(acc[j][1]), sigmoidf(acc[j][2]),
-                               sigmoidf(acc[j][3]));
-      }
+    for (int j = 0; j < NT; ++j) {
+      const int64_t s = base + 16 * j + col;
+      sink.put(s, g == 0 && s < M, acc[j][0], acc[j][1], acc[j][2], acc[j][3]);
     }
+    sink.end_tile(base);
   }
 }
 
-template <int D, int H, int L>
+template <int D, int H, int L, class Src, class Sink>
 __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
-    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
-    int64_t m_max, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
-    float4* __restrict__ out) {
+    Src src, const int32_t* m_dev, int64_t m_max, const bf16x8* __restrict__ wfrag,
+    const float* __restrict__ bias, Sink sink) {
   using Dm = Dims<D, H, L>;
   constexpr int NT = Dm::NT;
   constexpr bool kLds = weights_in_lds<D, H, L>();
@@ -298,7 +378,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
                       reinterpret_cast<unsigned char*>(lb) +
                       ((size_t(Dm::BIASES) * sizeof(float) + 15) & ~size_t(15))) +
                   (threadIdx.x >> 6) * (NT * Dm::KT0 * 64);
-  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t M = m_dev ? min(int64_t(*m_dev), m_max) : m_max;
   const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
   for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
        t += int64_t(gridDim.x) * 4) {
@@ -309,15 +389,12 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
       const int64_t s = base + 16 * j + col;
       float w[8];
       int32_t key[8];
-      SampleRec rec{-1, 0.0f, 0.0f, 0};
-      if (s < M) rec = samples[s];
-      const bool valid = rec.idx >= 0;  // -1: an unused slot of a ray's range
-      if (valid) corner_weights_fast(sc, rays, rec.idx, rec.depth, rec.ray, w, key);
+      const bool valid = s < M && src.template weights<true>(s, w, key);
 #pragma unroll
       for (int kt = 0; kt < Dm::KT0; ++kt) {
         float f[8];
         if (valid) {
-          gather8_fma<D>(sc, w, key, kt, g, f);
+          gather8_fma<D>(src.sc, w, key, kt, g, f);
         } else {
 #pragma unroll
           for (int i = 0; i < 8; ++i) f[i] = 0.0f;
@@ -336,7 +413,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
     for (int j = 0; j < NT; ++j)
 #pragma unroll
       for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = stage[(j * Dm::KT0 + kt) * 64 + lane];
-    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, base, M, out);
+    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, base, M, sink);
   }
 }
 
@@ -365,13 +442,10 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
 #pragma unroll
   for (int k = 0; k < 8; ++k) f[k] = 0.0f;
   if (s < M) {
-    const SampleRec rec = samples[s];
-    if (rec.idx >= 0) {
-      float w[8];
-      int32_t key[8];
-      corner_weights_fast(sc, rays, rec.idx, rec.depth, rec.ray, w, key);
-      gather8_fma<D>(sc, w, key, kt, lane >> 4, f);
-    }
+    const StreamSource src{sc, rays, samples};
+    float w[8];
+    int32_t key[8];
+    if (src.weights<true>(s, w, key)) gather8_fma<D>(sc, w, key, kt, lane >> 4, f);
   }
   const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
   const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
@@ -412,23 +486,23 @@ __global__ __launch_bounds__(kFieldBlock) void mlp_bf16_kernel(
     for (int j = 0; j < NT; ++j)
 #pragma unroll
       for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = emb[((t * NT + j) * Dm::KT0 + kt) * 64 + lane];
-    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, out);
+    StoreSink sink{out};
+    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, sink);
   }
 }
 
 // FP32 variant: same structure on v_mfma_f32_16x16x4_f32, weights read from
 // global memory (fragments of 64 floats).
-template <int D, int H, int L>
+template <int D, int H, int L, class Src, class Sink>
 __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
-    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
-    int64_t m_max, const float* __restrict__ wfrag, const float* __restrict__ bias,
-    float4* __restrict__ out) {
+    Src src, const int32_t* m_dev, int64_t m_max, const float* __restrict__ wfrag,
+    const float* __restrict__ bias, Sink sink) {
   constexpr int NT = 2;
   constexpr int KT0 = 4 * ((D + 15) / 16);  // k-tiles (t, i) of the input
   constexpr int KTH = H / 4;
   constexpr int MTH = H / 16;
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t M = m_dev ? min(int64_t(*m_dev), m_max) : m_max;
   const int64_t tiles = (M + 16 * NT - 1) / (16 * NT);
   for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
        t += int64_t(gridDim.x) * 4) {
@@ -439,15 +513,12 @@ __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
       const int64_t s = base + 16 * j + col;
       float w[8];
       int32_t key[8];
-      SampleRec rec{-1, 0.0f, 0.0f, 0};
-      if (s < M) rec = samples[s];
-      const bool valid = rec.idx >= 0;  // -1: an unused slot of a ray's range
-      if (valid) corner_weights(sc, rays, rec.idx, rec.depth, rec.ray, w, key);
+      const bool valid = s < M && src.template weights<false>(s, w, key);
 #pragma unroll
       for (int kt32 = 0; kt32 < (KT0 + 7) / 8; ++kt32) {
         float f[8];
         if (valid) {
-          gather8<D>(sc, w, key, kt32, g, f);
+          gather8<D>(src.sc, w, key, kt32, g, f);
         } else {
 #pragma unroll
           for (int i = 0; i < 8; ++i) f[i] = 0.0f;
@@ -528,16 +599,14 @@ __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
         for (int j = 0; j < NT; ++j)
           acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xh[kt][j], acc[j], 0, 0, 0);
       }
-      if (g == 0) {
-        const float* B = bias + H * L;
+      const float* B = bias + H * L;
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
-          const int64_t s = base + 16 * j + col;
-          if (s < M)
-            out[s] = make_float4(acc[j][0] + B[0], sigmoidf(acc[j][1] + B[1]),
-                                 sigmoidf(acc[j][2] + B[2]), sigmoidf(acc[j][3] + B[3]));
-        }
+      for (int j = 0; j < NT; ++j) {
+        const int64_t s = base + 16 * j + col;
+        sink.put(s, g == 0 && s < M, acc[j][0] + B[0], acc[j][1] + B[1], acc[j][2] + B[2],
+                 acc[j][3] + B[3]);
       }
+      sink.end_tile(base);
     }
   }
 }
@@ -621,32 +690,49 @@ int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
   return launch_mlp_bf16<D, H, L>(m_dev, m_max, wfrag, bias, out, workspace, stream);
 }
 
-template <int D, int H, int L>
-int launch_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples, const int32_t* m_dev,
-                int64_t m_max, const void* wfrag, const float* bias, float* out,
-                hipStream_t stream) {
+template <int D, int H, int L, class Src, class Sink>
+int launch_bf16(const Src& src, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                const float* bias, const Sink& sink, hipStream_t stream) {
   using Dm = Dims<D, H, L>;
   const size_t lds = (weights_in_lds<D, H, L>() ? size_t(Dm::FRAGS) * 1024 : 0) +
                      ((size_t(Dm::BIASES) * sizeof(float) + 15) & ~size_t(15)) +
                      size_t(4) * Dm::NT * Dm::KT0 * 64 * 16;
   const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
   const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
-  hipLaunchKernelGGL((field_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), lds,
-                     stream, sc, rays, samples, m_dev, m_max, static_cast<const bf16x8*>(wfrag), bias,
-                     reinterpret_cast<float4*>(out));
+  hipLaunchKernelGGL((field_bf16_kernel<D, H, L, Src, Sink>), dim3(unsigned(blocks)),
+                     dim3(kFieldBlock), lds, stream, src, m_dev, m_max,
+                     static_cast<const bf16x8*>(wfrag), bias, sink);
   return check_launch("field_bf16");
 }
 
-template <int D, int H, int L>
-int launch_f32(FieldScene sc, FieldRays rays, const SampleRec* samples, const int32_t* m_dev,
-               int64_t m_max, const void* wfrag, const float* bias, float* out,
-               hipStream_t stream) {
+template <int D, int H, int L, class Src, class Sink>
+int launch_f32(const Src& src, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+               const float* bias, const Sink& sink, hipStream_t stream) {
   const int64_t tiles = ceil_div(m_max, 32);
   const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
-  hipLaunchKernelGGL((field_f32_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), 0,
-                     stream, sc, rays, samples, m_dev, m_max, static_cast<const float*>(wfrag), bias,
-                     reinterpret_cast<float4*>(out));
+  hipLaunchKernelGGL((field_f32_kernel<D, H, L, Src, Sink>), dim3(unsigned(blocks)),
+                     dim3(kFieldBlock), 0, stream, src, m_dev, m_max,
+                     static_cast<const float*>(wfrag), bias, sink);
   return check_launch("field_f32");
+}
+
+__global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ p, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// Pruning scores of voxels [v0, v0 + n): max over each voxel's lattice of
+// relu(sigma), into vmax[0, n).
+template <int D, int H, int L>
+int launch_prune(int precision, const FieldScene& sc, int64_t v0, int64_t n, int bits,
+                 const void* wfrag, const float* bias, float* vmax, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(unsigned(ceil_div(n, 256))), dim3(256), 0, stream,
+                     vmax, n);
+  const int64_t per = int64_t(bits) * bits * bits;
+  const LatticeSource src{sc, v0, bits, sc.voxel_size * 0.5f};
+  const MaxSigmaSink sink{vmax, per, 0.0f};
+  if (precision == 0) return launch_bf16<D, H, L>(src, nullptr, n * per, wfrag, bias, sink, stream);
+  return launch_f32<D, H, L>(src, nullptr, n * per, wfrag, bias, sink, stream);
 }
 
 }  // namespace
@@ -686,6 +772,8 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
   FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
   FieldRays rays{ray_start, ray_dir};
   hipStream_t st = as_stream(stream);
+  const StreamSource src{sc, rays, static_cast<const SampleRec*>(samples)};
+  const StoreSink sink{reinterpret_cast<float4*>(out)};
 #define NSVF_FIELD_CASE(d, h, l)                                                               \
   if (D == d && H == h && L == l) {                                                            \
     if (precision == 0 && workspace &&                                                        \
@@ -693,10 +781,8 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
       return launch_bf16_split<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
                                         m_dev, m_max, wfrag, bias, out, workspace, st);        \
     if (precision == 0)                                                                        \
-      return launch_bf16<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), m_dev,     \
-                                  m_max, wfrag, bias, out, st);                                \
-    return launch_f32<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), m_dev, m_max, \
-                               wfrag, bias, out, st);                                          \
+      return launch_bf16<d, h, l>(src, m_dev, m_max, wfrag, bias, sink, st);                   \
+    return launch_f32<d, h, l>(src, m_dev, m_max, wfrag, bias, sink, st);                      \
   }
   NSVF_FIELD_CASE(32, 128, 3)
   NSVF_FIELD_CASE(48, 256, 4)
@@ -745,6 +831,31 @@ int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace, int64_t work
   NSVF_MLP_CASE(32, 256, 4)
 #undef NSVF_MLP_CASE
   return set_error(NSVF_EINVAL, "field_mlp_bf16: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_field_prune(int precision, int D, int H, int L, const float* centers,
+                     const int32_t* feats, const float* values, float voxel_size, int64_t v0,
+                     int64_t n, int bits, const void* wfrag, const float* bias, float* vmax,
+                     void* stream) {
+  clear_error();
+  if (n < 0 || v0 < 0) return set_error(NSVF_EINVAL, "field_prune: bad voxel range");
+  if (n == 0) return NSVF_OK;
+  if (bits < 4 || bits % 4 != 0 || bits > 64)
+    return set_error(NSVF_EINVAL, "field_prune: bits=%d (a multiple of 4 in [4, 64])", bits);
+  if (precision != 0 && precision != 1)
+    return set_error(NSVF_EINVAL, "field_prune: precision %d", precision);
+  if (!centers || !feats || !values || !wfrag || !bias || !vmax)
+    return set_error(NSVF_EINVAL, "field_prune: null pointer");
+  FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
+  hipStream_t st = as_stream(stream);
+#define NSVF_PRUNE_CASE(d, h, l) \
+  if (D == d && H == h && L == l) \
+    return launch_prune<d, h, l>(precision, sc, v0, n, bits, wfrag, bias, vmax, st);
+  NSVF_PRUNE_CASE(32, 128, 3)
+  NSVF_PRUNE_CASE(48, 256, 4)
+  NSVF_PRUNE_CASE(32, 256, 4)
+#undef NSVF_PRUNE_CASE
+  return set_error(NSVF_EINVAL, "field_prune: unsupported shape D=%d H=%d L=%d", D, H, L);
 }
 
 int nsvf_trilinear_gather(const int32_t* s_idx, const float* xyz, int64_t M, const float* centers,

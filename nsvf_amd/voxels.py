@@ -1,0 +1,109 @@
+"""Edits of the voxel set during training: pruning and splitting
+(fairnr/models/nsvf.py:143-155 -> fairnr/modules/encoder.py:605-676).
+
+  pruning   max of relu(sigma) over a 16^3 lattice per voxel, evaluated by
+            the field MLP on the matrix cores (one fused kernel per chunk of
+            voxels: lattice generation, gather, MLP, per-voxel max); a voxel
+            is kept when 1 - exp(-max) > th (encoder.py:610, th = 0.5).
+  splitting 8 children per voxel, rebuilt corner keys and interpolated
+            embeddings (geometry.py:250-274), on the device.
+
+Both work on a DeviceScene (the resident scene) and return the edited
+VoxelScene; the caller builds a new DeviceScene from it, which rebuilds the
+octree, as the reference's clean_caches() does lazily.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import replace
+
+import numpy as np
+import torch
+
+from ._lib import check, lib
+from .render import DeviceScene
+from .scene import VoxelScene
+
+PRECISIONS = {"bf16": 0, "fp32": 1}
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(dev):
+    return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+
+
+def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536):
+    """[N] float32: max over each voxel's bits^3 lattice of relu(sigma)
+    (encoder.py:620-646).  FP32 MFMA by default, as the reference evaluates
+    the field in FP32; "bf16" is faster and may flip voxels whose max is
+    within about 1% of the threshold."""
+    prec = PRECISIONS[precision]
+    N = sc.centers.shape[0]
+    out = torch.empty(N, dtype=torch.float32, device=sc.device)
+    w = sc.mlp["bf16"] if prec == 0 else sc.mlp["f32"]
+    st = _stream(sc.device)
+    for v0 in range(0, N, chunk):
+        n = min(chunk, N - v0)
+        check(lib().nsvf_field_prune(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats),
+                                     _p(sc.values), sc.voxel_size, v0, n, bits, _p(w),
+                                     _p(sc.mlp["bias"]), _p(out[v0:]), st), "field_prune")
+    return out
+
+
+def pruning(sc: DeviceScene, th=0.5, precision="fp32", bits=16):
+    """keep [N] bool: 1 - min(exp(-relu sigma)) > th (encoder.py:606-611),
+    computed from the max as 1 - exp(-max) > th (exp is monotone)."""
+    vmax = prune_scores(sc, precision, bits)
+    return (1 - torch.exp(-vmax)) > th
+
+
+def prune_scene(scene: VoxelScene, keep) -> VoxelScene:
+    """The scene with only the kept voxels (encoder.py:617 masks them out of
+    points and feats; the key table and embeddings stay as they are)."""
+    keep = np.asarray(keep.cpu().numpy() if torch.is_tensor(keep) else keep, bool)
+    return replace(scene, points=np.ascontiguousarray(scene.points[keep]),
+                   feats=np.ascontiguousarray(scene.feats[keep]), meta=dict(scene.meta))
+
+
+def split_scene(scene: VoxelScene, device) -> VoxelScene:
+    """encoder.py:656-676 + nsvf.py:148-155 on the device: 8 children per
+    voxel, new corner keys and embeddings, voxel size halved, max hits x1.5
+    (kept as a float, as the reference's buffer is; the intersection uses its
+    integer part).
+
+    As in the reference, the split starts from the encoder's precomputed
+    centres, which carry the gather's +vs/10 shift in x (encoder.py:383,
+    659-661), so the children's centres keep that shift."""
+    dev = torch.device(device)
+    L = lib()
+    vs = np.float32(scene.voxel_size)
+    pts = np.array(scene.points, np.float32, copy=True)
+    pts[:, 0] += vs / np.float32(10)
+    N = pts.shape[0]
+    D = scene.values.shape[1]
+    st = _stream(dev)
+    points = torch.from_numpy(pts).to(dev)
+    feats = torch.from_numpy(np.ascontiguousarray(scene.feats, np.int32)).to(dev)
+    values = torch.from_numpy(np.ascontiguousarray(scene.values, np.float32)).to(dev)
+    nbytes = int(L.nsvf_split_workspace_bytes(N, st))
+    if nbytes < 0:
+        check(-1, "split_workspace_bytes")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    new_points = torch.empty((8 * N, 3), dtype=torch.float32, device=dev)
+    new_feats = torch.empty((8 * N, 8), dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    check(L.nsvf_split_voxels(_p(points), N, float(vs), _p(ws), nbytes, _p(new_points),
+                              _p(new_feats), _p(count), st), "split_voxels")
+    K = int(count.item())
+    new_keys = torch.empty((K, 3), dtype=torch.int64, device=dev)
+    new_values = torch.empty((K, D), dtype=torch.float32, device=dev)
+    check(L.nsvf_split_values(_p(points), _p(feats), _p(values), N, D, float(vs), _p(ws), nbytes,
+                              K, _p(new_keys), _p(new_values), st), "split_values")
+    half = float(vs * np.float32(0.5))
+    return replace(scene, points=new_points.cpu().numpy(), voxel_size=half,
+                   max_hits=scene.max_hits * 1.5, keys=new_keys.cpu().numpy(),
+                   feats=new_feats.cpu().numpy().astype(np.int64),
+                   values=new_values.cpu().numpy(), meta=dict(scene.meta))

@@ -536,6 +536,21 @@ def prune_lattice(points, voxel_size, bits=16):
     return offset_points(np.asarray(points, np.float32), np.float32(voxel_size) / np.float32(2), bits=bits)
 
 
+def prune_max_sigma(centers_shifted, feats, values, voxel_size, mlp, voxels, bits=16,
+                    dtype=np.float64):
+    """encoder.py:620-646 for the voxels listed: the bits^3 lattice around
+    each (shifted) centre, the gather at that voxel, sigma from the MLP, and
+    the max of relu(sigma) per voxel (the keep test is 1 - exp(-max) > th,
+    encoder.py:610)."""
+    voxels = np.asarray(voxels, np.int64)
+    lat = prune_lattice(np.asarray(centers_shifted, np.float32)[voxels], voxel_size, bits)
+    idx = np.repeat(voxels, bits ** 3)
+    emb = trilinear_gather(idx, lat.reshape(-1, 3), centers_shifted, feats, values, voxel_size,
+                           dtype=dtype)
+    sigma, _ = mlp_forward(emb, mlp, dtype=dtype)
+    return np.maximum(sigma.reshape(len(voxels), -1), 0).max(1)
+
+
 def split_voxels(points, feats, values, voxel_size):
     """geometry.py:250-274: 8 children per voxel at +-vs/4, new unique sorted
     corner keys, new values by trilinear interpolation of the parent corners.

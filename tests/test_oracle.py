@@ -571,3 +571,31 @@ def test_split_reproduces_parent_corner_values():
     # children at parent +- vs/4
     d = np_.reshape(-1, 8, 3) - pts[:, None, :]
     assert np.allclose(np.abs(d), sc.voxel_size / 4)
+
+
+def test_prune_max_sigma_constant_field():
+    """A field whose sigma is a constant s (all weights zero, sigma bias s)
+    gives max relu(sigma) = max(s, 0) for every voxel; and with a sigma linear
+    in one embedding feature that is affine in position, the max sits at the
+    lattice corner (the lattice includes the corners)."""
+    sc = shell()
+    cen = O.shift_centers(sc.points, sc.voxel_size)
+    D = sc.values.shape[1]
+    ws = [np.zeros((4, D))]
+    for s in (-0.3, 1.25):
+        bs = [np.array([s, 0.0, 0.0, 0.0])]
+        m = O.prune_max_sigma(cen, sc.feats, sc.values, sc.voxel_size, O.MLPWeights(ws, bs),
+                              [0, 7, 100], bits=4)
+        assert np.allclose(m, max(s, 0.0), atol=0, rtol=0)
+    # values = x coordinate of each corner key (affine field): sigma = emb[0]
+    coords, res = O.discretize_points(sc.points, np.float32(sc.voxel_size) / 2)
+    key_pos = sc.keys.astype(np.float64) * (sc.voxel_size / 2) + res.astype(np.float64)
+    vals = np.zeros((sc.num_keys, D))
+    vals[:, 0] = key_pos[:, 0] + 5.0
+    w1 = np.zeros((4, D))
+    w1[0, 0] = 1.0
+    m = O.prune_max_sigma(cen, sc.feats, vals, sc.voxel_size,
+                          O.MLPWeights([w1], [np.zeros(4)]), [3, 50], bits=8)
+    # max x over the lattice = centre_x + vs/2 in the key frame
+    ctr = coords[[3, 50], 0].astype(np.float64) * (sc.voxel_size / 2) + res[0, 0]
+    assert np.abs(m - (ctr + sc.voxel_size / 2 + 5.0)).max() < 1e-5
