@@ -140,7 +140,7 @@ int nsvf_inverse_cdf_sampling_rows(const int32_t* pts_idx, const float* min_dept
 
 /* ------------------------------------------------------------------------
  * Pruning scores (fairnr/modules/encoder.py:605-654, nsvf.py:143-146): for
- * voxels [v0, v0 + n), the max of relu(sigma) over a bits^3 lattice inside
+ * voxels [v0, v0 + n) of the N voxels, the max of relu(sigma) over a bits^3 lattice inside
  * each voxel (offsets (2i+1-bits)/(bits-1) * vs/2 per axis, corners
  * included) evaluated with the field MLP, into vmax [n].  The reference keeps
  * a voxel when 1 - min exp(-relu sigma) > th, i.e. 1 - exp(-vmax) > th.
@@ -149,9 +149,10 @@ int nsvf_inverse_cdf_sampling_rows(const int32_t* pts_idx, const float* min_dept
  * bits a multiple of 4 (the reference uses 16).
  * ---------------------------------------------------------------------- */
 int nsvf_field_prune(int precision, int D, int H, int L, const float* centers,
-                     const int32_t* feats, const float* values, float voxel_size,
-                     int64_t v0, int64_t n, int bits, const void* wfrag,
-                     const float* bias, float* vmax, void* stream);
+                     const int32_t* feats, const float* values, int64_t N,
+                     float voxel_size, int64_t v0, int64_t n, int bits,
+                     const void* wfrag, const float* bias, float* vmax,
+                     void* stream);
 
 /* ------------------------------------------------------------------------
  * Splitting (fairnr/data/geometry.py:250-274, encoder.py:656-676): each of N

@@ -834,11 +834,11 @@ int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace, int64_t work
 }
 
 int nsvf_field_prune(int precision, int D, int H, int L, const float* centers,
-                     const int32_t* feats, const float* values, float voxel_size, int64_t v0,
-                     int64_t n, int bits, const void* wfrag, const float* bias, float* vmax,
-                     void* stream) {
+                     const int32_t* feats, const float* values, int64_t N, float voxel_size,
+                     int64_t v0, int64_t n, int bits, const void* wfrag, const float* bias,
+                     float* vmax, void* stream) {
   clear_error();
-  if (n < 0 || v0 < 0) return set_error(NSVF_EINVAL, "field_prune: bad voxel range");
+  if (n < 0 || v0 < 0 || v0 + n > N) return set_error(NSVF_EINVAL, "field_prune: bad voxel range");
   if (n == 0) return NSVF_OK;
   if (bits < 4 || bits % 4 != 0 || bits > 64)
     return set_error(NSVF_EINVAL, "field_prune: bits=%d (a multiple of 4 in [4, 64])", bits);

@@ -48,7 +48,7 @@ def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536):
     for v0 in range(0, N, chunk):
         n = min(chunk, N - v0)
         check(lib().nsvf_field_prune(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats),
-                                     _p(sc.values), sc.voxel_size, v0, n, bits, _p(w),
+                                     _p(sc.values), N, sc.voxel_size, v0, n, bits, _p(w),
                                      _p(sc.mlp["bias"]), _p(out[v0:]), st), "field_prune")
     return out
 
