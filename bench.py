@@ -41,6 +41,10 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--camera-seed", type=int, default=11)
+    ap.add_argument("--serve-requests", type=int, default=40,
+                    help="requests per arrival rate in the serving measurement (0 = skip)")
+    ap.add_argument("--serve-rates", default="10,20,40",
+                    help="Poisson arrival rates (requests/s) of the serving measurement")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB (a known-size kernel for "
                          "calibrating PMC byte counters)")
@@ -216,7 +220,89 @@ def main():
                                          f"FP32), {cpu_s:.1f} s"}
         log(f"cpu oracle: {len(sel) / cpu_s:.0f} rays/s on {cores} cores")
 
+    if args.serve_requests > 0:
+        out["serving"] = serving(args, dev)
+
     print(json.dumps(out), flush=True)
+
+
+SERVE_SCENES = [  # (grid resolution, embedding seed, MLP seed): shells of ~20k-400k voxels
+    (46, 11, 21), (79, 12, 22), (107, 13, 23), (125, 14, 24)]
+SERVE_RES = (256, 400, 512, 800)
+
+
+def serving(args, dev):
+    """Four resident shell scenes (spherical-shell generator at grids of 46,
+    79, 107 and 125 cells: about 20k, 100k, 250k and 400k voxels); requests
+    with a uniform scene, a resolution uniform over 256^2 / 400^2 / 512^2 /
+    800^2 and a camera at radius U(2, 3), arriving as a Poisson process at
+    each rate; rays are on the device before the clock starts.  Reports
+    rays/s, p50/p99 latency (arrival to completion), and the error of a
+    256^2 view of each scene against the plain FP32 path."""
+    import torch
+
+    from nsvf_amd import scene as S
+    from nsvf_amd.plain import PlainNSVF
+    from nsvf_amd.serve import RenderServer, latency_summary
+
+    t0 = time.time()
+    scenes = {f"shell{r}": S.shell_scene(res=r, emb_seed=e, mlp_seed=m) for r, e, m in SERVE_SCENES}
+    srv = RenderServer(dev, slots=4, precision="bf16", hierarchical=True)
+    for name, sc in scenes.items():
+        srv.load_scene(name, sc)
+    log(f"serving: {len(scenes)} scenes resident "
+        f"({', '.join(f'{n}: {sc.num_voxels}' for n, sc in scenes.items())} voxels; "
+        f"{time.time() - t0:.1f}s setup)")
+    names = list(scenes)
+    rng = np.random.default_rng(5)
+
+    def make_requests(n):
+        reqs = []
+        for _ in range(n):
+            name = names[int(rng.integers(len(names)))]
+            res = int(rng.choice(SERVE_RES))
+            cam = S.sphere_camera(float(rng.uniform(2.0, 3.0)), int(rng.integers(1 << 30)))
+            rs, rd = S.camera_rays(res, res, cam)
+            reqs.append((name, torch.from_numpy(rs).to(dev), torch.from_numpy(rd).to(dev)))
+        return reqs
+
+    # warm-up: one request of each scene at each size
+    warm = [(n, *make_requests(1)[0][1:]) for n in names]
+    for w in warm:
+        srv.submit(*w)
+    srv.drain()
+    torch.cuda.synchronize()
+    runs = []
+    for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+        items = make_requests(args.serve_requests)
+        gaps = rng.exponential(1.0 / rate, len(items))
+        arrivals = [(float(t), *it) for t, it in zip(np.cumsum(gaps) - gaps[0], items)]
+        torch.cuda.synchronize()
+        reqs = srv.run_arrivals(arrivals)
+        torch.cuda.synchronize()
+        summ = latency_summary(reqs)
+        summ["rate_req_per_s"] = rate
+        runs.append(summ)
+        log(f"serving @ {rate:g} req/s: {summ['rays_per_sec'] / 1e6:.1f} Mrays/s, "
+            f"p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms")
+    # error against the plain FP32 path, one 256^2 view per scene
+    errs = {}
+    for name in names:
+        rs_np, rd_np = S.camera_rays(256, 256, S.sphere_camera(2.5, 3))
+        rs, rd = torch.from_numpy(rs_np).to(dev), torch.from_numpy(rd_np).to(dev)
+        req = srv.submit(name, rs, rd)
+        srv.drain()
+        pl = PlainNSVF(scenes[name], dev)
+        prgb, _, _ = pl.render(rs, rd)
+        d = (req.result[0].double() - prgb.double())
+        mse = float(torch.mean(d * d))
+        errs[name] = {"max_abs_rgb": float(d.abs().max()),
+                      "psnr_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None}
+        del pl
+        torch.cuda.empty_cache()
+    return {"scenes": {n: sc.num_voxels for n, sc in scenes.items()}, "slots": 4,
+            "precision": "bf16", "resolutions": list(SERVE_RES), "runs": runs,
+            "error_vs_plain_fp32_256": errs, "budget_peak_bytes": srv.budget.peak}
 
 
 if __name__ == "__main__":

@@ -179,6 +179,8 @@ class Renderer:
         self.fine_steps = float(fine_steps)
         self.compat = int(bool(compat))
         self._ws = {}
+        self._host = None      # pinned [n_hit, slots] of the view in flight
+        self._pending = None   # the view between begin() and finish()
         self.last = None
         self.trace = None   # optional list of (stage, event) marks
         self._kernel_events = None  # optional list of (kernel, units, start, end)
@@ -192,7 +194,7 @@ class Renderer:
     def _buf(self, name, n, dtype):
         t = self._ws.get(name)
         if t is None or t.numel() < n or t.dtype != dtype:
-            t = torch.empty(max(int(n * 1.25), 1024), dtype=dtype, device=self.device)
+            t = torch.empty(self._alloc_elems(n), dtype=dtype, device=self.device)
             self._ws[name] = t
         return t[:n] if n else t[:0]
 
@@ -246,17 +248,81 @@ class Renderer:
         check(lib().nsvf_scan_exclusive_i32(_p(counts), _p(offsets), n, _p(ws), _p(total),
                                             _stream(self.device)), "scan")
 
+    # ---- workspace planning (exact bytes each phase allocates)
+
+    @staticmethod
+    def _alloc_elems(n):
+        return max(int(n * 1.25), 1024)
+
+    def _plan_bytes(self, sizes):
+        """Bytes that _buf would newly allocate for {name: (n, dtype)}."""
+        total = 0
+        for name, (n, dtype) in sizes.items():
+            t = self._ws.get(name)
+            if t is None or t.numel() < n or t.dtype != dtype:
+                total += self._alloc_elems(n) * torch.empty((), dtype=dtype).element_size()
+        return total
+
+    def workspace_bytes(self):
+        return sum(t.numel() * t.element_size() for t in self._ws.values())
+
+    def release_workspace(self):
+        """Drops every workspace buffer (back to torch's caching allocator)."""
+        self._ws = {}
+
+    def _geometry_sizes(self, R, P):
+        i32, f32 = torch.int32, torch.float32
+        return {"hits": (R * P * 4, i32), "count": (R, i32), "tot": (R, f32), "flag": (R, i32),
+                "c_bound": (R, i32), "scalars": (8, i32), "f_bound": (R, i32), "rank": (R, i32),
+                "hit_list": (R, i32), "c_off": (R, i32),
+                "scan_ws": (int(lib().nsvf_scan_workspace_ints(R)), i32),
+                "ray_rgb": (R * 3, f32), "ray_depth": (R, f32), "ray_missed": (R, f32),
+                "c_cnt": (R, i32), "f_off": (R, i32), "f_cnt": (R, i32),
+                "out": (R * 5, f32)}
+
+    def _shading_sizes(self, sc, R, n_hit, slots_c, slots_f):
+        i32, f32 = torch.int32, torch.float32
+        sizes = {"c_samples": (slots_c * 4, i32), "c_field": (slots_c * 4, f32),
+                 "c_prob": (slots_c, f32)}
+        m = slots_c
+        if self.hierarchical and n_hit > 0:
+            sizes.update({"f_samples": (slots_f * 4, i32), "f_field": (slots_f * 4, f32)})
+            m = max(m, slots_f)
+        if PRECISIONS[self.precision] == 0 and self.split_field and m > 0:
+            ws_bytes = int(lib().nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
+            sizes["field_ws"] = ((ws_bytes + 15) // 16 * 4, i32)
+        return sizes
+
+    def geometry_bytes(self, sc: DeviceScene, R):
+        """Workspace bytes begin() would allocate for R rays."""
+        return self._plan_bytes(self._geometry_sizes(R, sc.max_hits))
+
+    def shading_bytes(self):
+        """Workspace bytes finish() would allocate for the view begun last
+        (valid once geometry_ready())."""
+        pd = self._pending
+        n_hit, slots_c, slots_f = self._counts()
+        return self._plan_bytes(self._shading_sizes(pd["sc"], pd["R"], n_hit, slots_c, slots_f))
+
+    # ---- phase 1: intersection, compaction, slot offsets
+
     @torch.no_grad()
-    def coarse_stream(self, sc: DeviceScene, ray_start, ray_dir):
-        """The geometry stages of a view: intersection, hit compaction and the
-        coarse sample stream (encoder.py:498-556).  Returns a CoarseStream
-        whose tensors are workspace views, valid until the next call."""
+    def begin(self, sc: DeviceScene, ray_start, ray_dir):
+        """Enqueues the geometry stages of a view on the current stream:
+        intersection, hit compaction and the coarse slot offsets
+        (encoder.py:498-535), then an asynchronous copy of the hit-ray count
+        and slot total to pinned host memory.  geometry_ready() tells when
+        finish() can run without waiting."""
         L = lib()
         dev = self.device
         st = _stream(dev)
+        ray_start = ray_start.contiguous()
+        ray_dir = ray_dir.contiguous()
         R = ray_start.shape[0]
         P = sc.max_hits
-        fs = self.fine_steps
+        for name, (n, dtype) in self._geometry_sizes(R, P).items():
+            self._buf(name, n, dtype)
+        self._mark("start")
         hits = self._buf("hits", R * P * 4, torch.int32)
         count = self._buf("count", R, torch.int32)
         tot = self._buf("tot", R, torch.float32)
@@ -278,39 +344,77 @@ class Renderer:
         check(L.nsvf_render_compact(_p(flag), _p(rank), R, _p(hit_list), st), "render_compact")
         c_off = self._buf("c_off", R, torch.int32)
         self._scan(c_bound, c_off, R, scal[1:2])
+        if self._host is None:
+            self._host = torch.zeros(2, dtype=torch.int32, pin_memory=True)
         # the one device-to-host read of the view: hit rays and slot total
-        host = scal[:2].cpu()
-        n_hit, slots_c = int(host[0]), int(host[1])
+        self._host.copy_(scal[:2], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self._pending = {"sc": sc, "rs": ray_start, "rd": ray_dir, "R": R, "event": ev}
         self._mark("compact")
+
+    def geometry_ready(self):
+        return self._pending is not None and self._pending["event"].query()
+
+    def _counts(self):
+        self._pending["event"].synchronize()
+        n_hit, slots_c = int(self._host[0]), int(self._host[1])
+        slots_f = slots_c + n_hit * (int(math.ceil(self.fine_steps)) + 2)   # fine-slot bound
+        return n_hit, slots_c, slots_f
+
+    def _sample_coarse(self):
+        """Coarse sample stream of the view begun last (its geometry done)."""
+        L = lib()
+        pd = self._pending
+        sc, R = pd["sc"], pd["R"]
+        n_hit, slots_c, slots_f = self._counts()
+        for name, (n, dtype) in self._shading_sizes(sc, R, n_hit, slots_c, slots_f).items():
+            self._buf(name, n, dtype)
+        P = sc.max_hits
+        scal = self._buf("scalars", 8, torch.int32)
         cs = self._stream("c_samples", slots_c)
         c_cnt = self._buf("c_cnt", R, torch.int32)
-        check(L.nsvf_render_sample_coarse(_p(hit_list), _p(scal[0:1]), R, _p(hits), _p(count),
-                                          _p(tot), _p(c_bound), _p(c_off), P, sc.step_size,
-                                          self.compat, fs, _p(cs), _p(c_cnt), _p(f_bound),
-                                          _p(scal[2:3]), st), "sample_coarse")
+        check(L.nsvf_render_sample_coarse(
+            _p(self._buf("hit_list", R, torch.int32)), _p(scal[0:1]), R,
+            _p(self._buf("hits", R * P * 4, torch.int32)), _p(self._buf("count", R, torch.int32)),
+            _p(self._buf("tot", R, torch.float32)), _p(self._buf("c_bound", R, torch.int32)),
+            _p(self._buf("c_off", R, torch.int32)), P, sc.step_size, self.compat,
+            self.fine_steps, _p(cs), _p(c_cnt), _p(self._buf("f_bound", R, torch.int32)),
+            _p(scal[2:3]), _stream(self.device)), "sample_coarse")
         self._mark("coarse_sample")
-        return CoarseStream(R, n_hit, slots_c, hits, count, tot, flag, rank, hit_list, c_off,
-                            c_cnt, cs, f_bound, scal)
+        return CoarseStream(R, n_hit, slots_c, self._buf("hits", R * P * 4, torch.int32),
+                            self._buf("count", R, torch.int32), self._buf("tot", R, torch.float32),
+                            self._buf("flag", R, torch.int32), self._buf("rank", R, torch.int32),
+                            self._buf("hit_list", R, torch.int32),
+                            self._buf("c_off", R, torch.int32), c_cnt, cs,
+                            self._buf("f_bound", R, torch.int32), scal)
 
     @torch.no_grad()
-    def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False):
-        """ray_start, ray_dir: [R,3] float32 on the device.  Returns
-        (rgb [R,3], depth [R], missed [R]) -- the fine pass when hierarchical,
-        else the coarse pass -- after background fill."""
+    def coarse_stream(self, sc: DeviceScene, ray_start, ray_dir):
+        """The geometry stages of a view: intersection, hit compaction and the
+        coarse sample stream (encoder.py:498-556).  Returns a CoarseStream
+        whose tensors are workspace views, valid until the next call."""
+        self.begin(sc, ray_start, ray_dir)
+        g = self._sample_coarse()
+        self._pending = None
+        return g
+
+    # ---- phase 2: samples, field, compositing, background
+
+    @torch.no_grad()
+    def finish(self, keep=False):
+        """Enqueues the rest of the view begun last (waits for its geometry
+        if it is not done).  Returns (rgb [R,3], depth [R], missed [R])."""
         L = lib()
         dev = self.device
         st = _stream(dev)
-        ray_start = ray_start.contiguous()
-        ray_dir = ray_dir.contiguous()
-        R = ray_start.shape[0]
-        P = sc.max_hits
+        pd = self._pending
+        sc, ray_start, ray_dir, R = pd["sc"], pd["rs"], pd["rd"], pd["R"]
         fs = self.fine_steps
-        self._mark("start")
-        g = self.coarse_stream(sc, ray_start, ray_dir)
+        g = self._sample_coarse()
         n_hit, slots_c, scal, hit_list = g.n_hit, g.slots, g.scal, g.hit_list
-        hits, count, tot, flag, rank = g.hits, g.count, g.tot, g.flag, g.rank
-        c_off, c_cnt, cs, f_bound = g.c_off, g.c_cnt, g.samples, g.f_bound
-        slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)   # bound on the fine slots
+        flag, rank, c_off, c_cnt, cs, f_bound = g.flag, g.rank, g.off, g.cnt, g.samples, g.f_bound
+        slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)
         c_field = self._buf("c_field", slots_c * 4, torch.float32)
         self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
         self._mark("coarse_field")
@@ -339,19 +443,22 @@ class Renderer:
                                           _p(fsm), _p(f_field), None, _p(ray_rgb), _p(ray_depth),
                                           _p(ray_missed), st), "composite fine")
             self._mark("fine_composite")
-        rgb = torch.empty((R, 3), dtype=torch.float32, device=dev)
-        depth = torch.empty(R, dtype=torch.float32, device=dev)
-        missed = torch.empty(R, dtype=torch.float32, device=dev)
+        # the outputs leave the workspace: the caller owns them
+        out = self._buf("out", R * 5, torch.float32)
+        del self._ws["out"]
+        rgb, depth, missed = out[:R * 3].view(R, 3), out[R * 3:R * 4], out[R * 4:R * 5]
         check(L.nsvf_render_finish(R, _p(flag), _p(ray_rgb), _p(ray_depth), _p(ray_missed),
                                    BG_COLOR[0], BG_COLOR[1], BG_COLOR[2], BG_DEPTH, _p(rgb),
                                    _p(depth), _p(missed), st), "render_finish")
         self._mark("finish")
+        self._pending = None
         self.last = RenderStats(R, n_hit, slots_c, slots_f if self.hierarchical else 0, 0)
         if keep:
             hl = hit_list[:n_hit].long()
+            P = sc.max_hits
             self.kept = {
-                "hits": hits.view(R, P, 4), "count": count, "tot": tot, "flag": flag, "rank": rank,
-                "hit_list": hit_list[:n_hit], "c_cnt": c_cnt[hl], "c_off": c_off[hl],
+                "hits": g.hits.view(R, P, 4), "count": g.count, "tot": g.tot, "flag": flag,
+                "rank": rank, "hit_list": hit_list[:n_hit], "c_cnt": c_cnt[hl], "c_off": c_off[hl],
                 "c_samples": _columns(cs), "c_field": c_field.view(-1, 4), "c_prob": c_prob,
                 "c_max": int(scal[2]),
             }
@@ -359,6 +466,14 @@ class Renderer:
                 self.kept.update({"f_cnt": f_cnt[hl], "f_off": f_off[hl],
                                   "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4)})
         return rgb, depth, missed
+
+    @torch.no_grad()
+    def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False):
+        """ray_start, ray_dir: [R,3] float32 on the device.  Returns
+        (rgb [R,3], depth [R], missed [R]) -- the fine pass when hierarchical,
+        else the coarse pass -- after background fill."""
+        self.begin(sc, ray_start, ray_dir)
+        return self.finish(keep)
 
     def count_samples(self):
         """Exact sample counts of the last view (reads the device)."""

@@ -195,7 +195,8 @@ def test_training_gradients_match_plain_autograd(scene, c3, device):
     rgb, _, _ = pl.forward(t(c3["rs"]), t(c3["rd"]))
     loss = torch.mean((rgb - t(c3["target"])) ** 2)
     loss.backward()
-    assert abs(float(loss) - c3["loss"]) <= 1e-6 * abs(float(loss))
+    lv = loss.item()
+    assert abs(lv - c3["loss"]) <= 1e-6 * abs(lv)
     assert np.abs(N(rgb) - c3["rgb"]).max() < 1e-5
     ref = {"values": N(pl.values.grad)}
     for i, (w, b) in enumerate(pl.layers):
