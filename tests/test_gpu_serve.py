@@ -83,11 +83,13 @@ def test_tight_budget_queues_and_matches_alone(device):
     one = max(need(*it) for it in items)
     srv.budget.total = srv.budget.used + int(1.5 * one)
     reqs = [srv.submit(*it) for it in items]
-    in_flight = 0
+    held_back = False
     while not srv.scheduler.idle():
         srv.poll()
-        in_flight = max(in_flight, srv.scheduler.in_flight)
         assert srv.budget.used <= srv.budget.total
+        # a request waiting while a slot is free: the budget held it back
+        held_back |= bool(srv.scheduler.queue) and any(st.phase == "idle"
+                                                       for st in srv.scheduler.slots)
     torch.cuda.synchronize()
-    assert in_flight < 4, "the budget should have held requests back"
+    assert held_back or sum(r.preemptions for r in reqs) > 0, "the budget never held a request back"
     check_same(srv, reqs, items)
