@@ -68,11 +68,16 @@ def test_staggered_arrivals_match_alone(server):
 
 
 def test_tight_budget_queues_and_matches_alone(device):
+    """Six equal 64x64 requests under a budget of the scenes plus 1.5 times
+    one request's workspace: several geometries fit at once, but only one
+    request can shade at a time; every result still equals its render alone."""
     srv = RenderServer(device, slots=4, budget_bytes=1 << 40, precision="bf16")
     srv.load_scene("a", S.shell_scene(res=32))
     srv.load_scene("b", S.shell_scene(res=40, jitter_seed=1, emb_seed=5, mlp_seed=6))
-    items = requests(6, 3, device)
-    # a budget of the scenes plus 1.5 times the largest request's workspace
+    items = []
+    for i in range(6):
+        rs, rd = S.camera_rays(64, 64, S.sphere_camera(2.5, 50 + i))
+        items.append(("ab"[i % 2], torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device)))
 
     def need(scene, rs, rd):
         probe = Renderer(device, precision="bf16")
@@ -80,16 +85,15 @@ def test_tight_budget_queues_and_matches_alone(device):
         probe.begin(srv.scenes[scene], rs, rd)
         return g + probe.shading_bytes()
 
-    one = max(need(*it) for it in items)
-    srv.budget.total = srv.budget.used + int(1.5 * one)
+    needs = [need(*it) for it in items]
+    srv.budget.total = srv.budget.used + int(1.5 * min(needs))
+    assert max(needs) <= srv.budget.total - srv.budget.used
     reqs = [srv.submit(*it) for it in items]
-    held_back = False
+    shading = 0
     while not srv.scheduler.idle():
         srv.poll()
         assert srv.budget.used <= srv.budget.total
-        # a request waiting while a slot is free: the budget held it back
-        held_back |= bool(srv.scheduler.queue) and any(st.phase == "idle"
-                                                       for st in srv.scheduler.slots)
+        shading = max(shading, sum(st.phase == "shading" for st in srv.scheduler.slots))
     torch.cuda.synchronize()
-    assert held_back or sum(r.preemptions for r in reqs) > 0, "the budget never held a request back"
+    assert shading == 1, shading
     check_same(srv, reqs, items)
