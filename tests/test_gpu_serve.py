@@ -68,16 +68,15 @@ def test_staggered_arrivals_match_alone(server):
 
 
 def test_tight_budget_queues_and_matches_alone(device):
-    """Six equal 64x64 requests under a budget of the scenes plus 1.5 times
-    one request's workspace: several geometries fit at once, but only one
+    """Six identical 64x64 requests under a budget of the scenes plus 1.5
+    times one request's workspace: several geometries fit at once, but only one
     request can shade at a time; every result still equals its render alone."""
     srv = RenderServer(device, slots=4, budget_bytes=1 << 40, precision="bf16")
     srv.load_scene("a", S.shell_scene(res=32))
     srv.load_scene("b", S.shell_scene(res=40, jitter_seed=1, emb_seed=5, mlp_seed=6))
-    items = []
-    for i in range(6):
-        rs, rd = S.camera_rays(64, 64, S.sphere_camera(2.5, 50 + i))
-        items.append(("ab"[i % 2], torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device)))
+    rs, rd = S.camera_rays(64, 64, S.sphere_camera(2.5, 50))
+    items = [("a", torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device))
+             for _ in range(6)]
 
     def need(scene, rs, rd):
         probe = Renderer(device, precision="bf16")
@@ -86,8 +85,8 @@ def test_tight_budget_queues_and_matches_alone(device):
         return g + probe.shading_bytes()
 
     needs = [need(*it) for it in items]
-    srv.budget.total = srv.budget.used + int(1.5 * min(needs))
-    assert max(needs) <= srv.budget.total - srv.budget.used
+    assert len(set(needs)) == 1
+    srv.budget.total = srv.budget.used + int(1.5 * needs[0])
     reqs = [srv.submit(*it) for it in items]
     shading = 0
     while not srv.scheduler.idle():
