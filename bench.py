@@ -292,7 +292,7 @@ def serving(args, dev):
         rs, rd = torch.from_numpy(rs_np).to(dev), torch.from_numpy(rd_np).to(dev)
         req = srv.submit(name, rs, rd)
         srv.drain()
-        pl = PlainNSVF(scenes[name], dev)
+        pl = PlainNSVF(scenes[name], dev, compat=srv.compat)
         prgb, _, _ = pl.render(rs, rd)
         d = (req.result[0].double() - prgb.double())
         mse = float(torch.mean(d * d))
@@ -301,7 +301,8 @@ def serving(args, dev):
         del pl
         torch.cuda.empty_cache()
     return {"scenes": {n: sc.num_voxels for n, sc in scenes.items()}, "slots": 4,
-            "precision": "bf16", "resolutions": list(SERVE_RES), "runs": runs,
+            "precision": "bf16", "sampler_compat": bool(srv.compat),
+            "resolutions": list(SERVE_RES), "runs": runs,
             "error_vs_plain_fp32_256": errs, "budget_peak_bytes": srv.budget.peak}
 
 

@@ -372,9 +372,9 @@ int nsvf_plain_inverse_cdf_sampling(const int32_t* pts_idx,
                                     const float* max_depth, const float* noise,
                                     const float* probs, const float* steps,
                                     int B, int R, int P, int S,
-                                    float fixed_step_size, int32_t* out_idx,
-                                    float* out_depth, float* out_dists,
-                                    void* stream);
+                                    float fixed_step_size, int compat,
+                                    int32_t* out_idx, float* out_depth,
+                                    float* out_dists, void* stream);
 
 #ifdef __cplusplus
 }

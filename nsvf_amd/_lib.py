@@ -58,8 +58,8 @@ SIGNATURES = {
     "nsvf_render_finish": ([_L, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
     "nsvf_plain_aabb_intersect": ([_P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
     "nsvf_plain_svo_intersect": ([_P, _P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
-    "nsvf_plain_inverse_cdf_sampling": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _P, _P, _P,
-                                         _P], _I),
+    "nsvf_plain_inverse_cdf_sampling": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _F, _I, _P, _P,
+                                         _P, _P], _I),
 }
 
 

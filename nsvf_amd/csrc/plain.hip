@@ -116,7 +116,7 @@ struct PlainNoise {
   __device__ __forceinline__ float at(int k) const { return k < avail ? row[k] : 0.5f; }
 };
 
-__global__ void plain_icdf_kernel(int B, int R, int P, int S, float fixed_step_size,
+__global__ void plain_icdf_kernel(int B, int R, int P, int S, float fixed_step_size, int compat,
                                   const int32_t* __restrict__ pts_idx,
                                   const float* __restrict__ min_depth,
                                   const float* __restrict__ max_depth,
@@ -132,7 +132,7 @@ __global__ void plain_icdf_kernel(int B, int R, int P, int S, float fixed_step_s
                    pts_idx + int64_t(b) * R * P, (r + 1 < BR) ? pts_idx[H + P] : -1};
     PlainNoise nz{noise + r * S, (BR - r) * S};
     RowSink sink{out_idx + r * S, out_depth + r * S, out_dists + r * S, S, 0};
-    icdf_ray(bins, P, steps[r], fixed_step_size, nz, 1, S, sink);
+    icdf_ray(bins, P, steps[r], fixed_step_size, nz, compat, S, sink);
   }
 }
 
@@ -169,13 +169,13 @@ int nsvf_plain_svo_intersect(const float* ray_start, const float* ray_dir, const
 int nsvf_plain_inverse_cdf_sampling(const int32_t* pts_idx, const float* min_depth,
                                     const float* max_depth, const float* noise,
                                     const float* probs, const float* steps, int B, int R, int P,
-                                    int S, float fixed_step_size, int32_t* out_idx,
+                                    int S, float fixed_step_size, int compat, int32_t* out_idx,
                                     float* out_depth, float* out_dists, void* stream) {
   clear_error();
   if (B <= 0 || R <= 0 || S <= 0) return NSVF_OK;
   if (P <= 0) return set_error(NSVF_EINVAL, "plain_inverse_cdf_sampling: no bins");
   hipLaunchKernelGGL(plain_icdf_kernel, dim3(B), dim3(opt_threads(R)), 0, as_stream(stream), B, R,
-                     P, S, fixed_step_size, pts_idx, min_depth, max_depth, noise, probs, steps,
+                     P, S, fixed_step_size, compat, pts_idx, min_depth, max_depth, noise, probs, steps,
                      out_idx, out_depth, out_dists);
   return check_launch("plain_inverse_cdf_sampling");
 }

@@ -75,7 +75,7 @@ def _svo_ray_intersect(voxelsize, n_max, points, children, ray_start, ray_dir):
 
 
 def _inverse_cdf_sampling(pts_idx, min_depth, max_depth, probs, steps, fixed_step_size=-1,
-                          deterministic=True):
+                          deterministic=True, compat=True):
     """clib/__init__.py:231-300 with the naive kernel."""
     G, N, P = 200, pts_idx.size(0), pts_idx.size(1)
     H = int(np.ceil(N / G)) * G
@@ -111,7 +111,8 @@ def _inverse_cdf_sampling(pts_idx, min_depth, max_depth, probs, steps, fixed_ste
         ox = torch.zeros((B, R, max_steps), dtype=torch.float32, device=a[0].device)
         check(lib().nsvf_plain_inverse_cdf_sampling(_p(a[0]), _p(a[1]), _p(a[2]), _p(a[3]), _p(a[4]),
                                                     _p(a[5]), B, R, P, max_steps,
-                                                    float(fixed_step_size), _p(oi), _p(od), _p(ox),
+                                                    float(fixed_step_size), int(bool(compat)),
+                                                    _p(oi), _p(od), _p(ox),
                                                     _stream(a[0].device)), "plain_icdf")
         results.append((oi, od, ox))
     si, sd, sx = [torch.cat([r[i] for r in results], 1) for i in range(3)]
@@ -126,7 +127,7 @@ class PlainNSVF:
     """The reference NSVF model's eval-time rendering for one scene."""
 
     def __init__(self, scene: VoxelScene, device, chunk_size=64, hierarchical=True,
-                 fine_num_samples=64):
+                 fine_num_samples=64, compat=True):
         dev = torch.device(device)
         self.device = dev
         self.points = torch.from_numpy(scene.points).to(dev)
@@ -141,6 +142,9 @@ class PlainNSVF:
         self.chunk_size = 1024 * chunk_size
         self.hierarchical = hierarchical
         self.fine_num_samples = fine_num_samples
+        # compat=False: the sampler's evident intent (no trailing duplicate
+        # sample carrying the next ray's voxel id, sample_gpu.cu:188)
+        self.compat = compat
         self._octree = None
 
     # encoder.py:365-378 (runtime cache, built lazily from the kept points)
@@ -176,7 +180,8 @@ class PlainNSVF:
 
     def ray_sample(self, io):
         si, sd, sx = _inverse_cdf_sampling(io["intersected_voxel_idx"], io["min_depth"],
-                                           io["max_depth"], io["probs"], io["steps"], -1, True)
+                                           io["max_depth"], io["probs"], io["steps"], -1, True,
+                                           self.compat)
         sx = sx.clamp(min=0.0)
         sd.masked_fill_(si.eq(-1), MAX_DEPTH)
         sx.masked_fill_(si.eq(-1), 0.0)

@@ -34,7 +34,7 @@ def server(device):
 
 
 def alone(srv, scene, rs, rd):
-    r = Renderer(srv.device, precision="bf16", hierarchical=True)
+    r = Renderer(srv.device, precision="bf16", hierarchical=True, compat=srv.compat)
     rgb, depth, missed = r.render(srv.scenes[scene], rs, rd)
     torch.cuda.synchronize()
     return rgb.clone(), depth.clone(), missed.clone()
