@@ -16,7 +16,6 @@ Errors are stated per output (RGB, depth, alpha = 1 - missed):
 Bars: BF16 PSNR >= 45 dB end to end against FP64; FP32 end-to-end RGB
 within 1e-4 of FP64 given the same samples."""
 import math
-from dataclasses import replace
 
 import numpy as np
 import pytest
