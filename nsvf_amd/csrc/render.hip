@@ -20,8 +20,11 @@
 // depth, interval length, ray id}.  No padded [R, S] sample tensor exists on
 // this path; padding only appears in the op-level kernels that keep the
 // reference's layout.
+#include <algorithm>
+
 #include "render.h"
 #include "sample_core.h"
+#include "sample_wave.h"
 
 namespace nsvf {
 namespace {
@@ -344,24 +347,6 @@ struct SampleBins {
   __device__ __forceinline__ int32_t ray0_id(int k) const { return k < cnt0 ? rec0[k].idx : -1; }
 };
 
-// Writes the samples that carry a voxel id into the ray's slot range (dists
-// clamped at 0: encoder.py:547).
-struct StreamSink {
-  SampleRec* out;
-  int32_t ray_id;
-  int cap;  // the ray's slot count (never exceeded for valid bins; a guard)
-  int n = 0;
-  __device__ __forceinline__ void operator()(int, int32_t id, float d, float mid) {
-    if (id == -1 || n >= cap) return;
-    out[n] = SampleRec{id, mid, d > 0.0f ? d : 0.0f, ray_id};
-    ++n;
-  }
-  // Marks the slots past the written samples as unused.
-  __device__ __forceinline__ void close(int bound) {
-    for (int k = n; k < bound; ++k) out[k] = SampleRec{-1, 0.0f, 0.0f, ray_id};
-  }
-};
-
 __device__ __forceinline__ void grouped_neighbours(int64_t n, int64_t N, int64_t* ray0,
                                                    int64_t* next) {
   constexpr int G = 200, chunk = 800;  // clib/__init__.py:234,259
@@ -381,9 +366,12 @@ __device__ __forceinline__ void grouped_neighbours(int64_t n, int64_t N, int64_t
   *next = nn >= N ? 0 : nn;
 }
 
-// Coarse pass for hit ray n (ray r = hit_list[n]): samples into the slot
-// range [c_off[r], c_off[r] + bound[r]); count, fine-pass bound and the
-// longest row are recorded for the fine pass.
+// Waves per sampler block and the grid-stride loop over hit rays.
+constexpr int kSampleWaves = kBlock / 64;
+
+// Coarse pass: one wavefront per hit ray n (ray r = hit_list[n]) writes the
+// ray's samples into its slot range [c_off[r], c_off[r] + bound[r]); count,
+// fine-pass bound and the longest row are recorded for the fine pass.
 __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const HitRec* __restrict__ hits, const int32_t* __restrict__ count,
@@ -391,28 +379,44 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     const int32_t* __restrict__ c_off, int n_max, float step_size, int compat, float fine_steps,
     SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
     int32_t* __restrict__ max_count) {
-  const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  __shared__ float cdf_lds[kSampleWaves][kWaveBins];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
-  if (n >= N) return;
-  const int32_t r = hit_list[n];
-  int64_t n0, nn;
-  grouped_neighbours(n, N, &n0, &nn);
-  const int32_t r0 = hit_list[n0];
-  HitBins bins{hits + int64_t(r) * n_max, count[r], tot[r], hits + int64_t(r0) * n_max, count[r0],
-               nn < 0 ? -1 : hits[int64_t(hit_list[nn]) * n_max].idx};
-  const float steps = tot[r] / step_size;
-  ConstNoise noise{0.5f};
-  const int b = bound[r];
-  StreamSink sink{out + c_off[r], r, b};
-  icdf_ray(bins, n_max, steps, -1.0f, noise, compat, b, sink);
-  sink.close(b);
-  c_cnt[r] = sink.n;
-  f_bound[r] = int(ceilf(fine_steps)) + sink.n + 2;
-  atomicMax(max_count, sink.n);
+  for (int64_t n = int64_t(blockIdx.x) * kSampleWaves + wave; n < N;
+       n += int64_t(gridDim.x) * kSampleWaves) {
+    const int32_t r = hit_list[n];
+    int64_t n0, nn;
+    grouped_neighbours(n, N, &n0, &nn);
+    const int32_t r0 = hit_list[n0];
+    const int cnt = count[r];
+    HitBins bins{hits + int64_t(r) * n_max, cnt, tot[r], hits + int64_t(r0) * n_max, count[r0],
+                 nn < 0 ? -1 : hits[int64_t(hit_list[nn]) * n_max].idx};
+    const float steps = tot[r] / step_size;
+    const int m = icdf_wave(bins, n_max, cnt, steps, compat, bound[r], r, out + c_off[r],
+                            cdf_lds[wave]);
+    if (lane == 0) {
+      c_cnt[r] = m;
+      f_bound[r] = int(ceilf(fine_steps)) + m + 2;
+      atomicMax(max_count, m);
+    }
+  }
 }
 
-// Fine pass for hit ray n: bins are the ray's coarse samples and the *c_max
-// columns of the longest coarse row (nerf.py:64-79).
+// Column sum of (p + 1e-5) over the P bin columns of a fine-pass ray
+// (nerf.py:64-79 normalises over all columns, pads included), in a fixed
+// order: lane j sums columns j, j + 64, ... in turn, then an xor butterfly
+// over the lanes (d = 1, 2, ..., 32).  Every lane ends with the same value.
+__device__ __forceinline__ float fine_den(const float* prob, int cnt, int P) {
+  const int lane = threadIdx.x & 63;
+  float part = 0.0f;
+  for (int k = lane; k < P; k += 64) part = part + ((k < cnt ? prob[k] : 0.0f) + 1e-5f);
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) part = part + __shfl_xor(part, d, 64);
+  return part;
+}
+
+// Fine pass: one wavefront per hit ray; bins are the ray's coarse samples and
+// the *c_max columns of the longest coarse row (nerf.py:64-79).
 __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_cnt,
@@ -420,33 +424,35 @@ __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
     const float* __restrict__ c_prob, const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
     int32_t* __restrict__ f_cnt) {
-  const int64_t n = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  __shared__ float cdf_lds[kSampleWaves][kWaveBins];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
-  if (n >= N) return;
-  const int32_t r = hit_list[n];
   const int P = *c_max_dev;  // bin columns = longest coarse row
-  const int b = f_bound[r];
-  StreamSink sink{out + f_off[r], r, b};
-  if (P > 0) {
-    int64_t n0, nn;
-    grouped_neighbours(n, N, &n0, &nn);
-    const int32_t r0 = hit_list[n0];
-    const int64_t off = c_off[r];
-    const int cnt = c_cnt[r];
-    // Column sum of (p + 1e-5) over all P columns, left to right.
-    float den = 0.0f;
-    for (int k = 0; k < P; ++k) den = den + ((k < cnt ? c_prob[off + k] : 0.0f) + 1e-5f);
-    int32_t next = -1;
-    if (nn >= 0) {
-      const int32_t rn = hit_list[nn];
-      next = c_cnt[rn] > 0 ? coarse[c_off[rn]].idx : -1;
+  for (int64_t n = int64_t(blockIdx.x) * kSampleWaves + wave; n < N;
+       n += int64_t(gridDim.x) * kSampleWaves) {
+    const int32_t r = hit_list[n];
+    const int b = f_bound[r];
+    SampleRec* dst = out + f_off[r];
+    int m = 0;
+    if (P > 0) {
+      int64_t n0, nn;
+      grouped_neighbours(n, N, &n0, &nn);
+      const int32_t r0 = hit_list[n0];
+      const int64_t off = c_off[r];
+      const int cnt = c_cnt[r];
+      const float den = fine_den(c_prob + off, cnt, P);
+      int32_t next = -1;
+      if (nn >= 0) {
+        const int32_t rn = hit_list[nn];
+        next = c_cnt[rn] > 0 ? coarse[c_off[rn]].idx : -1;
+      }
+      SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
+      m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, cdf_lds[wave]);
+    } else {
+      for (int k = lane; k < b; k += 64) dst[k] = SampleRec{-1, 0.0f, 0.0f, r};
     }
-    SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
-    ConstNoise noise{0.5f};
-    icdf_ray(bins, P, fine_steps, -1.0f, noise, compat, b, sink);
+    if (lane == 0) f_cnt[r] = m;
   }
-  sink.close(b);
-  f_cnt[r] = sink.n;
 }
 
 // ---------------------------------------------------------------------------
@@ -668,7 +674,8 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
                               void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
-  hipLaunchKernelGGL(coarse_sample_kernel, dim3(unsigned(ceil_div(n_hit_max, kBlock))),
+  hipLaunchKernelGGL(coarse_sample_kernel,
+                     dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
                      dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev,
                      static_cast<const HitRec*>(hits), count, tot, bound, c_off, n_max, step_size,
                      compat, fine_steps, static_cast<SampleRec*>(samples), c_cnt, f_bound,
@@ -683,7 +690,8 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev, i
                             int32_t* f_cnt, void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
-  hipLaunchKernelGGL(fine_sample_kernel, dim3(unsigned(ceil_div(n_hit_max, kBlock))),
+  hipLaunchKernelGGL(fine_sample_kernel,
+                     dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
                      dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, c_off, c_cnt, c_max,
                      static_cast<const SampleRec*>(coarse), c_prob, f_bound, f_off, fine_steps,
                      compat, static_cast<SampleRec*>(samples), f_cnt);

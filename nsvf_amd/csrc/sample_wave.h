@@ -1,0 +1,199 @@
+// Inverse-CDF sampling of one ray by one wavefront, with the same outputs,
+// bit for bit, as the sequential walk of sample_core.h (icdf_ray with noise
+// 0.5 into a StreamSink).  The sequential walk interleaves two sorted
+// sequences -- the steps' cdf values (i + 0.5) / steps and the bins'
+// cumulative probabilities C_k -- so every output has a closed-form place:
+//
+//   b_i   = the first bin k with cdf_i <= C_k (binary search; C in LDS);
+//   step i's sample goes to position i + b_i (i steps and b_i bin tails
+//   precede it), with z = fma(u, hi - lo, lo), u = (cdf - C_{b-1}) /
+//   (C_b - C_{b-1}), and z_low = z_{i-1} if b_{i-1} == b_i, else lo_b;
+//   the tails of the bins crossed between steps i-1 and i, k in
+//   [b_{i-1}, b_i), go to positions i + k, z_low = z_{i-1} for
+//   k == b_{i-1}, else lo_k;
+//   the first step with cdf beyond C of the last bin ends the walk after
+//   the remaining tails ("done"); otherwise the trailing tails follow the
+//   last step, bin after bin, while each bin passes the same tests as the
+//   sequential loop.
+//
+// So each lane takes one step (64 per pass), positions are contiguous
+// across lanes (coalesced 16-byte stores), and only the prefix C -- whose
+// FP32 rounding order is the reference's left-to-right sum -- runs on one
+// lane.  Rows with more bins than the LDS holds, a NaN in C, or a step count
+// beyond the slot bound take the sequential walk on lane 0 instead.
+#pragma once
+
+#include "render.h"
+#include "sample_core.h"
+
+namespace nsvf {
+
+// Writes the samples that carry a voxel id into the ray's slot range (dists
+// clamped at 0: encoder.py:547).
+struct StreamSink {
+  SampleRec* out;
+  int32_t ray_id;
+  int cap;  // the ray's slot count (never exceeded for valid bins; a guard)
+  int n = 0;
+  __device__ __forceinline__ void operator()(int, int32_t id, float d, float mid) {
+    if (id == -1 || n >= cap) return;
+    out[n] = SampleRec{id, mid, d > 0.0f ? d : 0.0f, ray_id};
+    ++n;
+  }
+  // Marks the slots past the written samples as unused.
+  __device__ __forceinline__ void close(int bound) {
+    for (int k = n; k < bound; ++k) out[k] = SampleRec{-1, 0.0f, 0.0f, ray_id};
+  }
+};
+
+constexpr int kWaveBins = 2048;  // bins whose cumulative probability fits a wave's LDS
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int first_lane(uint64_t mask) {
+  return mask ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : 64;
+}
+
+// Writes one sample record (distance clamped at 0: encoder.py:547); slots
+// past the ray's bound are never written (a guard: valid bins stay within).
+__device__ __forceinline__ void put_rec(SampleRec* out, int bound, int pos, int32_t id, float d,
+                                        float mid, int32_t ray) {
+  if (pos < bound) out[pos] = SampleRec{id, mid, d > 0.0f ? d : 0.0f, ray};
+}
+
+// Samples ray `ray` with bins `bins` (P columns, the first Pv valid) into
+// out[0, bound); slots past the samples are marked unused.  Returns the
+// sample count (same value in every lane).  C: this wave's LDS array of
+// kWaveBins floats.
+template <class Bins>
+__device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compat, int bound,
+                         int32_t ray, SampleRec* out, float* C) {
+  const int lane = threadIdx.x & 63;
+  int n = 0;
+  bool seq = Pv <= 0 || Pv > kWaveBins || !(steps <= float(bound));
+  if (!seq) {
+    for (int k = lane; k < Pv; k += 64) C[k] = bins.prob(k);
+    wave_sync();
+    if (lane == 0) {
+      float c = C[0];
+      for (int k = 1; k < Pv; ++k) {
+        c = c + C[k];
+        C[k] = c;
+      }
+    }
+    wave_sync();
+    seq = isnan(C[Pv - 1]);
+  }
+  if (seq) {
+    if (lane == 0) {
+      StreamSink sink{out, ray, bound};
+      ConstNoise noise{0.5f};
+      icdf_ray(bins, P, steps, -1.0f, noise, compat, bound, sink);
+      n = sink.n;
+    }
+    n = __shfl(n, 0, 64);
+  } else {
+    const float ss = float(1.0 / double(steps));
+    const int T = int(ceilf(steps));
+    int bprev = 0;
+    float zprev = bins.lo(0);
+    bool done = false;
+    int cb = 0;
+    float zl = 0.0f, cmax = 0.0f;
+    for (int base = 0; base < T; base += 64) {
+      const int i = base + lane;
+      const bool act = i < T;
+      int b = Pv;
+      float z = 0.0f;
+      if (act) {
+        const float cdf = (float(i) + 0.5f) * ss;
+        int lo = 0, hi = Pv;
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (C[mid] < cdf)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        b = lo;
+        if (b < Pv) {
+          const float cmin = b > 0 ? C[b - 1] : 0.0f;
+          const float u = (cdf - cmin) / (C[b] - cmin);
+          const float lb = bins.lo(b);
+          z = __fmaf_rn(u, bins.hi(b) - lb, lb);
+        }
+      }
+      int bp = __shfl_up(b, 1, 64);
+      float zp = __shfl_up(z, 1, 64);
+      if (lane == 0) {
+        bp = bprev;
+        zp = zprev;
+      }
+      const uint64_t dmask = __ballot(act && b >= Pv);
+      const int dl = first_lane(dmask);
+      if (act && lane <= dl) {
+        const int end = b < Pv ? b : Pv;
+        for (int k = bp; k < end; ++k) {
+          const float hk = bins.hi(k);
+          const float zk = (k == bp) ? zp : bins.lo(k);
+          put_rec(out, bound, i + k, bins.id(k), hk - zk, (hk + zk) * 0.5f, ray);
+        }
+        if (b < Pv) {
+          const float zk = (b == bp) ? zp : bins.lo(b);
+          put_rec(out, bound, i + b, bins.id(b), z - zk, (z + zk) * 0.5f, ray);
+        }
+      }
+      if (dmask) {
+        const int bpd = __shfl(bp, dl, 64);
+        const float zpd = __shfl(zp, dl, 64);
+        done = true;
+        n = base + dl + Pv;
+        cb = Pv;
+        zl = (bpd == Pv - 1) ? zpd : bins.lo(Pv - 1);
+        cmax = bins.hi(Pv - 1);
+        break;
+      }
+      const int last = (T - 1 - base) < 63 ? (T - 1 - base) : 63;
+      bprev = __shfl(b, last, 64);
+      zprev = __shfl(z, last, 64);
+    }
+    if (!done) {
+      n = T > 0 ? T + bprev : 0;
+      cb = T > 0 ? bprev : 0;
+      zl = zprev;
+      cmax = bins.hi(cb);
+    }
+    // trailing tails (sample_core.h: the loop after the steps)
+    if (!(done && !compat) && zl < cmax) {
+      const int32_t id0 = cb < P ? bins.id(cb) : bins.next_first();
+      if (id0 != -1) {
+        if (lane == 0) put_rec(out, bound, n, id0, cmax - zl, (cmax + zl) * 0.5f, ray);
+        ++n;
+      }
+      const int k0 = cb + 1;
+      for (int base = k0; base < P; base += 64) {
+        const int k = base + lane;
+        bool ok = false;
+        float lk = 0.0f, hk = 0.0f;
+        if (k < P) {
+          const int32_t probe = compat ? bins.ray0_id(k) : bins.id(k);
+          lk = bins.lo(k);
+          hk = bins.hi(k);
+          ok = probe != -1 && lk < hk;
+        }
+        const int stop = first_lane(__ballot(!ok));
+        if (lane < stop) put_rec(out, bound, n + lane, bins.id(k), hk - lk, (hk + lk) * 0.5f, ray);
+        n += stop < 64 ? stop : 64;
+        if (stop < 64) break;
+      }
+    }
+  }
+  for (int k = n + lane; k < bound; k += 64) out[k] = SampleRec{-1, 0.0f, 0.0f, ray};
+  return n;
+}
+
+}  // namespace nsvf

@@ -643,16 +643,29 @@ def _field_eval(si, sd, sx, ray_start, ray_dir, centers_shifted, feats, values, 
     return S, C
 
 
+def fine_normaliser(safe):
+    """Sum over the columns of `safe` [R,P] (float32) in the fixed order the
+    fine sampler uses (the reference's torch sum leaves the order open): 64
+    partial sums, partial j adding columns j, j + 64, ... in turn, then an
+    xor butterfly over the partials, d = 1, 2, ..., 32."""
+    safe = np.asarray(safe, np.float32)
+    part = np.zeros((safe.shape[0], 64), np.float32)
+    for k in range(safe.shape[1]):
+        part[:, k % 64] = (part[:, k % 64] + safe[:, k]).astype(np.float32)
+    lanes = np.arange(64)
+    for d in (1, 2, 4, 8, 16, 32):
+        part = (part + part[:, lanes ^ d]).astype(np.float32)
+    return part[:, 0]
+
+
 def prepare_fine_bins(si, sd, sx, probs, fine_steps):
     """nerf.py:64-79 (+ nsvf.py:83-87 with its ratio unset): coarse samples as
     bins [z - d/2, z + d/2], probs (p + 1e-5) / sum over all columns (fp32,
-    left to right), steps = fine_steps."""
+    in fine_normaliser's order), steps = fine_steps."""
     lo = (sd - (sx * np.float32(0.5)).astype(np.float32)).astype(np.float32)
     hi = (sd + (sx * np.float32(0.5)).astype(np.float32)).astype(np.float32)
     safe = (np.asarray(probs, np.float32) + np.float32(1e-5)).astype(np.float32)
-    den = np.zeros(safe.shape[0], np.float32)
-    for k in range(safe.shape[1]):
-        den = (den + safe[:, k]).astype(np.float32)
+    den = fine_normaliser(safe)
     p = (safe / den[:, None]).astype(np.float32)
     steps = np.full(safe.shape[0], np.float32(fine_steps), np.float32)
     return si.astype(np.int32), lo, hi, p, steps

@@ -599,3 +599,20 @@ def test_prune_max_sigma_constant_field():
     # max x over the lattice = centre_x + vs/2 in the key frame
     ctr = coords[[3, 50], 0].astype(np.float64) * (sc.voxel_size / 2) + res[0, 0]
     assert np.abs(m - (ctr + sc.voxel_size / 2 + 5.0)).max() < 1e-5
+
+
+def test_fine_normaliser_order():
+    """The fine-pass normaliser: exact on values whose sums are exact, equal
+    to a hand-evaluated strided-then-butterfly order on a small case, and
+    within rounding of the float64 sum."""
+    x = np.full((1, 200), 0.25, np.float32)
+    assert O.fine_normaliser(x)[0] == np.float32(50.0)
+    rng = np.random.default_rng(0)
+    y = rng.uniform(0, 1, (3, 70)).astype(np.float32)
+    part = [np.float32(0)] * 64
+    for k in range(70):
+        part[k % 64] = np.float32(part[k % 64] + y[1, k])
+    for d in (1, 2, 4, 8, 16, 32):
+        part = [np.float32(part[j] + part[j ^ d]) for j in range(64)]
+    assert O.fine_normaliser(y)[1] == part[0]
+    assert np.abs(O.fine_normaliser(y) - y.astype(np.float64).sum(1)).max() < 1e-5
