@@ -417,43 +417,71 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
   }
 }
 
-// Gather kernel of the split BF16 path: one thread per (tile, n-tile, k-tile,
-// lane) writes the 8 bf16 operand values that lane of the MLP kernel loads, so
-// both the writes here and the MLP kernel's loads are contiguous 16 bytes per
-// lane.  Many waves per SIMD hide the gather's dependent loads.
+// Gather kernel of the split BF16 path: one thread per sample computes the
+// corner weights once, accumulates the 8 corner rows (contiguous float4
+// loads; neighbouring samples of a ray share rows, so a wave's loads mostly
+// coincide) with fused multiply-adds in corner order, and writes the sample's
+// operand values in the MLP kernel's tile layout: for k-tile kt and lane
+// group g, features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3} as one
+// 16-byte bf16x8 at ((t * NT + j) * KT0 + kt) * 64 + 16g + col, for sample
+// s = t * SAMPLES + 16j + col -- 16 consecutive samples give 256 contiguous
+// bytes per store.
 template <int D, int H, int L>
 __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRays rays,
                                                          const SampleRec* __restrict__ samples,
                                                          const int32_t* m_dev, int64_t m_max,
                                                          bf16x8* __restrict__ emb) {
   using Dm = Dims<D, H, L>;
+  static_assert(D % 4 == 0, "embedding width must be a multiple of 4");
   const int64_t M = min(int64_t(*m_dev), m_max);
-  const int64_t total = ((M + Dm::SAMPLES - 1) / Dm::SAMPLES) * Dm::NT * Dm::KT0 * 64;
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= total) return;
-  const int lane = int(i & 63);
-  const int64_t q = i >> 6;
-  const int kt = int(q % Dm::KT0);
-  const int64_t tj = q / Dm::KT0;           // t * NT + j
-  const int j = int(tj % Dm::NT);
-  const int64_t t = tj / Dm::NT;
-  const int64_t s = t * Dm::SAMPLES + 16 * j + (lane & 15);
-  float f[8];
+  const int64_t total = ((M + Dm::SAMPLES - 1) / Dm::SAMPLES) * Dm::SAMPLES;
+  const int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (s >= total) return;
+  float acc[D];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) f[k] = 0.0f;
-  if (s < M) {
-    const StreamSource src{sc, rays, samples};
-    float w[8];
-    int32_t key[8];
-    if (src.weights<true>(s, w, key)) gather8_fma<D>(sc, w, key, kt, lane >> 4, f);
+  for (int f = 0; f < D; ++f) acc[f] = 0.0f;
+  float w[8];
+  int32_t key[8];
+  const StreamSource src{sc, rays, samples};
+  if (s < M && src.weights<true>(s, w, key)) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const float4* row = reinterpret_cast<const float4*>(sc.values + int64_t(key[c]) * D);
+#pragma unroll
+      for (int q = 0; q < D / 4; ++q) {
+        const float4 e = row[q];
+        acc[4 * q + 0] = __fmaf_rn(w[c], e.x, acc[4 * q + 0]);
+        acc[4 * q + 1] = __fmaf_rn(w[c], e.y, acc[4 * q + 1]);
+        acc[4 * q + 2] = __fmaf_rn(w[c], e.z, acc[4 * q + 2]);
+        acc[4 * q + 3] = __fmaf_rn(w[c], e.w, acc[4 * q + 3]);
+      }
+    }
   }
-  const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
-  const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
-  const bf16x2 p2 = __builtin_convertvector(f32x2{f[4], f[5]}, bf16x2);
-  const bf16x2 p3 = __builtin_convertvector(f32x2{f[6], f[7]}, bf16x2);
-  const u32x4 u = {__builtin_bit_cast(unsigned int, p0), __builtin_bit_cast(unsigned int, p1),
-                   __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
-  emb[i] = __builtin_bit_cast(bf16x8, u);
+  const int64_t t = s / Dm::SAMPLES;
+  const int within = int(s - t * Dm::SAMPLES);
+  const int j = within >> 4, col = within & 15;
+  bf16x8* base = emb + ((t * Dm::NT + j) * Dm::KT0) * 64 + col;
+#pragma unroll
+  for (int kt = 0; kt < Dm::KT0; ++kt) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      float f[8];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int feat = 32 * kt + 16 * h + 4 * g + i;
+          f[4 * h + i] = feat < D ? acc[feat < D ? feat : 0] : 0.0f;
+        }
+      const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
+      const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
+      const bf16x2 p2 = __builtin_convertvector(f32x2{f[4], f[5]}, bf16x2);
+      const bf16x2 p3 = __builtin_convertvector(f32x2{f[6], f[7]}, bf16x2);
+      const u32x4 u = {__builtin_bit_cast(unsigned int, p0), __builtin_bit_cast(unsigned int, p1),
+                       __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
+      base[kt * 64 + 16 * g] = __builtin_bit_cast(bf16x8, u);
+    }
+  }
 }
 
 // MLP kernel of the split BF16 path: operand B of layer 0 read from the
@@ -660,7 +688,7 @@ int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
                       const int32_t* m_dev, int64_t m_max, void* workspace, hipStream_t stream) {
   using Dm = Dims<D, H, L>;
   const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
-  const int64_t threads = tiles * Dm::NT * Dm::KT0 * 64;
+  const int64_t threads = tiles * Dm::SAMPLES;
   hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(ceil_div(threads, 256))),
                      dim3(256), 0, stream, sc, rays, samples, m_dev, m_max,
                      static_cast<bf16x8*>(workspace));
