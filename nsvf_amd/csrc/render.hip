@@ -380,8 +380,12 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
     int32_t* __restrict__ max_count) {
   __shared__ float cdf_lds[kSampleWaves][kWaveBins];
+  __shared__ int block_max;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
+  if (threadIdx.x == 0) block_max = 0;
+  __syncthreads();
+  int wave_max = 0;  // longest row of this wave's rays (one atomic per block at the end)
   for (int64_t n = int64_t(blockIdx.x) * kSampleWaves + wave; n < N;
        n += int64_t(gridDim.x) * kSampleWaves) {
     const int32_t r = hit_list[n];
@@ -397,9 +401,12 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     if (lane == 0) {
       c_cnt[r] = m;
       f_bound[r] = int(ceilf(fine_steps)) + m + 2;
-      atomicMax(max_count, m);
     }
+    wave_max = m > wave_max ? m : wave_max;
   }
+  if (lane == 0) atomicMax(&block_max, wave_max);
+  __syncthreads();
+  if (threadIdx.x == 0 && block_max > 0) atomicMax(max_count, block_max);
 }
 
 // Column sum of (p + 1e-5) over the P bin columns of a fine-pass ray
