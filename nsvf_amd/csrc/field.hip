@@ -278,43 +278,60 @@ __device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][
                                          int col, int64_t base, int64_t M, Sink& sink) {
   using Dm = Dims<D, H, L>;
   constexpr int NT = Dm::NT;
+  // Weight fragments (and biases) are read one fragment ahead of the MFMAs
+  // that use them, so a read's latency hides behind the previous fragment's
+  // NT matrix instructions instead of stalling before them.
   // ---- layer 0: bias enters as the first MFMA's C operand
   bf16x8 bh[Dm::KTH][NT];
-#pragma unroll
-  for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
-    f32x4 acc[2][NT];
-#pragma unroll
-    for (int half = 0; half < 2; ++half) {
-      const int mt = 2 * kt2 + half;
-      const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + 16 * mt + 4 * g);
-#pragma unroll
-      for (int kt = 0; kt < Dm::KT0; ++kt) {
-        const bf16x8 a = lw[(mt * Dm::KT0 + kt) * 64 + lane];
-#pragma unroll
-        for (int j = 0; j < NT; ++j)
-          acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-              a, b0[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < NT; ++j) bh[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
-  }
-  // ---- hidden layers 1..L-1
-#pragma unroll 1
-  for (int l = 1; l < L; ++l) {
-    const bf16x8* A = lw + (Dm::F0 + (l - 1) * Dm::FH) * 64;
-    const float* B = lb + H * l;
-    bf16x8 nb[Dm::KTH][NT];
+  {
+    constexpr int NF = Dm::MTH * Dm::KT0;
+    bf16x8 a_next = lw[lane];
+    f32x4 bias_next = *reinterpret_cast<const f32x4*>(lb + 4 * g);
 #pragma unroll
     for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
       f32x4 acc[2][NT];
 #pragma unroll
       for (int half = 0; half < 2; ++half) {
         const int mt = 2 * kt2 + half;
-        const f32x4 bias4 = *reinterpret_cast<const f32x4*>(B + 16 * mt + 4 * g);
+        const f32x4 bias4 = bias_next;
+        if (mt + 1 < Dm::MTH) bias_next = *reinterpret_cast<const f32x4*>(lb + 16 * (mt + 1) + 4 * g);
+#pragma unroll
+        for (int kt = 0; kt < Dm::KT0; ++kt) {
+          const bf16x8 a = a_next;
+          const int nf = mt * Dm::KT0 + kt + 1;
+          if (nf < NF) a_next = lw[nf * 64 + lane];
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                a, b0[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bh[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
+    }
+  }
+  // ---- hidden layers 1..L-1
+#pragma unroll 1
+  for (int l = 1; l < L; ++l) {
+    constexpr int NF = Dm::MTH * Dm::KTH;
+    const bf16x8* A = lw + (Dm::F0 + (l - 1) * Dm::FH) * 64;
+    const float* B = lb + H * l;
+    bf16x8 nb[Dm::KTH][NT];
+    bf16x8 a_next = A[lane];
+    f32x4 bias_next = *reinterpret_cast<const f32x4*>(B + 4 * g);
+#pragma unroll
+    for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
+      f32x4 acc[2][NT];
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const int mt = 2 * kt2 + half;
+        const f32x4 bias4 = bias_next;
+        if (mt + 1 < Dm::MTH) bias_next = *reinterpret_cast<const f32x4*>(B + 16 * (mt + 1) + 4 * g);
 #pragma unroll
         for (int kt = 0; kt < Dm::KTH; ++kt) {
-          const bf16x8 a = A[(mt * Dm::KTH + kt) * 64 + lane];
+          const bf16x8 a = a_next;
+          const int nf = mt * Dm::KTH + kt + 1;
+          if (nf < NF) a_next = A[nf * 64 + lane];
 #pragma unroll
           for (int j = 0; j < NT; ++j)
             acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
@@ -334,9 +351,11 @@ __device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][
     const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
     const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + H * L + 4 * g);
     f32x4 acc[NT];
+    bf16x8 a_next = A[lane];
 #pragma unroll
     for (int kt = 0; kt < Dm::KTH; ++kt) {
-      const bf16x8 a = A[kt * 64 + lane];
+      const bf16x8 a = a_next;
+      if (kt + 1 < Dm::KTH) a_next = A[(kt + 1) * 64 + lane];
 #pragma unroll
       for (int j = 0; j < NT; ++j)
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], kt == 0 ? bias4 : acc[j],
