@@ -30,6 +30,14 @@ inline int check_launch(const char* what) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Makes this wavefront's earlier LDS writes visible to all its lanes (a
+// barrier for one wave; waves of a block do not wait for each other).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Slab test of one ray against one axis-aligned box (fairnr/clib/src/
 // intersect_gpu.cu:73-122): the same comparisons and ternaries, so an axis
 // whose bounds come out NaN (direction 0 and origin on the slab plane) fails

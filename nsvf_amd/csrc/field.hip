@@ -436,15 +436,17 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
   }
 }
 
-// Gather kernel of the split BF16 path: one thread per sample computes the
-// corner weights once, accumulates the 8 corner rows (contiguous float4
-// loads; neighbouring samples of a ray share rows, so a wave's loads mostly
-// coincide) with fused multiply-adds in corner order, and writes the sample's
-// operand values in the MLP kernel's tile layout: for k-tile kt and lane
-// group g, features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3} as one
-// 16-byte bf16x8 at ((t * NT + j) * KT0 + kt) * 64 + 16g + col, for sample
-// s = t * SAMPLES + 16j + col -- 16 consecutive samples give 256 contiguous
-// bytes per store.
+// Gather kernel of the split BF16 path.  A wavefront takes 64 consecutive
+// slots; consecutive samples of a ray mostly share their voxel, so the wave
+// finds its runs of equal voxel ids (about 8 per 64 samples), stages each
+// run's 8 corner rows in LDS with one coalesced load per row block, and each
+// lane then accumulates its sample from LDS -- instead of every lane pulling
+// 1 KB of rows through the texture path.  Per sample and feature the sum is
+// the same chain of fused multiply-adds in corner order as the fused kernel.
+// Output: the MLP kernel's tile layout -- for k-tile kt and lane group g,
+// features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3} as one 16-byte bf16x8
+// at ((t * NT + j) * KT0 + kt) * 64 + 16g + col, for sample
+// s = t * SAMPLES + 16j + col.
 template <int D, int H, int L>
 __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRays rays,
                                                          const SampleRec* __restrict__ samples,
@@ -452,30 +454,78 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
                                                          bf16x8* __restrict__ emb) {
   using Dm = Dims<D, H, L>;
   static_assert(D % 4 == 0, "embedding width must be a multiple of 4");
+  constexpr int RUNS = 8;        // voxels staged per round
+  constexpr int ROW4 = 2 * D;    // float4 per voxel: 8 corners x D / 4
+  __shared__ float4 rows_lds[4][RUNS * ROW4];
+  __shared__ int32_t run_vox[4][64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t M = min(int64_t(*m_dev), m_max);
   const int64_t total = ((M + Dm::SAMPLES - 1) / Dm::SAMPLES) * Dm::SAMPLES;
-  const int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (s >= total) return;
+  const int64_t s0 = (int64_t(blockIdx.x) * 4 + wave) * 64;
+  if (s0 >= total) return;  // the whole wave
+  const int64_t s = s0 + lane;
+  // the sample's voxel and corner weights (x = o + d * z; the fused kernel's
+  // corner_weights_fast arithmetic)
+  int32_t v = -1;
+  float w[8];
+  if (s < M) {
+    const SampleRec rec = samples[s];
+    v = rec.idx;
+    if (v >= 0) {
+      float p[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float x = rays.start[int64_t(rec.ray) * 3 + a] +
+                        rays.dir[int64_t(rec.ray) * 3 + a] * rec.depth;
+        const float d = x - sc.centers[int64_t(v) * 3 + a];
+        p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
+      }
+      const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+      const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
+    }
+  }
+  // runs of equal voxel ids among the wave's lanes
+  const int32_t vprev = __shfl_up(v, 1, 64);
+  const bool head = lane == 0 || v != vprev;
+  const uint64_t hmask = __ballot(head);
+  const int r = __popcll(hmask & ((2ull << lane) - 1)) - 1;
+  const int U = __popcll(hmask);
+  if (head) run_vox[wave][r] = v;
+  wave_sync();
   float acc[D];
 #pragma unroll
   for (int f = 0; f < D; ++f) acc[f] = 0.0f;
-  float w[8];
-  int32_t key[8];
-  const StreamSource src{sc, rays, samples};
-  if (s < M && src.weights<true>(s, w, key)) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const float4* row = reinterpret_cast<const float4*>(sc.values + int64_t(key[c]) * D);
-#pragma unroll
-      for (int q = 0; q < D / 4; ++q) {
-        const float4 e = row[q];
-        acc[4 * q + 0] = __fmaf_rn(w[c], e.x, acc[4 * q + 0]);
-        acc[4 * q + 1] = __fmaf_rn(w[c], e.y, acc[4 * q + 1]);
-        acc[4 * q + 2] = __fmaf_rn(w[c], e.z, acc[4 * q + 2]);
-        acc[4 * q + 3] = __fmaf_rn(w[c], e.w, acc[4 * q + 3]);
+  for (int u0 = 0; u0 < U; u0 += RUNS) {
+    const int nr = min(RUNS, U - u0);
+    for (int i = lane; i < nr * ROW4; i += 64) {
+      const int j = i / ROW4, e = i - j * ROW4;
+      const int32_t vu = run_vox[wave][u0 + j];
+      if (vu >= 0) {
+        const int c = e / (D / 4), qd = e - c * (D / 4);
+        const int32_t k = sc.feats[int64_t(vu) * 8 + c];
+        rows_lds[wave][i] = reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[qd];
       }
     }
+    wave_sync();
+    if (v >= 0 && r >= u0 && r < u0 + nr) {
+      const float4* rows = &rows_lds[wave][(r - u0) * ROW4];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+#pragma unroll
+        for (int qd = 0; qd < D / 4; ++qd) {
+          const float4 e = rows[c * (D / 4) + qd];
+          acc[4 * qd + 0] = __fmaf_rn(w[c], e.x, acc[4 * qd + 0]);
+          acc[4 * qd + 1] = __fmaf_rn(w[c], e.y, acc[4 * qd + 1]);
+          acc[4 * qd + 2] = __fmaf_rn(w[c], e.z, acc[4 * qd + 2]);
+          acc[4 * qd + 3] = __fmaf_rn(w[c], e.w, acc[4 * qd + 3]);
+        }
+      }
+    }
+    wave_sync();
   }
+  if (s >= total) return;
   const int64_t t = s / Dm::SAMPLES;
   const int within = int(s - t * Dm::SAMPLES);
   const int j = within >> 4, col = within & 15;
@@ -707,7 +757,7 @@ int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
                       const int32_t* m_dev, int64_t m_max, void* workspace, hipStream_t stream) {
   using Dm = Dims<D, H, L>;
   const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
-  const int64_t threads = tiles * Dm::SAMPLES;
+  const int64_t threads = ceil_div(tiles * Dm::SAMPLES, 64) * 64;
   hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(ceil_div(threads, 256))),
                      dim3(256), 0, stream, sc, rays, samples, m_dev, m_max,
                      static_cast<bf16x8*>(workspace));

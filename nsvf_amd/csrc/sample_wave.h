@@ -48,12 +48,6 @@ struct StreamSink {
 
 constexpr int kWaveBins = 2048;  // bins whose cumulative probability fits a wave's LDS
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ int first_lane(uint64_t mask) {
   return mask ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : 64;
 }
