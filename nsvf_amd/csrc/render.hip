@@ -118,6 +118,25 @@ __device__ __forceinline__ bool ray_box(const float o[3], const float inv[3], co
   return true;
 }
 
+// A ray's hit row as the traversal fills it, with what the kernel needs
+// afterwards kept on the fly: whether entry depths strictly increase, and
+// the left-to-right sum of the segment lengths (in append order, which is
+// the row order whenever the row is kept as is).
+struct HitRow {
+  HitRec* row;
+  int cnt;
+  float prev_lo;
+  float tot;
+  bool increasing;
+  __device__ __forceinline__ void push(float lo, float hi, int32_t id) {
+    row[cnt] = HitRec{lo, hi, id, 0};
+    increasing = increasing && (cnt == 0 || prev_lo < lo);
+    prev_lo = lo;
+    tot = tot + (hi - lo);
+    ++cnt;
+  }
+};
+
 // Front-to-back traversal over the packed tree: boxes[2k] = (min xyz, scale
 // bits), boxes[2k+1] = (max xyz, 0), kids[k] = the 8 child ids.  Leaf boxes
 // are the voxel boxes; an internal node's box is the exact bounding box of
@@ -134,7 +153,7 @@ __device__ __forceinline__ unsigned expand_node(int node, int psc, const float o
                                                 const float inv[3], int dmask,
                                                 const float4* __restrict__ boxes,
                                                 const int4* __restrict__ kids, int T, int n_max,
-                                                HitRec* __restrict__ row, int* cnt) {
+                                                HitRow& out) {
   const int4 k0 = kids[int64_t(node) * 2], k1 = kids[int64_t(node) * 2 + 1];
   int c[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
   // c[i] <- c[i ^ dmask] with static indices only
@@ -188,30 +207,29 @@ __device__ __forceinline__ unsigned expand_node(int node, int psc, const float o
   if ((hit & ~leaf) == 0) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      if (((hit >> i) & 1u) && *cnt < n_max) row[(*cnt)++] = HitRec{los[i], his[i], c[i], 0};
+      if (((hit >> i) & 1u) && out.cnt < n_max) out.push(los[i], his[i], c[i]);
     return 0u;
   }
   return hit;
 }
 
-__device__ int traverse_packed(const float o[3], const float inv[3], int dmask,
-                               const float4* __restrict__ boxes, const int4* __restrict__ kids,
-                               int T, int n_max, HitRec* __restrict__ row, int32_t* st_node,
-                               uint8_t* st_mask) {
-  int cnt = 0;
+__device__ void traverse_packed(const float o[3], const float inv[3], int dmask,
+                                const float4* __restrict__ boxes, const int4* __restrict__ kids,
+                                int T, int n_max, HitRow& out, int32_t* st_node,
+                                uint8_t* st_mask) {
   const int root = T - 1;
   const float4 rmn = boxes[int64_t(root) * 2], rmx = boxes[int64_t(root) * 2 + 1];
   const int rsc = __float_as_int(rmn.w);
   float lo, hi;
-  if (n_max <= 0 || !ray_box(o, inv, rmn, rmx, &lo, &hi)) return 0;
+  if (n_max <= 0 || !ray_box(o, inv, rmn, rmx, &lo, &hi)) return;
   if (rsc == 1) {
-    row[0] = HitRec{lo, hi, root, 0};
-    return 1;
+    out.push(lo, hi, root);
+    return;
   }
   int level = 0;
   st_node[0] = root;
-  st_mask[0] = uint8_t(expand_node(root, rsc, o, inv, dmask, boxes, kids, T, n_max, row, &cnt));
-  while (level >= 0 && cnt < n_max) {
+  st_mask[0] = uint8_t(expand_node(root, rsc, o, inv, dmask, boxes, kids, T, n_max, out));
+  while (level >= 0 && out.cnt < n_max) {
     const unsigned m = st_mask[level * kHitsBlock];
     if (m == 0) {
       --level;
@@ -225,17 +243,16 @@ __device__ int traverse_packed(const float o[3], const float inv[3], int dmask,
     const int sc = __float_as_int(mn.w);
     if (sc == 1) {
       const float4 mx = boxes[int64_t(id) * 2 + 1];
-      if (ray_box(o, inv, mn, mx, &lo, &hi)) row[cnt++] = HitRec{lo, hi, id, 0};
+      if (ray_box(o, inv, mn, mx, &lo, &hi)) out.push(lo, hi, id);
       continue;
     }
     if (level + 1 < kMaxLevels) {
       ++level;
       st_node[level * kHitsBlock] = id;
       st_mask[level * kHitsBlock] =
-          uint8_t(expand_node(id, sc, o, inv, dmask, boxes, kids, T, n_max, row, &cnt));
+          uint8_t(expand_node(id, sc, o, inv, dmask, boxes, kids, T, n_max, out));
     }
   }
-  return cnt;
 }
 
 // Per ray: hits sorted by entry depth, count, and the left-to-right sum of
@@ -267,11 +284,11 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
   }
   HitRec* row = hits + r * n_max;
   int32_t* st = stack + threadIdx.x;
-  int cnt = traverse_packed(o, inv, mask, boxes, kids, T, n_max, row, st,
-                            stack_mask + threadIdx.x);
-  bool ok = cnt < n_max;
-  for (int k = 1; ok && k < cnt; ++k) ok = row[k - 1].lo < row[k].lo;
-  if (!ok) {
+  HitRow hr{row, 0, 0.0f, 0.0f, true};
+  traverse_packed(o, inv, mask, boxes, kids, T, n_max, hr, st, stack_mask + threadIdx.x);
+  int cnt = hr.cnt;
+  float tot = hr.tot;
+  if (!(cnt < n_max && hr.increasing)) {
     cnt = traverse<false>(o, inv, mask, cen, ch, T, half_voxel, n_max, row, st);
     for (int k = 1; k < cnt; ++k) {  // stable insertion sort by entry depth
       const HitRec h = row[k];
@@ -282,9 +299,9 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
       }
       row[j + 1] = h;
     }
+    tot = 0.0f;
+    for (int k = 0; k < cnt; ++k) tot = tot + (row[k].hi - row[k].lo);
   }
-  float tot = 0.0f;
-  for (int k = 0; k < cnt; ++k) tot = tot + (row[k].hi - row[k].lo);
   count[r] = cnt;
   tot_out[r] = tot;
   hit_flag[r] = cnt > 0 ? 1 : 0;
