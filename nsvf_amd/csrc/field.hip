@@ -262,6 +262,9 @@ struct Dims {
   static constexpr int FHEAD = KTH;
   static constexpr int FRAGS = F0 + (L - 1) * FH + FHEAD;
   static constexpr int BIASES = H * L + 16;
+  // blocks per CU the MLP kernel is built for: two waves per SIMD when the
+  // layer width lets the activations fit half the register file
+  static constexpr int MIN_BLOCKS = H <= 128 ? 2 : 1;
 };
 
 template <int D, int H, int L>
@@ -272,10 +275,15 @@ constexpr bool weights_in_lds() {
 // The MLP on one wave's tile of SAMPLES samples: operand B of layer 0 in
 // b0, weights lw / biases lb (LDS or global), head results of samples < M
 // to the sink.
+// One wave's layer-0 operand tile: [k-tile][n-tile] fragments.
+template <int D, int H, int L>
+using OperandTile = bf16x8[Dims<D, H, L>::KT0][Dims<D, H, L>::NT];
+
 template <int D, int H, int L, class Sink>
-__device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][Dims<D, H, L>::NT],
+__device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
                                          const bf16x8* lw, const float* lb, int lane, int g,
-                                         int col, int64_t base, int64_t M, Sink& sink) {
+                                         int col, int64_t base, int64_t M, Sink& sink,
+                                         const bf16x8* __restrict__ next_b0 = nullptr) {
   using Dm = Dims<D, H, L>;
   constexpr int NT = Dm::NT;
   // Weight fragments (and biases) are read one fragment ahead of the MFMAs
@@ -345,6 +353,14 @@ __device__ __forceinline__ void mlp_tile(const bf16x8 (&b0)[Dims<D, H, L>::KT0][
     for (int kt = 0; kt < Dm::KTH; ++kt)
 #pragma unroll
       for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
+  }
+  // b0 is dead since layer 0 and the hidden layers' double buffer now too:
+  // the next tile's operands load into b0 while the head and the stores run
+  if (next_b0) {
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = next_b0[(j * Dm::KT0 + kt) * 64 + lane];
   }
   // ---- head (4 outputs in rows 0..3 of one m-tile)
   {
@@ -556,7 +572,7 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
 // MLP kernel of the split BF16 path: operand B of layer 0 read from the
 // gather kernel's output.
 template <int D, int H, int L>
-__global__ __launch_bounds__(kFieldBlock) void mlp_bf16_kernel(
+__global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_bf16_kernel(
     const bf16x8* __restrict__ emb, const int32_t* m_dev, int64_t m_max,
     const bf16x8* __restrict__ wfrag, const float* __restrict__ bias, float4* __restrict__ out) {
   using Dm = Dims<D, H, L>;
@@ -576,15 +592,19 @@ __global__ __launch_bounds__(kFieldBlock) void mlp_bf16_kernel(
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
   const int64_t M = min(int64_t(*m_dev), m_max);
   const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
-  for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
-       t += int64_t(gridDim.x) * 4) {
-    bf16x8 b0[Dm::KT0][NT];
+  const int64_t stride = int64_t(gridDim.x) * 4;
+  int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (t >= tiles) return;
+  bf16x8 b0[Dm::KT0][NT];
 #pragma unroll
-    for (int j = 0; j < NT; ++j)
+  for (int j = 0; j < NT; ++j)
 #pragma unroll
-      for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = emb[((t * NT + j) * Dm::KT0 + kt) * 64 + lane];
+    for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = emb[((t * NT + j) * Dm::KT0 + kt) * 64 + lane];
+  for (; t < tiles; t += stride) {
+    const int64_t tn = t + stride;
     StoreSink sink{out};
-    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, sink);
+    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, sink,
+                      tn < tiles ? emb + tn * NT * Dm::KT0 * 64 : nullptr);
   }
 }
 
