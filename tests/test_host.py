@@ -1,0 +1,117 @@
+"""Host-side packers on the CPU: the MLP fragments against the matrix-core
+operand layout they are read with, and the packed octree boxes."""
+import numpy as np
+
+from _cases import shell, shell_octree
+from nsvf_amd.field import bf16_bits, bf16_round, pack_layer_bf16, pack_layer_f32
+from nsvf_amd.render import pack_tree
+
+
+def _perm(g, i):
+    return 4 * g + i if i < 4 else 16 + 4 * g + (i - 4)
+
+
+def mfma_16x16x32(a_frag, b_frag):
+    """v_mfma_f32_16x16x32_bf16 on one tile: lane 16g + m of A holds row m,
+    K-slots 8g..8g+7; lane 16g + n of B holds column n, the same K-slots;
+    D[m][n] = sum over the 32 K-slots."""
+    D = np.zeros((16, 16))
+    for g in range(4):
+        for i in range(8):
+            D += np.outer(a_frag[16 * g:16 * g + 16, i], b_frag[16 * g:16 * g + 16, i])
+    return D
+
+
+def test_bf16_fragments_compute_w_times_x():
+    """A layer packed by pack_layer_bf16 and an input laid out as the gather
+    kernel writes it (lane 16g + n, element i = feature 32kt + perm(g, i) of
+    sample n) give W @ X through the MFMA's operand semantics."""
+    rng = np.random.default_rng(0)
+    W = bf16_round(rng.standard_normal((40, 70)).astype(np.float32))   # pads to 48 x 96
+    X = bf16_round(rng.standard_normal((70, 16)).astype(np.float32))
+    frag = pack_layer_bf16(W, 96)                                        # [3,3,64,8]
+    assert frag.shape == (3, 3, 64, 8)
+    Xp = np.zeros((96, 16), np.float32)
+    Xp[:70] = X
+    out = np.zeros((48, 16))
+    for mt in range(3):
+        for kt in range(3):
+            b = np.zeros((64, 8))
+            for g in range(4):
+                for n in range(16):
+                    for i in range(8):
+                        b[16 * g + n, i] = Xp[32 * kt + _perm(g, i), n]
+            out[16 * mt:16 * mt + 16] += mfma_16x16x32(frag[mt, kt].astype(np.float64), b)
+    ref = W.astype(np.float64) @ X.astype(np.float64)
+    assert np.abs(out[:40] - ref).max() < 1e-9
+    assert not out[40:].any()
+
+
+def test_bf16_layer_output_feeds_next_layer():
+    """The accumulator layout (lane 16g + n holds rows 4g..4g+3 of column n)
+    of two consecutive m-tiles, packed as the kernel's epilogue does, is the
+    next layer's B operand for one k-tile: features 32kt + perm(g, i)."""
+    rng = np.random.default_rng(1)
+    Y = rng.standard_normal((32, 16))        # rows of m-tiles 2kt and 2kt + 1
+    b = np.zeros((64, 8))
+    for g in range(4):
+        for n in range(16):
+            for j in range(4):
+                b[16 * g + n, j] = Y[4 * g + j, n]            # half 0: acc[0][j]
+                b[16 * g + n, 4 + j] = Y[16 + 4 * g + j, n]   # half 1: acc[1][j]
+    for g in range(4):
+        for i in range(8):
+            assert np.array_equal(b[16 * g:16 * g + 16, i], Y[_perm(g, i)])
+
+
+def test_f32_fragments_compute_w_times_x():
+    """FP32 fragments (v_mfma_f32_16x16x4_f32: lane 16g + m holds K-slot g):
+    k-tile kt = 4t + i, lane 16g + m holds W[16mt + m][16t + 4g + i]."""
+    rng = np.random.default_rng(2)
+    W = rng.standard_normal((20, 36)).astype(np.float32)    # pads to 32 x 48
+    X = rng.standard_normal((36, 16)).astype(np.float32)
+    frag = pack_layer_f32(W, 48)                            # [2,12,64]
+    Xp = np.zeros((48, 16), np.float32)
+    Xp[:36] = X
+    out = np.zeros((32, 16))
+    for mt in range(2):
+        for kt in range(12):
+            t, i = divmod(kt, 4)
+            for g in range(4):
+                a = frag[mt, kt, 16 * g:16 * g + 16].astype(np.float64)
+                x = Xp[16 * t + 4 * g + i].astype(np.float64)
+                out[16 * mt:16 * mt + 16] += np.outer(a, x)
+    assert np.abs(out[:20] - W.astype(np.float64) @ X).max() < 1e-9
+
+
+def test_bf16_rounding_is_nearest_even():
+    x = np.array([1.0, 1.00390625, 1.005859375, -2.5, 3.0e38, 1e-40], np.float32)
+    bits = bf16_bits(x)
+    back = (bits.astype(np.uint32) << 16).view(np.float32)
+    assert back[0] == 1.0 and back[1] == 1.0         # exactly halfway: to even
+    assert back[2] == np.float32(1.0078125)          # above halfway: up
+    assert back[3] == -2.5
+    assert np.abs(back[:4] - x[:4]).max() <= 2 ** -8
+
+
+def test_packed_tree_boxes():
+    """Leaf boxes are fp32(c - h), fp32(c + h); an internal node's box is the
+    exact min/max over its children's boxes, so it contains every descendant
+    leaf; the kid lists are the reference children."""
+    sc = shell()
+    cen, ch = shell_octree()
+    h = np.float32(sc.voxel_size) * np.float32(0.5)
+    boxes, kids = pack_tree(cen.astype(np.float32), ch.astype(np.int32), h)
+    T = ch.shape[0]
+    mn, mx = boxes[0::2, :3], boxes[1::2, :3]
+    scale = boxes[0::2, 3].view(np.int32)
+    assert np.array_equal(scale, ch[:, 8])
+    assert np.array_equal(kids, ch[:, :8].astype(np.int32))
+    leaf = scale == 1
+    assert np.array_equal(mn[leaf], (cen[leaf] - h).astype(np.float32))
+    assert np.array_equal(mx[leaf], (cen[leaf] + h).astype(np.float32))
+    for node in np.nonzero(~leaf)[0][:2000]:
+        k = kids[node][kids[node] >= 0]
+        assert np.array_equal(mn[node], mn[k].min(0)) and np.array_equal(mx[node], mx[k].max(0))
+    root = T - 1
+    assert (mn[root] <= mn[leaf]).all() and (mx[root] >= mx[leaf]).all()
