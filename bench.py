@@ -45,6 +45,8 @@ def parse():
                     help="requests per arrival rate in the serving measurement (0 = skip)")
     ap.add_argument("--serve-rates", default="10,20,40",
                     help="Poisson arrival rates (requests/s) of the serving measurement")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the backward / training / pruning / splitting measurements")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB (a known-size kernel for "
                          "calibrating PMC byte counters)")
@@ -220,10 +222,118 @@ def main():
                                          f"FP32), {cpu_s:.1f} s"}
         log(f"cpu oracle: {len(sel) / cpu_s:.0f} rays/s on {cores} cores")
 
+    if not args.no_rows:
+        out["rows"] = rows(args, dev, sc, dsc)
+
     if args.serve_requests > 0:
         out["serving"] = serving(args, dev)
 
     print(json.dumps(out), flush=True)
+
+
+def _time_ms(fn, reps):
+    """Mean device time of fn() over reps calls (HIP events on the current
+    stream, after one untimed call)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def rows(args, dev, sc, dsc):
+    """The other rows of the path on the same 100k-voxel scene: backward
+    kernels and one training step on a 256x256 batch of rays, the pruning
+    pass (BF16 and FP32 MFMA) and the split."""
+    import ctypes
+
+    import torch
+
+    from nsvf_amd import scene as S
+    from nsvf_amd._lib import lib
+    from nsvf_amd.render import Renderer
+    from nsvf_amd.train import TrainableField, TrainRenderer, mse_loss, trilinear_gather_backward
+    from nsvf_amd.voxels import prune_scores, split_scene
+
+    res = {}
+    rs_np, rd_np = S.camera_rays(256, 256, S.sphere_camera(2.5, args.camera_seed))
+    rs, rd = torch.from_numpy(rs_np).to(dev), torch.from_numpy(rd_np).to(dev)
+    R = rs.shape[0]
+    D = dsc.D
+    # the coarse stream of the batch (what training differentiates through)
+    g = Renderer(dev, precision="fp32", hierarchical=False).coarse_stream(dsc, rs, rd).owned()
+    M = g.slots
+    rec = g.samples
+    s_idx = rec[:, 0].contiguous()
+    ray = rec[:, 3].long()
+    xyz = (rs.index_select(0, ray) + rd.index_select(0, ray) * rec[:, 1].view(torch.float32)[:, None]).contiguous()
+    grad_emb = torch.randn((M, D), device=dev)
+    K, N = dsc.values.shape[0], dsc.centers.shape[0]
+    ms = _time_ms(lambda: trilinear_gather_backward(s_idx, xyz, dsc.centers, dsc.feats, K, grad_emb,
+                                                    dsc.voxel_size), 10)
+    nbytes = M * 4 * D + M * (4 + 12) + N * 44 + K * 4 * D
+    res["gather_backward"] = {"ms": round(ms, 3), "samples": M, "algorithmic_bytes": nbytes,
+                              "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                              "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    field = torch.rand((M, 4), device=dev)
+    field[:, 0] = field[:, 0] * 20 - 5
+    gr, gd, gm = torch.randn((R, 3), device=dev), torch.randn(R, device=dev), torch.randn(R, device=dev)
+    gf = torch.zeros_like(field)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    ms = _time_ms(lambda: lib().nsvf_render_composite_backward(
+        p(g.hit_list), p(g.scal[0:1]), g.n_hit, p(g.off), p(g.cnt), p(g.samples), p(field), p(gr),
+        p(gd), p(gm), p(gf), st), 20)
+    nbytes = 2 * M * 32 + M * 16 + R * 20
+    res["composite_backward"] = {"ms": round(ms, 3), "samples": M, "algorithmic_bytes": nbytes,
+                                 "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                                 "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    # one training step: forward, MSE against U(0,1) targets, backward
+    tf = TrainableField(sc, dev)
+    tr = TrainRenderer(dev)
+    target = torch.rand((R, 3), device=dev)
+
+    def step():
+        tf.zero_grad(set_to_none=True)
+        rgb, _, _ = tr.forward(dsc, tf, rs, rd)
+        mse_loss(rgb, target).backward()
+
+    torch.cuda.synchronize()
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        step()
+    torch.cuda.synchronize()
+    res["train_step"] = {"ms": round((time.perf_counter() - t0) * 1e3 / 5, 2), "rays": R,
+                         "samples": M, "mlp": "torch FP32 nn.Linear",
+                         "kernels": "HIP gather / composite forward and backward"}
+    del tf, tr
+    torch.cuda.empty_cache()
+    # pruning: 16^3 lattice per voxel through the field MLP, max relu(sigma)
+    flops = sc.num_voxels * 4096 * sc.mlp.flops_per_sample()
+    for prec, peak in (("bf16", PEAK_BF16_TFLOPS), ("fp32", 157.3)):
+        ms = _time_ms(lambda: prune_scores(dsc, prec), 2)
+        tf_s = flops / (ms * 1e-3) / 1e12
+        res[f"prune_{prec}"] = {"ms": round(ms, 2), "lattice_points": sc.num_voxels * 4096,
+                                "achieved_TFLOPs": round(tf_s, 1), "peak_TFLOPs": peak,
+                                "frac": round(tf_s / peak, 4)}
+    # splitting (device key rebuild + interpolation; includes the uploads and
+    # the readback of the new scene)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    new = split_scene(sc, dev)
+    torch.cuda.synchronize()
+    res["split"] = {"ms_including_transfers": round((time.perf_counter() - t0) * 1e3, 1),
+                    "voxels": sc.num_voxels, "new_voxels": new.num_voxels, "new_keys": new.num_keys}
+    log("rows: " + ", ".join(f"{k} {v.get('ms', v.get('ms_including_transfers'))} ms"
+                              for k, v in res.items()))
+    return res
 
 
 SERVE_SCENES = [  # (grid resolution, embedding seed, MLP seed): shells of ~20k-400k voxels
