@@ -396,7 +396,9 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     const int32_t* __restrict__ c_off, int n_max, float step_size, int compat, float fine_steps,
     SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
     int32_t* __restrict__ max_count) {
-  __shared__ float cdf_lds[kSampleWaves][kWaveBins];
+  __shared__ float lds_c[kSampleWaves][kWaveBins], lds_lo[kSampleWaves][kWaveBins],
+      lds_hi[kSampleWaves][kWaveBins];
+  __shared__ int32_t lds_id[kSampleWaves][kWaveBins];
   __shared__ int block_max;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
                  nn < 0 ? -1 : hits[int64_t(hit_list[nn]) * n_max].idx};
     const float steps = tot[r] / step_size;
     const int m = icdf_wave(bins, n_max, cnt, steps, compat, bound[r], r, out + c_off[r],
-                            cdf_lds[wave]);
+                            WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave]});
     if (lane == 0) {
       c_cnt[r] = m;
       f_bound[r] = int(ceilf(fine_steps)) + m + 2;
@@ -448,7 +450,9 @@ __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
     const float* __restrict__ c_prob, const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
     int32_t* __restrict__ f_cnt) {
-  __shared__ float cdf_lds[kSampleWaves][kWaveBins];
+  __shared__ float lds_c[kSampleWaves][kWaveBins], lds_lo[kSampleWaves][kWaveBins],
+      lds_hi[kSampleWaves][kWaveBins];
+  __shared__ int32_t lds_id[kSampleWaves][kWaveBins];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
   const int P = *c_max_dev;  // bin columns = longest coarse row
@@ -471,7 +475,8 @@ __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
         next = c_cnt[rn] > 0 ? coarse[c_off[rn]].idx : -1;
       }
       SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
-      m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, cdf_lds[wave]);
+      m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst,
+                    WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave]});
     } else {
       for (int k = lane; k < b; k += 64) dst[k] = SampleRec{-1, 0.0f, 0.0f, r};
     }

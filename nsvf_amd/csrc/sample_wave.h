@@ -46,7 +46,16 @@ struct StreamSink {
   }
 };
 
-constexpr int kWaveBins = 2048;  // bins whose cumulative probability fits a wave's LDS
+constexpr int kWaveBins = 512;  // bins of a ray staged in a wave's LDS
+
+// A wave's LDS staging of one ray's bins: cumulative probability, bounds and
+// voxel ids of bins [0, kWaveBins).
+struct WaveBinsLds {
+  float* C;
+  float* lo;
+  float* hi;
+  int32_t* id;
+};
 
 __device__ __forceinline__ int first_lane(uint64_t mask) {
   return mask ? __ffsll(static_cast<unsigned long long>(mask)) - 1 : 64;
@@ -61,26 +70,38 @@ __device__ __forceinline__ void put_rec(SampleRec* out, int bound, int pos, int3
 
 // Samples ray `ray` with bins `bins` (P columns, the first Pv valid) into
 // out[0, bound); slots past the samples are marked unused.  Returns the
-// sample count (same value in every lane).  C: this wave's LDS array of
-// kWaveBins floats.
+// sample count (same value in every lane).  lds: this wave's staging arrays.
 template <class Bins>
 __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compat, int bound,
-                         int32_t ray, SampleRec* out, float* C) {
+                         int32_t ray, SampleRec* out, const WaveBinsLds& lds) {
   const int lane = threadIdx.x & 63;
   int n = 0;
   bool seq = Pv <= 0 || Pv > kWaveBins || !(steps <= float(bound));
   if (!seq) {
-    for (int k = lane; k < Pv; k += 64) C[k] = bins.prob(k);
-    wave_sync();
-    if (lane == 0) {
-      float c = C[0];
-      for (int k = 1; k < Pv; ++k) {
-        c = c + C[k];
-        C[k] = c;
+    // stage the bins; the prefix C_k = C_{k-1} + p_k runs left to right (the
+    // reference kernel's order) over values pulled from their lanes, every
+    // lane computing the same sum, and lane k keeps C_k
+    float c = 0.0f;
+    for (int t0 = 0; t0 < Pv; t0 += 64) {
+      const int k = t0 + lane;
+      float pk = 0.0f;
+      if (k < Pv) {
+        pk = bins.prob(k);
+        lds.lo[k] = bins.lo(k);
+        lds.hi[k] = bins.hi(k);
+        lds.id[k] = bins.id(k);
       }
+      const int m = Pv - t0 < 64 ? Pv - t0 : 64;
+      float mine = 0.0f;
+      for (int j = 0; j < m; ++j) {
+        const float pj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pk), j));
+        c = (t0 + j == 0) ? pj : c + pj;
+        mine = lane == j ? c : mine;
+      }
+      if (k < Pv) lds.C[k] = mine;
     }
     wave_sync();
-    seq = isnan(C[Pv - 1]);
+    seq = isnan(c);
   }
   if (seq) {
     if (lane == 0) {
@@ -91,10 +112,11 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
     }
     n = __shfl(n, 0, 64);
   } else {
+    const float* C = lds.C;
     const float ss = float(1.0 / double(steps));
     const int T = int(ceilf(steps));
     int bprev = 0;
-    float zprev = bins.lo(0);
+    float zprev = lds.lo[0];
     bool done = false;
     int cb = 0;
     float zl = 0.0f, cmax = 0.0f;
@@ -117,8 +139,8 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
         if (b < Pv) {
           const float cmin = b > 0 ? C[b - 1] : 0.0f;
           const float u = (cdf - cmin) / (C[b] - cmin);
-          const float lb = bins.lo(b);
-          z = __fmaf_rn(u, bins.hi(b) - lb, lb);
+          const float lb = lds.lo[b];
+          z = __fmaf_rn(u, lds.hi[b] - lb, lb);
         }
       }
       int bp = __shfl_up(b, 1, 64);
@@ -132,13 +154,13 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       if (act && lane <= dl) {
         const int end = b < Pv ? b : Pv;
         for (int k = bp; k < end; ++k) {
-          const float hk = bins.hi(k);
-          const float zk = (k == bp) ? zp : bins.lo(k);
-          put_rec(out, bound, i + k, bins.id(k), hk - zk, (hk + zk) * 0.5f, ray);
+          const float hk = lds.hi[k];
+          const float zk = (k == bp) ? zp : lds.lo[k];
+          put_rec(out, bound, i + k, lds.id[k], hk - zk, (hk + zk) * 0.5f, ray);
         }
         if (b < Pv) {
-          const float zk = (b == bp) ? zp : bins.lo(b);
-          put_rec(out, bound, i + b, bins.id(b), z - zk, (z + zk) * 0.5f, ray);
+          const float zk = (b == bp) ? zp : lds.lo[b];
+          put_rec(out, bound, i + b, lds.id[b], z - zk, (z + zk) * 0.5f, ray);
         }
       }
       if (dmask) {
@@ -147,8 +169,8 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
         done = true;
         n = base + dl + Pv;
         cb = Pv;
-        zl = (bpd == Pv - 1) ? zpd : bins.lo(Pv - 1);
-        cmax = bins.hi(Pv - 1);
+        zl = (bpd == Pv - 1) ? zpd : lds.lo[Pv - 1];
+        cmax = lds.hi[Pv - 1];
         break;
       }
       const int last = (T - 1 - base) < 63 ? (T - 1 - base) : 63;
@@ -159,11 +181,12 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       n = T > 0 ? T + bprev : 0;
       cb = T > 0 ? bprev : 0;
       zl = zprev;
-      cmax = bins.hi(cb);
+      cmax = lds.hi[cb];
     }
-    // trailing tails (sample_core.h: the loop after the steps)
+    // trailing tails (sample_core.h: the loop after the steps); bins past
+    // Pv are read through `bins` (pads)
     if (!(done && !compat) && zl < cmax) {
-      const int32_t id0 = cb < P ? bins.id(cb) : bins.next_first();
+      const int32_t id0 = cb < Pv ? lds.id[cb] : (cb < P ? bins.id(cb) : bins.next_first());
       if (id0 != -1) {
         if (lane == 0) put_rec(out, bound, n, id0, cmax - zl, (cmax + zl) * 0.5f, ray);
         ++n;
@@ -173,20 +196,23 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
         const int k = base + lane;
         bool ok = false;
         float lk = 0.0f, hk = 0.0f;
+        int32_t idk = -1;
         if (k < P) {
-          const int32_t probe = compat ? bins.ray0_id(k) : bins.id(k);
-          lk = bins.lo(k);
-          hk = bins.hi(k);
+          const int32_t probe = compat ? bins.ray0_id(k) : (k < Pv ? lds.id[k] : bins.id(k));
+          lk = k < Pv ? lds.lo[k] : bins.lo(k);
+          hk = k < Pv ? lds.hi[k] : bins.hi(k);
+          idk = k < Pv ? lds.id[k] : bins.id(k);
           ok = probe != -1 && lk < hk;
         }
         const int stop = first_lane(__ballot(!ok));
-        if (lane < stop) put_rec(out, bound, n + lane, bins.id(k), hk - lk, (hk + lk) * 0.5f, ray);
+        if (lane < stop) put_rec(out, bound, n + lane, idk, hk - lk, (hk + lk) * 0.5f, ray);
         n += stop < 64 ? stop : 64;
         if (stop < 64) break;
       }
     }
   }
   for (int k = n + lane; k < bound; k += 64) out[k] = SampleRec{-1, 0.0f, 0.0f, ray};
+  wave_sync();  // the next ray's staging must not overtake this ray's reads
   return n;
 }
 
