@@ -472,7 +472,10 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
   static_assert(D % 4 == 0, "embedding width must be a multiple of 4");
   constexpr int RUNS = 8;        // voxels staged per round
   constexpr int ROW4 = 2 * D;    // float4 per voxel: 8 corners x D / 4
-  __shared__ float4 rows_lds[4][RUNS * ROW4];
+  // per-run stride padded by one float4, so the same element of different
+  // runs falls in different LDS banks
+  constexpr int RUN_STRIDE = ROW4 + 1;
+  __shared__ float4 rows_lds[4][RUNS * RUN_STRIDE];
   __shared__ int32_t run_vox[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t M = min(int64_t(*m_dev), m_max);
@@ -521,12 +524,13 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
       if (vu >= 0) {
         const int c = e / (D / 4), qd = e - c * (D / 4);
         const int32_t k = sc.feats[int64_t(vu) * 8 + c];
-        rows_lds[wave][i] = reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[qd];
+        rows_lds[wave][j * RUN_STRIDE + e] =
+            reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[qd];
       }
     }
     wave_sync();
     if (v >= 0 && r >= u0 && r < u0 + nr) {
-      const float4* rows = &rows_lds[wave][(r - u0) * ROW4];
+      const float4* rows = &rows_lds[wave][(r - u0) * RUN_STRIDE];
 #pragma unroll
       for (int c = 0; c < 8; ++c) {
 #pragma unroll
