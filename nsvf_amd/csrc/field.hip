@@ -483,26 +483,22 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
   const int64_t s0 = (int64_t(blockIdx.x) * 4 + wave) * 64;
   if (s0 >= total) return;  // the whole wave
   const int64_t s = s0 + lane;
-  // the sample's voxel and corner weights (x = o + d * z; the fused kernel's
-  // corner_weights_fast arithmetic)
+  // the sample's voxel; the loads its corner weights need (ray and centre)
+  // are issued here and used after the row staging below, so their latency
+  // overlaps the staging's
   int32_t v = -1;
-  float w[8];
+  float ro[3] = {0.f, 0.f, 0.f}, rdir[3] = {0.f, 0.f, 0.f}, ctr[3] = {0.f, 0.f, 0.f}, z = 0.f;
   if (s < M) {
     const SampleRec rec = samples[s];
     v = rec.idx;
+    z = rec.depth;
     if (v >= 0) {
-      float p[3];
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const float x = rays.start[int64_t(rec.ray) * 3 + a] +
-                        rays.dir[int64_t(rec.ray) * 3 + a] * rec.depth;
-        const float d = x - sc.centers[int64_t(v) * 3 + a];
-        p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
+        ro[a] = rays.start[int64_t(rec.ray) * 3 + a];
+        rdir[a] = rays.dir[int64_t(rec.ray) * 3 + a];
+        ctr[a] = sc.centers[int64_t(v) * 3 + a];
       }
-      const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
-      const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
-#pragma unroll
-      for (int c = 0; c < 8; ++c) w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
     }
   }
   // runs of equal voxel ids among the wave's lanes
@@ -516,30 +512,54 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
   float acc[D];
 #pragma unroll
   for (int f = 0; f < D; ++f) acc[f] = 0.0f;
+  float w[8];
+  bool have_w = false;
+  constexpr int SLOTS = RUNS * ROW4 / 64;  // staging slots per lane and round
   for (int u0 = 0; u0 < U; u0 += RUNS) {
     const int nr = min(RUNS, U - u0);
-    for (int i = lane; i < nr * ROW4; i += 64) {
+    // one unrolled, branch-free pass over the round's staging slots (slots of
+    // absent runs load a valid row and are never read), so the corner-key
+    // and row loads of all slots are in flight together
+#pragma unroll
+    for (int t = 0; t < SLOTS; ++t) {
+      const int i = lane + 64 * t;
       const int j = i / ROW4, e = i - j * ROW4;
-      const int32_t vu = run_vox[wave][u0 + j];
-      if (vu >= 0) {
-        const int c = e / (D / 4), qd = e - c * (D / 4);
-        const int32_t k = sc.feats[int64_t(vu) * 8 + c];
-        rows_lds[wave][j * RUN_STRIDE + e] =
-            reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[qd];
+      const int32_t vr = run_vox[wave][(u0 + j) & 63];  // in bounds for any j
+      const int32_t vu = (j < nr && vr >= 0) ? vr : 0;
+      const int32_t k = sc.feats[int64_t(vu) * 8 + e / (D / 4)];
+      rows_lds[wave][j * RUN_STRIDE + e] =
+          reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[e % (D / 4)];
+    }
+    if (!have_w) {
+      // corner weights: x = o + d * z, p = (x - c) / vs + 0.5 (the fused
+      // kernel's corner_weights_fast arithmetic)
+      float p[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const float x = ro[a] + rdir[a] * z;
+        const float d = x - ctr[a];
+        p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
       }
+      const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+      const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
+#pragma unroll
+      for (int c = 0; c < 8; ++c) w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
+      have_w = true;
     }
     wave_sync();
     if (v >= 0 && r >= u0 && r < u0 + nr) {
       const float4* rows = &rows_lds[wave][(r - u0) * RUN_STRIDE];
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
+      for (int qd = 0; qd < D / 4; ++qd) {
+        float4 e[8];
 #pragma unroll
-        for (int qd = 0; qd < D / 4; ++qd) {
-          const float4 e = rows[c * (D / 4) + qd];
-          acc[4 * qd + 0] = __fmaf_rn(w[c], e.x, acc[4 * qd + 0]);
-          acc[4 * qd + 1] = __fmaf_rn(w[c], e.y, acc[4 * qd + 1]);
-          acc[4 * qd + 2] = __fmaf_rn(w[c], e.z, acc[4 * qd + 2]);
-          acc[4 * qd + 3] = __fmaf_rn(w[c], e.w, acc[4 * qd + 3]);
+        for (int c = 0; c < 8; ++c) e[c] = rows[c * (D / 4) + qd];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          acc[4 * qd + 0] = __fmaf_rn(w[c], e[c].x, acc[4 * qd + 0]);
+          acc[4 * qd + 1] = __fmaf_rn(w[c], e[c].y, acc[4 * qd + 1]);
+          acc[4 * qd + 2] = __fmaf_rn(w[c], e[c].z, acc[4 * qd + 2]);
+          acc[4 * qd + 3] = __fmaf_rn(w[c], e[c].w, acc[4 * qd + 3]);
         }
       }
     }
