@@ -232,23 +232,24 @@ int nsvf_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n,
  * sorted by entry depth (stable with respect to the reference's traversal
  * order), their count, the sum of segment lengths, a hit flag, and an upper
  * bound on the ray's coarse sample count (ceil(sum / step_size) + count + 2,
- * 0 for rays that hit nothing).
- *   centers [T,3] / children [T,9]: the octree in the reference layout;
+ * 0 for rays that hit nothing).  Replaces the octree branch of
+ * fairnr/modules/encoder.py:498-523 (svo_ray_intersect, intersect.cpp:84-112
+ * and intersect_gpu.cu:170-237, then the depth sort) and the segment sum
+ * of fairnr/models/nsvf.py:66-74.
  *   boxes [T] {float minx, miny, minz; int32 scale; float maxx, maxy, maxz;
- *   float 0} and kids [T,8]: the same tree packed for a front-to-back
- *   traversal (scale = children[k][8], kids = children[k][0..7]; a leaf's
- *   box is fp32(c -+ h), an internal node's the exact bounding box of its
- *   leaves' boxes), 16-byte aligned.  Rays whose result depends on traversal
- *   order (n_max reached, equal entry depths) are redone over the reference
- *   layout in the reference's order.
+ *   float 0} and kids [T,8]: the octree of the reference layout (centres
+ *   [T,3], children [T,9]) packed for the traversal (scale = children[k][8],
+ *   kids = children[k][0..7]; a leaf's box is fp32(c -+ h), an internal
+ *   node's the exact bounding box of its leaves' boxes), 16-byte aligned;
+ *   the root is node T-1.  Rays whose result depends on traversal order
+ *   (n_max reached, equal entry depths) are traversed again in the
+ *   reference's child order.
  *   hits: [R, n_max] records {float lo, float hi, int32 idx, int32 pad};
  *   only the first count[r] records of a row are written.                  */
 int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
-                     const float* centers, const int32_t* children,
-                     const void* boxes, const int32_t* kids, int T,
-                     float voxelsize, int n_max, float step_size, void* hits,
-                     int32_t* count, float* tot, int32_t* hit_flag,
-                     int32_t* bound, void* stream);
+                     const void* boxes, const int32_t* kids, int T, int n_max,
+                     float step_size, void* hits, int32_t* count, float* tot,
+                     int32_t* hit_flag, int32_t* bound, void* stream);
 
 /* list[rank[r]] = r for every ray with hit_flag[r] != 0.                  */
 int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,

@@ -36,59 +36,6 @@ constexpr int kMaxLevels = 24;
 // ---------------------------------------------------------------------------
 // hits
 
-// Traverses the octree for one ray, appending leaf hits to `row`.  Child
-// slots are visited in the order (i ^ mask), i = 0..7, with mask the ray's
-// direction-sign octant (front to back), or 7..0 (the reference's order).
-// Returns the hit count (<= n_max).
-template <bool kFrontToBack>
-__device__ int traverse(const float o[3], const float inv[3], int mask,
-                        const float* __restrict__ cen, const int32_t* __restrict__ ch, int T,
-                        float half_voxel, int n_max, HitRec* __restrict__ row, int32_t* stack) {
-  int cnt = 0, level = -1;
-  const int root = T - 1;
-  {
-    const int sc = ch[int64_t(root) * 9 + 8];
-    float lo, hi;
-    if (n_max > 0 && ray_aabb(o, inv, cen[int64_t(root) * 3], cen[int64_t(root) * 3 + 1],
-                              cen[int64_t(root) * 3 + 2], half_voxel * float(sc), &lo, &hi)) {
-      if (sc == 1) {
-        row[0] = HitRec{lo, hi, root, 0};
-        return 1;
-      }
-      level = 0;
-      stack[0] = (root << 4) | 8;
-    }
-  }
-  while (level >= 0 && cnt < n_max) {
-    const int32_t e = stack[level * kHitsBlock];
-    const int node = e >> 4;
-    const int left = e & 15;
-    if (left == 0) {
-      --level;
-      continue;
-    }
-    stack[level * kHitsBlock] = (node << 4) | (left - 1);
-    const int slot = kFrontToBack ? ((8 - left) ^ mask) : (left - 1);
-    const int c = ch[int64_t(node) * 9 + slot];
-    if (c < 0 || c >= T) continue;
-    const int sc = ch[int64_t(c) * 9 + 8];
-    float lo, hi;
-    if (!ray_aabb(o, inv, cen[int64_t(c) * 3], cen[int64_t(c) * 3 + 1], cen[int64_t(c) * 3 + 2],
-                  half_voxel * float(sc), &lo, &hi))
-      continue;
-    if (sc == 1) {
-      row[cnt++] = HitRec{lo, hi, c, 0};
-      continue;
-    }
-    const int ps = ch[int64_t(node) * 9 + 8];
-    if (sc < ps && level + 1 < kMaxLevels) {
-      ++level;
-      stack[level * kHitsBlock] = (c << 4) | 8;
-    }
-  }
-  return cnt;
-}
-
 // Slab test against a box given by its corners; with mn = fp32(c - h) and
 // mx = fp32(c + h) this is the reference's test (intersect_gpu.cu:93-94
 // evaluates (c - h) - o and (c + h) - o) bit for bit.
@@ -261,12 +208,15 @@ __device__ void traverse_packed(const float o[3], const float inv[3], int dmask,
 // (slot ^ mask) order), so it is the common path.  A ray whose hits reach
 // n_max (the kept subset then depends on traversal order), or whose entry
 // depths are not strictly increasing (equal entries: the tie order must be
-// the reference's), is redone in the reference's order and stably
-// insertion-sorted, which is exactly the reference pipeline's result.
+// the reference's), is traversed again in the reference's order -- child
+// slots 7..0 (intersect_gpu.cu:201-230), which is slot = i ^ 7 -- and stably
+// insertion-sorted, which is exactly the reference pipeline's result: the
+// exact boxes prune only subtrees without a hit, so both traversals emit the
+// same leaves in the same order and stop at the same n_max-th hit.
 __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
     const float* __restrict__ ray_start, const float* __restrict__ ray_dir, int64_t R,
-    const float* __restrict__ cen, const int32_t* __restrict__ ch, const float4* __restrict__ boxes,
-    const int4* __restrict__ kids, int T, float half_voxel, int n_max, float step_size,
+    const float4* __restrict__ boxes, const int4* __restrict__ kids, int T, int n_max,
+    float step_size,
     HitRec* __restrict__ hits, int32_t* __restrict__ count, float* __restrict__ tot_out,
     int32_t* __restrict__ hit_flag, int32_t* __restrict__ bound) {
   __shared__ int32_t stack[kMaxLevels * kHitsBlock];
@@ -289,7 +239,9 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
   int cnt = hr.cnt;
   float tot = hr.tot;
   if (!(cnt < n_max && hr.increasing)) {
-    cnt = traverse<false>(o, inv, mask, cen, ch, T, half_voxel, n_max, row, st);
+    HitRow ref{row, 0, 0.0f, 0.0f, true};
+    traverse_packed(o, inv, 7, boxes, kids, T, n_max, ref, st, stack_mask + threadIdx.x);
+    cnt = ref.cnt;
     for (int k = 1; k < cnt; ++k) {  // stable insertion sort by entry depth
       const HitRec h = row[k];
       int j = k - 1;
@@ -666,23 +618,20 @@ using namespace nsvf;
 
 extern "C" {
 
-int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
-                     const float* centers, const int32_t* children, const void* boxes,
-                     const int32_t* kids, int T, float voxelsize, int n_max, float step_size,
-                     void* hits, int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
+int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R, const void* boxes,
+                     const int32_t* kids, int T, int n_max, float step_size, void* hits,
+                     int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
                      void* stream) {
   clear_error();
   if (R < 0 || T <= 0 || n_max <= 0) return set_error(NSVF_EINVAL, "render_hits: bad size");
   if (R == 0) return NSVF_OK;
-  if (!ray_start || !ray_dir || !centers || !children || !boxes || !kids || !hits || !count ||
-      !tot || !hit_flag || !bound)
+  if (!ray_start || !ray_dir || !boxes || !kids || !hits || !count || !tot || !hit_flag || !bound)
     return set_error(NSVF_EINVAL, "render_hits: null pointer");
   if (T >= (1 << 27)) return set_error(NSVF_EINVAL, "render_hits: more than 2^27 nodes");
   hipLaunchKernelGGL(hits_kernel, dim3(unsigned(ceil_div(R, kHitsBlock))), dim3(kHitsBlock), 0,
-                     as_stream(stream), ray_start, ray_dir, R, centers, children,
-                     static_cast<const float4*>(boxes), reinterpret_cast<const int4*>(kids), T,
-                     voxelsize * 0.5f, n_max, step_size, static_cast<HitRec*>(hits), count, tot,
-                     hit_flag, bound);
+                     as_stream(stream), ray_start, ray_dir, R, static_cast<const float4*>(boxes),
+                     reinterpret_cast<const int4*>(kids), T, n_max, step_size,
+                     static_cast<HitRec*>(hits), count, tot, hit_flag, bound);
   return check_launch("render_hits");
 }
 

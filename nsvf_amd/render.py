@@ -88,9 +88,7 @@ class DeviceScene:
         shifted[:, 0] += torch.tensor(vs) / 10
         cen, ch = geometry.build_easy_octree(pts, torch.tensor(vs) / 2)
         self.centers = shifted.contiguous().to(self.device)
-        self.tree_centers = cen.float().contiguous().to(self.device)
-        self.tree_children = ch.int().contiguous().to(self.device)
-        # packed copy for the front-to-back traversal
+        # the octree packed for the traversal (boxes and child ids)
         boxes, kids = pack_tree(cen.float().numpy(), ch.int().numpy(), np.float32(vs) * np.float32(0.5))
         self.tree_boxes = torch.from_numpy(boxes).to(self.device)
         self.tree_kids = torch.from_numpy(kids).to(self.device)
@@ -106,10 +104,10 @@ class DeviceScene:
 
     @property
     def num_nodes(self):
-        return self.tree_children.shape[0]
+        return self.tree_kids.shape[0]
 
     def nbytes(self):
-        ts = [self.centers, self.tree_centers, self.tree_children, self.tree_boxes, self.tree_kids,
+        ts = [self.centers, self.tree_boxes, self.tree_kids,
               self.feats, self.values,
               self.mlp["bf16"], self.mlp["f32"], self.mlp["bias"]]
         return sum(t.numel() * t.element_size() for t in ts)
@@ -328,11 +326,9 @@ class Renderer:
         tot = self._buf("tot", R, torch.float32)
         flag = self._buf("flag", R, torch.int32)
         c_bound = self._buf("c_bound", R, torch.int32)
-        check(L.nsvf_render_hits(_p(ray_start), _p(ray_dir), R, _p(sc.tree_centers),
-                                 _p(sc.tree_children), _p(sc.tree_boxes), _p(sc.tree_kids),
-                                 sc.num_nodes, sc.voxel_size, P,
-                                 sc.step_size, _p(hits), _p(count), _p(tot), _p(flag),
-                                 _p(c_bound), st), "render_hits")
+        check(L.nsvf_render_hits(_p(ray_start), _p(ray_dir), R, _p(sc.tree_boxes),
+                                 _p(sc.tree_kids), sc.num_nodes, P, sc.step_size, _p(hits),
+                                 _p(count), _p(tot), _p(flag), _p(c_bound), st), "render_hits")
         self._mark("intersect")
         scal = self._buf("scalars", 8, torch.int32)   # [n_hit, slots_c, max_c, slots_f]
         scal.zero_()
