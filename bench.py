@@ -102,7 +102,7 @@ def main():
     # ---- fused BF16 path
     rnd = Renderer(dev, precision="bf16", hierarchical=True, fine_steps=64.0)
     for _ in range(args.warmup):
-        rnd.render(dsc, rs, rd)
+        rnd.render(dsc, rs, rd, width=args.res)
     torch.cuda.synchronize()
     kev = rnd.record_kernels(True)
     lat = []
@@ -112,7 +112,7 @@ def main():
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        rnd.render(dsc, rs, rd)
+        rnd.render(dsc, rs, rd, width=args.res)
         e1.record()
         lat.append((e0, e1))
     torch.cuda.synchronize()
@@ -123,7 +123,7 @@ def main():
     value = R / (wall / args.steps)
     st = rnd.last
     # exact sample counts of the view (identical for every repetition)
-    rnd.render(dsc, rs, rd, keep=True)
+    rnd.render(dsc, rs, rd, keep=True, width=args.res)
     n_coarse, n_fine = rnd.count_samples()
     rnd.kept = None
     log(f"fused bf16: {ms_per_step:.2f} ms/view, {value / 1e6:.2f} Mrays/s; hit rays {st.hit_rays}, "
@@ -157,7 +157,7 @@ def main():
 
     # per-stage breakdown of one traced view (outside the timed region)
     rnd.trace = []
-    rnd.render(dsc, rs, rd)
+    rnd.render(dsc, rs, rd, width=args.res)
     torch.cuda.synchronize()
     stages = {}
     for (n0, e0), (n1, e1) in zip(rnd.trace[:-1], rnd.trace[1:]):
@@ -194,7 +194,7 @@ def main():
             prgb, _, _ = pl.render(rs, rd)
         torch.cuda.synchronize()
         pms = (time.perf_counter() - t1) * 1000.0 / args.plain_views
-        frgb, _, _ = rnd.render(dsc, rs, rd)
+        frgb, _, _ = rnd.render(dsc, rs, rd, width=args.res)
         mse = float(torch.mean((frgb.double() - prgb.double()) ** 2))
         out["plain_fp32"] = {"rays_per_sec": round(R / (pms / 1000.0), 1), "ms_per_view": round(pms, 2),
                              "views": args.plain_views,
@@ -373,7 +373,7 @@ def serving(args, dev):
             res = int(rng.choice(SERVE_RES))
             cam = S.sphere_camera(float(rng.uniform(2.0, 3.0)), int(rng.integers(1 << 30)))
             rs, rd = S.camera_rays(res, res, cam)
-            reqs.append((name, torch.from_numpy(rs).to(dev), torch.from_numpy(rd).to(dev)))
+            reqs.append((name, torch.from_numpy(rs).to(dev), torch.from_numpy(rd).to(dev), res))
         return reqs
 
     # warm-up: one request of each scene at each size
@@ -400,7 +400,7 @@ def serving(args, dev):
     for name in names:
         rs_np, rd_np = S.camera_rays(256, 256, S.sphere_camera(2.5, 3))
         rs, rd = torch.from_numpy(rs_np).to(dev), torch.from_numpy(rd_np).to(dev)
-        req = srv.submit(name, rs, rd)
+        req = srv.submit(name, rs, rd, width=256)
         srv.drain()
         pl = PlainNSVF(scenes[name], dev, compat=srv.compat)
         prgb, _, _ = pl.render(rs, rd)

@@ -236,20 +236,25 @@ int nsvf_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n,
  * fairnr/modules/encoder.py:498-523 (svo_ray_intersect, intersect.cpp:84-112
  * and intersect_gpu.cu:170-237, then the depth sort) and the segment sum
  * of fairnr/models/nsvf.py:66-74.
- *   boxes [T] {float minx, miny, minz; int32 scale; float maxx, maxy, maxz;
- *   float 0} and kids [T,8]: the octree of the reference layout (centres
- *   [T,3], children [T,9]) packed for the traversal (scale = children[k][8],
- *   kids = children[k][0..7]; a leaf's box is fp32(c -+ h), an internal
- *   node's the exact bounding box of its leaves' boxes), 16-byte aligned;
- *   the root is node T-1.  Rays whose result depends on traversal order
- *   (n_max reached, equal entry depths) are traversed again in the
- *   reference's child order.
+ *   width: 0, or the pixel width of the image whose rows the R rays are
+ *   (R a multiple of width; several views stack as more rows): lanes then
+ *   take the rays in 8x8 pixel tiles, for coherent traversal.  The outputs
+ *   do not depend on it.
+ *   records [T + 1][8][2] float4: the octree of the reference layout
+ *   (centres [T,3], children [T,9], root T-1) packed as node records -- for
+ *   node k and child slot s, {min x, y, z, scale bits} and {max x, y, z,
+ *   child id bits} of child children[k][s] (id -1: empty slot); record T
+ *   holds the root in slot 0.  A leaf's box is fp32(c -+ h), an internal
+ *   node's the exact bounding box of its leaves' boxes; scale =
+ *   children[k][8].  16-byte aligned.  Rays whose result depends on
+ *   traversal order (n_max reached, equal entry depths) are traversed again
+ *   in the reference's child order.
  *   hits: [R, n_max] records {float lo, float hi, int32 idx, int32 pad};
  *   only the first count[r] records of a row are written.                  */
 int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
-                     const void* boxes, const int32_t* kids, int T, int n_max,
-                     float step_size, void* hits, int32_t* count, float* tot,
-                     int32_t* hit_flag, int32_t* bound, void* stream);
+                     int width, const void* records, int T, int n_max, float step_size,
+                     void* hits, int32_t* count, float* tot, int32_t* hit_flag,
+                     int32_t* bound, void* stream);
 
 /* list[rank[r]] = r for every ray with hit_flag[r] != 0.                  */
 int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,

@@ -84,120 +84,113 @@ struct HitRow {
   }
 };
 
-// Front-to-back traversal over the packed tree: boxes[2k] = (min xyz, scale
-// bits), boxes[2k+1] = (max xyz, 0), kids[k] = the 8 child ids.  Leaf boxes
-// are the voxel boxes; an internal node's box is the exact bounding box of
-// its leaves' boxes (computed with the same fp32 values), so it prunes every
-// subtree the ray misses while keeping every leaf the ray hits -- the slab
-// test is monotone under box containment -- and the hit set equals the
-// reference's, whose loose boxes (twice the octant) only prune less.
-// Expanding a node tests all its children at once in visit order (slot =
-// i ^ dmask, dmask = the ray's direction-sign octant).  If every hit child is
-// a leaf they are emitted right away in that order; otherwise the hit mask
-// goes on the level stack and the children are visited from it in order (a
-// leaf among them is re-tested when visited).  Returns the hit count.
-__device__ __forceinline__ unsigned expand_node(int node, int psc, const float o[3],
-                                                const float inv[3], int dmask,
-                                                const float4* __restrict__ boxes,
-                                                const int4* __restrict__ kids, int T, int n_max,
-                                                HitRow& out) {
-  const int4 k0 = kids[int64_t(node) * 2], k1 = kids[int64_t(node) * 2 + 1];
-  int c[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
-  // c[i] <- c[i ^ dmask] with static indices only
-  if (dmask & 1) {
-#pragma unroll
-    for (int i = 0; i < 8; i += 2) {
-      const int t = c[i];
-      c[i] = c[i + 1];
-      c[i + 1] = t;
-    }
-  }
-  if (dmask & 2) {
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-      if (!(i & 2)) {
-        const int t = c[i];
-        c[i] = c[i + 2];
-        c[i + 2] = t;
-      }
-  }
-  if (dmask & 4) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int t = c[i];
-      c[i] = c[i + 4];
-      c[i + 4] = t;
-    }
-  }
-  unsigned hit = 0, leaf = 0;
+// Wave-union traversal of the octree packed as node records: record k (16
+// float4) holds, for child slot s, piece 2s = (box min xyz, scale bits) and
+// piece 2s+1 = (box max xyz, child id bits), id -1 for an empty slot; record
+// T holds the root in slot 0.  Leaf boxes are the voxel boxes; an internal
+// node's box is the exact bounding box of its leaves' boxes (computed with
+// the same fp32 values), so it prunes every subtree the ray misses while
+// keeping every leaf the ray hits -- the slab test is monotone under box
+// containment -- and the hit set equals the reference's, whose loose boxes
+// (twice the octant) only prune less.
+//
+// The lanes of `group` (rays sharing the child order dmask) walk ONE node
+// stack: a node is expanded when any of them hits it, so its record is
+// wave-uniform -- one coalesced 256-byte load into this level's LDS slot,
+// read back by broadcast -- and each lane only runs its own slab tests.  A
+// lane's hits come out in the same order as from a walk of its own -- depth
+// first, children in order slot = i ^ dmask -- because the union walk visits
+// a superset of its nodes in that order and a lane that misses a node misses
+// all of its descendants (containment again).  Expanding a node tests all
+// its children at once; if every child hit by some lane is a leaf, each lane
+// emits its hits right away in order, otherwise the union hit mask goes on
+// the stack and the children are visited from it in order (a leaf among
+// them is re-tested when visited).  The stack of masks lives in one register
+// across the lanes (level l in lane l), the records of the open nodes in
+// LDS.
+__device__ __forceinline__ unsigned expand_union(int node, int psc, const float o[3],
+                                                 const float inv[3], int dmask, bool act,
+                                                 const float4* __restrict__ recs, int T,
+                                                 int n_max, float4* lrec, HitRow& out) {
+  const int lane = threadIdx.x & 63;
+  if (lane < 16) lrec[lane] = recs[int64_t(node) * 16 + lane];
+  wave_sync();
+  const bool live = act && out.cnt < n_max;
+  unsigned U = 0, leafs = 0, mine = 0;
   float los[8], his[8];
+  int ids[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     los[i] = 0.0f;
     his[i] = 0.0f;
-    const int id = c[i];
-    if (id >= 0 && id < T) {
-      const float4 mn = boxes[int64_t(id) * 2];
-      const int sc = __float_as_int(mn.w);
-      if (sc < psc) {
-        const float4 mx = boxes[int64_t(id) * 2 + 1];
-        float lo, hi;
-        if (ray_box(o, inv, mn, mx, &lo, &hi)) {
-          hit |= 1u << i;
-          if (sc == 1) leaf |= 1u << i;
-          los[i] = lo;
-          his[i] = hi;
-        }
+    const int s = i ^ dmask;
+    const float4 mn = lrec[2 * s], mx = lrec[2 * s + 1];
+    const int id = __builtin_amdgcn_readfirstlane(__float_as_int(mx.w));
+    const int sc = __builtin_amdgcn_readfirstlane(__float_as_int(mn.w));
+    ids[i] = id;
+    if (id >= 0 && id < T && sc < psc) {
+      float lo = 0.0f, hi = 0.0f;
+      const bool h = live && ray_box(o, inv, mn, mx, &lo, &hi);
+      if (h) {
+        mine |= 1u << i;
+        los[i] = lo;
+        his[i] = hi;
+      }
+      if (__ballot(h)) {
+        U |= 1u << i;
+        if (sc == 1) leafs |= 1u << i;
       }
     }
   }
-  if ((hit & ~leaf) == 0) {
+  if ((U & ~leafs) == 0) {
 #pragma unroll
     for (int i = 0; i < 8; ++i)
-      if (((hit >> i) & 1u) && out.cnt < n_max) out.push(los[i], his[i], c[i]);
+      if (((mine >> i) & 1u) && out.cnt < n_max) out.push(los[i], his[i], ids[i]);
     return 0u;
   }
-  return hit;
+  return U;
 }
 
-__device__ void traverse_packed(const float o[3], const float inv[3], int dmask,
-                                const float4* __restrict__ boxes, const int4* __restrict__ kids,
-                                int T, int n_max, HitRow& out, int32_t* st_node,
-                                uint8_t* st_mask) {
-  const int root = T - 1;
-  const float4 rmn = boxes[int64_t(root) * 2], rmx = boxes[int64_t(root) * 2 + 1];
-  const int rsc = __float_as_int(rmn.w);
-  float lo, hi;
-  if (n_max <= 0 || !ray_box(o, inv, rmn, rmx, &lo, &hi)) return;
-  if (rsc == 1) {
-    out.push(lo, hi, root);
-    return;
-  }
+// Appends the hits of the lanes with `act` set, all of which share dmask.
+// lrec: this wave's LDS, kMaxLevels records.
+__device__ void traverse_union(const float o[3], const float inv[3], int dmask, bool act,
+                               const float4* __restrict__ recs, int T, int n_max, float4* lrec,
+                               HitRow& out) {
+  const bool here0 = (threadIdx.x & 63) == 0;
+  int st_mask = 0;  // level l's remaining children in lane l
   int level = 0;
-  st_node[0] = root;
-  st_mask[0] = uint8_t(expand_node(root, rsc, o, inv, dmask, boxes, kids, T, n_max, out));
-  while (level >= 0 && out.cnt < n_max) {
-    const unsigned m = st_mask[level * kHitsBlock];
+  {
+    // record T: the root in slot 0 under a parent of unbounded scale
+    const unsigned m = expand_union(T, 0x7fffffff, o, inv, 0, act, recs, T, n_max, lrec, out);
+    st_mask = here0 ? int(m) : st_mask;
+  }
+  while (level >= 0) {
+    if (!__ballot(act && out.cnt < n_max)) break;
+    const unsigned m = unsigned(__builtin_amdgcn_readlane(st_mask, level));
     if (m == 0) {
       --level;
       continue;
     }
     const int i = __builtin_ctz(m);
-    st_mask[level * kHitsBlock] = uint8_t(m & (m - 1));
-    const int node = st_node[level * kHitsBlock];
-    const int id = reinterpret_cast<const int*>(kids)[int64_t(node) * 8 + (i ^ dmask)];
-    const float4 mn = boxes[int64_t(id) * 2];
-    const int sc = __float_as_int(mn.w);
+    const bool here = int(threadIdx.x & 63) == level;
+    st_mask = here ? int(m & (m - 1)) : st_mask;
+    // record T's only child is the root, visited in slot 0 whatever dmask
+    const int s = level == 0 ? i : (i ^ dmask);
+    const float4* rec = lrec + level * 16;
+    const float4 mn = rec[2 * s], mx = rec[2 * s + 1];
+    const int id = __builtin_amdgcn_readfirstlane(__float_as_int(mx.w));
+    const int sc = __builtin_amdgcn_readfirstlane(__float_as_int(mn.w));
     if (sc == 1) {
-      const float4 mx = boxes[int64_t(id) * 2 + 1];
-      if (ray_box(o, inv, mn, mx, &lo, &hi)) out.push(lo, hi, id);
+      float lo = 0.0f, hi = 0.0f;
+      if (act && out.cnt < n_max && ray_box(o, inv, mn, mx, &lo, &hi)) out.push(lo, hi, id);
       continue;
     }
     if (level + 1 < kMaxLevels) {
       ++level;
-      st_node[level * kHitsBlock] = id;
-      st_mask[level * kHitsBlock] =
-          uint8_t(expand_node(id, sc, o, inv, dmask, boxes, kids, T, n_max, out));
+      const unsigned mm =
+          expand_union(id, sc, o, inv, dmask, act, recs, T, n_max, lrec + level * 16, out);
+      const bool there = int(threadIdx.x & 63) == level;
+      st_mask = there ? int(mm) : st_mask;
     }
   }
 }
@@ -214,46 +207,69 @@ __device__ void traverse_packed(const float o[3], const float inv[3], int dmask,
 // exact boxes prune only subtrees without a hit, so both traversals emit the
 // same leaves in the same order and stop at the same n_max-th hit.
 __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
-    const float* __restrict__ ray_start, const float* __restrict__ ray_dir, int64_t R,
-    const float4* __restrict__ boxes, const int4* __restrict__ kids, int T, int n_max,
-    float step_size,
+    const float* __restrict__ ray_start, const float* __restrict__ ray_dir, int64_t R, int width,
+    const float4* __restrict__ recs, int T, int n_max, float step_size,
     HitRec* __restrict__ hits, int32_t* __restrict__ count, float* __restrict__ tot_out,
     int32_t* __restrict__ hit_flag, int32_t* __restrict__ bound) {
-  __shared__ int32_t stack[kMaxLevels * kHitsBlock];
-  __shared__ uint8_t stack_mask[kMaxLevels * kHitsBlock];
-  const int64_t r = int64_t(blockIdx.x) * kHitsBlock + threadIdx.x;
-  if (r >= R) return;
-  float o[3], inv[3];
-  int mask = 0;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    o[a] = ray_start[r * 3 + a];
-    const float d = ray_dir[r * 3 + a];
-    inv[a] = 1.0f / d;
-    mask |= (d < 0.0f ? 1 : 0) << a;
+  __shared__ float4 lrec_all[kHitsBlock / 64][kMaxLevels * 16];
+  float4* lrec = lrec_all[threadIdx.x >> 6];
+  // the lane's ray: consecutive rays, or for rays that are the rows of an
+  // image `width` pixels wide, one 8x8 pixel tile per wave (neighbouring
+  // pixels share most of their octree paths, which keeps the union small);
+  // lanes outside R or the image take part in the wave's votes only
+  const int64_t t = int64_t(blockIdx.x) * kHitsBlock + threadIdx.x;
+  int64_t r = t;
+  bool valid = t < R;
+  if (width > 0) {
+    const int64_t tiles_x = (width + 7) / 8, k = t >> 6;
+    const int q = int(t & 63);
+    const int64_t y = (k / tiles_x) * 8 + (q >> 3), x = (k % tiles_x) * 8 + (q & 7);
+    valid = x < width && y < R / width;
+    r = y * width + x;
   }
-  HitRec* row = hits + r * n_max;
-  int32_t* st = stack + threadIdx.x;
+  float o[3] = {0.0f, 0.0f, 0.0f}, inv[3] = {1.0f, 1.0f, 1.0f};
+  int mask = 0;
+  if (valid) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      o[a] = ray_start[r * 3 + a];
+      const float d = ray_dir[r * 3 + a];
+      inv[a] = 1.0f / d;
+      mask |= (d < 0.0f ? 1 : 0) << a;
+    }
+  }
+  HitRec* row = hits + (valid ? r : 0) * n_max;
   HitRow hr{row, 0, 0.0f, 0.0f, true};
-  traverse_packed(o, inv, mask, boxes, kids, T, n_max, hr, st, stack_mask + threadIdx.x);
+  // one union walk per child order present in the wave
+  uint64_t pending = __ballot(valid);
+  while (pending) {
+    const int d = __builtin_amdgcn_readlane(mask, first_lane(pending));
+    const bool grp = valid && mask == d;
+    pending &= ~__ballot(grp);
+    traverse_union(o, inv, d, grp, recs, T, n_max, lrec, hr);
+  }
   int cnt = hr.cnt;
   float tot = hr.tot;
-  if (!(cnt < n_max && hr.increasing)) {
+  const bool redo = valid && !(cnt < n_max && hr.increasing);
+  if (__ballot(redo)) {
     HitRow ref{row, 0, 0.0f, 0.0f, true};
-    traverse_packed(o, inv, 7, boxes, kids, T, n_max, ref, st, stack_mask + threadIdx.x);
-    cnt = ref.cnt;
-    for (int k = 1; k < cnt; ++k) {  // stable insertion sort by entry depth
-      const HitRec h = row[k];
-      int j = k - 1;
-      while (j >= 0 && row[j].lo > h.lo) {
-        row[j + 1] = row[j];
-        --j;
+    traverse_union(o, inv, 7, redo, recs, T, n_max, lrec, ref);
+    if (redo) {
+      cnt = ref.cnt;
+      for (int k = 1; k < cnt; ++k) {  // stable insertion sort by entry depth
+        const HitRec h = row[k];
+        int j = k - 1;
+        while (j >= 0 && row[j].lo > h.lo) {
+          row[j + 1] = row[j];
+          --j;
+        }
+        row[j + 1] = h;
       }
-      row[j + 1] = h;
+      tot = 0.0f;
+      for (int k = 0; k < cnt; ++k) tot = tot + (row[k].hi - row[k].lo);
     }
-    tot = 0.0f;
-    for (int k = 0; k < cnt; ++k) tot = tot + (row[k].hi - row[k].lo);
   }
+  if (!valid) return;
   count[r] = cnt;
   tot_out[r] = tot;
   hit_flag[r] = cnt > 0 ? 1 : 0;
@@ -618,19 +634,24 @@ using namespace nsvf;
 
 extern "C" {
 
-int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R, const void* boxes,
-                     const int32_t* kids, int T, int n_max, float step_size, void* hits,
+int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R, int width,
+                     const void* records, int T, int n_max, float step_size, void* hits,
                      int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
                      void* stream) {
   clear_error();
   if (R < 0 || T <= 0 || n_max <= 0) return set_error(NSVF_EINVAL, "render_hits: bad size");
+  if (width < 0 || (width > 0 && R % width != 0))
+    return set_error(NSVF_EINVAL, "render_hits: %lld rays are not rows of width %d",
+                     static_cast<long long>(R), width);
   if (R == 0) return NSVF_OK;
-  if (!ray_start || !ray_dir || !boxes || !kids || !hits || !count || !tot || !hit_flag || !bound)
+  if (!ray_start || !ray_dir || !records || !hits || !count || !tot || !hit_flag || !bound)
     return set_error(NSVF_EINVAL, "render_hits: null pointer");
   if (T >= (1 << 27)) return set_error(NSVF_EINVAL, "render_hits: more than 2^27 nodes");
-  hipLaunchKernelGGL(hits_kernel, dim3(unsigned(ceil_div(R, kHitsBlock))), dim3(kHitsBlock), 0,
-                     as_stream(stream), ray_start, ray_dir, R, static_cast<const float4*>(boxes),
-                     reinterpret_cast<const int4*>(kids), T, n_max, step_size,
+  const int64_t threads =
+      width > 0 ? ceil_div(R / width, 8) * ceil_div(width, 8) * 64 : R;
+  hipLaunchKernelGGL(hits_kernel, dim3(unsigned(ceil_div(threads, kHitsBlock))), dim3(kHitsBlock),
+                     0, as_stream(stream), ray_start, ray_dir, R, width,
+                     static_cast<const float4*>(records), T, n_max, step_size,
                      static_cast<HitRec*>(hits), count, tot, hit_flag, bound);
   return check_launch("render_hits");
 }

@@ -41,15 +41,14 @@ def _stream(dev):
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
-def pack_tree(centers, children, half_voxel):
-    """Packed octree for the front-to-back traversal: per node two float4
-    (min xyz + scale bits, max xyz + 0) and the 8 child ids.  Leaf boxes are
-    fp32(c - h), fp32(c + h); an internal node's box is the bounding box of
-    its children's boxes (children have smaller ids than their parent, so one
-    pass per depth from the leaves up fills it)."""
-    T = children.shape[0]
+def node_boxes(centers, children, half_voxel):
+    """Per-node boxes of the octree: leaf boxes are fp32(c - h), fp32(c + h);
+    an internal node's box is the bounding box of its children's boxes
+    (children have smaller ids than their parent, so one pass per depth from
+    the leaves up fills it).  Returns mn [T,3], mx [T,3] f32, scale [T] i32."""
     h = np.float32(half_voxel)
     scale = children[:, 8].astype(np.int32)
+    T = children.shape[0]
     mn = np.zeros((T, 3), np.float32)
     mx = np.zeros((T, 3), np.float32)
     leaf = scale == 1
@@ -65,11 +64,27 @@ def pack_tree(centers, children, half_voxel):
         cmx = np.where(valid[..., None], mx[kk], np.float32(-np.inf))
         mn[nodes] = cmn.min(1)
         mx[nodes] = cmx.max(1)
-    boxes = np.zeros((T, 2, 4), np.float32)
-    boxes[:, 0, :3] = mn
-    boxes[:, 0, 3] = scale.view(np.float32)
-    boxes[:, 1, :3] = mx
-    return boxes.reshape(T * 2, 4), np.ascontiguousarray(kids)
+    return mn, mx, scale
+
+
+def pack_tree(centers, children, half_voxel):
+    """The octree packed as node records for the traversal: record k holds,
+    for child slot s, (min xyz, scale bits) and (max xyz, id bits) of child
+    children[k][s] (id -1 for an empty slot); record T holds the root (node
+    T-1) in slot 0.  Returns float32 [T + 1, 16, 4]."""
+    T = children.shape[0]
+    mn, mx, scale = node_boxes(centers, children, half_voxel)
+    ids = np.full((T + 1, 8), -1, np.int32)
+    ids[:T] = children[:, :8]
+    ids[T, 0] = T - 1
+    ok = ids >= 0
+    k = np.where(ok, ids, 0)
+    rec = np.zeros((T + 1, 8, 2, 4), np.float32)
+    rec[:, :, 0, :3] = np.where(ok[..., None], mn[k], np.float32(0))
+    rec[:, :, 0, 3] = np.where(ok, scale[k], 0).astype(np.int32).view(np.float32)
+    rec[:, :, 1, :3] = np.where(ok[..., None], mx[k], np.float32(0))
+    rec[:, :, 1, 3] = ids.view(np.float32)
+    return rec.reshape(T + 1, 16, 4)
 
 
 class DeviceScene:
@@ -89,9 +104,8 @@ class DeviceScene:
         cen, ch = geometry.build_easy_octree(pts, torch.tensor(vs) / 2)
         self.centers = shifted.contiguous().to(self.device)
         # the octree packed for the traversal (boxes and child ids)
-        boxes, kids = pack_tree(cen.float().numpy(), ch.int().numpy(), np.float32(vs) * np.float32(0.5))
-        self.tree_boxes = torch.from_numpy(boxes).to(self.device)
-        self.tree_kids = torch.from_numpy(kids).to(self.device)
+        rec = pack_tree(cen.float().numpy(), ch.int().numpy(), np.float32(vs) * np.float32(0.5))
+        self.tree_records = torch.from_numpy(rec).to(self.device)
         self.feats = torch.from_numpy(scene.feats.astype(np.int32)).to(self.device)
         self.values = torch.from_numpy(scene.values).to(self.device)
         self.voxel_size = float(vs)
@@ -104,10 +118,10 @@ class DeviceScene:
 
     @property
     def num_nodes(self):
-        return self.tree_kids.shape[0]
+        return self.tree_records.shape[0] - 1
 
     def nbytes(self):
-        ts = [self.centers, self.tree_boxes, self.tree_kids,
+        ts = [self.centers, self.tree_records,
               self.feats, self.values,
               self.mlp["bf16"], self.mlp["f32"], self.mlp["bias"]]
         return sum(t.numel() * t.element_size() for t in ts)
@@ -305,12 +319,14 @@ class Renderer:
     # ---- phase 1: intersection, compaction, slot offsets
 
     @torch.no_grad()
-    def begin(self, sc: DeviceScene, ray_start, ray_dir):
+    def begin(self, sc: DeviceScene, ray_start, ray_dir, width=0):
         """Enqueues the geometry stages of a view on the current stream:
         intersection, hit compaction and the coarse slot offsets
         (encoder.py:498-535), then an asynchronous copy of the hit-ray count
         and slot total to pinned host memory.  geometry_ready() tells when
-        finish() can run without waiting."""
+        finish() can run without waiting.  width: 0, or the pixel width of
+        the image(s) whose rows the rays are (a traversal-order hint; the
+        results do not depend on it)."""
         L = lib()
         dev = self.device
         st = _stream(dev)
@@ -326,8 +342,10 @@ class Renderer:
         tot = self._buf("tot", R, torch.float32)
         flag = self._buf("flag", R, torch.int32)
         c_bound = self._buf("c_bound", R, torch.int32)
-        check(L.nsvf_render_hits(_p(ray_start), _p(ray_dir), R, _p(sc.tree_boxes),
-                                 _p(sc.tree_kids), sc.num_nodes, P, sc.step_size, _p(hits),
+        if width and R % width:
+            raise ValueError(f"{R} rays are not rows of width {width}")
+        check(L.nsvf_render_hits(_p(ray_start), _p(ray_dir), R, int(width), _p(sc.tree_records),
+                                 sc.num_nodes, P, sc.step_size, _p(hits),
                                  _p(count), _p(tot), _p(flag), _p(c_bound), st), "render_hits")
         self._mark("intersect")
         scal = self._buf("scalars", 8, torch.int32)   # [n_hit, slots_c, max_c, slots_f]
@@ -464,11 +482,11 @@ class Renderer:
         return rgb, depth, missed
 
     @torch.no_grad()
-    def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False):
+    def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False, width=0):
         """ray_start, ray_dir: [R,3] float32 on the device.  Returns
         (rgb [R,3], depth [R], missed [R]) -- the fine pass when hierarchical,
-        else the coarse pass -- after background fill."""
-        self.begin(sc, ray_start, ray_dir)
+        else the coarse pass -- after background fill.  width: as in begin()."""
+        self.begin(sc, ray_start, ray_dir, width)
         return self.finish(keep)
 
     def count_samples(self):

@@ -79,6 +79,7 @@ class Request:
     result: Any = None
     error: Optional[str] = None
     preemptions: int = 0
+    width: int = 0            # image width of the rays (a traversal-order hint), 0 = none
 
     @property
     def rays(self) -> int:
@@ -126,9 +127,9 @@ class Scheduler:
         self.done = []
         self._ids = itertools.count()
 
-    def submit(self, scene, ray_start, ray_dir, arrival=None) -> Request:
+    def submit(self, scene, ray_start, ray_dir, arrival=None, width=0) -> Request:
         req = Request(next(self._ids), scene, ray_start, ray_dir,
-                      self.clock() if arrival is None else arrival)
+                      self.clock() if arrival is None else arrival, width=int(width))
         self.queue.append(req)
         return req
 
@@ -253,7 +254,7 @@ class RenderSlot(Slot):
         # the request's rays were written on the submitting stream
         self.stream.wait_stream(torch.cuda.current_stream(self.server.device))
         with torch.cuda.stream(self.stream):
-            self.renderer.begin(sc, req.ray_start, req.ray_dir)
+            self.renderer.begin(sc, req.ray_start, req.ray_dir, req.width)
 
     def geometry_ready(self):
         return self.renderer.geometry_ready()
@@ -328,10 +329,12 @@ class RenderServer:
         del self.scenes[name]
         self.budget.release(self._scene_bytes.pop(name))
 
-    def submit(self, scene, ray_start, ray_dir, arrival=None):
+    def submit(self, scene, ray_start, ray_dir, arrival=None, width=0):
+        """Queues a view: rays [R,3] on the device; width as in
+        Renderer.begin()."""
         if scene not in self.scenes:
             raise KeyError(f"scene {scene} is not resident")
-        return self.scheduler.submit(scene, ray_start, ray_dir, arrival)
+        return self.scheduler.submit(scene, ray_start, ray_dir, arrival, width)
 
     def poll(self):
         return self.scheduler.poll()
@@ -340,9 +343,10 @@ class RenderServer:
         self.scheduler.drain()
 
     def run_arrivals(self, arrivals):
-        """Replays (offset_seconds, scene, ray_start, ray_dir) arrivals against
-        the wall clock -- each submitted when its time comes, while earlier
-        requests are in flight -- and returns the requests when all are done."""
+        """Replays (offset_seconds, scene, ray_start, ray_dir[, width])
+        arrivals against the wall clock -- each submitted when its time comes,
+        while earlier requests are in flight -- and returns the requests when
+        all are done."""
         arrivals = sorted(arrivals, key=lambda a: a[0])
         t0 = time.perf_counter()
         reqs = []
@@ -350,8 +354,8 @@ class RenderServer:
         while i < len(arrivals) or not self.scheduler.idle():
             now = time.perf_counter() - t0
             while i < len(arrivals) and arrivals[i][0] <= now:
-                off, scene, rs, rd = arrivals[i]
-                reqs.append(self.submit(scene, rs, rd, arrival=t0 + off))
+                off, scene, rs, rd, *w = arrivals[i]
+                reqs.append(self.submit(scene, rs, rd, arrival=t0 + off, width=w[0] if w else 0))
                 i += 1
             self.poll()
         return reqs

@@ -15,6 +15,7 @@ import torch
 import nsvf_oracle as O
 from _cases import shell, shell_rays
 from nsvf_amd import _lib
+from nsvf_amd import scene as S
 from nsvf_amd.render import DeviceScene, Renderer
 
 pytestmark = pytest.mark.gpu
@@ -259,3 +260,27 @@ def test_hits_truncation_and_ties(device, n_max):
         assert np.array_equal(rows[ray, :n, 0].view(np.float32), lo[ray, :n]), ray
     if n_max == 7:
         assert full.sum() > 100
+
+
+def test_tiled_traversal_order_is_invisible(device):
+    """The image-width hint (lanes take 8x8 pixel tiles) changes which rays
+    share a wave, never a result: hits, counts and pixels equal the run in
+    ray order bit for bit; 60 x 45 leaves partial tiles on both edges."""
+    sc = shell()
+    dsc = DeviceScene(sc, device)
+    rs_np, rd_np = S.camera_rays(60, 45, S.sphere_camera(2.5, 9))
+    rs, rd = torch.from_numpy(rs_np).to(device), torch.from_numpy(rd_np).to(device)
+    outs = []
+    for width in (0, 60):
+        r = Renderer(device, precision="fp32", hierarchical=True)
+        img = [N(t) for t in r.render(dsc, rs, rd, keep=True, width=width)]
+        k = r.kept
+        outs.append((img, N(k["count"]), N(k["hits"]), N(k["tot"])))
+    (i0, c0, h0, t0), (i1, c1, h1, t1) = outs
+    assert c0.sum() > 0 and np.array_equal(c0, c1) and np.array_equal(t0, t1)
+    for ray in np.nonzero(c0)[0]:
+        assert np.array_equal(h0[ray, :c0[ray]], h1[ray, :c0[ray]]), ray
+    for a, b in zip(i0, i1):
+        assert np.array_equal(a, b)
+    with pytest.raises(ValueError):
+        Renderer(device).render(dsc, rs, rd, width=7)

@@ -4,7 +4,7 @@ import numpy as np
 
 from _cases import shell, shell_octree
 from nsvf_amd.field import bf16_bits, bf16_round, pack_layer_bf16, pack_layer_f32
-from nsvf_amd.render import pack_tree
+from nsvf_amd.render import node_boxes, pack_tree
 
 
 def _perm(g, i):
@@ -97,21 +97,38 @@ def test_bf16_rounding_is_nearest_even():
 def test_packed_tree_boxes():
     """Leaf boxes are fp32(c - h), fp32(c + h); an internal node's box is the
     exact min/max over its children's boxes, so it contains every descendant
-    leaf; the kid lists are the reference children."""
+    leaf."""
     sc = shell()
     cen, ch = shell_octree()
     h = np.float32(sc.voxel_size) * np.float32(0.5)
-    boxes, kids = pack_tree(cen.astype(np.float32), ch.astype(np.int32), h)
-    T = ch.shape[0]
-    mn, mx = boxes[0::2, :3], boxes[1::2, :3]
-    scale = boxes[0::2, 3].view(np.int32)
+    mn, mx, scale = node_boxes(cen.astype(np.float32), ch.astype(np.int32), h)
+    kids = ch[:, :8]
     assert np.array_equal(scale, ch[:, 8])
-    assert np.array_equal(kids, ch[:, :8].astype(np.int32))
     leaf = scale == 1
     assert np.array_equal(mn[leaf], (cen[leaf] - h).astype(np.float32))
     assert np.array_equal(mx[leaf], (cen[leaf] + h).astype(np.float32))
     for node in np.nonzero(~leaf)[0][:2000]:
         k = kids[node][kids[node] >= 0]
         assert np.array_equal(mn[node], mn[k].min(0)) and np.array_equal(mx[node], mx[k].max(0))
-    root = T - 1
+    root = ch.shape[0] - 1
     assert (mn[root] <= mn[leaf]).all() and (mx[root] >= mx[leaf]).all()
+
+
+def test_packed_tree_records():
+    """Record k, slot s: child children[k][s]'s (min, scale) and (max, id);
+    empty slots carry id -1; record T holds the root in slot 0 only."""
+    sc = shell()
+    cen, ch = shell_octree()
+    h = np.float32(sc.voxel_size) * np.float32(0.5)
+    mn, mx, scale = node_boxes(cen.astype(np.float32), ch.astype(np.int32), h)
+    rec = pack_tree(cen.astype(np.float32), ch.astype(np.int32), h)
+    T = ch.shape[0]
+    assert rec.shape == (T + 1, 16, 4) and rec.dtype == np.float32
+    lo, hi = rec[:, 0::2], rec[:, 1::2]                 # [T+1, 8, 4]
+    ids = hi[..., 3].view(np.int32)
+    assert np.array_equal(ids[:T], ch[:, :8])
+    assert ids[T, 0] == T - 1 and (ids[T, 1:] == -1).all()
+    ok = ids >= 0
+    k = ids[ok]
+    assert np.array_equal(lo[ok][:, :3], mn[k]) and np.array_equal(hi[ok][:, :3], mx[k])
+    assert np.array_equal(lo[ok][:, 3].view(np.int32), scale[k])
