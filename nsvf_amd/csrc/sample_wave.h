@@ -10,16 +10,17 @@
 //   (C_b - C_{b-1}), and z_low = z_{i-1} if b_{i-1} == b_i, else lo_b;
 //   the tails of the bins crossed between steps i-1 and i, k in
 //   [b_{i-1}, b_i), go to positions i + k, z_low = z_{i-1} for
-//   k == b_{i-1}, else lo_k;
+//   k == b_{i-1}, else lo_k -- computed per bin: i = i_k, the number of
+//   steps with cdf <= C_k;
 //   the first step with cdf beyond C of the last bin ends the walk after
 //   the remaining tails ("done"); otherwise the trailing tails follow the
 //   last step, bin after bin, while each bin passes the same tests as the
 //   sequential loop.
 //
-// So each lane takes one step (64 per pass), positions are contiguous
-// across lanes (coalesced 16-byte stores), and only the prefix C -- whose
-// FP32 rounding order is the reference's left-to-right sum -- runs on one
-// lane.  Rows with more bins than the LDS holds, a NaN in C, or a step count
+// So each lane takes one step, then one bin (64 per pass), positions are
+// contiguous across lanes (coalesced 16-byte stores), and only the prefix C
+// -- whose FP32 rounding order is the reference's left-to-right sum -- is a
+// serial chain.  Rows with more bins than the LDS holds, a NaN in C, or a step count
 // beyond the slot bound take the sequential walk on lane 0 instead.
 #pragma once
 
@@ -75,6 +76,12 @@ template <class Bins>
 __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compat, int bound,
                          int32_t ray, SampleRec* out, const WaveBinsLds& lds) {
   const int lane = threadIdx.x & 63;
+  // the row's sizes are the same in every lane; scalar copies keep the
+  // loops below uniform (no per-lane exit masks)
+  P = __builtin_amdgcn_readfirstlane(P);
+  Pv = __builtin_amdgcn_readfirstlane(Pv);
+  bound = __builtin_amdgcn_readfirstlane(bound);
+  steps = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(steps)));
   int n = 0;
   bool seq = Pv <= 0 || Pv > kWaveBins || !(steps <= float(bound));
   if (!seq) {
@@ -93,9 +100,14 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       }
       const int m = Pv - t0 < 64 ? Pv - t0 : 64;
       float mine = 0.0f;
-      for (int j = 0; j < m; ++j) {
-        const float pj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pk), j));
-        c = (t0 + j == 0) ? pj : c + pj;
+      int j = 0;
+      if (t0 == 0) {  // C_0 = p_0
+        c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pk), 0));
+        mine = c;
+        j = 1;
+      }
+      for (; j < m; ++j) {
+        c = c + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pk), j));
         mine = lane == j ? c : mine;
       }
       if (k < Pv) lds.C[k] = mine;
@@ -151,17 +163,9 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       }
       const uint64_t dmask = __ballot(act && b >= Pv);
       const int dl = first_lane(dmask);
-      if (act && lane <= dl) {
-        const int end = b < Pv ? b : Pv;
-        for (int k = bp; k < end; ++k) {
-          const float hk = lds.hi[k];
-          const float zk = (k == bp) ? zp : lds.lo[k];
-          put_rec(out, bound, i + k, lds.id[k], hk - zk, (hk + zk) * 0.5f, ray);
-        }
-        if (b < Pv) {
-          const float zk = (b == bp) ? zp : lds.lo[b];
-          put_rec(out, bound, i + b, lds.id[b], z - zk, (z + zk) * 0.5f, ray);
-        }
+      if (act && lane <= dl && b < Pv) {
+        const float zk = (b == bp) ? zp : lds.lo[b];
+        put_rec(out, bound, i + b, lds.id[b], z - zk, (z + zk) * 0.5f, ray);
       }
       if (dmask) {
         const int bpd = __shfl(bp, dl, 64);
@@ -176,6 +180,32 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       const int last = (T - 1 - base) < 63 ? (T - 1 - base) : 63;
       bprev = __shfl(b, last, 64);
       zprev = __shfl(z, last, 64);
+    }
+    // the tails of the bins crossed by a step, one lane per bin: bin k's
+    // tail follows the i_k steps with cdf <= C_k and the k earlier tails
+    // (position i_k + k); its lower end is the depth of step i_k - 1 when
+    // that step fell in bin k (C_{k-1} < cdf <= C_k), else the bin's entry.
+    // i_k comes from an estimate corrected against the exact, monotone cdf.
+    for (int k0 = 0; k0 < Pv; k0 += 64) {
+      const int k = k0 + lane;
+      if (k < Pv) {
+        const float ck = C[k];
+        float e = floorf(ck * steps - 0.5f) + 1.0f;
+        e = fminf(fmaxf(e, 0.0f), float(T));
+        int ik = int(e);
+        while (ik > 0 && (float(ik - 1) + 0.5f) * ss > ck) --ik;
+        while (ik < T && (float(ik) + 0.5f) * ss <= ck) ++ik;
+        if (ik < T) {
+          const float lk = lds.lo[k], hk = lds.hi[k];
+          float zk = lk;
+          if (ik > 0) {
+            const float cdfp = (float(ik - 1) + 0.5f) * ss;
+            const float cmin = k > 0 ? C[k - 1] : 0.0f;
+            if (k == 0 || cdfp > cmin) zk = __fmaf_rn((cdfp - cmin) / (ck - cmin), hk - lk, lk);
+          }
+          put_rec(out, bound, ik + k, lds.id[k], hk - zk, (hk + zk) * 0.5f, ray);
+        }
+      }
     }
     if (!done) {
       n = T > 0 ? T + bprev : 0;
