@@ -158,7 +158,13 @@ __device__ __forceinline__ bf16x8 relu_pack8(const f32x4& lo, const f32x4& hi) {
   return __builtin_bit_cast(bf16x8, u);
 }
 
-__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+// 1 / (1 + e^-x) from the hardware base-2 exponential and reciprocal (a few
+// ulp; the head's three sigmoids otherwise cost as much VALU as a hidden
+// layer's epilogue).  Saturates to 0 / 1 for large |x|.
+__device__ __forceinline__ float sigmoidf(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * -1.44269504088896341f);
+  return __builtin_amdgcn_rcpf(1.0f + e);
+}
 
 // ---------------------------------------------------------------------------
 // Sample sources: sample s -> (voxel, corner weights and keys), false for a
