@@ -155,6 +155,18 @@ def main():
                                      "achieved_GBs": round(emb_bytes / (emb_ms * 1e-3) / 1e9, 1),
                                      "bound": "hbm/l2"}}
 
+    # the same view from its camera: rays generated inside the traversal
+    # kernel (outside the timed region; reported beside the metric)
+    from nsvf_amd.rays import ViewCamera
+    vcam = ViewCamera.looking_at(cam, args.res, args.res)
+    rnd.render_camera(dsc, vcam)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        rnd.render_camera(dsc, vcam)
+    torch.cuda.synchronize()
+    cam_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
+
     # per-stage breakdown of one traced view (outside the timed region)
     rnd.trace = []
     rnd.render(dsc, rs, rd, width=args.res)
@@ -181,6 +193,8 @@ def main():
         "stages_ms": stages,
         "roofline": roofline,
         "kernels": kernels,
+        "from_camera": {"ms_per_view": round(cam_ms, 3),
+                        "note": "rays generated inside the traversal kernel from the camera"},
     }
 
     # ---- plain FP32 path (the comparator)

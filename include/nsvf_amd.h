@@ -256,6 +256,37 @@ int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
                      void* hits, int32_t* count, float* tot, int32_t* hit_flag,
                      int32_t* bound, void* stream);
 
+/* One view's pinhole camera, as the reference builds its rays
+ * (fairnr/renderer.py:91-112, data/geometry.py:68-122, data_utils.py:57-65):
+ * intrinsics fx, fy, cx, cy of the full H x W image; inv_rt the
+ * camera-to-world pose [R | t], row-major 3 x 4; the rays are the h x w
+ * grid, pixel (row i, column j) at uv = (j * float(W / w), i * float(H / h))
+ * (float32 products), in row-major order r = i * w + j.                    */
+typedef struct nsvf_camera {
+  float fx, fy, cx, cy;
+  float inv_rt[12];
+  int H, W, h, w;
+} nsvf_camera;
+
+/* Rays of a camera (ImageReader.forward / get_ray_direction,
+ * fairnr/modules/reader.py:92-119, geometry.py:116-122): ray_start[r] = t,
+ * ray_dir[r] = d / (|d| + 1e-8) with d = (R [x, y, 1] + t) - t,
+ * x = (u - cx) / fx, y = (v - cy) / fy; FP32, products and sums rounded
+ * separately, sums left to right.  cam is a host pointer; outputs
+ * [h * w, 3] on the device.                                               */
+int nsvf_generate_rays(const nsvf_camera* cam, float* ray_start,
+                       float* ray_dir, void* stream);
+
+/* nsvf_render_hits for the rays of a camera, generated inside the traversal
+ * kernel (same arithmetic as nsvf_generate_rays) and written to ray_start /
+ * ray_dir [h * w, 3] for the later stages; lanes take 8 x 8 pixel tiles.
+ * Outputs as nsvf_render_hits with R = h * w.                             */
+int nsvf_render_hits_camera(const nsvf_camera* cam, const void* records, int T,
+                            int n_max, float step_size, float* ray_start,
+                            float* ray_dir, void* hits, int32_t* count,
+                            float* tot, int32_t* hit_flag, int32_t* bound,
+                            void* stream);
+
 /* list[rank[r]] = r for every ray with hit_flag[r] != 0.                  */
 int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,
                         int64_t R, int32_t* list, void* stream);

@@ -21,6 +21,15 @@ _I = ctypes.c_int
 _F = ctypes.c_float
 _L = ctypes.c_int64
 
+
+class Camera(ctypes.Structure):
+    """``nsvf_camera``: fx, fy, cx, cy; inv_rt (3 x 4 row-major); H, W, h, w."""
+    _fields_ = [("fx", _F), ("fy", _F), ("cx", _F), ("cy", _F), ("inv_rt", _F * 12),
+                ("H", _I), ("W", _I), ("h", _I), ("w", _I)]
+
+
+_CAM = ctypes.POINTER(Camera)
+
 # name -> (argtypes, restype); every entry point declared in include/nsvf_amd.h
 SIGNATURES = {
     "nsvf_last_error": ([], ctypes.c_char_p),
@@ -43,6 +52,8 @@ SIGNATURES = {
     "nsvf_scan_workspace_ints": ([_L], _L),
     "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_render_hits": ([_P, _P, _L, _I, _P, _I, _I, _F, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_generate_rays": ([_CAM, _P, _P, _P], _I),
+    "nsvf_render_hits_camera": ([_CAM, _P, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_sample_coarse": ([_P, _P, _L, _P, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P,
                                    _P, _P], _I),

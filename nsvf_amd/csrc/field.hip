@@ -486,25 +486,29 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t M = min(int64_t(*m_dev), m_max);
   const int64_t total = ((M + Dm::SAMPLES - 1) / Dm::SAMPLES) * Dm::SAMPLES;
-  const int64_t s0 = (int64_t(blockIdx.x) * 4 + wave) * 64;
-  if (s0 >= total) return;  // the whole wave
-  const int64_t s = s0 + lane;
+  // persistent waves over 64-slot tiles; each tile's sample records are
+  // loaded one tile ahead, so a tile's dependent chain starts at its keys
+  const int64_t ntile = total / 64, stride = int64_t(gridDim.x) * 4;
+  int64_t tile = int64_t(blockIdx.x) * 4 + wave;
+  if (tile >= ntile) return;  // the whole wave
+  SampleRec nrec{-1, 0.0f, 0.0f, 0};
+  if (tile * 64 + lane < M) nrec = samples[tile * 64 + lane];
+  for (; tile < ntile; tile += stride) {
+  const int64_t s = tile * 64 + lane;
+  const SampleRec rec = nrec;
+  if (tile + stride < ntile && s + stride * 64 < M) nrec = samples[s + stride * 64];
   // the sample's voxel; the loads its corner weights need (ray and centre)
   // are issued here and used after the row staging below, so their latency
   // overlaps the staging's
-  int32_t v = -1;
-  float ro[3] = {0.f, 0.f, 0.f}, rdir[3] = {0.f, 0.f, 0.f}, ctr[3] = {0.f, 0.f, 0.f}, z = 0.f;
-  if (s < M) {
-    const SampleRec rec = samples[s];
-    v = rec.idx;
-    z = rec.depth;
-    if (v >= 0) {
+  const int32_t v = s < M ? rec.idx : -1;
+  float ro[3] = {0.f, 0.f, 0.f}, rdir[3] = {0.f, 0.f, 0.f}, ctr[3] = {0.f, 0.f, 0.f};
+  const float z = rec.depth;
+  if (v >= 0) {
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        ro[a] = rays.start[int64_t(rec.ray) * 3 + a];
-        rdir[a] = rays.dir[int64_t(rec.ray) * 3 + a];
-        ctr[a] = sc.centers[int64_t(v) * 3 + a];
-      }
+    for (int a = 0; a < 3; ++a) {
+      ro[a] = rays.start[int64_t(rec.ray) * 3 + a];
+      rdir[a] = rays.dir[int64_t(rec.ray) * 3 + a];
+      ctr[a] = sc.centers[int64_t(v) * 3 + a];
     }
   }
   // runs of equal voxel ids among the wave's lanes
@@ -571,7 +575,6 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
     }
     wave_sync();
   }
-  if (s >= total) return;
   const int64_t t = s / Dm::SAMPLES;
   const int within = int(s - t * Dm::SAMPLES);
   const int j = within >> 4, col = within & 15;
@@ -596,6 +599,7 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
                        __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
       base[kt * 64 + 16 * g] = __builtin_bit_cast(bf16x8, u);
     }
+  }
   }
 }
 
@@ -802,15 +806,30 @@ int64_t bf16_workspace_bytes(int64_t m_max) {
   return ceil_div(m_max, Dm::SAMPLES) * Dm::NT * Dm::KT0 * 64 * 16;
 }
 
+// Blocks of `kernel` (256 threads, static LDS only) that the current device
+// holds at once: its occupancy per CU times the CU count.
+template <class K>
+int64_t resident_blocks(K kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+      cus <= 0 || per_cu <= 0)
+    return 1024;
+  return int64_t(cus) * per_cu;
+}
+
 template <int D, int H, int L>
 int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
                       const int32_t* m_dev, int64_t m_max, void* workspace, hipStream_t stream) {
   using Dm = Dims<D, H, L>;
-  const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
-  const int64_t threads = ceil_div(tiles * Dm::SAMPLES, 64) * 64;
-  hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(ceil_div(threads, 256))),
-                     dim3(256), 0, stream, sc, rays, samples, m_dev, m_max,
-                     static_cast<bf16x8*>(workspace));
+  // persistent waves: one resident round of blocks, each wave looping over
+  // 64-slot tiles
+  static const int64_t resident = resident_blocks(embed_bf16_kernel<D, H, L>);
+  const int64_t tiles64 = ceil_div(m_max, Dm::SAMPLES) * (Dm::SAMPLES / 64);
+  const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles64, 4), resident));
+  hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(256), 0, stream,
+                     sc, rays, samples, m_dev, m_max, static_cast<bf16x8*>(workspace));
   return check_launch("field_embed_bf16");
 }
 

@@ -195,6 +195,42 @@ __device__ void traverse_union(const float o[3], const float inv[3], int dmask, 
   }
 }
 
+// The ray of pixel (row i, column j) of a camera (geometry.py:76-89,116-122,
+// data_utils.py:57-65): uv = (j * su, i * sv) with su = float(W / w), sv =
+// float(H / h); x = (u - cx) / fx, y = (v - cy) / fy (times z = 1); rt =
+// [R | t] [x, y, 1, 1]; d = rt - t; dir = d / (|d| + 1e-8).  Every product
+// and sum rounded on its own, sums left to right.
+__device__ __forceinline__ void camera_ray(const nsvf_camera& cam, float su, float sv, int i,
+                                           int j, float o[3], float d[3]) {
+  const float u = float(j) * su, v = float(i) * sv;
+  const float x = (u - cam.cx) / cam.fx, y = (v - cam.cy) / cam.fy;
+  float q[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float* m = cam.inv_rt + 4 * a;
+    const float rt = ((m[0] * x + m[1] * y) + m[2]) + m[3];
+    o[a] = m[3];
+    q[a] = rt - m[3];
+  }
+  const float n = sqrtf((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + 1e-8f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) d[a] = q[a] / n;
+}
+
+__global__ __launch_bounds__(kBlock) void ray_gen_kernel(nsvf_camera cam, float su, float sv,
+                                                         float* __restrict__ ray_start,
+                                                         float* __restrict__ ray_dir) {
+  const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (r >= int64_t(cam.h) * cam.w) return;
+  float o[3], d[3];
+  camera_ray(cam, su, sv, int(r / cam.w), int(r % cam.w), o, d);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    ray_start[r * 3 + a] = o[a];
+    ray_dir[r * 3 + a] = d[a];
+  }
+}
+
 // Per ray: hits sorted by entry depth, count, and the left-to-right sum of
 // the segment lengths.  The front-to-back traversal yields the hits already
 // sorted (leaves are disjoint voxels and a ray crosses octants in increasing
@@ -206,9 +242,12 @@ __device__ void traverse_union(const float o[3], const float inv[3], int dmask, 
 // insertion-sorted, which is exactly the reference pipeline's result: the
 // exact boxes prune only subtrees without a hit, so both traversals emit the
 // same leaves in the same order and stop at the same n_max-th hit.
+template <bool kCamera>
 __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
     const float* __restrict__ ray_start, const float* __restrict__ ray_dir, int64_t R, int width,
-    const float4* __restrict__ recs, int T, int n_max, float step_size,
+    nsvf_camera cam, float su, float sv, float* __restrict__ gen_start,
+    float* __restrict__ gen_dir, const float4* __restrict__ recs, int T, int n_max,
+    float step_size,
     HitRec* __restrict__ hits, int32_t* __restrict__ count, float* __restrict__ tot_out,
     int32_t* __restrict__ hit_flag, int32_t* __restrict__ bound) {
   __shared__ float4 lrec_all[kHitsBlock / 64][kMaxLevels * 16];
@@ -230,12 +269,26 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
   float o[3] = {0.0f, 0.0f, 0.0f}, inv[3] = {1.0f, 1.0f, 1.0f};
   int mask = 0;
   if (valid) {
+    float dir[3];
+    if constexpr (kCamera) {
+      // the ray of pixel r of the camera, also stored for the later stages
+      camera_ray(cam, su, sv, int(r / width), int(r % width), o, dir);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        gen_start[r * 3 + a] = o[a];
+        gen_dir[r * 3 + a] = dir[a];
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        o[a] = ray_start[r * 3 + a];
+        dir[a] = ray_dir[r * 3 + a];
+      }
+    }
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      o[a] = ray_start[r * 3 + a];
-      const float d = ray_dir[r * 3 + a];
-      inv[a] = 1.0f / d;
-      mask |= (d < 0.0f ? 1 : 0) << a;
+      inv[a] = 1.0f / dir[a];
+      mask |= (dir[a] < 0.0f ? 1 : 0) << a;
     }
   }
   HitRec* row = hits + (valid ? r : 0) * n_max;
@@ -649,11 +702,54 @@ int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R, in
   if (T >= (1 << 27)) return set_error(NSVF_EINVAL, "render_hits: more than 2^27 nodes");
   const int64_t threads =
       width > 0 ? ceil_div(R / width, 8) * ceil_div(width, 8) * 64 : R;
-  hipLaunchKernelGGL(hits_kernel, dim3(unsigned(ceil_div(threads, kHitsBlock))), dim3(kHitsBlock),
-                     0, as_stream(stream), ray_start, ray_dir, R, width,
+  hipLaunchKernelGGL(hits_kernel<false>, dim3(unsigned(ceil_div(threads, kHitsBlock))),
+                     dim3(kHitsBlock), 0, as_stream(stream), ray_start, ray_dir, R, width,
+                     nsvf_camera{}, 0.0f, 0.0f, nullptr, nullptr,
                      static_cast<const float4*>(records), T, n_max, step_size,
                      static_cast<HitRec*>(hits), count, tot, hit_flag, bound);
   return check_launch("render_hits");
+}
+
+static int check_camera(const nsvf_camera* cam, const char* what) {
+  if (!cam) return set_error(NSVF_EINVAL, "%s: null camera", what);
+  if (cam->h <= 0 || cam->w <= 0 || cam->H <= 0 || cam->W <= 0)
+    return set_error(NSVF_EINVAL, "%s: bad image size %dx%d of %dx%d", what, cam->h, cam->w,
+                     cam->H, cam->W);
+  if (int64_t(cam->h) * cam->w >= (int64_t(1) << 31))
+    return set_error(NSVF_EINVAL, "%s: more than 2^31 rays", what);
+  return NSVF_OK;
+}
+
+int nsvf_generate_rays(const nsvf_camera* cam, float* ray_start, float* ray_dir, void* stream) {
+  clear_error();
+  if (const int rc = check_camera(cam, "generate_rays")) return rc;
+  if (!ray_start || !ray_dir) return set_error(NSVF_EINVAL, "generate_rays: null pointer");
+  // float(W / w): the quotient in double, rounded to float (data_utils.py:63-64)
+  const float su = float(double(cam->W) / cam->w), sv = float(double(cam->H) / cam->h);
+  const int64_t R = int64_t(cam->h) * cam->w;
+  hipLaunchKernelGGL(ray_gen_kernel, dim3(unsigned(ceil_div(R, kBlock))), dim3(kBlock), 0,
+                     as_stream(stream), *cam, su, sv, ray_start, ray_dir);
+  return check_launch("generate_rays");
+}
+
+int nsvf_render_hits_camera(const nsvf_camera* cam, const void* records, int T, int n_max,
+                            float step_size, float* ray_start, float* ray_dir, void* hits,
+                            int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
+                            void* stream) {
+  clear_error();
+  if (const int rc = check_camera(cam, "render_hits_camera")) return rc;
+  if (T <= 0 || n_max <= 0) return set_error(NSVF_EINVAL, "render_hits_camera: bad size");
+  if (!records || !ray_start || !ray_dir || !hits || !count || !tot || !hit_flag || !bound)
+    return set_error(NSVF_EINVAL, "render_hits_camera: null pointer");
+  if (T >= (1 << 27)) return set_error(NSVF_EINVAL, "render_hits_camera: more than 2^27 nodes");
+  const float su = float(double(cam->W) / cam->w), sv = float(double(cam->H) / cam->h);
+  const int64_t R = int64_t(cam->h) * cam->w;
+  const int64_t threads = ceil_div(cam->h, 8) * ceil_div(cam->w, 8) * 64;
+  hipLaunchKernelGGL(hits_kernel<true>, dim3(unsigned(ceil_div(threads, kHitsBlock))),
+                     dim3(kHitsBlock), 0, as_stream(stream), nullptr, nullptr, R, cam->w, *cam,
+                     su, sv, ray_start, ray_dir, static_cast<const float4*>(records), T, n_max,
+                     step_size, static_cast<HitRec*>(hits), count, tot, hit_flag, bound);
+  return check_launch("render_hits_camera");
 }
 
 int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank, int64_t R, int32_t* list,

@@ -327,12 +327,40 @@ class Renderer:
         finish() can run without waiting.  width: 0, or the pixel width of
         the image(s) whose rows the rays are (a traversal-order hint; the
         results do not depend on it)."""
-        L = lib()
-        dev = self.device
-        st = _stream(dev)
         ray_start = ray_start.contiguous()
         ray_dir = ray_dir.contiguous()
         R = ray_start.shape[0]
+        if width and R % width:
+            raise ValueError(f"{R} rays are not rows of width {width}")
+
+        def intersect(sc, P, hits, count, tot, flag, c_bound, st):
+            check(lib().nsvf_render_hits(_p(ray_start), _p(ray_dir), R, int(width),
+                                         _p(sc.tree_records), sc.num_nodes, P, sc.step_size,
+                                         _p(hits), _p(count), _p(tot), _p(flag), _p(c_bound), st),
+                  "render_hits")
+        self._begin(sc, R, ray_start, ray_dir, intersect)
+
+    @torch.no_grad()
+    def begin_camera(self, sc: DeviceScene, camera):
+        """begin() for the rays of a camera (nsvf_amd.rays.ViewCamera): they
+        are generated inside the traversal kernel (renderer.py:91-112,
+        reader.py:92-119) into the workspace, in row-major pixel order."""
+        R = camera.num_rays
+        rs = self._buf("cam_rs", R * 3, torch.float32).view(R, 3)
+        rd = self._buf("cam_rd", R * 3, torch.float32).view(R, 3)
+        cam = camera.c()
+
+        def intersect(sc, P, hits, count, tot, flag, c_bound, st):
+            check(lib().nsvf_render_hits_camera(ctypes.byref(cam), _p(sc.tree_records),
+                                                sc.num_nodes, P, sc.step_size, _p(rs), _p(rd),
+                                                _p(hits), _p(count), _p(tot), _p(flag),
+                                                _p(c_bound), st), "render_hits_camera")
+        self._begin(sc, R, rs, rd, intersect)
+
+    def _begin(self, sc, R, ray_start, ray_dir, intersect):
+        L = lib()
+        dev = self.device
+        st = _stream(dev)
         P = sc.max_hits
         for name, (n, dtype) in self._geometry_sizes(R, P).items():
             self._buf(name, n, dtype)
@@ -342,11 +370,7 @@ class Renderer:
         tot = self._buf("tot", R, torch.float32)
         flag = self._buf("flag", R, torch.int32)
         c_bound = self._buf("c_bound", R, torch.int32)
-        if width and R % width:
-            raise ValueError(f"{R} rays are not rows of width {width}")
-        check(L.nsvf_render_hits(_p(ray_start), _p(ray_dir), R, int(width), _p(sc.tree_records),
-                                 sc.num_nodes, P, sc.step_size, _p(hits),
-                                 _p(count), _p(tot), _p(flag), _p(c_bound), st), "render_hits")
+        intersect(sc, P, hits, count, tot, flag, c_bound, st)
         self._mark("intersect")
         scal = self._buf("scalars", 8, torch.int32)   # [n_hit, slots_c, max_c, slots_f]
         scal.zero_()
@@ -480,6 +504,13 @@ class Renderer:
                 self.kept.update({"f_cnt": f_cnt[hl], "f_off": f_off[hl],
                                   "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4)})
         return rgb, depth, missed
+
+    @torch.no_grad()
+    def render_camera(self, sc: DeviceScene, camera, keep=False):
+        """render() of the rays of a camera (nsvf_amd.rays.ViewCamera), which
+        the traversal kernel generates; outputs in row-major pixel order."""
+        self.begin_camera(sc, camera)
+        return self.finish(keep)
 
     @torch.no_grad()
     def render(self, sc: DeviceScene, ray_start, ray_dir, keep=False, width=0):

@@ -512,6 +512,34 @@ def trilinear_backward(sample_idx, xyz, centers, feats, num_keys, grad_out, voxe
     return out
 
 
+def camera_uv(H, W, h, w):
+    """data_utils.py:57-65 ``get_uv``: [2, h, w] float32 pixel coordinates,
+    u = column * float32(W / w), v = row * float32(H / h)."""
+    uv = np.flip(np.mgrid[0:h, 0:w], axis=0).astype(np.float32)
+    uv[0] = uv[0] * np.float32(W / w)
+    uv[1] = uv[1] * np.float32(H / h)
+    return uv
+
+
+def camera_rays(H, W, h, w, fx, fy, cx, cy, inv_RT, dtype=np.float64):
+    """Rays of one view (renderer.py:91-112, reader.py:92-119,
+    geometry.py:76-89,116-122): uv from ``camera_uv``; x = (u - cx) / fx,
+    y = (v - cy) / fy, z = 1 (``uv2cam``); rt = inv_RT [x, y, 1, 1]
+    (``cam2world``); d = rt - t; dir = d / (|d| + 1e-8) (``normalize``).
+    Returns ray_start, ray_dir [h*w, 3] in row-major pixel order."""
+    uv = camera_uv(H, W, h, w).reshape(2, -1).astype(dtype)
+    M = np.asarray(inv_RT, dtype)[:3, :4]
+    x = (uv[0] - dtype(cx)) / dtype(fx)
+    y = (uv[1] - dtype(cy)) / dtype(fy)
+    cam = np.stack([x, y, np.ones_like(x), np.ones_like(x)])
+    rt = M @ cam
+    t = M[:, 3:4]
+    d = rt - t
+    n = np.sqrt((d * d).sum(0, keepdims=True)) + 1e-8
+    start = np.broadcast_to(t.T, (uv.shape[1], 3))
+    return np.ascontiguousarray(start), np.ascontiguousarray((d / n).T)
+
+
 def fill_background(hits, rgb, depth, missed, bg=(1.0, 1.0, 1.0), bg_depth=BG_DEPTH, dtype=np.float64):
     """nsvf.py:89-101 with geometry.py:303-317: scatter hit rays back to the
     full frame (missed=1, rgb=0, depth=0 elsewhere), then add the background."""

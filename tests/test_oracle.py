@@ -10,6 +10,7 @@ import pytest
 
 import nsvf_oracle as O
 from _cases import axis_rays, random_rays, shell, shell_octree, shell_rays
+from nsvf_amd import scene as S
 
 
 # ---------------------------------------------------------------------------
@@ -616,3 +617,46 @@ def test_fine_normaliser_order():
         part = [np.float32(part[j] + part[j ^ d]) for j in range(64)]
     assert O.fine_normaliser(y)[1] == part[0]
     assert np.abs(O.fine_normaliser(y) - y.astype(np.float64).sum(1)).max() < 1e-5
+
+
+# ---- camera rays (renderer.py:91-112, reader.py:92-119, geometry.py:76-122)
+
+def test_camera_uv_enumerates_pixels():
+    """get_uv: u = column * W/w, v = row * H/h, as a float32 [2, h, w]."""
+    uv = O.camera_uv(9, 4, 3, 2)      # rows scaled by 3, columns by 2
+    assert uv.dtype == np.float32 and uv.shape == (2, 3, 2)
+    for i in range(3):
+        for j in range(2):
+            assert uv[0, i, j] == np.float32(j * 2.0) and uv[1, i, j] == np.float32(i * 3.0)
+
+
+def test_camera_rays_closed_forms():
+    """Identity pose, f = 1, principal point 0: pixel (i, j) looks along
+    (j, i, 1) / |.|; the principal-point pixel of any pose looks along the
+    third column of R; directions are unit; starts are the translation."""
+    rs, rd = O.camera_rays(3, 4, 3, 4, 1.0, 1.0, 0.0, 0.0, np.eye(4))
+    i, j = np.divmod(np.arange(12), 4)
+    ref = np.stack([j, i, np.ones(12)], 1).astype(np.float64)
+    ref /= np.linalg.norm(ref, axis=1, keepdims=True) + 1e-8     # normalize(): |d| + 1e-8
+    assert np.abs(rd - ref).max() < 1e-12 and not rs.any()
+    rng = np.random.default_rng(0)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    RT = np.eye(4)
+    RT[:3, :3] = q
+    RT[:3, 3] = [0.5, -2.0, 1.5]
+    rs, rd = O.camera_rays(8, 10, 8, 10, 7.0, 7.0, 5.0, 4.0, RT)
+    assert np.abs(rd[4 * 10 + 5] - q[:, 2] / (1 + 1e-8)).max() < 1e-12
+    assert np.abs(np.linalg.norm(rd, axis=1) - 1).max() < 1e-7
+    assert np.array_equal(rs, np.broadcast_to(RT[:3, 3], rs.shape))
+
+
+def test_camera_rays_match_the_scene_cameras():
+    """Against the scenes' own look-at camera (an independent float64
+    restatement: R [(u - cx)/f, (v - cy)/f, 1], normalised)."""
+    from nsvf_amd.rays import ViewCamera
+    cam_pos = S.sphere_camera(2.5, 11)
+    rs0, rd0 = S.camera_rays(40, 30, cam_pos)
+    vc = ViewCamera.looking_at(cam_pos, 30, 40)
+    fx, fy, cx, cy = vc.intrinsics[0, 0], vc.intrinsics[1, 1], vc.intrinsics[0, 2], vc.intrinsics[1, 2]
+    rs, rd = O.camera_rays(30, 40, 30, 40, fx, fy, cx, cy, vc.inv_RT)
+    assert np.abs(rd - rd0).max() < 1e-6 and np.abs(rs - rs0).max() < 1e-6
