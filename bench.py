@@ -167,6 +167,19 @@ def main():
     torch.cuda.synchronize()
     cam_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
 
+    # the same view with train-set voxel statistics on (both passes fold
+    # their sample probabilities into a per-voxel running max)
+    rnd.track = torch.zeros(sc.num_voxels, dtype=torch.float32, device=dev)
+    rnd.render(dsc, rs, rd, width=args.res)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        rnd.render(dsc, rs, rd, width=args.res)
+    torch.cuda.synchronize()
+    track_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
+    tracked_voxels = int((rnd.track > 0).sum())
+    rnd.track = None
+
     # per-stage breakdown of one traced view (outside the timed region)
     rnd.trace = []
     rnd.render(dsc, rs, rd, width=args.res)
@@ -195,6 +208,8 @@ def main():
         "kernels": kernels,
         "from_camera": {"ms_per_view": round(cam_ms, 3),
                         "note": "rays generated inside the traversal kernel from the camera"},
+        "with_voxel_stats": {"ms_per_view": round(track_ms, 3), "voxels_seen": tracked_voxels,
+                             "note": "track_voxel_probs after both passes"},
     }
 
     # ---- plain FP32 path (the comparator)

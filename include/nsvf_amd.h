@@ -366,6 +366,20 @@ int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev,
                           const float* field, float* prob_out, float* ray_rgb,
                           float* ray_depth, float* ray_missed, void* stream);
 
+/* Train-set voxel statistics (encoder.py:594-603 track_voxel_probs, called
+ * at renderer.py:250-251): max_voxel_probs[v] = max(max_voxel_probs[v], max
+ * over the hit rays of the sum of the ray's sample probabilities in voxel
+ * v), probs slot-aligned with the stream (nsvf_render_composite's
+ * prob_out); -1 samples ignored.  Sums run left to right in sample order
+ * (the reference's scatter_add order is unspecified).  max_voxel_probs
+ * [N] on the device, values >= 0; pruning keeps voxels above th
+ * (encoder.py:614-615).                                                    */
+int nsvf_track_voxel_probs(const int32_t* hit_list, const int32_t* n_hit_dev,
+                           int64_t n_hit_max, const int32_t* off,
+                           const int32_t* cnt, const void* samples,
+                           const float* probs, float* max_voxel_probs,
+                           int64_t N, void* stream);
+
 /* Backward of nsvf_render_composite (the autograd of renderer.py:119-121,
  * 195-218): from the gradients of the ray outputs (grad_rgb [R,3],
  * grad_depth [R], grad_missed [R], indexed by ray id) to grad_field [M,4]

@@ -66,6 +66,7 @@ SIGNATURES = {
     "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P], _I),
     "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_composite_backward": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_track_voxel_probs": ([_P, _P, _L, _P, _P, _P, _P, _P, _L, _P], _I),
     "nsvf_render_finish": ([_L, _P, _P, _P, _P, _F, _F, _F, _F, _P, _P, _P, _P], _I),
     "nsvf_plain_aabb_intersect": ([_P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),
     "nsvf_plain_svo_intersect": ([_P, _P, _P, _P, _I, _I, _I, _F, _I, _P, _P, _P, _P], _I),

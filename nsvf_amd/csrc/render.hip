@@ -514,6 +514,103 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+// Per-voxel maximum over rays of a ray's summed sample probabilities
+// (encoder.py:594-603 track_voxel_probs: scatter_add of p over each ray's
+// samples into voxel columns, then max over rays, then a running max; -1
+// samples ignored).  One wavefront per hit ray.  A ray crosses a convex
+// voxel once and its samples are depth-ordered, so a voxel's samples are
+// one contiguous run: lane j sums run j left to right.  If an id comes back
+// later in the ray (the compat-mode sample with the next ray's id), the
+// lane of its first run sums every sample of that id to the end of the ray,
+// again left to right.  Sums are >= 0, so an integer max on their bits
+// orders them.
+constexpr int kTrackRuns = 256;
+
+__global__ __launch_bounds__(kBlock) void track_probs_kernel(
+    const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
+    const int32_t* __restrict__ off_arr, const int32_t* __restrict__ cnt_arr,
+    const SampleRec* __restrict__ samples, const float* __restrict__ probs,
+    float* __restrict__ vmax, int64_t N) {
+  __shared__ int32_t run_start[kSampleWaves][kTrackRuns + 1];
+  __shared__ int32_t run_id[kSampleWaves][kTrackRuns];
+  __shared__ int32_t run_dup[kSampleWaves][kTrackRuns];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t NH = *n_hit_dev;
+  for (int64_t n = int64_t(blockIdx.x) * kSampleWaves + wave; n < NH;
+       n += int64_t(gridDim.x) * kSampleWaves) {
+    const int32_t r = hit_list[n];
+    const SampleRec* rec = samples + off_arr[r];
+    const float* pr = probs + off_arr[r];
+    const int c = __builtin_amdgcn_readfirstlane(cnt_arr[r]);
+    // run heads
+    int nruns = 0;
+    int32_t prev = INT32_MIN;  // no sample has this id
+    for (int k0 = 0; k0 < c; k0 += 64) {
+      const int k = k0 + lane;
+      const int32_t id = k < c ? rec[k].idx : INT32_MIN;
+      int32_t idp = __shfl_up(id, 1, 64);
+      if (lane == 0) idp = prev;
+      const bool head = k < c && id != idp;
+      const uint64_t hm = __ballot(head);
+      const int pos = nruns + __popcll(hm & ((1ull << lane) - 1));
+      if (head && pos < kTrackRuns) {
+        run_start[wave][pos] = k;
+        run_id[wave][pos] = id;
+      }
+      nruns += __popcll(hm);
+      prev = __shfl(id, 63, 64);
+    }
+    if (nruns > kTrackRuns) {
+      // more runs than LDS holds (not met on rendered rays): lane 0 does the
+      // whole ray, first occurrence by first occurrence
+      if (lane == 0) {
+        for (int s0 = 0; s0 < c; ++s0) {
+          const int32_t id = rec[s0].idx;
+          if (id < 0 || id >= N) continue;
+          bool seen = false;
+          for (int t = 0; t < s0 && !seen; ++t) seen = rec[t].idx == id;
+          if (seen) continue;
+          float acc = 0.0f;
+          for (int s = s0; s < c; ++s)
+            if (rec[s].idx == id) acc = acc + pr[s];
+          atomicMax(reinterpret_cast<int*>(vmax + id), __float_as_int(acc));
+        }
+      }
+      wave_sync();
+      continue;
+    }
+    if (lane == 0) run_start[wave][nruns] = c;
+    wave_sync();
+    // runs whose id an earlier run already has
+    bool any_dup = false;
+    for (int j0 = 0; j0 < nruns; j0 += 64) {
+      const int j = j0 + lane;
+      bool dup = false;
+      if (j < nruns) {
+        const int32_t idj = run_id[wave][j];
+        for (int i = 0; i < j && !dup; ++i) dup = run_id[wave][i] == idj;
+        run_dup[wave][j] = dup ? 1 : 0;
+      }
+      any_dup = any_dup || __ballot(dup) != 0;
+    }
+    wave_sync();
+    for (int j0 = 0; j0 < nruns; j0 += 64) {
+      const int j = j0 + lane;
+      if (j < nruns && !run_dup[wave][j]) {
+        const int32_t id = run_id[wave][j];
+        if (id >= 0 && id < N) {
+          float acc = 0.0f;
+          const int e = any_dup ? c : run_start[wave][j + 1];
+          for (int s = run_start[wave][j]; s < e; ++s)
+            if (!any_dup || rec[s].idx == id) acc = acc + pr[s];
+          atomicMax(reinterpret_cast<int*>(vmax + id), __float_as_int(acc));
+        }
+      }
+    }
+    wave_sync();  // the next ray's run lists must not overtake this ray's reads
+  }
+}
+
 // One wavefront per hit ray, 64 samples per step.  Free energy
 // fe = relu(sigma) * dist * 7 (renderer.py:119-121); transmittance before a
 // sample is exp(-(sum of the earlier fe)), an exclusive prefix sum across the
@@ -805,6 +902,21 @@ int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev, int
                      static_cast<const SampleRec*>(samples), reinterpret_cast<const float4*>(field),
                      prob_out, ray_rgb, ray_depth, ray_missed);
   return check_launch("render_composite");
+}
+
+int nsvf_track_voxel_probs(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
+                           const int32_t* off, const int32_t* cnt, const void* samples,
+                           const float* probs, float* max_voxel_probs, int64_t N, void* stream) {
+  clear_error();
+  if (n_hit_max <= 0) return NSVF_OK;
+  if (N <= 0) return set_error(NSVF_EINVAL, "track_voxel_probs: no voxels");
+  if (!hit_list || !n_hit_dev || !off || !cnt || !samples || !probs || !max_voxel_probs)
+    return set_error(NSVF_EINVAL, "track_voxel_probs: null pointer");
+  hipLaunchKernelGGL(track_probs_kernel,
+                     dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, off, cnt,
+                     static_cast<const SampleRec*>(samples), probs, max_voxel_probs, N);
+  return check_launch("track_voxel_probs");
 }
 
 int nsvf_render_composite_backward(const int32_t* hit_list, const int32_t* n_hit_dev,

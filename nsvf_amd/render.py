@@ -196,6 +196,9 @@ class Renderer:
         self.last = None
         self.trace = None   # optional list of (stage, event) marks
         self._kernel_events = None  # optional list of (kernel, units, start, end)
+        # optional float32 [num_voxels] running max of per-ray voxel
+        # probabilities, updated by every pass (encoder.py:594-603)
+        self.track = None
 
     def record_kernels(self, on=True):
         """Start (or stop) recording events around the field-kernel launches;
@@ -219,6 +222,18 @@ class Renderer:
     def _stream(self, name, m):
         """m sample records {idx, depth, dist, ray} as an int32 [m, 4] buffer."""
         return self._buf(name, m * 4, torch.int32).view(m, 4)
+
+    def _track(self, sc, hit_list, scal, n_hit, off, cnt, samples, probs, st):
+        """Folds a pass's sample probabilities into self.track (renderer.py:
+        250-251 calls track_voxel_probs after each pass when tracking)."""
+        if self.track is None or n_hit <= 0:
+            return
+        t = self.track
+        if t.dtype != torch.float32 or t.device != self.device or t.numel() != sc.scene.num_voxels:
+            raise ValueError("track must be a float32 [num_voxels] tensor on the renderer's device")
+        check(lib().nsvf_track_voxel_probs(_p(hit_list), _p(scal[0:1]), n_hit, _p(off), _p(cnt),
+                                           _p(samples), _p(probs), _p(t), t.numel(), st),
+              "track_voxel_probs")
 
     def _field(self, sc, ray_start, ray_dir, samples, m_dev, m, out):
         L = lib()
@@ -463,6 +478,7 @@ class Renderer:
         check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), max(n_hit, 1), _p(c_off),
                                       _p(c_cnt), _p(cs), _p(c_field), _p(c_prob), _p(ray_rgb),
                                       _p(ray_depth), _p(ray_missed), st), "composite coarse")
+        self._track(sc, hit_list, scal, n_hit, c_off, c_cnt, cs, c_prob, st)
         self._mark("coarse_composite")
         if self.hierarchical and n_hit > 0:
             f_off = self._buf("f_off", R, torch.int32)
@@ -477,9 +493,11 @@ class Renderer:
             f_field = self._buf("f_field", slots_f * 4, torch.float32)
             self._field(sc, ray_start, ray_dir, fsm, scal[3:4], slots_f, f_field)
             self._mark("fine_field")
+            f_prob = self._buf("f_prob", slots_f, torch.float32) if self.track is not None else None
             check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), n_hit, _p(f_off), _p(f_cnt),
-                                          _p(fsm), _p(f_field), None, _p(ray_rgb), _p(ray_depth),
-                                          _p(ray_missed), st), "composite fine")
+                                          _p(fsm), _p(f_field), _p(f_prob), _p(ray_rgb),
+                                          _p(ray_depth), _p(ray_missed), st), "composite fine")
+            self._track(sc, hit_list, scal, n_hit, f_off, f_cnt, fsm, f_prob, st)
             self._mark("fine_composite")
         # the outputs leave the workspace: the caller owns them
         out = self._buf("out", R * 5, torch.float32)
@@ -503,6 +521,8 @@ class Renderer:
             if self.hierarchical and n_hit > 0:
                 self.kept.update({"f_cnt": f_cnt[hl], "f_off": f_off[hl],
                                   "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4)})
+                if self.track is not None:
+                    self.kept["f_prob"] = f_prob
         return rgb, depth, missed
 
     @torch.no_grad()

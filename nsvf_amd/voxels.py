@@ -53,9 +53,17 @@ def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536):
     return out
 
 
-def pruning(sc: DeviceScene, th=0.5, precision="fp32", bits=16):
-    """keep [N] bool: 1 - min(exp(-relu sigma)) > th (encoder.py:606-611),
-    computed from the max as 1 - exp(-max) > th (exp is monotone)."""
+def pruning(sc: DeviceScene, th=0.5, precision="fp32", bits=16, train_stats=False,
+            max_voxel_probs=None):
+    """keep [N] bool (encoder.py:605-615).  From the field: 1 - min(exp(-relu
+    sigma)) > th over each voxel's lattice, computed from the max as
+    1 - exp(-max) > th (exp is monotone).  With train_stats: max_voxel_probs
+    > th, the statistics a Renderer gathers in its ``track`` tensor while
+    rendering the training views (encoder.py:594-603)."""
+    if train_stats:
+        if max_voxel_probs is None or max_voxel_probs.numel() != sc.scene.num_voxels:
+            raise ValueError("train_stats pruning needs max_voxel_probs [num_voxels]")
+        return max_voxel_probs > th
     vmax = prune_scores(sc, precision, bits)
     return (1 - torch.exp(-vmax)) > th
 

@@ -512,6 +512,26 @@ def trilinear_backward(sample_idx, xyz, centers, feats, num_keys, grad_out, voxe
     return out
 
 
+def track_voxel_probs(idx, probs, num_voxels, prev=None):
+    """encoder.py:594-603: per ray, the sum of the sample probabilities in
+    each voxel (scatter_add into voxel columns; -1 samples go to a dropped
+    column), then the max over rays, then the max with the running values.
+    Sums run left to right in sample order, in float32."""
+    out = np.zeros(num_voxels, np.float32) if prev is None else np.array(prev, np.float32)
+    idx = np.asarray(idx)
+    probs = np.asarray(probs, np.float32)
+    for r in range(idx.shape[0]):
+        sums = {}
+        for s in range(idx.shape[1]):
+            v = int(idx[r, s])
+            if v < 0:
+                continue
+            sums[v] = np.float32(sums.get(v, np.float32(0)) + probs[r, s])
+        for v, x in sums.items():
+            out[v] = max(out[v], x)
+    return out
+
+
 def camera_uv(H, W, h, w):
     """data_utils.py:57-65 ``get_uv``: [2, h, w] float32 pixel coordinates,
     u = column * float32(W / w), v = row * float32(H / h)."""

@@ -660,3 +660,35 @@ def test_camera_rays_match_the_scene_cameras():
     fx, fy, cx, cy = vc.intrinsics[0, 0], vc.intrinsics[1, 1], vc.intrinsics[0, 2], vc.intrinsics[1, 2]
     rs, rd = O.camera_rays(30, 40, 30, 40, fx, fy, cx, cy, vc.inv_RT)
     assert np.abs(rd - rd0).max() < 1e-6 and np.abs(rs - rs0).max() < 1e-6
+
+
+# ---- train-set voxel statistics (encoder.py:594-603)
+
+def test_track_voxel_probs_closed_form():
+    """A voxel's samples on one ray add up (also when split into two runs),
+    rays take the max, -1 samples are dropped, and the running values only
+    grow."""
+    idx = np.array([[3, 3, 5, 3, -1], [5, 1, -1, -1, -1]])
+    p = np.array([[0.1, 0.2, 0.4, 0.3, 0.9], [0.5, 0.05, 0.7, 0.0, 0.0]], np.float32)
+    out = O.track_voxel_probs(idx, p, 6)
+    s3 = np.float32(np.float32(np.float32(0.1) + np.float32(0.2)) + np.float32(0.3))
+    assert out[3] == s3 and out[5] == np.float32(0.5) and out[1] == np.float32(0.05)
+    assert out[0] == 0 and out[2] == 0 and out[4] == 0
+    prev = np.array([0, 0.01, 0, 0.7, 0, 0.2], np.float32)
+    out2 = O.track_voxel_probs(idx, p, 6, prev)
+    assert out2[3] == np.float32(0.7) and out2[5] == np.float32(0.5) and out2[1] == np.float32(0.05)
+
+
+def test_track_voxel_probs_matches_scatter_add():
+    """Against an independent form: numpy's unbuffered in-order add.at per
+    ray into N + 1 columns (the reference's dropped -1 column), then max."""
+    rng = np.random.default_rng(3)
+    N, R, S = 40, 30, 25
+    idx = rng.integers(-1, N, (R, S))
+    p = rng.random((R, S)).astype(np.float32)
+    ref = np.zeros(N, np.float32)
+    for r in range(R):
+        col = np.zeros(N + 1, np.float32)
+        np.add.at(col, np.where(idx[r] < 0, N, idx[r]), p[r])
+        ref = np.maximum(ref, col[:N])
+    assert np.array_equal(O.track_voxel_probs(idx, p, N), ref)
