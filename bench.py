@@ -360,6 +360,26 @@ def rows(args, dev, sc, dsc):
     torch.cuda.synchronize()
     res["split"] = {"ms_including_transfers": round((time.perf_counter() - t0) * 1e3, 1),
                     "voxels": sc.num_voxels, "new_voxels": new.num_voxels, "new_keys": new.num_keys}
+
+    # scene files: the encoder's checkpoint buffers written and read back,
+    # then made resident (octree build, node records, upload)
+    import tempfile
+
+    from nsvf_amd import scene_io as IO
+    from nsvf_amd.render import DeviceScene
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "scene.pt")
+        t0 = time.perf_counter()
+        IO.save_checkpoint(sc, path)
+        t1 = time.perf_counter()
+        loaded = IO.load_checkpoint(path, mlp=sc.mlp)
+        t2 = time.perf_counter()
+        DeviceScene(loaded, dev)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+    res["scene_files"] = {"ms": round((t3 - t1) * 1e3, 1), "save_ms": round((t1 - t0) * 1e3, 1),
+                          "load_ms": round((t2 - t1) * 1e3, 1),
+                          "resident_ms": round((t3 - t2) * 1e3, 1), "voxels": loaded.num_voxels}
     log("rows: " + ", ".join(f"{k} {v.get('ms', v.get('ms_including_transfers'))} ms"
                               for k, v in res.items()))
     return res
