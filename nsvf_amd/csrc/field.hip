@@ -642,6 +642,171 @@ __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_
   }
 }
 
+// Fused BF16 field of the render path: the split path's run-staged gather
+// and the MLP in one kernel, so the layer-0 operands never go through HBM.
+// A block of 8 waves shares one LDS copy of the weights.  Each wave takes 64
+// consecutive slots (one MLP tile), stages the corner rows of their runs of
+// equal voxel ids in its own LDS area, and each lane computes its own
+// operand-B elements -- features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3}
+// of samples 16j + col -- with the gather kernel's corner-order chain of
+// fused multiply-adds, so operands and field equal the split path's bit for
+// bit.
+constexpr int kFusedWaves = 8;
+constexpr int kFusedRuns = 6;   // voxels staged per round
+
+template <int D>
+constexpr size_t fused_wave_bytes() {
+  // rows (padded stride), per-sample weights, per-sample run, run voxels
+  return size_t(kFusedRuns * (2 * D + 1)) * 16 + 64 * 8 * 4 + 64 * 4 + 64 * 4;
+}
+
+template <int D, int H, int L>
+constexpr size_t fused_lds_bytes() {
+  using Dm = Dims<D, H, L>;
+  return ((size_t(Dm::FRAGS) * 1024 + size_t(Dm::BIASES) * 4 + 15) & ~size_t(15)) +
+         kFusedWaves * fused_wave_bytes<D>();
+}
+
+template <int D, int H, int L>
+__global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
+    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
+    int64_t m_max, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
+    float4* __restrict__ out) {
+  using Dm = Dims<D, H, L>;
+  static_assert(Dm::SAMPLES == 64, "one MLP tile per wave");
+  static_assert(weights_in_lds<D, H, L>(), "weights staged in LDS");
+  constexpr int NT = Dm::NT, KT0 = Dm::KT0, Q = D / 4;
+  constexpr int ROW4 = 2 * D, RUN_STRIDE = ROW4 + 1, RUNS = kFusedRuns;
+  constexpr int SLOTS = (RUNS * ROW4 + 63) / 64;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  bf16x8* lw = reinterpret_cast<bf16x8*>(smem_raw);
+  float* lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 1024);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  unsigned char* wb = smem_raw + ((size_t(Dm::FRAGS) * 1024 + size_t(Dm::BIASES) * 4 + 15) &
+                                  ~size_t(15)) + size_t(wave) * fused_wave_bytes<D>();
+  float4* rows = reinterpret_cast<float4*>(wb);
+  float* wts = reinterpret_cast<float*>(wb + size_t(RUNS * RUN_STRIDE) * 16);
+  int32_t* rix = reinterpret_cast<int32_t*>(wts + 64 * 8);
+  int32_t* run_vox = rix + 64;
+  for (int i = threadIdx.x; i < Dm::FRAGS * 64; i += 64 * kFusedWaves) lw[i] = wfrag[i];
+  for (int i = threadIdx.x; i < Dm::BIASES; i += 64 * kFusedWaves) lb[i] = bias[i];
+  __syncthreads();
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + 63) / 64;
+  for (int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave; t < tiles;
+       t += int64_t(gridDim.x) * kFusedWaves) {
+    const int64_t s = t * 64 + lane;
+    // the lane's sample: voxel, corner weights (the gather kernel's
+    // arithmetic), and its run of equal voxel ids
+    int32_t v = -1;
+    float w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = 0.0f;
+    if (s < M) {
+      const SampleRec rec = samples[s];
+      v = rec.idx;
+      if (v >= 0) {
+        float p[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const float x = rays.start[int64_t(rec.ray) * 3 + a] +
+                          rays.dir[int64_t(rec.ray) * 3 + a] * rec.depth;
+          const float d = x - sc.centers[int64_t(v) * 3 + a];
+          p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
+        }
+        const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+        const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
+      }
+    }
+    const int32_t vprev = __shfl_up(v, 1, 64);
+    const bool head = lane == 0 || v != vprev;
+    const uint64_t hmask = __ballot(head);
+    const int r = __popcll(hmask & ((2ull << lane) - 1)) - 1;
+    const int U = __popcll(hmask);
+    if (head) run_vox[r] = v;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) wts[lane * 8 + c] = w[c];
+    rix[lane] = v >= 0 ? r : -1;
+    wave_sync();
+    float f[NT][KT0][8];
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int kt = 0; kt < KT0; ++kt)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) f[j][kt][i] = 0.0f;
+    for (int u0 = 0; u0 < U; u0 += RUNS) {
+      const int nr = min(RUNS, U - u0);
+      // all staging loads in flight together (absent runs load a valid row)
+#pragma unroll
+      for (int tt = 0; tt < SLOTS; ++tt) {
+        const int i = lane + 64 * tt;
+        if (i < RUNS * ROW4) {
+          const int jr = i / ROW4, e = i - jr * ROW4;
+          const int32_t vr = run_vox[(u0 + jr) & 63];
+          const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
+          const int32_t k = sc.feats[int64_t(vu) * 8 + e / Q];
+          rows[jr * RUN_STRIDE + e] = reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[e % Q];
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int qs = 16 * j + col;
+        const int rq = rix[qs];
+        if (rq >= u0 && rq < u0 + nr) {
+          float wq[8];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) wq[c] = wts[qs * 8 + c];
+          const float4* rw = rows + (rq - u0) * RUN_STRIDE;
+#pragma unroll
+          for (int kt = 0; kt < KT0; ++kt)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+              const int quad = 8 * kt + 4 * h + g;
+              if (quad < Q) {
+                float4 e[8];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) e[c] = rw[c * Q + quad];
+                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                  a0 = __fmaf_rn(wq[c], e[c].x, a0);
+                  a1 = __fmaf_rn(wq[c], e[c].y, a1);
+                  a2 = __fmaf_rn(wq[c], e[c].z, a2);
+                  a3 = __fmaf_rn(wq[c], e[c].w, a3);
+                }
+                f[j][kt][4 * h + 0] = a0;
+                f[j][kt][4 * h + 1] = a1;
+                f[j][kt][4 * h + 2] = a2;
+                f[j][kt][4 * h + 3] = a3;
+              }
+            }
+        }
+      }
+      wave_sync();
+    }
+    OperandTile<D, H, L> b0;
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int kt = 0; kt < KT0; ++kt) {
+        const float* ff = f[j][kt];
+        const bf16x2 p0 = __builtin_convertvector(f32x2{ff[0], ff[1]}, bf16x2);
+        const bf16x2 p1 = __builtin_convertvector(f32x2{ff[2], ff[3]}, bf16x2);
+        const bf16x2 p2 = __builtin_convertvector(f32x2{ff[4], ff[5]}, bf16x2);
+        const bf16x2 p3 = __builtin_convertvector(f32x2{ff[6], ff[7]}, bf16x2);
+        const u32x4 u = {__builtin_bit_cast(unsigned int, p0), __builtin_bit_cast(unsigned int, p1),
+                         __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
+        b0[kt][j] = __builtin_bit_cast(bf16x8, u);
+      }
+    StoreSink sink{out};
+    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * 64, M, sink);
+  }
+}
+
 // FP32 variant: same structure on v_mfma_f32_16x16x4_f32, weights read from
 // global memory (fragments of 64 floats).
 template <int D, int H, int L, class Src, class Sink>
@@ -806,16 +971,16 @@ int64_t bf16_workspace_bytes(int64_t m_max) {
   return ceil_div(m_max, Dm::SAMPLES) * Dm::NT * Dm::KT0 * 64 * 16;
 }
 
-// Blocks of `kernel` (256 threads, static LDS only) that the current device
-// holds at once: its occupancy per CU times the CU count.
+// Blocks of `kernel` (`threads` per block, `lds` bytes of dynamic LDS) that
+// the current device holds at once: its occupancy per CU times the CU count.
 template <class K>
-int64_t resident_blocks(K kernel) {
+int64_t resident_blocks(K kernel, int threads = 256, size_t lds = 0) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess ||
       cus <= 0 || per_cu <= 0)
-    return 1024;
+    return 256;
   return int64_t(cus) * per_cu;
 }
 
@@ -856,6 +1021,13 @@ int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
   return launch_mlp_bf16<D, H, L>(m_dev, m_max, wfrag, bias, out, workspace, stream);
 }
 
+// The fused BF16 field over a sample stream: the run-staged gather + MLP
+// kernel where the weights fit in LDS, else the per-lane gather kernel.
+template <int D, int H, int L>
+int launch_bf16_fused(FieldScene sc, FieldRays rays, const SampleRec* samples,
+                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                      float* out, hipStream_t stream);
+
 template <int D, int H, int L, class Src, class Sink>
 int launch_bf16(const Src& src, const int32_t* m_dev, int64_t m_max, const void* wfrag,
                 const float* bias, const Sink& sink, hipStream_t stream) {
@@ -880,6 +1052,27 @@ int launch_f32(const Src& src, const int32_t* m_dev, int64_t m_max, const void* 
                      dim3(kFieldBlock), 0, stream, src, m_dev, m_max,
                      static_cast<const float*>(wfrag), bias, sink);
   return check_launch("field_f32");
+}
+
+template <int D, int H, int L>
+int launch_bf16_fused(FieldScene sc, FieldRays rays, const SampleRec* samples,
+                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                      float* out, hipStream_t stream) {
+  if constexpr (weights_in_lds<D, H, L>() && Dims<D, H, L>::SAMPLES == 64) {
+    constexpr size_t lds = fused_lds_bytes<D, H, L>();
+    static const int64_t resident =
+        resident_blocks(field_fused_kernel<D, H, L>, 64 * kFusedWaves, lds);
+    const int64_t tiles = ceil_div(m_max, 64);
+    const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles, kFusedWaves), resident));
+    hipLaunchKernelGGL((field_fused_kernel<D, H, L>), dim3(unsigned(blocks)),
+                       dim3(64 * kFusedWaves), lds, stream, sc, rays, samples, m_dev, m_max,
+                       static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out));
+    return check_launch("field_fused_bf16");
+  } else {
+    const StreamSource src{sc, rays, samples};
+    const StoreSink sink{reinterpret_cast<float4*>(out)};
+    return launch_bf16<D, H, L>(src, m_dev, m_max, wfrag, bias, sink, stream);
+  }
 }
 
 __global__ __launch_bounds__(256) void fill_zero_kernel(float* __restrict__ p, int64_t n) {
@@ -947,7 +1140,8 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
       return launch_bf16_split<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
                                         m_dev, m_max, wfrag, bias, out, workspace, st);        \
     if (precision == 0)                                                                        \
-      return launch_bf16<d, h, l>(src, m_dev, m_max, wfrag, bias, sink, st);                   \
+      return launch_bf16_fused<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
+                                        m_dev, m_max, wfrag, bias, out, st);                   \
     return launch_f32<d, h, l>(src, m_dev, m_max, wfrag, bias, sink, st);                      \
   }
   NSVF_FIELD_CASE(32, 128, 3)

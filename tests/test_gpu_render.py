@@ -214,20 +214,25 @@ def test_plain_path_matches_fused(setup, device):
 
 
 def test_bf16_split_and_fused_field_agree(setup, device):
-    """The split (gather kernel + MLP kernel) and fused BF16 field paths give
-    the same sample outputs up to FMA-vs-separate rounding of the gather."""
+    """The split (gather kernel + MLP kernel) and fused BF16 field paths run
+    the same gather arithmetic (corner-order fused multiply-adds) and the
+    same MLP: their field outputs and images agree bit for bit."""
     sc, rs, rd, dsc, ref = setup
-    outs = []
+    fields, images = [], []
     for split in (True, False):
         r = Renderer(device, precision="bf16", split_field=split)
-        r.render(dsc, torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device), keep=True)
+        img = r.render(dsc, torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device),
+                       keep=True)
         torch.cuda.synchronize()
         k = r.kept
         c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
         W = ref["c_idx"].shape[1]
         f = N(k["c_field"])
-        outs.append(csr_to_padded(c_cnt, c_off, f[:, 0], W, np.float32(0)))
-    assert np.abs(outs[0] - outs[1]).max() < 1e-5
+        fields.append(csr_to_padded(c_cnt, c_off, f[:, 0], W, np.float32(0)))
+        images.append([N(t) for t in img])
+    assert np.array_equal(fields[0], fields[1])
+    for a, b in zip(*images):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("n_max", [60, 7])
