@@ -41,8 +41,9 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--camera-seed", type=int, default=11)
-    ap.add_argument("--field", choices=["split", "fused"], default="split",
-                    help="BF16 field as a gather kernel + an MLP kernel, or one fused kernel")
+    ap.add_argument("--field", choices=["auto", "split", "fused"], default="auto",
+                    help="BF16 field as a gather kernel + an MLP kernel, one fused kernel, or "
+                         "fused where the fused kernel supports the field's shape")
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests per arrival rate in the serving measurement (0 = skip)")
     ap.add_argument("--serve-rates", default="10,20,40",
@@ -103,7 +104,7 @@ def main():
 
     # ---- fused BF16 path
     rnd = Renderer(dev, precision="bf16", hierarchical=True, fine_steps=64.0,
-                   split_field=args.field == "split")
+                   split_field={"auto": None, "split": True, "fused": False}[args.field])
     for _ in range(args.warmup):
         rnd.render(dsc, rs, rd, width=args.res)
     torch.cuda.synchronize()
@@ -133,12 +134,13 @@ def main():
         f"coarse {n_coarse} samples ({st.coarse_slots} slots), fine {n_fine} samples "
         f"({st.fine_slots} slots allocated)")
 
-    # roofline of the dominant kernel: the BF16 MLP kernel (MFMA-bound), with
-    # algorithmic FLOPs of the samples it evaluates / its HIP-event time; the
-    # gather kernel feeding it reported beside it (HBM/L2-bound)
+    # roofline of the dominant kernel: the BF16 field kernel (MFMA-bound) --
+    # the fused gather + MLP kernel, or the MLP kernel of the split path --
+    # with the MLP's algorithmic FLOPs of the samples it evaluates / its
+    # HIP-event time; a split path's gather kernel reported beside it
     flops_per_sample = sc.mlp.flops_per_sample()
     n_views = args.steps
-    split = args.field == "split"
+    split = rnd.uses_split_field(dsc)
     mlp_ev = [(m, a, b) for k, m, a, b in kev if k == ("mlp" if split else "field")]
     emb_ev = [(m, a, b) for k, m, a, b in kev if k == "embed"]
     mlp_ms = sum(a.elapsed_time(b) for _, a, b in mlp_ev)

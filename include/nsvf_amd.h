@@ -329,9 +329,13 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev,
  *   out [M,4] = (sigma, sigmoid(r), sigmoid(g), sigmoid(b)).
  *   workspace (device, optional): with at least
  *   nsvf_field_workspace_bytes(...) bytes, BF16 runs as a gather kernel
- *   writing operand tiles plus an MLP kernel; with NULL, as one fused kernel.
+ *   writing operand tiles plus an MLP kernel; with NULL, as one fused kernel
+ *   (the run-staged gather and the MLP in one kernel where
+ *   nsvf_field_fused_supported(D, H, L), else a per-lane gather).  Both
+ *   give the same bits.
  *   Supported shapes: nsvf_field_supported(D, H, L).                        */
 int nsvf_field_supported(int D, int H, int L);
+int nsvf_field_fused_supported(int D, int H, int L);
 int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L,
                                    int64_t m_max);
 int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,

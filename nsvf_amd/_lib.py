@@ -59,6 +59,7 @@ SIGNATURES = {
                                    _P, _P], _I),
     "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P, _P], _I),
     "nsvf_field_supported": ([_I, _I, _I], _I),
+    "nsvf_field_fused_supported": ([_I, _I, _I], _I),
     "nsvf_field_workspace_bytes": ([_I, _I, _I, _I, _L], _L),
     "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _P, _P, _P, _L,
                          _P], _I),

@@ -285,11 +285,33 @@ constexpr bool weights_in_lds() {
 template <int D, int H, int L>
 using OperandTile = bf16x8[Dims<D, H, L>::KT0][Dims<D, H, L>::NT];
 
+// Work a caller interleaves with the MLP's layers (hook->layer_done(l)
+// after layer l, hook->before_head() before the head); none by default.
+struct NoHook {
+  __device__ __forceinline__ void layer_done(int) {}
+  __device__ __forceinline__ void before_head() {}
+};
+
+template <int D, int H, int L, class Sink, class Hook>
+__device__ __forceinline__ void mlp_tile_hooked(OperandTile<D, H, L>& b0, const bf16x8* lw,
+                                                const float* lb, int lane, int g, int col,
+                                                int64_t base, int64_t M, Sink& sink,
+                                                const bf16x8* __restrict__ next_b0, Hook& hook);
+
 template <int D, int H, int L, class Sink>
 __device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
                                          const bf16x8* lw, const float* lb, int lane, int g,
                                          int col, int64_t base, int64_t M, Sink& sink,
                                          const bf16x8* __restrict__ next_b0 = nullptr) {
+  NoHook none;
+  mlp_tile_hooked<D, H, L>(b0, lw, lb, lane, g, col, base, M, sink, next_b0, none);
+}
+
+template <int D, int H, int L, class Sink, class Hook>
+__device__ __forceinline__ void mlp_tile_hooked(OperandTile<D, H, L>& b0, const bf16x8* lw,
+                                                const float* lb, int lane, int g, int col,
+                                                int64_t base, int64_t M, Sink& sink,
+                                                const bf16x8* __restrict__ next_b0, Hook& hook) {
   using Dm = Dims<D, H, L>;
   constexpr int NT = Dm::NT;
   // Weight fragments (and biases) are read one fragment ahead of the MFMAs
@@ -324,6 +346,7 @@ __device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
       for (int j = 0; j < NT; ++j) bh[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
     }
   }
+  hook.layer_done(0);
   // ---- hidden layers 1..L-1
 #pragma unroll 1
   for (int l = 1; l < L; ++l) {
@@ -359,6 +382,7 @@ __device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
     for (int kt = 0; kt < Dm::KTH; ++kt)
 #pragma unroll
       for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
+    hook.layer_done(l);
   }
   // b0 is dead since layer 0 and the hidden layers' double buffer now too:
   // the next tile's operands load into b0 while the head and the stores run
@@ -368,6 +392,7 @@ __device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
 #pragma unroll
       for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = next_b0[(j * Dm::KT0 + kt) * 64 + lane];
   }
+  hook.before_head();
   // ---- head (4 outputs in rows 0..3 of one m-tile)
   {
     const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
@@ -667,17 +692,132 @@ constexpr size_t fused_lds_bytes() {
          kFusedWaves * fused_wave_bytes<D>();
 }
 
+// The corner weights of a sample at depth z on ray (o, d) in voxel c (the
+// gather kernel's arithmetic: x = o + d z, p = (x - c) / vs + 0.5).
+__device__ __forceinline__ void fused_weights(const FieldScene& sc, const float o[3],
+                                              const float d[3], const float c[3], float z,
+                                              float w[8]) {
+  float p[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float x = o[a] + d[a] * z;
+    const float e = x - c[a];
+    p[a] = (sc.voxel_pow2 ? e * sc.inv_voxel : e / sc.voxel_size) + 0.5f;
+  }
+  const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
+  const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
+}
+
+// The next tile's gather, in stages around the current tile's MLP so that
+// each stage's loads land while matrix work runs and nothing it holds stays
+// live across the hidden layers: the tile's runs, ray / centre loads and the
+// corner keys of its first staging round before the MLP (from sample
+// records loaded one tile earlier), its corner rows straight into LDS (a
+// run's rows are one 64-lane load when D = 32, so they take no registers)
+// and its weights into LDS after layer 0, and the records of the tile after
+// it before the head.
+template <int D, int H, int L>
+struct FusedPrefetch {
+  static constexpr int ROW4 = 2 * D, RUN_STRIDE = ROW4 + 1, RUNS = kFusedRuns, Q = D / 4;
+  static constexpr int SLOTS = (RUNS * ROW4 + 63) / 64;
+  static_assert(ROW4 == 64, "one run per 64-lane load");
+  FieldScene sc;
+  FieldRays rays;
+  const SampleRec* samples;
+  float4* rows;
+  float* wts;
+  int32_t* rix;
+  int32_t* run_vox;
+  int64_t M, tiles;
+  int lane;
+  int64_t rtile = -1;  // the tile whose records are in rec
+  int64_t tile = -1;   // the tile being staged, -1 for none
+  SampleRec rec{-1, 0.0f, 0.0f, 0};
+  int32_t v = -1;
+  int U = 1;
+  float z = 0.0f;
+  float ro[3] = {0.f, 0.f, 0.f}, rd[3] = {0.f, 0.f, 0.f}, ctr[3] = {0.f, 0.f, 0.f};
+  int32_t key[SLOTS];
+
+  __device__ __forceinline__ void issue_rec(int64_t t) {
+    rtile = t;
+    rec = SampleRec{-1, 0.0f, 0.0f, 0};
+    const int64_t s = t * 64 + lane;
+    if (t < tiles && s < M) rec = samples[s];
+  }
+  // runs into LDS; ray / centre and corner-key loads issued
+  __device__ __forceinline__ void stage_a() {
+    tile = rtile < tiles ? rtile : -1;
+    if (tile < 0) return;
+    v = rec.idx;
+    z = rec.depth;
+    if (v >= 0) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        ro[a] = rays.start[int64_t(rec.ray) * 3 + a];
+        rd[a] = rays.dir[int64_t(rec.ray) * 3 + a];
+        ctr[a] = sc.centers[int64_t(v) * 3 + a];
+      }
+    }
+    const int32_t vprev = __shfl_up(v, 1, 64);
+    const bool head = lane == 0 || v != vprev;
+    const uint64_t hmask = __ballot(head);
+    const int r = __popcll(hmask & ((2ull << lane) - 1)) - 1;
+    U = __popcll(hmask);
+    if (head) run_vox[r] = v;
+    rix[lane] = v >= 0 ? r : -1;
+    wave_sync();
+#pragma unroll
+    for (int tt = 0; tt < SLOTS; ++tt) {
+      const int32_t vr = run_vox[tt];
+      const int32_t vu = (tt < U && vr >= 0) ? vr : 0;
+      key[tt] = sc.feats[int64_t(vu) * 8 + lane / Q];
+    }
+  }
+  // the first round's rows straight into LDS (lane e of load tt: element e
+  // of run tt at rows + tt * RUN_STRIDE + e); the weights into LDS
+  __device__ __forceinline__ void stage_b() {
+    if (tile < 0) return;
+#pragma unroll
+    for (int tt = 0; tt < SLOTS; ++tt) {
+      const float4* src =
+          reinterpret_cast<const float4*>(sc.values + int64_t(key[tt]) * D) + lane % Q;
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(src),
+          (__attribute__((address_space(3))) void*)(rows + tt * RUN_STRIDE), 16, 0, 0);
+    }
+    float w[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) w[c] = 0.0f;
+    if (v >= 0) fused_weights(sc, ro, rd, ctr, z, w);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) wts[lane * 8 + c] = w[c];
+  }
+  // the direct row loads have landed
+  __device__ __forceinline__ void rows_ready() { __builtin_amdgcn_s_waitcnt(0); }
+
+  int64_t next_rec = -1;  // the records to load before the head
+  __device__ __forceinline__ void layer_done(int l) {
+    if (l == 0) stage_b();
+  }
+  __device__ __forceinline__ void before_head() {
+    if (next_rec >= 0) issue_rec(next_rec);
+  }
+};
+
 template <int D, int H, int L>
 __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
     FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
     int64_t m_max, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
     float4* __restrict__ out) {
   using Dm = Dims<D, H, L>;
+  using PF = FusedPrefetch<D, H, L>;
   static_assert(Dm::SAMPLES == 64, "one MLP tile per wave");
   static_assert(weights_in_lds<D, H, L>(), "weights staged in LDS");
   constexpr int NT = Dm::NT, KT0 = Dm::KT0, Q = D / 4;
-  constexpr int ROW4 = 2 * D, RUN_STRIDE = ROW4 + 1, RUNS = kFusedRuns;
-  constexpr int SLOTS = (RUNS * ROW4 + 63) / 64;
+  constexpr int ROW4 = PF::ROW4, RUN_STRIDE = PF::RUN_STRIDE, RUNS = PF::RUNS, SLOTS = PF::SLOTS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   bf16x8* lw = reinterpret_cast<bf16x8*>(smem_raw);
   float* lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 1024);
@@ -693,42 +833,23 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   __syncthreads();
   const int64_t M = min(int64_t(*m_dev), m_max);
   const int64_t tiles = (M + 63) / 64;
-  for (int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave; t < tiles;
-       t += int64_t(gridDim.x) * kFusedWaves) {
-    const int64_t s = t * 64 + lane;
-    // the lane's sample: voxel, corner weights (the gather kernel's
-    // arithmetic), and its run of equal voxel ids
-    int32_t v = -1;
-    float w[8];
-#pragma unroll
-    for (int c = 0; c < 8; ++c) w[c] = 0.0f;
-    if (s < M) {
-      const SampleRec rec = samples[s];
-      v = rec.idx;
-      if (v >= 0) {
-        float p[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const float x = rays.start[int64_t(rec.ray) * 3 + a] +
-                          rays.dir[int64_t(rec.ray) * 3 + a] * rec.depth;
-          const float d = x - sc.centers[int64_t(v) * 3 + a];
-          p[a] = (sc.voxel_pow2 ? d * sc.inv_voxel : d / sc.voxel_size) + 0.5f;
-        }
-        const float q[3] = {1.0f - p[0], 1.0f - p[1], 1.0f - p[2]};
-        const float xy[4] = {q[0] * q[1], q[0] * p[1], p[0] * q[1], p[0] * p[1]};
-#pragma unroll
-        for (int c = 0; c < 8; ++c) w[c] = xy[c >> 1] * ((c & 1) ? p[2] : q[2]);
-      }
-    }
-    const int32_t vprev = __shfl_up(v, 1, 64);
-    const bool head = lane == 0 || v != vprev;
-    const uint64_t hmask = __ballot(head);
-    const int r = __popcll(hmask & ((2ull << lane) - 1)) - 1;
-    const int U = __popcll(hmask);
-    if (head) run_vox[r] = v;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) wts[lane * 8 + c] = w[c];
-    rix[lane] = v >= 0 ? r : -1;
+  const int64_t stride = int64_t(gridDim.x) * kFusedWaves;
+  int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave;
+  if (t >= tiles) return;  // the whole wave
+  PF pf{sc, rays, samples, rows, wts, rix, run_vox, M, tiles, lane};
+  // the first tile's gather stages back to back, the second tile's records
+  pf.issue_rec(t);
+  pf.stage_a();
+  pf.stage_b();
+  pf.issue_rec(t + stride);
+  // the two waves of a SIMD (waves w and w + 4) start half a tile apart, and
+  // a wave in its MLP issues first: the other wave's gather (VALU and LDS)
+  // then fills the matrix pipe's gaps instead of both waves alternating in
+  // step
+  if (wave >= kFusedWaves / 2) __builtin_amdgcn_s_sleep(64);
+  for (; t < tiles; t += stride) {
+    const int U = pf.U;
+    pf.rows_ready();
     wave_sync();
     float f[NT][KT0][8];
 #pragma unroll
@@ -739,19 +860,22 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
         for (int i = 0; i < 8; ++i) f[j][kt][i] = 0.0f;
     for (int u0 = 0; u0 < U; u0 += RUNS) {
       const int nr = min(RUNS, U - u0);
-      // all staging loads in flight together (absent runs load a valid row)
+      if (u0 > 0) {
+        // further staging rounds of this tile (more runs than one round holds)
 #pragma unroll
-      for (int tt = 0; tt < SLOTS; ++tt) {
-        const int i = lane + 64 * tt;
-        if (i < RUNS * ROW4) {
-          const int jr = i / ROW4, e = i - jr * ROW4;
-          const int32_t vr = run_vox[(u0 + jr) & 63];
-          const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
-          const int32_t k = sc.feats[int64_t(vu) * 8 + e / Q];
-          rows[jr * RUN_STRIDE + e] = reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[e % Q];
+        for (int tt = 0; tt < SLOTS; ++tt) {
+          const int i = lane + 64 * tt;
+          if (i < RUNS * ROW4) {
+            const int jr = i / ROW4, e = i - jr * ROW4;
+            const int32_t vr = run_vox[(u0 + jr) & 63];
+            const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
+            const int32_t k = sc.feats[int64_t(vu) * 8 + e / Q];
+            rows[jr * RUN_STRIDE + e] =
+                reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[e % Q];
+          }
         }
+        wave_sync();
       }
-      wave_sync();
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         const int qs = 16 * j + col;
@@ -802,8 +926,13 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
                          __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
         b0[kt][j] = __builtin_bit_cast(bf16x8, u);
       }
+    // the next tile's gather runs in stages around this tile's MLP
+    pf.stage_a();
+    pf.next_rec = t + 2 * stride;
     StoreSink sink{out};
-    mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * 64, M, sink);
+    __builtin_amdgcn_s_setprio(1);
+    mlp_tile_hooked<D, H, L>(b0, lw, lb, lane, g, col, t * 64, M, sink, nullptr, pf);
+    __builtin_amdgcn_s_setprio(0);
   }
 }
 
@@ -1105,6 +1234,17 @@ extern "C" {
 int nsvf_field_supported(int D, int H, int L) {
   return (D == 32 && H == 128 && L == 3) || (D == 48 && H == 256 && L == 4) ||
          (D == 32 && H == 256 && L == 4);
+}
+
+int nsvf_field_fused_supported(int D, int H, int L) {
+#define NSVF_FUSED_CASE(d, h, l) \
+  if (D == d && H == h && L == l) \
+    return (weights_in_lds<d, h, l>() && Dims<d, h, l>::SAMPLES == 64) ? 1 : 0;
+  NSVF_FUSED_CASE(32, 128, 3)
+  NSVF_FUSED_CASE(48, 256, 4)
+  NSVF_FUSED_CASE(32, 256, 4)
+#undef NSVF_FUSED_CASE
+  return 0;
 }
 
 int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L, int64_t m_max) {

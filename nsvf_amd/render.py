@@ -181,10 +181,11 @@ class Renderer:
     grown as needed, so repeated views of the same size allocate nothing."""
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
-                 compat=True, split_field=True):
+                 compat=True, split_field=None):
         self.device = torch.device(device)
-        # BF16: gather kernel + MLP kernel through a workspace (True) or one
-        # fused kernel (False)
+        # BF16: gather kernel + MLP kernel through a workspace (True), one
+        # fused kernel (False), or per scene (None): fused where the fused
+        # kernel stages the field's weights in LDS, else split
         self.split_field = split_field
         self.precision = precision
         self.hierarchical = hierarchical
@@ -235,6 +236,12 @@ class Renderer:
                                            _p(samples), _p(probs), _p(t), t.numel(), st),
               "track_voxel_probs")
 
+    def uses_split_field(self, sc):
+        """Whether the BF16 field of scene `sc` runs as gather + MLP kernels."""
+        if self.split_field is not None:
+            return bool(self.split_field)
+        return not lib().nsvf_field_fused_supported(sc.D, sc.H, sc.L)
+
     def _field(self, sc, ray_start, ray_dir, samples, m_dev, m, out):
         L = lib()
         prec = PRECISIONS[self.precision]
@@ -247,7 +254,7 @@ class Renderer:
             e.record()
             return e
 
-        if prec == 0 and self.split_field:
+        if prec == 0 and self.uses_split_field(sc):
             ws_bytes = int(L.nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
             ws = self._buf("field_ws", (ws_bytes + 15) // 16 * 4, torch.int32)
             e0 = event() if ev else None
@@ -315,7 +322,7 @@ class Renderer:
         if self.hierarchical and n_hit > 0:
             sizes.update({"f_samples": (slots_f * 4, i32), "f_field": (slots_f * 4, f32)})
             m = max(m, slots_f)
-        if PRECISIONS[self.precision] == 0 and self.split_field and m > 0:
+        if PRECISIONS[self.precision] == 0 and self.uses_split_field(sc) and m > 0:
             ws_bytes = int(lib().nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
             sizes["field_ws"] = ((ws_bytes + 15) // 16 * 4, i32)
         return sizes
