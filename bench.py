@@ -129,10 +129,12 @@ def main():
     # exact sample counts of the view (identical for every repetition)
     rnd.render(dsc, rs, rd, keep=True, width=args.res)
     n_coarse, n_fine = rnd.count_samples()
+    # fine samples the field evaluates (the rest reuse their coarse value)
+    n_fine_eval = rnd.kept["f_evaluated"]
     rnd.kept = None
     log(f"fused bf16: {ms_per_step:.2f} ms/view, {value / 1e6:.2f} Mrays/s; hit rays {st.hit_rays}, "
         f"coarse {n_coarse} samples ({st.coarse_slots} slots), fine {n_fine} samples "
-        f"({st.fine_slots} slots allocated)")
+        f"({st.fine_slots} slots allocated, {n_fine_eval} evaluated)")
 
     # roofline of the dominant kernel: the BF16 field kernel (MFMA-bound) --
     # the fused gather + MLP kernel, or the MLP kernel of the split path --
@@ -145,7 +147,7 @@ def main():
     emb_ev = [(m, a, b) for k, m, a, b in kev if k == "embed"]
     mlp_ms = sum(a.elapsed_time(b) for _, a, b in mlp_ev)
     emb_ms = sum(a.elapsed_time(b) for _, a, b in emb_ev)
-    samples = (n_coarse + n_fine) * n_views
+    samples = (n_coarse + n_fine_eval) * n_views
     achieved = flops_per_sample * samples / (mlp_ms * 1e-3) / 1e12
     D = dsc.D
     emb_bytes = samples * (16 + 2 * D) + n_views * 2 * (
@@ -211,6 +213,7 @@ def main():
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "samples_per_view": {"hit_rays": st.hit_rays, "coarse": n_coarse, "fine": n_fine,
+                             "fine_evaluated": n_fine_eval,
                              "coarse_slots": st.coarse_slots},
         "stages_ms": stages,
         "roofline": roofline,

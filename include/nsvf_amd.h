@@ -312,20 +312,56 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
 /* Fine (hierarchical) inverse-CDF sampling: bins are each ray's coarse
  * samples [depth -+ length/2] with probs (p + 1e-5) / column sum over the
  * *c_max (device) columns; `fine_steps` CDF steps (fairnr/models/nerf.py:
- * 64-79).  Writes the fine stream and f_cnt[r].                             */
+ * 64-79).  Writes the fine stream and f_cnt[r].
+ * Field reuse (optional src, int16, fine-slot aligned): the fine pass
+ * evaluates the coarse pass's field (nerf.py:58), whose value depends only
+ * on ray, voxel and depth, and a coarse bin that no fine step splits comes
+ * back as a fine sample equal to its coarse sample whenever (lo + hi) / 2
+ * rounds back to the coarse depth.  src[slot] = k: the sample is coarse
+ * bin k of its ray again (its field value is c_field[c_off[r] + k]);
+ * -1: to evaluate; -2: unused slot.  The field list of the -1 slots comes
+ * from nsvf_render_eval_list; nsvf_render_composite_reuse reads the reused
+ * values from the coarse field.                                             */
 int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev,
                             int64_t n_hit_max, const int32_t* c_off,
                             const int32_t* c_cnt, const int32_t* c_max,
                             const void* coarse, const float* c_prob,
                             const int32_t* f_bound, const int32_t* f_off,
                             float fine_steps, int compat, void* samples,
-                            int32_t* f_cnt, void* stream);
+                            int32_t* f_cnt, int16_t* src, void* stream);
+
+/* The field's evaluation list of a marked fine stream: the slots s < *n_dev
+ * with src[s] == -1, in slot order.  Three steps:
+ * nsvf_render_eval_list_count writes per-block counts (block_cnt[
+ * nsvf_render_eval_list_blocks(n_max)]), nsvf_scan_exclusive_i32 turns them
+ * into block_off (its total = the list length), nsvf_render_eval_list
+ * writes the list.  No counterpart in the reference, which evaluates every
+ * fine sample.                                                              */
+int64_t nsvf_render_eval_list_blocks(int64_t n_max);
+int nsvf_render_eval_list_count(const int16_t* src, const int32_t* n_dev,
+                                int64_t n_max, int32_t* block_cnt,
+                                void* stream);
+int nsvf_render_eval_list(const int16_t* src, const int32_t* n_dev,
+                          int64_t n_max, const int32_t* block_off,
+                          int32_t* list, void* stream);
+
+/* f_field[slot] = c_field[c_off[r] + src[slot]] for the reused slots of
+ * each hit ray (for callers that keep the fine field; the reuse composite
+ * does not need it).                                                        */
+int nsvf_render_reuse_fill(const int32_t* hit_list, const int32_t* n_hit_dev,
+                           int64_t n_hit_max, const int32_t* c_off,
+                           const int32_t* f_off, const int32_t* f_cnt,
+                           const int16_t* src, const float* c_field,
+                           float* f_field, void* stream);
 
 /* Field evaluation of a sample stream: trilinear gather fused with the MLP
  * (L hidden ReLU layers of width H on a D-wide embedding, 4-wide head).
  *   precision 0: BF16 MFMA with FP32 accumulation, wfrag = packed bf16;
  *   precision 1: FP32 MFMA, wfrag = packed f32.
  *   samples: *m_dev records (m_max >= *m_dev); unused slots are skipped.
+ *   list (optional): *m_dev slot indices -- then sample s is the record at
+ *   slot list[s] and its result goes to out[list[s]]; other slots of out
+ *   are left as they are (nsvf_render_eval_list builds such a list).
  *   out [M,4] = (sigma, sigmoid(r), sigmoid(g), sigmoid(b)).
  *   workspace (device, optional): with at least
  *   nsvf_field_workspace_bytes(...) bytes, BF16 runs as a gather kernel
@@ -341,9 +377,10 @@ int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L,
 int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
                     const int32_t* feats, const float* values, float voxel_size,
                     const float* ray_start, const float* ray_dir,
-                    const void* samples, const int32_t* m_dev, int64_t m_max,
-                    const void* wfrag, const float* bias, float* out,
-                    void* workspace, int64_t workspace_bytes, void* stream);
+                    const void* samples, const int32_t* list,
+                    const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                    const float* bias, float* out, void* workspace,
+                    int64_t workspace_bytes, void* stream);
 
 /* The two kernels of the workspace BF16 path, callable one by one (so a
  * caller can time or overlap them): the gather writes the MLP's operand
@@ -353,12 +390,13 @@ int nsvf_field_embed_bf16(int D, int H, int L, const float* centers,
                           const int32_t* feats, const float* values,
                           float voxel_size, const float* ray_start,
                           const float* ray_dir, const void* samples,
-                          const int32_t* m_dev, int64_t m_max, void* workspace,
+                          const int32_t* list, const int32_t* m_dev,
+                          int64_t m_max, void* workspace,
                           int64_t workspace_bytes, void* stream);
 int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace,
                         int64_t workspace_bytes, const int32_t* m_dev,
                         int64_t m_max, const void* wfrag, const float* bias,
-                        float* out, void* stream);
+                        float* out, const int32_t* list, void* stream);
 
 /* Front-to-back compositing of each hit ray's samples (one wavefront per
  * ray): free energy relu(sigma) * length * 7, probs, and the ray's rgb,
@@ -369,6 +407,18 @@ int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev,
                           const int32_t* cnt, const void* samples,
                           const float* field, float* prob_out, float* ray_rgb,
                           float* ray_depth, float* ray_missed, void* stream);
+
+/* nsvf_render_composite over a fine stream marked for field reuse: a slot
+ * with src[slot] = k >= 0 takes its field value from c_field[c_off[r] + k]
+ * (coarse bin k of its ray), the others from field.                        */
+int nsvf_render_composite_reuse(const int32_t* hit_list,
+                                const int32_t* n_hit_dev, int64_t n_hit_max,
+                                const int32_t* off, const int32_t* cnt,
+                                const void* samples, const float* field,
+                                const int16_t* src, const int32_t* c_off,
+                                const float* c_field, float* prob_out,
+                                float* ray_rgb, float* ray_depth,
+                                float* ray_missed, void* stream);
 
 /* Train-set voxel statistics (encoder.py:594-603 track_voxel_probs, called
  * at renderer.py:250-251): max_voxel_probs[v] = max(max_voxel_probs[v], max

@@ -57,14 +57,22 @@ SIGNATURES = {
     "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_sample_coarse": ([_P, _P, _L, _P, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P,
                                    _P, _P], _I),
-    "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P, _P], _I),
+    "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P, _P,
+                                 _P, _P], _I),
+    "nsvf_render_eval_list_blocks": ([_L], _L),
+    "nsvf_render_eval_list_count": ([_P, _P, _L, _P, _P], _I),
+    "nsvf_render_eval_list": ([_P, _P, _L, _P, _P, _P], _I),
+    "nsvf_render_reuse_fill": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_composite_reuse": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+                                     _P], _I),
     "nsvf_field_supported": ([_I, _I, _I], _I),
     "nsvf_field_fused_supported": ([_I, _I, _I], _I),
     "nsvf_field_workspace_bytes": ([_I, _I, _I, _I, _L], _L),
-    "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _P, _P, _P, _L,
-                         _P], _I),
-    "nsvf_field_embed_bf16": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _L, _P, _L, _P], _I),
-    "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P], _I),
+    "nsvf_field_eval": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _P, _P, _P,
+                         _L, _P], _I),
+    "nsvf_field_embed_bf16": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _L, _P],
+                              _I),
+    "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P, _P], _I),
     "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_composite_backward": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_track_voxel_probs": ([_P, _P, _L, _P, _P, _P, _P, _P, _L, _P], _I),

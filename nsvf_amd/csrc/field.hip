@@ -172,13 +172,15 @@ __device__ __forceinline__ float sigmoidf(float x) {
 
 // Samples of a sample stream: x = o + d * z of the record's ray
 // (geometry.py:160-161), each operation rounded.
+// With a list, stream sample s is the record at slot list[s].
 struct StreamSource {
   FieldScene sc;
   FieldRays rays;
   const SampleRec* samples;
+  const int32_t* list;
   template <bool kFast>
   __device__ __forceinline__ bool weights(int64_t s, float w[8], int32_t key[8]) const {
-    const SampleRec rec = samples[s];
+    const SampleRec rec = samples[list ? list[s] : s];
     if (rec.idx < 0) return false;  // an unused slot of a ray's range
     float x[3];
 #pragma unroll
@@ -221,15 +223,29 @@ struct LatticeSource {
   }
 };
 
-// Output sinks: put() receives each head result (sigma and the three RGB
-// logits, biases added) for lanes with ok; end_tile() runs on every lane of
-// the wave after each tile.
+// Output sinks: begin_tile() runs on every lane of the wave when a tile of n
+// samples at base starts; put() receives each head result (sigma and the
+// three RGB logits, biases added) of sample s = base + 16 j + (lane % 16) on
+// every lane, to be kept where ok; end_tile() runs on every lane after each
+// tile.
 
-// Writes (sigma, sigmoid(rgb)) per sample.
+// Writes (sigma, sigmoid(rgb)) per sample: to slot s, or with a list to
+// slot list[s].  A tile's slots are loaded when it starts (one per lane:
+// sample base + lane) and handed to the storing lanes by a lane shuffle, so
+// the stores never wait on the list.
 struct StoreSink {
   float4* out;
-  __device__ __forceinline__ void put(int64_t s, bool ok, float sg, float r, float g, float b) {
-    if (ok) out[s] = make_float4(sg, sigmoidf(r), sigmoidf(g), sigmoidf(b));
+  const int32_t* list;
+  int32_t mine = 0;
+  __device__ __forceinline__ void begin_tile(int64_t base, int64_t M, int n) {
+    const int lane = threadIdx.x & 63;
+    if (list) mine = (lane < n && base + lane < M) ? list[base + lane] : 0;
+  }
+  __device__ __forceinline__ void put(int j, int64_t s, bool ok, float sg, float r, float g,
+                                      float b) {
+    int64_t o = s;
+    if (list) o = __shfl(mine, 16 * j + (threadIdx.x & 15), 64);
+    if (ok) out[o] = make_float4(sg, sigmoidf(r), sigmoidf(g), sigmoidf(b));
   }
   __device__ __forceinline__ void end_tile(int64_t) {}
 };
@@ -243,7 +259,8 @@ struct MaxSigmaSink {
   float* vmax;   // [voxels of the launch]
   int64_t per;   // samples per voxel
   float m;
-  __device__ __forceinline__ void put(int64_t, bool ok, float sg, float, float, float) {
+  __device__ __forceinline__ void begin_tile(int64_t, int64_t, int) {}
+  __device__ __forceinline__ void put(int, int64_t, bool ok, float sg, float, float, float) {
     if (ok && sg > m) m = sg;
   }
   __device__ __forceinline__ void end_tile(int64_t base) {
@@ -307,82 +324,100 @@ __device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
   mlp_tile_hooked<D, H, L>(b0, lw, lb, lane, g, col, base, M, sink, next_b0, none);
 }
 
+// One layer's matrix work, out = relu(A in + bias), over m-tile pairs: each
+// weight fragment, read from A kLead fragments ahead, feeds NT MFMAs.  With
+// kPin (weights in LDS) the fragments are read two ahead and a scheduling
+// barrier closes each fragment's MFMAs, which pins this order: left to
+// itself the compiler runs the k-tiles outermost over all m-tiles, which
+// keeps every accumulator live and pushes each fragment read down to just
+// before its use, so the wave waits out an LDS read every NT MFMAs.  (With
+// the weights in global memory the pinned form makes the compiler hoist the
+// loads of later layers and spill, so there the fragments go one ahead,
+// unpinned.)  The accumulation order per output is k-tile 0, 1, ... either
+// way.
+template <int KT, int MT, int NT, bool kPin>
+__device__ __forceinline__ void mfma_relu_layer(const bf16x8* A, const float* B, int lane, int g,
+                                                const bf16x8 (&in)[KT][NT],
+                                                bf16x8 (&out)[MT / 2][NT]) {
+  constexpr int kFragLead = kPin ? 2 : 1;
+  constexpr int NF = MT * KT, NB = kFragLead + 1;
+  bf16x8 a[NB];
+#pragma unroll
+  for (int p = 0; p < kFragLead; ++p)
+    if (p < NF) a[p] = A[p * 64 + lane];
+  f32x4 bias_next = *reinterpret_cast<const f32x4*>(B + 4 * g);
+#pragma unroll
+  for (int kt2 = 0; kt2 < MT / 2; ++kt2) {
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int mt = 2 * kt2 + half;
+      const f32x4 bias4 = bias_next;
+      if (mt + 1 < MT) bias_next = *reinterpret_cast<const f32x4*>(B + 16 * (mt + 1) + 4 * g);
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt) {
+        const int f = mt * KT + kt;
+        if (f + kFragLead < NF) a[(f + kFragLead) % NB] = A[(f + kFragLead) * 64 + lane];
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              a[f % NB], in[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
+        if constexpr (kPin) __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) out[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
+  }
+}
+
 template <int D, int H, int L, class Sink, class Hook>
 __device__ __forceinline__ void mlp_tile_hooked(OperandTile<D, H, L>& b0, const bf16x8* lw,
                                                 const float* lb, int lane, int g, int col,
                                                 int64_t base, int64_t M, Sink& sink,
                                                 const bf16x8* __restrict__ next_b0, Hook& hook) {
   using Dm = Dims<D, H, L>;
-  constexpr int NT = Dm::NT;
-  // Weight fragments (and biases) are read one fragment ahead of the MFMAs
-  // that use them, so a read's latency hides behind the previous fragment's
-  // NT matrix instructions instead of stalling before them.
+  constexpr int NT = Dm::NT, KTH = Dm::KTH, MTH = Dm::MTH;
+  constexpr bool kPin = weights_in_lds<D, H, L>();
+  static_assert(MTH == 2 * KTH, "a layer's output m-tile pairs are the next layer's k-tiles");
+  bf16x8 bh[KTH][NT];
   // ---- layer 0: bias enters as the first MFMA's C operand
-  bf16x8 bh[Dm::KTH][NT];
-  {
-    constexpr int NF = Dm::MTH * Dm::KT0;
-    bf16x8 a_next = lw[lane];
-    f32x4 bias_next = *reinterpret_cast<const f32x4*>(lb + 4 * g);
-#pragma unroll
-    for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
-      f32x4 acc[2][NT];
-#pragma unroll
-      for (int half = 0; half < 2; ++half) {
-        const int mt = 2 * kt2 + half;
-        const f32x4 bias4 = bias_next;
-        if (mt + 1 < Dm::MTH) bias_next = *reinterpret_cast<const f32x4*>(lb + 16 * (mt + 1) + 4 * g);
-#pragma unroll
-        for (int kt = 0; kt < Dm::KT0; ++kt) {
-          const bf16x8 a = a_next;
-          const int nf = mt * Dm::KT0 + kt + 1;
-          if (nf < NF) a_next = lw[nf * 64 + lane];
-#pragma unroll
-          for (int j = 0; j < NT; ++j)
-            acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                a, b0[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < NT; ++j) bh[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
-    }
-  }
+  mfma_relu_layer<Dm::KT0, MTH, NT, kPin>(lw, lb, lane, g, b0, bh);
   hook.layer_done(0);
   // ---- hidden layers 1..L-1
+  if constexpr (kPin) {
+    // two per trip, so that the activations alternate between bh and nb
+    // without copies
+    bf16x8 nb[KTH][NT];
 #pragma unroll 1
-  for (int l = 1; l < L; ++l) {
-    constexpr int NF = Dm::MTH * Dm::KTH;
-    const bf16x8* A = lw + (Dm::F0 + (l - 1) * Dm::FH) * 64;
-    const float* B = lb + H * l;
-    bf16x8 nb[Dm::KTH][NT];
-    bf16x8 a_next = A[lane];
-    f32x4 bias_next = *reinterpret_cast<const f32x4*>(B + 4 * g);
-#pragma unroll
-    for (int kt2 = 0; kt2 < Dm::KTH; ++kt2) {
-      f32x4 acc[2][NT];
-#pragma unroll
-      for (int half = 0; half < 2; ++half) {
-        const int mt = 2 * kt2 + half;
-        const f32x4 bias4 = bias_next;
-        if (mt + 1 < Dm::MTH) bias_next = *reinterpret_cast<const f32x4*>(B + 16 * (mt + 1) + 4 * g);
-#pragma unroll
-        for (int kt = 0; kt < Dm::KTH; ++kt) {
-          const bf16x8 a = a_next;
-          const int nf = mt * Dm::KTH + kt + 1;
-          if (nf < NF) a_next = A[nf * 64 + lane];
-#pragma unroll
-          for (int j = 0; j < NT; ++j)
-            acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                a, bh[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < NT; ++j) nb[kt2][j] = relu_pack8(acc[0][j], acc[1][j]);
+    for (int l = 1; l + 1 < L; l += 2) {
+      mfma_relu_layer<KTH, MTH, NT, true>(lw + (Dm::F0 + (l - 1) * Dm::FH) * 64, lb + H * l,
+                                          lane, g, bh, nb);
+      hook.layer_done(l);
+      mfma_relu_layer<KTH, MTH, NT, true>(lw + (Dm::F0 + l * Dm::FH) * 64, lb + H * (l + 1),
+                                          lane, g, nb, bh);
+      hook.layer_done(l + 1);
     }
+    if ((L - 1) % 2) {
+      mfma_relu_layer<KTH, MTH, NT, true>(lw + (Dm::F0 + (L - 2) * Dm::FH) * 64,
+                                          lb + H * (L - 1), lane, g, bh, nb);
 #pragma unroll
-    for (int kt = 0; kt < Dm::KTH; ++kt)
+      for (int kt = 0; kt < KTH; ++kt)
 #pragma unroll
-      for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
-    hook.layer_done(l);
+        for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
+      hook.layer_done(L - 1);
+    }
+  } else {
+#pragma unroll 1
+    for (int l = 1; l < L; ++l) {
+      bf16x8 nb[KTH][NT];
+      mfma_relu_layer<KTH, MTH, NT, false>(lw + (Dm::F0 + (l - 1) * Dm::FH) * 64, lb + H * l,
+                                           lane, g, bh, nb);
+#pragma unroll
+      for (int kt = 0; kt < KTH; ++kt)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) bh[kt][j] = nb[kt][j];
+      hook.layer_done(l);
+    }
   }
   // b0 is dead since layer 0 and the hidden layers' double buffer now too:
   // the next tile's operands load into b0 while the head and the stores run
@@ -397,21 +432,26 @@ __device__ __forceinline__ void mlp_tile_hooked(OperandTile<D, H, L>& b0, const 
   {
     const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
     const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + H * L + 4 * g);
+    constexpr int kFragLead = kPin ? 2 : 1;
+    constexpr int NB = kFragLead + 1;
     f32x4 acc[NT];
-    bf16x8 a_next = A[lane];
+    bf16x8 a[NB];
 #pragma unroll
-    for (int kt = 0; kt < Dm::KTH; ++kt) {
-      const bf16x8 a = a_next;
-      if (kt + 1 < Dm::KTH) a_next = A[(kt + 1) * 64 + lane];
+    for (int p = 0; p < kFragLead; ++p)
+      if (p < KTH) a[p] = A[p * 64 + lane];
+#pragma unroll
+    for (int kt = 0; kt < KTH; ++kt) {
+      if (kt + kFragLead < KTH) a[(kt + kFragLead) % NB] = A[(kt + kFragLead) * 64 + lane];
 #pragma unroll
       for (int j = 0; j < NT; ++j)
-        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bh[kt][j], kt == 0 ? bias4 : acc[j],
-                                                         0, 0, 0);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[kt % NB], bh[kt][j],
+                                                         kt == 0 ? bias4 : acc[j], 0, 0, 0);
+      if constexpr (kPin) __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
       const int64_t s = base + 16 * j + col;
-      sink.put(s, g == 0 && s < M, acc[j][0], acc[j][1], acc[j][2], acc[j][3]);
+      sink.put(j, s, g == 0 && s < M, acc[j][0], acc[j][1], acc[j][2], acc[j][3]);
     }
     sink.end_tile(base);
   }
@@ -449,6 +489,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
   for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
        t += int64_t(gridDim.x) * 4) {
     const int64_t base = t * Dm::SAMPLES;
+    sink.begin_tile(base, M, Dm::SAMPLES);
     // ---- gather -> operand B of layer 0
 #pragma unroll 1
     for (int j = 0; j < NT; ++j) {
@@ -497,6 +538,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
 template <int D, int H, int L>
 __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRays rays,
                                                          const SampleRec* __restrict__ samples,
+                                                         const int32_t* __restrict__ list,
                                                          const int32_t* m_dev, int64_t m_max,
                                                          bf16x8* __restrict__ emb) {
   using Dm = Dims<D, H, L>;
@@ -517,11 +559,17 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
   int64_t tile = int64_t(blockIdx.x) * 4 + wave;
   if (tile >= ntile) return;  // the whole wave
   SampleRec nrec{-1, 0.0f, 0.0f, 0};
-  if (tile * 64 + lane < M) nrec = samples[tile * 64 + lane];
+  if (tile * 64 + lane < M) {
+    const int64_t s0 = tile * 64 + lane;
+    nrec = samples[list ? list[s0] : s0];
+  }
   for (; tile < ntile; tile += stride) {
   const int64_t s = tile * 64 + lane;
   const SampleRec rec = nrec;
-  if (tile + stride < ntile && s + stride * 64 < M) nrec = samples[s + stride * 64];
+  if (tile + stride < ntile && s + stride * 64 < M) {
+    const int64_t sn = s + stride * 64;
+    nrec = samples[list ? list[sn] : sn];
+  }
   // the sample's voxel; the loads its corner weights need (ray and centre)
   // are issued here and used after the row staging below, so their latency
   // overlaps the staging's
@@ -633,7 +681,8 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
 template <int D, int H, int L>
 __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_bf16_kernel(
     const bf16x8* __restrict__ emb, const int32_t* m_dev, int64_t m_max,
-    const bf16x8* __restrict__ wfrag, const float* __restrict__ bias, float4* __restrict__ out) {
+    const bf16x8* __restrict__ wfrag, const float* __restrict__ bias, float4* __restrict__ out,
+    const int32_t* __restrict__ list) {
   using Dm = Dims<D, H, L>;
   constexpr int NT = Dm::NT;
   constexpr bool kLds = weights_in_lds<D, H, L>();
@@ -661,7 +710,8 @@ __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_
     for (int kt = 0; kt < Dm::KT0; ++kt) b0[kt][j] = emb[((t * NT + j) * Dm::KT0 + kt) * 64 + lane];
   for (; t < tiles; t += stride) {
     const int64_t tn = t + stride;
-    StoreSink sink{out};
+    StoreSink sink{out, list};
+    sink.begin_tile(t * Dm::SAMPLES, M, Dm::SAMPLES);
     mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, sink,
                       tn < tiles ? emb + tn * NT * Dm::KT0 * 64 : nullptr);
   }
@@ -726,6 +776,7 @@ struct FusedPrefetch {
   FieldScene sc;
   FieldRays rays;
   const SampleRec* samples;
+  const int32_t* list;  // stream sample s is the record at slot list[s]; null: s
   float4* rows;
   float* wts;
   int32_t* rix;
@@ -745,7 +796,7 @@ struct FusedPrefetch {
     rtile = t;
     rec = SampleRec{-1, 0.0f, 0.0f, 0};
     const int64_t s = t * 64 + lane;
-    if (t < tiles && s < M) rec = samples[s];
+    if (t < tiles && s < M) rec = samples[list ? list[s] : s];
   }
   // runs into LDS; ray / centre and corner-key loads issued
   __device__ __forceinline__ void stage_a() {
@@ -809,9 +860,9 @@ struct FusedPrefetch {
 
 template <int D, int H, int L>
 __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
-    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples, const int32_t* m_dev,
-    int64_t m_max, const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
-    float4* __restrict__ out) {
+    FieldScene sc, FieldRays rays, const SampleRec* __restrict__ samples,
+    const int32_t* __restrict__ list, const int32_t* m_dev, int64_t m_max,
+    const bf16x8* __restrict__ wfrag, const float* __restrict__ bias, float4* __restrict__ out) {
   using Dm = Dims<D, H, L>;
   using PF = FusedPrefetch<D, H, L>;
   static_assert(Dm::SAMPLES == 64, "one MLP tile per wave");
@@ -836,7 +887,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   const int64_t stride = int64_t(gridDim.x) * kFusedWaves;
   int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave;
   if (t >= tiles) return;  // the whole wave
-  PF pf{sc, rays, samples, rows, wts, rix, run_vox, M, tiles, lane};
+  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, M, tiles, lane};
   // the first tile's gather stages back to back, the second tile's records
   pf.issue_rec(t);
   pf.stage_a();
@@ -849,6 +900,8 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   if (wave >= kFusedWaves / 2) __builtin_amdgcn_s_sleep(64);
   for (; t < tiles; t += stride) {
     const int U = pf.U;
+    StoreSink sink{out, list};
+    sink.begin_tile(t * 64, M, 64);
     pf.rows_ready();
     wave_sync();
     float f[NT][KT0][8];
@@ -929,7 +982,6 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
     // the next tile's gather runs in stages around this tile's MLP
     pf.stage_a();
     pf.next_rec = t + 2 * stride;
-    StoreSink sink{out};
     __builtin_amdgcn_s_setprio(1);
     mlp_tile_hooked<D, H, L>(b0, lw, lb, lane, g, col, t * 64, M, sink, nullptr, pf);
     __builtin_amdgcn_s_setprio(0);
@@ -952,6 +1004,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
   for (int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6); t < tiles;
        t += int64_t(gridDim.x) * 4) {
     const int64_t base = t * 16 * NT;
+    sink.begin_tile(base, M, 16 * NT);
     float x0[KT0][NT];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
@@ -1048,7 +1101,7 @@ __global__ __launch_bounds__(kFieldBlock) void field_f32_kernel(
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         const int64_t s = base + 16 * j + col;
-        sink.put(s, g == 0 && s < M, acc[j][0] + B[0], acc[j][1] + B[1], acc[j][2] + B[2],
+        sink.put(j, s, g == 0 && s < M, acc[j][0] + B[0], acc[j][1] + B[1], acc[j][2] + B[2],
                  acc[j][3] + B[3]);
       }
       sink.end_tile(base);
@@ -1115,7 +1168,8 @@ int64_t resident_blocks(K kernel, int threads = 256, size_t lds = 0) {
 
 template <int D, int H, int L>
 int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
-                      const int32_t* m_dev, int64_t m_max, void* workspace, hipStream_t stream) {
+                      const int32_t* list, const int32_t* m_dev, int64_t m_max, void* workspace,
+                      hipStream_t stream) {
   using Dm = Dims<D, H, L>;
   // persistent waves: one resident round of blocks, each wave looping over
   // 64-slot tiles
@@ -1123,13 +1177,13 @@ int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
   const int64_t tiles64 = ceil_div(m_max, Dm::SAMPLES) * (Dm::SAMPLES / 64);
   const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles64, 4), resident));
   hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(256), 0, stream,
-                     sc, rays, samples, m_dev, m_max, static_cast<bf16x8*>(workspace));
+                     sc, rays, samples, list, m_dev, m_max, static_cast<bf16x8*>(workspace));
   return check_launch("field_embed_bf16");
 }
 
 template <int D, int H, int L>
 int launch_mlp_bf16(const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                    float* out, const void* workspace, hipStream_t stream) {
+                    float* out, const int32_t* list, const void* workspace, hipStream_t stream) {
   using Dm = Dims<D, H, L>;
   const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
   const size_t lds = (weights_in_lds<D, H, L>() ? size_t(Dm::FRAGS) * 1024 : 0) +
@@ -1137,25 +1191,26 @@ int launch_mlp_bf16(const int32_t* m_dev, int64_t m_max, const void* wfrag, cons
   const int64_t blocks = std::min<int64_t>(ceil_div(tiles, 4), 2048);
   hipLaunchKernelGGL((mlp_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(kFieldBlock), lds,
                      stream, static_cast<const bf16x8*>(workspace), m_dev, m_max,
-                     static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out));
+                     static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out), list);
   return check_launch("field_mlp_bf16");
 }
 
 template <int D, int H, int L>
 int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
-                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                      float* out, void* workspace, hipStream_t stream) {
-  const int rc = launch_embed_bf16<D, H, L>(sc, rays, samples, m_dev, m_max, workspace, stream);
+                      const int32_t* list, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                      const float* bias, float* out, void* workspace, hipStream_t stream) {
+  const int rc =
+      launch_embed_bf16<D, H, L>(sc, rays, samples, list, m_dev, m_max, workspace, stream);
   if (rc) return rc;
-  return launch_mlp_bf16<D, H, L>(m_dev, m_max, wfrag, bias, out, workspace, stream);
+  return launch_mlp_bf16<D, H, L>(m_dev, m_max, wfrag, bias, out, list, workspace, stream);
 }
 
 // The fused BF16 field over a sample stream: the run-staged gather + MLP
 // kernel where the weights fit in LDS, else the per-lane gather kernel.
 template <int D, int H, int L>
 int launch_bf16_fused(FieldScene sc, FieldRays rays, const SampleRec* samples,
-                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                      float* out, hipStream_t stream);
+                      const int32_t* list, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                      const float* bias, float* out, hipStream_t stream);
 
 template <int D, int H, int L, class Src, class Sink>
 int launch_bf16(const Src& src, const int32_t* m_dev, int64_t m_max, const void* wfrag,
@@ -1185,8 +1240,8 @@ int launch_f32(const Src& src, const int32_t* m_dev, int64_t m_max, const void* 
 
 template <int D, int H, int L>
 int launch_bf16_fused(FieldScene sc, FieldRays rays, const SampleRec* samples,
-                      const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                      float* out, hipStream_t stream) {
+                      const int32_t* list, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                      const float* bias, float* out, hipStream_t stream) {
   if constexpr (weights_in_lds<D, H, L>() && Dims<D, H, L>::SAMPLES == 64) {
     constexpr size_t lds = fused_lds_bytes<D, H, L>();
     static const int64_t resident =
@@ -1194,12 +1249,12 @@ int launch_bf16_fused(FieldScene sc, FieldRays rays, const SampleRec* samples,
     const int64_t tiles = ceil_div(m_max, 64);
     const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles, kFusedWaves), resident));
     hipLaunchKernelGGL((field_fused_kernel<D, H, L>), dim3(unsigned(blocks)),
-                       dim3(64 * kFusedWaves), lds, stream, sc, rays, samples, m_dev, m_max,
+                       dim3(64 * kFusedWaves), lds, stream, sc, rays, samples, list, m_dev, m_max,
                        static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out));
     return check_launch("field_fused_bf16");
   } else {
-    const StreamSource src{sc, rays, samples};
-    const StoreSink sink{reinterpret_cast<float4*>(out)};
+    const StreamSource src{sc, rays, samples, list};
+    const StoreSink sink{reinterpret_cast<float4*>(out), list};
     return launch_bf16<D, H, L>(src, m_dev, m_max, wfrag, bias, sink, stream);
   }
 }
@@ -1261,8 +1316,9 @@ int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L, int64_t m
 int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
                     const int32_t* feats, const float* values, float voxel_size,
                     const float* ray_start, const float* ray_dir, const void* samples,
-                    const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                    float* out, void* workspace, int64_t workspace_bytes, void* stream) {
+                    const int32_t* list, const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                    const float* bias, float* out, void* workspace, int64_t workspace_bytes,
+                    void* stream) {
   clear_error();
   if (m_max <= 0) return NSVF_OK;
   if (!centers || !feats || !values || !ray_start || !ray_dir || !samples || !m_dev || !wfrag ||
@@ -1271,17 +1327,17 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
   FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
   FieldRays rays{ray_start, ray_dir};
   hipStream_t st = as_stream(stream);
-  const StreamSource src{sc, rays, static_cast<const SampleRec*>(samples)};
-  const StoreSink sink{reinterpret_cast<float4*>(out)};
+  const StreamSource src{sc, rays, static_cast<const SampleRec*>(samples), list};
+  const StoreSink sink{reinterpret_cast<float4*>(out), list};
 #define NSVF_FIELD_CASE(d, h, l)                                                               \
   if (D == d && H == h && L == l) {                                                            \
     if (precision == 0 && workspace &&                                                        \
         workspace_bytes >= bf16_workspace_bytes<d, h, l>(m_max))                               \
       return launch_bf16_split<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
-                                        m_dev, m_max, wfrag, bias, out, workspace, st);        \
+                                        list, m_dev, m_max, wfrag, bias, out, workspace, st);  \
     if (precision == 0)                                                                        \
       return launch_bf16_fused<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
-                                        m_dev, m_max, wfrag, bias, out, st);                   \
+                                        list, m_dev, m_max, wfrag, bias, out, st);             \
     return launch_f32<d, h, l>(src, m_dev, m_max, wfrag, bias, sink, st);                      \
   }
   NSVF_FIELD_CASE(32, 128, 3)
@@ -1293,8 +1349,9 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
 
 int nsvf_field_embed_bf16(int D, int H, int L, const float* centers, const int32_t* feats,
                           const float* values, float voxel_size, const float* ray_start,
-                          const float* ray_dir, const void* samples, const int32_t* m_dev,
-                          int64_t m_max, void* workspace, int64_t workspace_bytes, void* stream) {
+                          const float* ray_dir, const void* samples, const int32_t* list,
+                          const int32_t* m_dev, int64_t m_max, void* workspace,
+                          int64_t workspace_bytes, void* stream) {
   clear_error();
   if (m_max <= 0) return NSVF_OK;
   if (!centers || !feats || !values || !ray_start || !ray_dir || !samples || !m_dev || !workspace)
@@ -1305,8 +1362,8 @@ int nsvf_field_embed_bf16(int D, int H, int L, const float* centers, const int32
   FieldRays rays{ray_start, ray_dir};
 #define NSVF_EMB_CASE(d, h, l)                                                                \
   if (D == d && H == h && L == l)                                                             \
-    return launch_embed_bf16<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), m_dev, \
-                                      m_max, workspace, as_stream(stream));
+    return launch_embed_bf16<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), list, \
+                                      m_dev, m_max, workspace, as_stream(stream));
   NSVF_EMB_CASE(32, 128, 3)
   NSVF_EMB_CASE(48, 256, 4)
   NSVF_EMB_CASE(32, 256, 4)
@@ -1316,7 +1373,7 @@ int nsvf_field_embed_bf16(int D, int H, int L, const float* centers, const int32
 
 int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace, int64_t workspace_bytes,
                         const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
-                        float* out, void* stream) {
+                        float* out, const int32_t* list, void* stream) {
   clear_error();
   if (m_max <= 0) return NSVF_OK;
   if (!workspace || !m_dev || !wfrag || !bias || !out)
@@ -1325,7 +1382,8 @@ int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace, int64_t work
     return set_error(NSVF_EINVAL, "field_mlp_bf16: workspace too small");
 #define NSVF_MLP_CASE(d, h, l) \
   if (D == d && H == h && L == l) \
-    return launch_mlp_bf16<d, h, l>(m_dev, m_max, wfrag, bias, out, workspace, as_stream(stream));
+    return launch_mlp_bf16<d, h, l>(m_dev, m_max, wfrag, bias, out, list, workspace,  \
+                                    as_stream(stream));
   NSVF_MLP_CASE(32, 128, 3)
   NSVF_MLP_CASE(48, 256, 4)
   NSVF_MLP_CASE(32, 256, 4)

@@ -381,6 +381,7 @@ struct SampleBins {
   __device__ __forceinline__ float prob(int k) const {
     return (k < cnt ? prob_c[k] + 1e-5f : 1e-5f) / den;
   }
+  __device__ __forceinline__ float depth(int k) const { return k < cnt ? rec[k].depth : 0.0f; }
   __device__ __forceinline__ int32_t next_first() const { return next; }
   __device__ __forceinline__ int32_t ray0_id(int k) const { return k < cnt0 ? rec0[k].idx : -1; }
 };
@@ -462,15 +463,40 @@ __device__ __forceinline__ float fine_den(const float* prob, int cnt, int P) {
   return part;
 }
 
+// Field reuse in the fine pass.  The fine pass evaluates the same field as
+// the coarse pass (nerf.py:58: field_fine only with --use-fine-model), and
+// the field of a sample depends only on its ray, voxel id and depth.  A
+// coarse bin that no fine step splits comes back as a fine sample of the
+// same voxel whose midpoint (lo + hi) / 2 of [depth - dist/2, depth +
+// dist/2] is nearly always the coarse depth again; where it is bit for bit,
+// the sample's field value is the coarse one.  The sampler only marks the
+// slots: src[slot] = k reads bin k's coarse value, -1 = evaluate, -2 =
+// unused (stores only: its serial walk never waits on an extra load).
+struct FieldReuse {
+  static constexpr bool kBins = true;
+  int16_t* src;  // the ray's fine slots, or null for no reuse
+  __device__ __forceinline__ void put(int pos, int bound, int k) {
+    if (src && pos < bound) src[pos] = int16_t(k >= 0 ? k : -1);
+  }
+  __device__ __forceinline__ void seq(int n) {
+    if (src)
+      for (int k = threadIdx.x & 63; k < n; k += 64) src[k] = -1;
+  }
+  __device__ __forceinline__ void unused(int pos) {
+    if (src) src[pos] = -2;
+  }
+};
+
 // Fine pass: one wavefront per hit ray; bins are the ray's coarse samples and
-// the *c_max columns of the longest coarse row (nerf.py:64-79).
+// the *c_max columns of the longest coarse row (nerf.py:64-79).  With src
+// (optional), the slots are marked for field reuse as FieldReuse describes.
 __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_cnt,
     const int32_t* __restrict__ c_max_dev, const SampleRec* __restrict__ coarse,
     const float* __restrict__ c_prob, const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
-    int32_t* __restrict__ f_cnt) {
+    int32_t* __restrict__ f_cnt, int16_t* __restrict__ src) {
   __shared__ float lds_c[kSampleWaves][kWaveBins], lds_lo[kSampleWaves][kWaveBins],
       lds_hi[kSampleWaves][kWaveBins];
   __shared__ int32_t lds_id[kSampleWaves][kWaveBins];
@@ -483,6 +509,7 @@ __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
     const int b = f_bound[r];
     SampleRec* dst = out + f_off[r];
     int m = 0;
+    FieldReuse reuse{src ? src + f_off[r] : nullptr};
     if (P > 0) {
       int64_t n0, nn;
       grouped_neighbours(n, N, &n0, &nn);
@@ -497,11 +524,105 @@ __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
       }
       SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
       m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst,
-                    WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave]});
+                    WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave]}, reuse);
     } else {
-      for (int k = lane; k < b; k += 64) dst[k] = SampleRec{-1, 0.0f, 0.0f, r};
+      for (int k = lane; k < b; k += 64) {
+        dst[k] = SampleRec{-1, 0.0f, 0.0f, r};
+        reuse.unused(k);
+      }
     }
     if (lane == 0) f_cnt[r] = m;
+  }
+}
+
+// The field's evaluation list of a marked fine stream: the slots s < *n_dev
+// with src[s] == -1, in slot order.  A stream compaction over all slots in
+// two kernels around a scan of per-block counts: a block of 4 waves takes
+// 4096 consecutive slots, each wave 16 groups of 64 (one slot per lane, so
+// every load and store of a group is one coalesced access), and a group's
+// list entries come from a ballot.
+constexpr int kMarkGroups = 16, kMarkPerBlock = kBlock * kMarkGroups;
+
+// The wave's ballots of eval slots, group by group; returns their count.
+__device__ __forceinline__ int mark_ballots(const int16_t* __restrict__ src, int64_t s0,
+                                            int64_t n, uint64_t (&mask)[kMarkGroups]) {
+  const int lane = threadIdx.x & 63;
+  int16_t v[kMarkGroups];
+#pragma unroll
+  for (int q = 0; q < kMarkGroups; ++q) {
+    const int64_t s = s0 + 64 * q + lane;
+    v[q] = s < n ? src[s] : int16_t(-2);
+  }
+  int c = 0;
+#pragma unroll
+  for (int q = 0; q < kMarkGroups; ++q) {
+    mask[q] = __ballot(v[q] == -1);
+    c += __popcll(mask[q]);
+  }
+  return c;
+}
+
+__global__ __launch_bounds__(kBlock) void mark_count_kernel(const int16_t* __restrict__ src,
+                                                            const int32_t* __restrict__ n_dev,
+                                                            int64_t n_max,
+                                                            int32_t* __restrict__ block_cnt) {
+  __shared__ int wcnt[kBlock / 64];
+  const int64_t n = min(int64_t(*n_dev), n_max);
+  const int wave = threadIdx.x >> 6;
+  const int64_t s0 = int64_t(blockIdx.x) * kMarkPerBlock + int64_t(wave) * 64 * kMarkGroups;
+  uint64_t mask[kMarkGroups];
+  const int c = mark_ballots(src, s0, n, mask);
+  if ((threadIdx.x & 63) == 0) wcnt[wave] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += wcnt[w];
+    block_cnt[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void mark_compact_kernel(const int16_t* __restrict__ src,
+                                                              const int32_t* __restrict__ n_dev,
+                                                              int64_t n_max,
+                                                              const int32_t* __restrict__ block_off,
+                                                              int32_t* __restrict__ list) {
+  __shared__ int wcnt[kBlock / 64];
+  const int64_t n = min(int64_t(*n_dev), n_max);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t s0 = int64_t(blockIdx.x) * kMarkPerBlock + int64_t(wave) * 64 * kMarkGroups;
+  uint64_t mask[kMarkGroups];
+  const int c = mark_ballots(src, s0, n, mask);
+  if (lane == 0) wcnt[wave] = c;
+  __syncthreads();
+  int pos = block_off[blockIdx.x];
+  for (int w = 0; w < wave; ++w) pos += wcnt[w];
+  const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+  for (int q = 0; q < kMarkGroups; ++q) {
+    if ((mask[q] >> lane) & 1ull)
+      list[pos + __popcll(mask[q] & below)] = int32_t(s0 + 64 * q + lane);
+    pos += __popcll(mask[q]);
+  }
+}
+
+// The reused fine field values written out (f_field[slot] = the coarse
+// value it reads), for callers that keep the fine field; one wavefront per
+// hit ray.
+__global__ __launch_bounds__(kBlock) void reuse_fill_kernel(
+    const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
+    const int32_t* __restrict__ c_off, const int32_t* __restrict__ f_off,
+    const int32_t* __restrict__ f_cnt, const int16_t* __restrict__ src,
+    const float4* __restrict__ c_field, float4* __restrict__ f_field) {
+  const int64_t n = int64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (n >= *n_hit_dev) return;
+  const int32_t r = hit_list[n];
+  const int64_t coff = c_off[r], off = f_off[r];
+  const int m = f_cnt[r];
+  for (int k = lane; k < m; k += 64) {
+    const int sk = src[off + k];
+    if (sk >= 0) f_field[off + k] = c_field[coff + sk];
   }
 }
 
@@ -615,18 +736,30 @@ __global__ __launch_bounds__(kBlock) void track_probs_kernel(
 // fe = relu(sigma) * dist * 7 (renderer.py:119-121); transmittance before a
 // sample is exp(-(sum of the earlier fe)), an exclusive prefix sum across the
 // wave plus the carry of earlier steps (renderer.py:195-197).
+template <bool kReuse>
 __global__ __launch_bounds__(kBlock) void composite_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ off_arr, const int32_t* __restrict__ cnt_arr,
     const SampleRec* __restrict__ samples, const float4* __restrict__ field,
     float* __restrict__ prob_out, float* __restrict__ ray_rgb, float* __restrict__ ray_depth,
-    float* __restrict__ ray_missed) {
+    float* __restrict__ ray_missed, const int16_t* __restrict__ src,
+    const int32_t* __restrict__ src_off, const float4* __restrict__ src_field) {
   const int64_t n = int64_t(blockIdx.x) * (kBlock / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (n >= *n_hit_dev) return;
   const int32_t r = hit_list[n];
   const int64_t off = off_arr[r];
   const int cnt = cnt_arr[r];
+  const int64_t soff = kReuse ? src_off[r] : 0;
+  // the marks of the ray's first 4 passes, loaded together up front so a
+  // pass's field load does not wait on its own mark load
+  constexpr int kPre = 4;
+  int16_t pre[kPre];
+#pragma unroll
+  for (int q = 0; q < kPre; ++q) {
+    const int k = 64 * q + lane;
+    pre[q] = (kReuse && k < cnt) ? src[off + k] : int16_t(-1);
+  }
   float carry = 0.0f;
   float acc_p = 0.0f, acc_z = 0.0f, acc_r = 0.0f, acc_g = 0.0f, acc_b = 0.0f;
   for (int base = 0; base < cnt; base += 64) {
@@ -635,7 +768,17 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(
     float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
     float z = 0.0f;
     if (k < cnt) {
-      f = field[off + k];
+      const int q = base >> 6;
+      int sk = -1;
+      if constexpr (kReuse) {
+        sk = pre[0];
+#pragma unroll
+        for (int j = 1; j < kPre; ++j) sk = q == j ? pre[j] : sk;
+        if (q >= kPre) sk = src[off + k];
+      }
+      // one load from whichever array holds the value
+      const float4* fp = sk >= 0 ? src_field + soff + sk : field + off + k;
+      f = *fp;
       const SampleRec sr = samples[off + k];
       const float sg = f.x > 0.0f ? f.x : 0.0f;
       fe = (sg * sr.dist) * 7.0f;
@@ -879,15 +1022,56 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev, i
                             const int32_t* c_off, const int32_t* c_cnt, const int32_t* c_max,
                             const void* coarse, const float* c_prob, const int32_t* f_bound,
                             const int32_t* f_off, float fine_steps, int compat, void* samples,
-                            int32_t* f_cnt, void* stream) {
+                            int32_t* f_cnt, int16_t* src, void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
   hipLaunchKernelGGL(fine_sample_kernel,
                      dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
                      dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, c_off, c_cnt, c_max,
                      static_cast<const SampleRec*>(coarse), c_prob, f_bound, f_off, fine_steps,
-                     compat, static_cast<SampleRec*>(samples), f_cnt);
+                     compat, static_cast<SampleRec*>(samples), f_cnt, src);
   return check_launch("render_sample_fine");
+}
+
+int64_t nsvf_render_eval_list_blocks(int64_t n_max) {
+  return n_max > 0 ? ceil_div(n_max, kMarkPerBlock) : 0;
+}
+
+int nsvf_render_eval_list_count(const int16_t* src, const int32_t* n_dev, int64_t n_max,
+                                int32_t* block_cnt, void* stream) {
+  clear_error();
+  if (n_max <= 0) return NSVF_OK;
+  if (!src || !n_dev || !block_cnt) return set_error(NSVF_EINVAL, "eval_list_count: null pointer");
+  hipLaunchKernelGGL(mark_count_kernel, dim3(unsigned(nsvf_render_eval_list_blocks(n_max))),
+                     dim3(kBlock), 0, as_stream(stream), src, n_dev, n_max, block_cnt);
+  return check_launch("render_eval_list_count");
+}
+
+int nsvf_render_eval_list(const int16_t* src, const int32_t* n_dev, int64_t n_max,
+                          const int32_t* block_off, int32_t* list, void* stream) {
+  clear_error();
+  if (n_max <= 0) return NSVF_OK;
+  if (!src || !n_dev || !block_off || !list)
+    return set_error(NSVF_EINVAL, "render_eval_list: null pointer");
+  hipLaunchKernelGGL(mark_compact_kernel, dim3(unsigned(nsvf_render_eval_list_blocks(n_max))),
+                     dim3(kBlock), 0, as_stream(stream), src, n_dev, n_max, block_off, list);
+  return check_launch("render_eval_list");
+}
+
+int nsvf_render_reuse_fill(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
+                           const int32_t* c_off, const int32_t* f_off, const int32_t* f_cnt,
+                           const int16_t* src, const float* c_field, float* f_field,
+                           void* stream) {
+  clear_error();
+  if (n_hit_max <= 0) return NSVF_OK;
+  if (!hit_list || !n_hit_dev || !c_off || !f_off || !f_cnt || !src || !c_field || !f_field)
+    return set_error(NSVF_EINVAL, "render_reuse_fill: null pointer");
+  const int per_block = kBlock / 64;
+  hipLaunchKernelGGL(reuse_fill_kernel, dim3(unsigned(ceil_div(n_hit_max, per_block))),
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, c_off, f_off, f_cnt,
+                     src, reinterpret_cast<const float4*>(c_field),
+                     reinterpret_cast<float4*>(f_field));
+  return check_launch("render_reuse_fill");
 }
 
 int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
@@ -897,11 +1081,30 @@ int nsvf_render_composite(const int32_t* hit_list, const int32_t* n_hit_dev, int
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
   const int per_block = kBlock / 64;
-  hipLaunchKernelGGL(composite_kernel, dim3(unsigned(ceil_div(n_hit_max, per_block))),
+  hipLaunchKernelGGL(composite_kernel<false>, dim3(unsigned(ceil_div(n_hit_max, per_block))),
                      dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, off, cnt,
                      static_cast<const SampleRec*>(samples), reinterpret_cast<const float4*>(field),
-                     prob_out, ray_rgb, ray_depth, ray_missed);
+                     prob_out, ray_rgb, ray_depth, ray_missed, nullptr, nullptr, nullptr);
   return check_launch("render_composite");
+}
+
+int nsvf_render_composite_reuse(const int32_t* hit_list, const int32_t* n_hit_dev,
+                                int64_t n_hit_max, const int32_t* off, const int32_t* cnt,
+                                const void* samples, const float* field, const int16_t* src,
+                                const int32_t* c_off, const float* c_field, float* prob_out,
+                                float* ray_rgb, float* ray_depth, float* ray_missed,
+                                void* stream) {
+  clear_error();
+  if (n_hit_max <= 0) return NSVF_OK;
+  if (!src || !c_off || !c_field)
+    return set_error(NSVF_EINVAL, "render_composite_reuse: null pointer");
+  const int per_block = kBlock / 64;
+  hipLaunchKernelGGL(composite_kernel<true>, dim3(unsigned(ceil_div(n_hit_max, per_block))),
+                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, off, cnt,
+                     static_cast<const SampleRec*>(samples), reinterpret_cast<const float4*>(field),
+                     prob_out, ray_rgb, ray_depth, ray_missed, src, c_off,
+                     reinterpret_cast<const float4*>(c_field));
+  return check_launch("render_composite_reuse");
 }
 
 int nsvf_track_voxel_probs(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,

@@ -24,6 +24,8 @@
 // beyond the slot bound take the sequential walk on lane 0 instead.
 #pragma once
 
+#include <type_traits>
+
 #include "render.h"
 #include "sample_core.h"
 
@@ -69,12 +71,27 @@ __device__ __forceinline__ void put_rec(SampleRec* out, int bound, int pos, int3
   if (pos < bound) out[pos] = SampleRec{id, mid, d > 0.0f ? d : 0.0f, ray};
 }
 
+// Work done beside each record a sampler writes: put(pos, bound, k) for the
+// sample at slot pos, k >= 0 when that sample is bin k's own sample again
+// (kBins hooks: the whole bin [lo_k, hi_k] with bins.depth(k) == (lo_k +
+// hi_k) / 2 bit for bit, so voxel id and depth equal bin k's), else -1;
+// seq(n) after the sequential walk wrote slots [0, n); unused(pos) for each
+// slot past the samples.  NoSlotHook does nothing.
+struct NoSlotHook {
+  static constexpr bool kBins = false;
+  __device__ __forceinline__ void put(int, int, int) {}
+  __device__ __forceinline__ void seq(int) {}
+  __device__ __forceinline__ void unused(int) {}
+};
+
 // Samples ray `ray` with bins `bins` (P columns, the first Pv valid) into
 // out[0, bound); slots past the samples are marked unused.  Returns the
-// sample count (same value in every lane).  lds: this wave's staging arrays.
-template <class Bins>
+// sample count (same value in every lane).  lds: this wave's staging arrays;
+// hook: called beside every slot written (the caller's lane decides nothing).
+template <class Bins, class Hook = NoSlotHook>
 __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compat, int bound,
-                         int32_t ray, SampleRec* out, const WaveBinsLds& lds) {
+                         int32_t ray, SampleRec* out, const WaveBinsLds& lds,
+                         Hook&& hook = NoSlotHook{}) {
   const int lane = threadIdx.x & 63;
   // the row's sizes are the same in every lane; scalar copies keep the
   // loops below uniform (no per-lane exit masks)
@@ -82,7 +99,10 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
   Pv = __builtin_amdgcn_readfirstlane(Pv);
   bound = __builtin_amdgcn_readfirstlane(bound);
   steps = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(steps)));
+  using HookT = std::remove_reference_t<Hook>;
   int n = 0;
+  // bit p of lane l: bin 64 p + l is its own sample when whole (kBins)
+  uint32_t same = 0;
   bool seq = Pv <= 0 || Pv > kWaveBins || !(steps <= float(bound));
   if (!seq) {
     // stage the bins; the prefix C_k = C_{k-1} + p_k runs left to right (the
@@ -94,9 +114,13 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       float pk = 0.0f;
       if (k < Pv) {
         pk = bins.prob(k);
-        lds.lo[k] = bins.lo(k);
-        lds.hi[k] = bins.hi(k);
+        const float lk = bins.lo(k), hk = bins.hi(k);
+        lds.lo[k] = lk;
+        lds.hi[k] = hk;
         lds.id[k] = bins.id(k);
+        if constexpr (HookT::kBins)
+          if (__float_as_int((hk + lk) * 0.5f) == __float_as_int(bins.depth(k)))
+            same |= 1u << (t0 >> 6);
       }
       const int m = Pv - t0 < 64 ? Pv - t0 : 64;
       float mine = 0.0f;
@@ -123,6 +147,7 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       n = sink.n;
     }
     n = __shfl(n, 0, 64);
+    hook.seq(n);
   } else {
     const float* C = lds.C;
     const float ss = float(1.0 / double(steps));
@@ -166,6 +191,7 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       if (act && lane <= dl && b < Pv) {
         const float zk = (b == bp) ? zp : lds.lo[b];
         put_rec(out, bound, i + b, lds.id[b], z - zk, (z + zk) * 0.5f, ray);
+        hook.put(i + b, bound, -1);
       }
       if (dmask) {
         const int bpd = __shfl(bp, dl, 64);
@@ -204,6 +230,9 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
             if (k == 0 || cdfp > cmin) zk = __fmaf_rn((cdfp - cmin) / (ck - cmin), hk - lk, lk);
           }
           put_rec(out, bound, ik + k, lds.id[k], hk - zk, (hk + zk) * 0.5f, ray);
+          const bool whole = __float_as_int(zk) == __float_as_int(lk);
+          const bool own = whole && ((same >> ((k0 >> 6) & 31)) & 1u);
+          hook.put(ik + k, bound, own ? k : -1);
         }
       }
     }
@@ -218,7 +247,15 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
     if (!(done && !compat) && zl < cmax) {
       const int32_t id0 = cb < Pv ? lds.id[cb] : (cb < P ? bins.id(cb) : bins.next_first());
       if (id0 != -1) {
-        if (lane == 0) put_rec(out, bound, n, id0, cmax - zl, (cmax + zl) * 0.5f, ray);
+        // every lane takes part in the shuffles (their source lanes must be active)
+        const uint32_t s_cb = __shfl(same, cb & 63, 64);
+        const bool same_cb = cb < Pv && ((s_cb >> ((cb >> 6) & 31)) & 1u);
+        if (lane == 0) {
+          put_rec(out, bound, n, id0, cmax - zl, (cmax + zl) * 0.5f, ray);
+          const bool whole = same_cb && __float_as_int(zl) == __float_as_int(lds.lo[cb]) &&
+                             __float_as_int(cmax) == __float_as_int(lds.hi[cb]);
+          hook.put(n, bound, whole ? cb : -1);
+        }
         ++n;
       }
       const int k0 = cb + 1;
@@ -235,13 +272,21 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
           ok = probe != -1 && lk < hk;
         }
         const int stop = first_lane(__ballot(!ok));
-        if (lane < stop) put_rec(out, bound, n + lane, idk, hk - lk, (hk + lk) * 0.5f, ray);
+        const uint32_t s_k = __shfl(same, k & 63, 64);
+        const bool same_k = k < Pv && ((s_k >> ((k >> 6) & 31)) & 1u);
+        if (lane < stop) {
+          put_rec(out, bound, n + lane, idk, hk - lk, (hk + lk) * 0.5f, ray);
+          hook.put(n + lane, bound, same_k ? k : -1);
+        }
         n += stop < 64 ? stop : 64;
         if (stop < 64) break;
       }
     }
   }
-  for (int k = n + lane; k < bound; k += 64) out[k] = SampleRec{-1, 0.0f, 0.0f, ray};
+  for (int k = n + lane; k < bound; k += 64) {
+    out[k] = SampleRec{-1, 0.0f, 0.0f, ray};
+    hook.unused(k);
+  }
   wave_sync();  // the next ray's staging must not overtake this ray's reads
   return n;
 }

@@ -181,8 +181,12 @@ class Renderer:
     grown as needed, so repeated views of the same size allocate nothing."""
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
-                 compat=True, split_field=None):
+                 compat=True, split_field=None, reuse_coarse=True):
         self.device = torch.device(device)
+        # fine pass: copy the coarse field value of every fine sample that is
+        # its bin's coarse sample again (same voxel and depth bits) and
+        # evaluate the field on the other fine samples only (same results)
+        self.reuse_coarse = bool(reuse_coarse)
         # BF16: gather kernel + MLP kernel through a workspace (True), one
         # fused kernel (False), or per scene (None): fused where the fused
         # kernel stages the field's weights in LDS, else split
@@ -242,7 +246,9 @@ class Renderer:
             return bool(self.split_field)
         return not lib().nsvf_field_fused_supported(sc.D, sc.H, sc.L)
 
-    def _field(self, sc, ray_start, ray_dir, samples, m_dev, m, out):
+    def _field(self, sc, ray_start, ray_dir, samples, m_dev, m, out, slots=None):
+        """Field of the m_dev samples of a stream (slots: optional evaluation
+        list of stream slots; then only those slots of out are written)."""
         L = lib()
         prec = PRECISIONS[self.precision]
         w = sc.mlp["bf16"] if prec == 0 else sc.mlp["f32"]
@@ -260,11 +266,12 @@ class Renderer:
             e0 = event() if ev else None
             check(L.nsvf_field_embed_bf16(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats),
                                           _p(sc.values), sc.voxel_size, _p(ray_start), _p(ray_dir),
-                                          _p(samples), _p(m_dev), m, _p(ws), ws_bytes, st),
-                  "field_embed_bf16")
+                                          _p(samples), _p(slots), _p(m_dev), m, _p(ws), ws_bytes,
+                                          st), "field_embed_bf16")
             e1 = event() if ev else None
             check(L.nsvf_field_mlp_bf16(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(m_dev), m, _p(w),
-                                        _p(sc.mlp["bias"]), _p(out), st), "field_mlp_bf16")
+                                        _p(sc.mlp["bias"]), _p(out), _p(slots), st),
+                  "field_mlp_bf16")
             if ev:
                 e2 = event()
                 self._kernel_events.append(("embed", m, e0, e1))
@@ -272,8 +279,9 @@ class Renderer:
             return
         e0 = event() if ev else None
         check(L.nsvf_field_eval(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
-                                sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples), _p(m_dev), m,
-                                _p(w), _p(sc.mlp["bias"]), _p(out), None, 0, st), "field_eval")
+                                sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples), _p(slots),
+                                _p(m_dev), m, _p(w), _p(sc.mlp["bias"]), _p(out), None, 0, st),
+              "field_eval")
         if ev:
             self._kernel_events.append(("field", m, e0, event()))
 
@@ -321,6 +329,11 @@ class Renderer:
         m = slots_c
         if self.hierarchical and n_hit > 0:
             sizes.update({"f_samples": (slots_f * 4, i32), "f_field": (slots_f * 4, f32)})
+            if self.reuse_coarse:
+                nb = int(lib().nsvf_render_eval_list_blocks(slots_f))
+                sizes.update({"f_src": (slots_f, torch.int16), "e_bcnt": (nb, i32),
+                              "e_boff": (nb, i32), "e_list": (slots_f, i32),
+                              "scan_ws": (int(lib().nsvf_scan_workspace_ints(max(R, nb))), i32)})
             m = max(m, slots_f)
         if PRECISIONS[self.precision] == 0 and self.uses_split_field(sc) and m > 0:
             ws_bytes = int(lib().nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
@@ -492,18 +505,44 @@ class Renderer:
             self._scan(f_bound, f_off, R, scal[3:4])
             fsm = self._stream("f_samples", slots_f)
             f_cnt = self._buf("f_cnt", R, torch.int32)
+            f_field = self._buf("f_field", slots_f * 4, torch.float32)
+            reuse = self.reuse_coarse
+            src = None
+            if reuse:
+                src = self._buf("f_src", slots_f, torch.int16)
             check(L.nsvf_render_sample_fine(_p(hit_list), _p(scal[0:1]), R, _p(c_off), _p(c_cnt),
                                             _p(scal[2:3]), _p(cs), _p(c_prob), _p(f_bound),
-                                            _p(f_off), fs, self.compat, _p(fsm), _p(f_cnt), st),
-                  "sample_fine")
+                                            _p(f_off), fs, self.compat, _p(fsm), _p(f_cnt),
+                                            _p(src), st), "sample_fine")
+            if reuse:
+                # the field's list of the fine slots to evaluate (src == -1),
+                # its length in scal[4]
+                nb = int(L.nsvf_render_eval_list_blocks(slots_f))
+                bcnt = self._buf("e_bcnt", nb, torch.int32)
+                boff = self._buf("e_boff", nb, torch.int32)
+                e_list = self._buf("e_list", slots_f, torch.int32)
+                check(L.nsvf_render_eval_list_count(_p(src), _p(scal[3:4]), slots_f, _p(bcnt), st),
+                      "eval_list_count")
+                self._scan(bcnt, boff, nb, scal[4:5])
+                check(L.nsvf_render_eval_list(_p(src), _p(scal[3:4]), slots_f, _p(boff),
+                                              _p(e_list), st), "eval_list")
             self._mark("fine_sample")
-            f_field = self._buf("f_field", slots_f * 4, torch.float32)
-            self._field(sc, ray_start, ray_dir, fsm, scal[3:4], slots_f, f_field)
+            if reuse:
+                self._field(sc, ray_start, ray_dir, fsm, scal[4:5], slots_f, f_field, e_list)
+            else:
+                self._field(sc, ray_start, ray_dir, fsm, scal[3:4], slots_f, f_field)
             self._mark("fine_field")
             f_prob = self._buf("f_prob", slots_f, torch.float32) if self.track is not None else None
-            check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), n_hit, _p(f_off), _p(f_cnt),
-                                          _p(fsm), _p(f_field), _p(f_prob), _p(ray_rgb),
-                                          _p(ray_depth), _p(ray_missed), st), "composite fine")
+            if reuse:
+                check(L.nsvf_render_composite_reuse(
+                    _p(hit_list), _p(scal[0:1]), n_hit, _p(f_off), _p(f_cnt), _p(fsm),
+                    _p(f_field), _p(src), _p(c_off), _p(c_field), _p(f_prob), _p(ray_rgb),
+                    _p(ray_depth), _p(ray_missed), st), "composite fine")
+            else:
+                check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), n_hit, _p(f_off),
+                                              _p(f_cnt), _p(fsm), _p(f_field), _p(f_prob),
+                                              _p(ray_rgb), _p(ray_depth), _p(ray_missed), st),
+                      "composite fine")
             self._track(sc, hit_list, scal, n_hit, f_off, f_cnt, fsm, f_prob, st)
             self._mark("fine_composite")
         # the outputs leave the workspace: the caller owns them
@@ -526,8 +565,13 @@ class Renderer:
                 "c_max": int(scal[2]),
             }
             if self.hierarchical and n_hit > 0:
+                if reuse:   # the kept fine field holds every sample's value
+                    check(L.nsvf_render_reuse_fill(_p(hit_list), _p(scal[0:1]), n_hit, _p(c_off),
+                                                   _p(f_off), _p(f_cnt), _p(src), _p(c_field),
+                                                   _p(f_field), st), "reuse_fill")
                 self.kept.update({"f_cnt": f_cnt[hl], "f_off": f_off[hl],
-                                  "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4)})
+                                  "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4),
+                                  "f_evaluated": int(scal[4]) if reuse else int(f_cnt[hl].sum())})
                 if self.track is not None:
                     self.kept["f_prob"] = f_prob
         return rgb, depth, missed

@@ -235,6 +235,34 @@ def test_bf16_split_and_fused_field_agree(setup, device):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("compat", [False, True])
+@pytest.mark.parametrize("precision,split", [("fp32", None), ("bf16", True), ("bf16", False)])
+def test_fine_field_reuse_is_invisible(setup, device, precision, split, compat):
+    """Fine samples that are their bin's coarse sample again (same voxel and
+    depth bits) take the coarse field value instead of a second evaluation:
+    the fine field of every written sample and the images are the same bits
+    with and without the reuse, and with it part of the samples is copied."""
+    sc, rs, rd, dsc, ref = setup
+    runs = []
+    for reuse in (False, True):
+        r = Renderer(device, precision=precision, split_field=split, compat=compat,
+                     reuse_coarse=reuse)
+        img = r.render(dsc, torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device),
+                       keep=True)
+        torch.cuda.synchronize()
+        k = r.kept
+        f_cnt, f_off = N(k["f_cnt"]), N(k["f_off"])
+        slots = np.concatenate([np.arange(o, o + c) for o, c in zip(f_off, f_cnt)])
+        runs.append((N(k["f_field"])[slots], [N(t) for t in img], k["f_evaluated"],
+                     int(f_cnt.sum())))
+    (f0, img0, ev0, n0), (f1, img1, ev1, n1) = runs
+    assert n0 == n1 and ev0 == n0
+    assert 0 < ev1 < n1
+    assert np.array_equal(f0.view(np.int32), f1.view(np.int32))
+    for a, b in zip(img0, img1):
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
 @pytest.mark.parametrize("n_max", [60, 7])
 def test_hits_truncation_and_ties(device, n_max):
     """Rays with equal entry depths (axis-aligned rays on voxel face planes)
