@@ -407,6 +407,12 @@ __device__ __forceinline__ void grouped_neighbours(int64_t n, int64_t N, int64_t
 
 // Waves per sampler block and the grid-stride loop over hit rays.
 constexpr int kSampleWaves = kBlock / 64;
+// Bins a sampler wave stages in LDS (rows with more take the sequential
+// walk).  Coarse rows have at most n_max bins (60 by default); fine rows as
+// many as the ray's coarse samples.  The LDS per block sets the occupancy:
+// 16 bytes per bin and wave, so 128 bins leave the coarse kernel limited
+// by registers, and 384 give the fine kernel 6 blocks (6 waves per SIMD).
+constexpr int kCoarseBins = 128, kFineBins = 384;
 
 // Coarse pass: one wavefront per hit ray n (ray r = hit_list[n]) writes the
 // ray's samples into its slot range [c_off[r], c_off[r] + bound[r]); count,
@@ -418,9 +424,9 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
     const int32_t* __restrict__ c_off, int n_max, float step_size, int compat, float fine_steps,
     SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
     int32_t* __restrict__ max_count) {
-  __shared__ float lds_c[kSampleWaves][kWaveBins], lds_lo[kSampleWaves][kWaveBins],
-      lds_hi[kSampleWaves][kWaveBins];
-  __shared__ int32_t lds_id[kSampleWaves][kWaveBins];
+  __shared__ float lds_c[kSampleWaves][kCoarseBins], lds_lo[kSampleWaves][kCoarseBins],
+      lds_hi[kSampleWaves][kCoarseBins];
+  __shared__ int32_t lds_id[kSampleWaves][kCoarseBins];
   __shared__ int block_max;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
@@ -438,7 +444,8 @@ __global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
                  nn < 0 ? -1 : hits[int64_t(hit_list[nn]) * n_max].idx};
     const float steps = tot[r] / step_size;
     const int m = icdf_wave(bins, n_max, cnt, steps, compat, bound[r], r, out + c_off[r],
-                            WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave]});
+                            WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave],
+                                        kCoarseBins});
     if (lane == 0) {
       c_cnt[r] = m;
       f_bound[r] = int(ceilf(fine_steps)) + m + 2;
@@ -490,16 +497,16 @@ struct FieldReuse {
 // Fine pass: one wavefront per hit ray; bins are the ray's coarse samples and
 // the *c_max columns of the longest coarse row (nerf.py:64-79).  With src
 // (optional), the slots are marked for field reuse as FieldReuse describes.
-__global__ __launch_bounds__(kBlock) void fine_sample_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void fine_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_cnt,
     const int32_t* __restrict__ c_max_dev, const SampleRec* __restrict__ coarse,
     const float* __restrict__ c_prob, const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
     int32_t* __restrict__ f_cnt, int16_t* __restrict__ src) {
-  __shared__ float lds_c[kSampleWaves][kWaveBins], lds_lo[kSampleWaves][kWaveBins],
-      lds_hi[kSampleWaves][kWaveBins];
-  __shared__ int32_t lds_id[kSampleWaves][kWaveBins];
+  __shared__ float lds_c[kSampleWaves][kFineBins], lds_lo[kSampleWaves][kFineBins],
+      lds_hi[kSampleWaves][kFineBins];
+  __shared__ int32_t lds_id[kSampleWaves][kFineBins];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
   const int P = *c_max_dev;  // bin columns = longest coarse row
@@ -524,7 +531,8 @@ __global__ __launch_bounds__(kBlock) void fine_sample_kernel(
       }
       SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
       m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst,
-                    WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave]}, reuse);
+                    WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave], kFineBins},
+                    reuse);
     } else {
       for (int k = lane; k < b; k += 64) {
         dst[k] = SampleRec{-1, 0.0f, 0.0f, r};

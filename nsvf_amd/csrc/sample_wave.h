@@ -49,15 +49,14 @@ struct StreamSink {
   }
 };
 
-constexpr int kWaveBins = 512;  // bins of a ray staged in a wave's LDS
-
 // A wave's LDS staging of one ray's bins: cumulative probability, bounds and
-// voxel ids of bins [0, kWaveBins).
+// voxel ids of bins [0, cap); rows with more bins take the sequential walk.
 struct WaveBinsLds {
   float* C;
   float* lo;
   float* hi;
   int32_t* id;
+  int cap;
 };
 
 __device__ __forceinline__ int first_lane(uint64_t mask) {
@@ -103,7 +102,7 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
   int n = 0;
   // bit p of lane l: bin 64 p + l is its own sample when whole (kBins)
   uint32_t same = 0;
-  bool seq = Pv <= 0 || Pv > kWaveBins || !(steps <= float(bound));
+  bool seq = Pv <= 0 || Pv > lds.cap || !(steps <= float(bound));
   if (!seq) {
     // stage the bins; the prefix C_k = C_{k-1} + p_k runs left to right (the
     // reference kernel's order) over values pulled from their lanes, every

@@ -114,3 +114,67 @@ def test_one_hit_budget(device):
     ref = check_hits(r, sc, rs, rd)
     assert ref["hits"].sum() > 50
     assert np.abs(rgb - ref["rgb"]).max() < 1e-5
+
+
+def padded(cnt, off, arr, width, fill):
+    out = np.full((len(cnt), width), fill, dtype=arr.dtype)
+    for n in range(len(cnt)):
+        out[n, : cnt[n]] = arr[off[n]: off[n] + cnt[n]]
+    return out
+
+
+@pytest.fixture(scope="module")
+def cube():
+    """A dense 56^3 cube of voxels and rays through it, many along the
+    diagonals: rows of more than 128 hits (past the coarse sampler's staged
+    bins) and of more than 384 coarse samples (past the fine sampler's)."""
+    n = 56
+    g = (np.arange(n) - (n - 1) / 2) * 0.1
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    sc = dataclasses.replace(IO.scene_from_points(pts, 0.1, embed_dim=32), max_hits=200,
+                             step_size=np.float32(0.025))
+    rng = np.random.default_rng(5)
+    R = 96
+    d = rng.normal(size=(R, 3)).astype(np.float32)
+    d[: R // 2] = np.sign(d[: R // 2]) * (1 + 0.05 * rng.random((R // 2, 3)))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return sc, (-d * 3.0).astype(np.float32), d.astype(np.float32)
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_rows_past_the_staged_bins(device, cube, compat):
+    """Rows too long for the samplers' LDS staging take the sequential walk;
+    hit rows, coarse and fine streams still equal the oracle's bit for bit
+    and the FP32 image is within 1e-5 of the FP64 one."""
+    sc, rs, rd = cube
+    r = Renderer(device, precision="fp32", hierarchical=True, compat=compat)
+    rgb, depth, missed = (N(t) for t in r.render(
+        DeviceScene(sc, device), torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device),
+        keep=True))
+    k = r.kept
+    ref = O.render_view(sc, rs, rd, compat=compat)
+    hit_cnt = (ref["sorted_idx"] != -1).sum(1)
+    assert (hit_cnt > 128).sum() > 10
+    cnt, rows = N(k["count"]), N(k["hits"])
+    assert np.array_equal(cnt, hit_cnt)
+    for ray in np.nonzero(ref["hits"])[0]:
+        m = cnt[ray]
+        assert np.array_equal(rows[ray, :m, 2], ref["sorted_idx"][ray, :m])
+        assert np.array_equal(rows[ray, :m, 0].view(np.float32), ref["sorted_lo"][ray, :m])
+    c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
+    ci = ref["c_idx"]
+    assert (c_cnt > 384).sum() > 10
+    W = ci.shape[1]
+    assert np.array_equal(padded(c_cnt, c_off, N(k["c_samples"][0]), W, -1), ci)
+    assert np.array_equal(padded(c_cnt, c_off, N(k["c_samples"][1]), W,
+                                 np.float32(O.MAX_DEPTH)), ref["c_depth"])
+    gp = padded(c_cnt, c_off, N(k["c_prob"])[: c_off[-1] + c_cnt[-1]], W, np.float32(0))
+    ref2 = O.render_view(sc, rs, rd, compat=compat, coarse_probs=gp)
+    f_cnt, f_off = N(k["f_cnt"]), N(k["f_off"])
+    fi = ref2["f_idx"]
+    W2 = fi.shape[1]
+    assert np.array_equal(padded(f_cnt, f_off, N(k["f_samples"][0]), W2, -1), fi)
+    assert np.array_equal(padded(f_cnt, f_off, N(k["f_samples"][1]), W2,
+                                 np.float32(O.MAX_DEPTH)), ref2["f_depth"])
+    assert np.abs(rgb - ref2["rgb"]).max() < 1e-5
+    assert np.abs(missed - ref2["missed"]).max() < 1e-5
