@@ -105,8 +105,10 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
   bool seq = Pv <= 0 || Pv > lds.cap || !(steps <= float(bound));
   if (!seq) {
     // stage the bins; the prefix C_k = C_{k-1} + p_k runs left to right (the
-    // reference kernel's order) over values pulled from their lanes, every
-    // lane computing the same sum, and lane k keeps C_k
+    // reference kernel's order) across the lanes: a whole-wave shift right by
+    // one lane (DPP wave_shr:1) hands lane k the running sum of lane k-1, so
+    // after step i lanes 0..i hold C_k, each sum rounded once in the order of
+    // a sequential loop; lane 0 starts from the previous 64 bins' carry
     float c = 0.0f;
     for (int t0 = 0; t0 < Pv; t0 += 64) {
       const int k = t0 + lane;
@@ -122,18 +124,17 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
             same |= 1u << (t0 >> 6);
       }
       const int m = Pv - t0 < 64 ? Pv - t0 : 64;
-      float mine = 0.0f;
-      int j = 0;
-      if (t0 == 0) {  // C_0 = p_0
-        c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pk), 0));
-        mine = c;
-        j = 1;
+      const float q = (lane == 0 && t0 > 0) ? c + pk : pk;
+      float run = q;
+      for (int i = 1; i < m; ++i) {
+        // lane 0 has no left neighbour and reads 0: 0 + q == q
+        const float left = __int_as_float(
+            __builtin_amdgcn_update_dpp(0, __float_as_int(run), 0x138 /* wave_shr:1 */, 0xf, 0xf,
+                                        false));
+        run = left + q;
       }
-      for (; j < m; ++j) {
-        c = c + __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pk), j));
-        mine = lane == j ? c : mine;
-      }
-      if (k < Pv) lds.C[k] = mine;
+      c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(run), m - 1));
+      if (k < Pv) lds.C[k] = run;
     }
     wave_sync();
     seq = isnan(c);
