@@ -409,15 +409,16 @@ __device__ __forceinline__ void grouped_neighbours(int64_t n, int64_t N, int64_t
 constexpr int kSampleWaves = kBlock / 64;
 // Bins a sampler wave stages in LDS (rows with more take the sequential
 // walk).  Coarse rows have at most n_max bins (60 by default); fine rows as
-// many as the ray's coarse samples.  The LDS per block sets the occupancy:
-// 16 bytes per bin and wave, so 128 bins leave the coarse kernel limited
-// by registers, and 384 give the fine kernel 6 blocks (6 waves per SIMD).
+// many as the ray's coarse samples.  The LDS per block (16 bytes per bin and
+// wave) and the registers set the occupancy: the coarse kernel runs 8 waves
+// per SIMD (128 bins, 63 VGPRs), the fine kernel 6 (384 bins; with fewer
+// bins and registers for 7 or 8 it spills and runs slower).
 constexpr int kCoarseBins = 128, kFineBins = 384;
 
 // Coarse pass: one wavefront per hit ray n (ray r = hit_list[n]) writes the
 // ray's samples into its slot range [c_off[r], c_off[r] + bound[r]); count,
 // fine-pass bound and the longest row are recorded for the fine pass.
-__global__ __launch_bounds__(kBlock) void coarse_sample_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void coarse_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const HitRec* __restrict__ hits, const int32_t* __restrict__ count,
     const float* __restrict__ tot, const int32_t* __restrict__ bound,
