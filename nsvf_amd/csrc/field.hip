@@ -781,6 +781,7 @@ struct FusedPrefetch {
   float* wts;
   int32_t* rix;
   int32_t* run_vox;
+  int* in_mlp;  // MLPs of this wave's SIMD pair in flight (LDS)
   int64_t M, tiles;
   int lane;
   int64_t rtile = -1;  // the tile whose records are in rec
@@ -852,6 +853,8 @@ struct FusedPrefetch {
   int64_t next_rec = -1;  // the records to load before the head
   __device__ __forceinline__ void layer_done(int l) {
     if (l == 0) stage_b();
+    // the partner's MLP has ended: this one is alone on the matrix pipe
+    if (__builtin_amdgcn_readfirstlane(*(volatile int*)in_mlp) == 1) __builtin_amdgcn_s_setprio(2);
   }
   __device__ __forceinline__ void before_head() {
     if (next_rec >= 0) issue_rec(next_rec);
@@ -879,15 +882,22 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   float* wts = reinterpret_cast<float*>(wb + size_t(RUNS * RUN_STRIDE) * 16);
   int32_t* rix = reinterpret_cast<int32_t*>(wts + 64 * 8);
   int32_t* run_vox = rix + 64;
+  // MLPs in flight per SIMD pair (waves w and w + 4): a wave that starts
+  // its MLP while its partner's runs takes the lower priority, so the
+  // earlier MLP keeps the matrix pipe and that wave returns to its gather
+  // sooner (a hint only: nothing waits on it)
+  __shared__ int mlp_count[kFusedWaves / 2];
   for (int i = threadIdx.x; i < Dm::FRAGS * 64; i += 64 * kFusedWaves) lw[i] = wfrag[i];
   for (int i = threadIdx.x; i < Dm::BIASES; i += 64 * kFusedWaves) lb[i] = bias[i];
+  if (threadIdx.x < kFusedWaves / 2) mlp_count[threadIdx.x] = 0;
   __syncthreads();
   const int64_t M = min(int64_t(*m_dev), m_max);
   const int64_t tiles = (M + 63) / 64;
   const int64_t stride = int64_t(gridDim.x) * kFusedWaves;
   int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave;
   if (t >= tiles) return;  // the whole wave
-  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, M, tiles, lane};
+  int* in_mlp = mlp_count + wave % (kFusedWaves / 2);
+  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, in_mlp, M, tiles, lane};
   // the first tile's gather stages back to back, the second tile's records
   pf.issue_rec(t);
   pf.stage_a();
@@ -982,8 +992,14 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
     // the next tile's gather runs in stages around this tile's MLP
     pf.stage_a();
     pf.next_rec = t + 2 * stride;
-    __builtin_amdgcn_s_setprio(1);
+    int before = 0;
+    if (lane == 0) before = atomicAdd(in_mlp, 1);
+    if (__builtin_amdgcn_readfirstlane(before) == 0)
+      __builtin_amdgcn_s_setprio(2);
+    else
+      __builtin_amdgcn_s_setprio(1);
     mlp_tile_hooked<D, H, L>(b0, lw, lb, lane, g, col, t * 64, M, sink, nullptr, pf);
+    if (lane == 0) atomicSub(in_mlp, 1);
     __builtin_amdgcn_s_setprio(0);
   }
 }
