@@ -903,11 +903,9 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   pf.stage_a();
   pf.stage_b();
   pf.issue_rec(t + stride);
-  // the two waves of a SIMD (waves w and w + 4) start half a tile apart, and
   // a wave in its MLP issues first: the other wave's gather (VALU and LDS)
-  // then fills the matrix pipe's gaps instead of both waves alternating in
-  // step
-  if (wave >= kFusedWaves / 2) __builtin_amdgcn_s_sleep(64);
+  // fills the matrix pipe's gaps (the MLP priorities below also keep the two
+  // waves of a SIMD out of step, so they start together)
   for (; t < tiles; t += stride) {
     const int U = pf.U;
     StoreSink sink{out, list};
