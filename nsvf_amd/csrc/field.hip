@@ -875,7 +875,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   bf16x8* lw = reinterpret_cast<bf16x8*>(smem_raw);
   float* lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 1024);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   unsigned char* wb = smem_raw + ((size_t(Dm::FRAGS) * 1024 + size_t(Dm::BIASES) * 4 + 15) &
                                   ~size_t(15)) + size_t(wave) * fused_wave_bytes<D>();
   float4* rows = reinterpret_cast<float4*>(wb);
@@ -907,6 +907,13 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   // fills the matrix pipe's gaps (the MLP priorities below also keep the two
   // waves of a SIMD out of step, so they start together)
   for (; t < tiles; t += stride) {
+    // the lane index made opaque once per tile: addresses derived from it are
+    // recomputed in the loop (a few ALU ops) instead of being hoisted out of
+    // it and spilled
+    int ln = lane;
+    asm volatile("" : "+v"(ln));
+    pf.lane = ln;
+    const int g = ln >> 4, col = ln & 15;
     const int U = pf.U;
     StoreSink sink{out, list};
     sink.begin_tile(t * 64, M, 64);
@@ -996,7 +1003,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
       __builtin_amdgcn_s_setprio(2);
     else
       __builtin_amdgcn_s_setprio(1);
-    mlp_tile_hooked<D, H, L>(b0, lw, lb, lane, g, col, t * 64, M, sink, nullptr, pf);
+    mlp_tile_hooked<D, H, L>(b0, lw, lb, ln, g, col, t * 64, M, sink, nullptr, pf);
     if (lane == 0) atomicSub(in_mlp, 1);
     __builtin_amdgcn_s_setprio(0);
   }
