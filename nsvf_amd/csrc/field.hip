@@ -325,21 +325,20 @@ __device__ __forceinline__ void mlp_tile(OperandTile<D, H, L>& b0,
 }
 
 // One layer's matrix work, out = relu(A in + bias), over m-tile pairs: each
-// weight fragment, read from A one fragment ahead, feeds NT MFMAs.  With
-// kPin (weights in LDS) a scheduling barrier closes each fragment's MFMAs,
+// weight fragment feeds NT MFMAs.  With kPin (weights in LDS) the fragments
+// are read two ahead and a scheduling barrier closes each fragment's MFMAs,
 // which pins this order: left to itself the compiler runs the k-tiles
 // outermost over all m-tiles, which keeps every accumulator live and pushes
 // each fragment read down to just before its use, so the wave waits out an
-// LDS read every NT MFMAs.  (Reading two fragments ahead measured slower in
-// the fused kernel: its four more live registers spill.  With the weights
-// in global memory the pinned form makes the compiler hoist the loads of
-// later layers and spill, so there the order is left to the compiler.)  The
-// accumulation order per output is k-tile 0, 1, ... either way.
+// LDS read every NT MFMAs.  (With the weights in global memory the pinned
+// form makes the compiler hoist the loads of later layers and spill, so
+// there the fragments go one ahead and the order is left to the compiler.)
+// The accumulation order per output is k-tile 0, 1, ... either way.
 template <int KT, int MT, int NT, bool kPin>
 __device__ __forceinline__ void mfma_relu_layer(const bf16x8* A, const float* B, int lane, int g,
                                                 const bf16x8 (&in)[KT][NT],
                                                 bf16x8 (&out)[MT / 2][NT]) {
-  constexpr int kFragLead = 1;
+  constexpr int kFragLead = kPin ? 2 : 1;
   constexpr int NF = MT * KT, NB = kFragLead + 1;
   bf16x8 a[NB];
 #pragma unroll
@@ -432,7 +431,7 @@ __device__ __forceinline__ void mlp_tile_hooked(OperandTile<D, H, L>& b0, const 
   {
     const bf16x8* A = lw + (Dm::F0 + (L - 1) * Dm::FH) * 64;
     const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + H * L + 4 * g);
-    constexpr int kFragLead = 1;
+    constexpr int kFragLead = kPin ? 2 : 1;
     constexpr int NB = kFragLead + 1;
     f32x4 acc[NT];
     bf16x8 a[NB];
