@@ -730,8 +730,9 @@ constexpr int kFusedRuns = 6;   // voxels staged per round
 
 template <int D>
 constexpr size_t fused_wave_bytes() {
-  // rows (padded stride), per-sample weights, per-sample run, run voxels
-  return size_t(kFusedRuns * (2 * D + 1)) * 16 + 64 * 8 * 4 + 64 * 4 + 64 * 4;
+  // rows (padded stride), per-sample weights, per-sample run, run voxels,
+  // the second round's corner keys
+  return size_t(kFusedRuns * (2 * D + 1)) * 16 + 64 * 8 * 4 + 64 * 4 + 64 * 4 + 64 * 4;
 }
 
 template <int D, int H, int L>
@@ -780,6 +781,7 @@ struct FusedPrefetch {
   float* wts;
   int32_t* rix;
   int32_t* run_vox;
+  int32_t* keys2;  // corner c of run RUNS + j at keys2[8 j + c] (tiles of more than RUNS runs)
   int* in_mlp;  // MLPs of this wave's SIMD pair in flight (LDS)
   int64_t M, tiles;
   int lane;
@@ -839,6 +841,17 @@ struct FusedPrefetch {
           (const __attribute__((address_space(1))) void*)(src),
           (__attribute__((address_space(3))) void*)(rows + tt * RUN_STRIDE), 16, 0, 0);
     }
+    // a tile of more runs than a round holds: the second round's corner keys
+    // straight into LDS too (lane l: corner l % 8 of run RUNS + l / 8), so
+    // that round waits on its row loads only
+    if (U > RUNS) {
+      const int jr = lane >> 3;
+      const int32_t vr = run_vox[(RUNS + jr) & 63];
+      const int32_t vu = (jr < RUNS && RUNS + jr < U && vr >= 0) ? vr : 0;
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(sc.feats + int64_t(vu) * 8 + (lane & 7)),
+          (__attribute__((address_space(3))) void*)(keys2), 4, 0, 0);
+    }
     float w[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) w[c] = 0.0f;
@@ -881,6 +894,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   float* wts = reinterpret_cast<float*>(wb + size_t(RUNS * RUN_STRIDE) * 16);
   int32_t* rix = reinterpret_cast<int32_t*>(wts + 64 * 8);
   int32_t* run_vox = rix + 64;
+  int32_t* keys2 = run_vox + 64;
   // MLPs in flight per SIMD pair (waves w and w + 4): a wave that starts
   // its MLP while its partner's runs takes the lower priority, so the
   // earlier MLP keeps the matrix pipe and that wave returns to its gather
@@ -896,7 +910,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave;
   if (t >= tiles) return;  // the whole wave
   int* in_mlp = mlp_count + wave % (kFusedWaves / 2);
-  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, in_mlp, M, tiles, lane};
+  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, keys2, in_mlp, M, tiles, lane};
   // the first tile's gather stages back to back, the second tile's records
   pf.issue_rec(t);
   pf.stage_a();
@@ -934,9 +948,14 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
           const int i = lane + 64 * tt;
           if (i < RUNS * ROW4) {
             const int jr = i / ROW4, e = i - jr * ROW4;
-            const int32_t vr = run_vox[(u0 + jr) & 63];
-            const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
-            const int32_t k = sc.feats[int64_t(vu) * 8 + e / Q];
+            int32_t k;
+            if (u0 == RUNS) {  // keys loaded with the first round's rows
+              k = keys2[jr * 8 + e / Q];
+            } else {
+              const int32_t vr = run_vox[(u0 + jr) & 63];
+              const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
+              k = sc.feats[int64_t(vu) * 8 + e / Q];
+            }
             rows[jr * RUN_STRIDE + e] =
                 reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[e % Q];
           }
