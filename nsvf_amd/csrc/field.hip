@@ -726,12 +726,12 @@ __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_
 // fused multiply-adds, so operands and field equal the split path's bit for
 // bit.
 constexpr int kFusedWaves = 8;
-constexpr int kFusedRuns = 6;   // voxels staged per round
+constexpr int kFusedRuns = 7;   // voxels staged per round
 
 template <int D>
 constexpr size_t fused_wave_bytes() {
   // rows (padded stride), per-sample weights, per-sample run, run voxels,
-  // the second round's corner keys
+  // the corner keys of a staging round
   return size_t(kFusedRuns * (2 * D + 1)) * 16 + 64 * 8 * 4 + 64 * 4 + 64 * 4 + 64 * 4;
 }
 
@@ -773,6 +773,7 @@ struct FusedPrefetch {
   static constexpr int ROW4 = 2 * D, RUN_STRIDE = ROW4 + 1, RUNS = kFusedRuns, Q = D / 4;
   static constexpr int SLOTS = (RUNS * ROW4 + 63) / 64;
   static_assert(ROW4 == 64, "one run per 64-lane load");
+  static_assert(RUNS <= 8, "a round's corner keys in one 64-lane load");
   FieldScene sc;
   FieldRays rays;
   const SampleRec* samples;
@@ -781,7 +782,7 @@ struct FusedPrefetch {
   float* wts;
   int32_t* rix;
   int32_t* run_vox;
-  int32_t* keys2;  // corner c of run RUNS + j at keys2[8 j + c] (tiles of more than RUNS runs)
+  int32_t* keys;  // corner c of the round's run j at keys[8 j + c]
   int* in_mlp;  // MLPs of this wave's SIMD pair in flight (LDS)
   int64_t M, tiles;
   int lane;
@@ -792,7 +793,6 @@ struct FusedPrefetch {
   int U = 1;
   float z = 0.0f;
   float ro[3] = {0.f, 0.f, 0.f}, rd[3] = {0.f, 0.f, 0.f}, ctr[3] = {0.f, 0.f, 0.f};
-  int32_t key[SLOTS];
 
   __device__ __forceinline__ void issue_rec(int64_t t) {
     rtile = t;
@@ -800,7 +800,8 @@ struct FusedPrefetch {
     const int64_t s = t * 64 + lane;
     if (t < tiles && s < M) rec = samples[list ? list[s] : s];
   }
-  // runs into LDS; ray / centre and corner-key loads issued
+  // runs into LDS; ray / centre loads issued, the first round's corner keys
+  // straight into LDS
   __device__ __forceinline__ void stage_a() {
     tile = rtile < tiles ? rtile : -1;
     if (tile < 0) return;
@@ -822,36 +823,37 @@ struct FusedPrefetch {
     if (head) run_vox[r] = v;
     rix[lane] = v >= 0 ? r : -1;
     wave_sync();
-#pragma unroll
-    for (int tt = 0; tt < SLOTS; ++tt) {
-      const int32_t vr = run_vox[tt];
-      const int32_t vu = (tt < U && vr >= 0) ? vr : 0;
-      key[tt] = sc.feats[int64_t(vu) * 8 + lane / Q];
-    }
+    load_keys(0);
+  }
+  // lane l: corner l % 8 of run u0 + l / 8 (padding runs read voxel 0's)
+  __device__ __forceinline__ void load_keys(int u0) {
+    const int jr = lane >> 3;
+    const int32_t vr = run_vox[(u0 + jr) & 63];
+    const int32_t vu = (jr < RUNS && u0 + jr < U && vr >= 0) ? vr : 0;
+    __builtin_amdgcn_global_load_lds(
+        (const __attribute__((address_space(1))) void*)(sc.feats + int64_t(vu) * 8 + (lane & 7)),
+        (__attribute__((address_space(3))) void*)(keys), 4, 0, 0);
   }
   // the first round's rows straight into LDS (lane e of load tt: element e
   // of run tt at rows + tt * RUN_STRIDE + e); the weights into LDS
   __device__ __forceinline__ void stage_b() {
     if (tile < 0) return;
+    __builtin_amdgcn_s_waitcnt(0);  // the keys have landed
+    wave_sync();
 #pragma unroll
     for (int tt = 0; tt < SLOTS; ++tt) {
+      const int32_t k = keys[tt * 8 + lane / Q];
       const float4* src =
-          reinterpret_cast<const float4*>(sc.values + int64_t(key[tt]) * D) + lane % Q;
+          reinterpret_cast<const float4*>(sc.values + int64_t(k) * D) + lane % Q;
       __builtin_amdgcn_global_load_lds(
           (const __attribute__((address_space(1))) void*)(src),
           (__attribute__((address_space(3))) void*)(rows + tt * RUN_STRIDE), 16, 0, 0);
     }
     // a tile of more runs than a round holds: the second round's corner keys
-    // straight into LDS too (lane l: corner l % 8 of run RUNS + l / 8), so
-    // that round waits on its row loads only
-    if (U > RUNS) {
-      const int jr = lane >> 3;
-      const int32_t vr = run_vox[(RUNS + jr) & 63];
-      const int32_t vu = (jr < RUNS && RUNS + jr < U && vr >= 0) ? vr : 0;
-      __builtin_amdgcn_global_load_lds(
-          (const __attribute__((address_space(1))) void*)(sc.feats + int64_t(vu) * 8 + (lane & 7)),
-          (__attribute__((address_space(3))) void*)(keys2), 4, 0, 0);
-    }
+    // into the same buffer (the first round's are read: the row loads above
+    // take their addresses from them), so that round waits on its row loads
+    // only
+    if (U > RUNS) load_keys(RUNS);
     float w[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) w[c] = 0.0f;
@@ -894,7 +896,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   float* wts = reinterpret_cast<float*>(wb + size_t(RUNS * RUN_STRIDE) * 16);
   int32_t* rix = reinterpret_cast<int32_t*>(wts + 64 * 8);
   int32_t* run_vox = rix + 64;
-  int32_t* keys2 = run_vox + 64;
+  int32_t* keys = run_vox + 64;
   // MLPs in flight per SIMD pair (waves w and w + 4): a wave that starts
   // its MLP while its partner's runs takes the lower priority, so the
   // earlier MLP keeps the matrix pipe and that wave returns to its gather
@@ -910,7 +912,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   int64_t t = int64_t(blockIdx.x) * kFusedWaves + wave;
   if (t >= tiles) return;  // the whole wave
   int* in_mlp = mlp_count + wave % (kFusedWaves / 2);
-  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, keys2, in_mlp, M, tiles, lane};
+  PF pf{sc, rays, samples, list, rows, wts, rix, run_vox, keys, in_mlp, M, tiles, lane};
   // the first tile's gather stages back to back, the second tile's records
   pf.issue_rec(t);
   pf.stage_a();
@@ -950,7 +952,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
             const int jr = i / ROW4, e = i - jr * ROW4;
             int32_t k;
             if (u0 == RUNS) {  // keys loaded with the first round's rows
-              k = keys2[jr * 8 + e / Q];
+              k = keys[jr * 8 + e / Q];
             } else {
               const int32_t vr = run_vox[(u0 + jr) & 63];
               const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
