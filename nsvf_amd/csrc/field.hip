@@ -944,24 +944,24 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
     for (int u0 = 0; u0 < U; u0 += RUNS) {
       const int nr = min(RUNS, U - u0);
       if (u0 > 0) {
-        // further staging rounds of this tile (more runs than one round holds)
-#pragma unroll
-        for (int tt = 0; tt < SLOTS; ++tt) {
-          const int i = lane + 64 * tt;
-          if (i < RUNS * ROW4) {
-            const int jr = i / ROW4, e = i - jr * ROW4;
-            int32_t k;
-            if (u0 == RUNS) {  // keys loaded with the first round's rows
-              k = keys[jr * 8 + e / Q];
-            } else {
-              const int32_t vr = run_vox[(u0 + jr) & 63];
-              const int32_t vu = (jr < nr && vr >= 0) ? vr : 0;
-              k = sc.feats[int64_t(vu) * 8 + e / Q];
-            }
-            rows[jr * RUN_STRIDE + e] =
-                reinterpret_cast<const float4*>(sc.values + int64_t(k) * D)[e % Q];
-          }
+        // further staging rounds of this tile (more runs than one round
+        // holds), rows straight into LDS as in the first; the second round's
+        // keys came with the first round's rows
+        __builtin_amdgcn_s_waitcnt(0);  // the last round's row reads are done
+        if (u0 > RUNS) {
+          pf.load_keys(u0);
+          __builtin_amdgcn_s_waitcnt(0);
         }
+        wave_sync();
+#pragma unroll
+        for (int tt = 0; tt < RUNS; ++tt) {
+          const int32_t k = keys[tt * 8 + ln / Q];
+          __builtin_amdgcn_global_load_lds(
+              (const __attribute__((address_space(1))) void*)(
+                  reinterpret_cast<const float4*>(sc.values + int64_t(k) * D) + ln % Q),
+              (__attribute__((address_space(3))) void*)(rows + tt * RUN_STRIDE), 16, 0, 0);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
         wave_sync();
       }
 #pragma unroll

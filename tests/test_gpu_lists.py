@@ -64,3 +64,32 @@ def test_field_over_a_list(device, precision, split):
     assert np.array_equal(o[sel].view(np.int32), f[sel].view(np.int32))
     rest = np.setdiff1d(np.arange(m), sel)
     assert np.isnan(o[rest]).all()
+
+
+def test_fused_field_over_a_shuffled_list(device):
+    """A list in random order: nearly every sample of a 64-sample tile is a
+    run of its own, so the fused field stages its corner rows in many rounds
+    per tile; it still equals the split path (gather kernel + MLP kernel)
+    bit for bit, slot by slot."""
+    sc = shell()
+    rs, rd = shell_rays(width=64)
+    dsc = DeviceScene(sc, device)
+    rs_d, rd_d = torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device)
+    outs = []
+    for split in (True, False):
+        r = Renderer(device, precision="bf16", split_field=split)
+        r.render(dsc, rs_d, rd_d, keep=True)
+        torch.cuda.synchronize()
+        m = int(r._ws["scalars"][1])
+        samples = r._ws["c_samples"][: m * 4].view(m, 4).clone()
+        perm = np.random.default_rng(11).permutation(m).astype(np.int32)
+        idx = samples[:, 0].cpu().numpy()[perm]
+        runs = 1 + (np.diff(idx.reshape(-1)[: (m // 64) * 64].reshape(-1, 64), axis=1) != 0).sum(1)
+        assert runs.max() > 3 * 7  # past the third staging round
+        n_dev = torch.tensor([m], dtype=torch.int32, device=device)
+        out = torch.full((m, 4), float("nan"), dtype=torch.float32, device=device)
+        r._field(dsc, rs_d, rd_d, samples, n_dev, m, out.view(-1), torch.from_numpy(perm).to(device))
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    assert not np.isnan(outs[1]).any()
+    assert np.array_equal(outs[0].view(np.int32), outs[1].view(np.int32))
