@@ -217,7 +217,7 @@ def main():
                              "coarse_slots": st.coarse_slots},
         "stages_ms": stages,
         "roofline": roofline,
-        "kernels": kernels,
+        **({"kernels": kernels} if kernels else {}),
         "from_camera": {"ms_per_view": round(cam_ms, 3),
                         "note": "rays generated inside the traversal kernel from the camera"},
         "with_voxel_stats": {"ms_per_view": round(track_ms, 3), "voxels_seen": tracked_voxels,
