@@ -321,14 +321,22 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
  * bin k of its ray again (its field value is c_field[c_off[r] + k]);
  * -1: to evaluate; -2: unused slot.  The field list of the -1 slots comes
  * from nsvf_render_eval_list; nsvf_render_composite_reuse reads the reused
- * values from the coarse field.                                             */
+ * values from the coarse field.
+ * c_prob: the coarse samples' compositing weights, slot-aligned with the
+ * coarse stream.  Without c_field it holds them (nsvf_render_composite's
+ * prob_out).  With c_field (float4 per coarse slot, the coarse field) the
+ * sampler composites each ray's coarse row itself, with the same arithmetic
+ * and the same bits as nsvf_render_composite, and writes the weights to
+ * c_prob: in a hierarchical view the coarse composite's colour is replaced
+ * by the fine one, so the coarse composite is then not needed.              */
 int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev,
                             int64_t n_hit_max, const int32_t* c_off,
                             const int32_t* c_cnt, const int32_t* c_max,
-                            const void* coarse, const float* c_prob,
-                            const int32_t* f_bound, const int32_t* f_off,
-                            float fine_steps, int compat, void* samples,
-                            int32_t* f_cnt, int16_t* src, void* stream);
+                            const void* coarse, const void* c_field,
+                            float* c_prob, const int32_t* f_bound,
+                            const int32_t* f_off, float fine_steps,
+                            int compat, void* samples, int32_t* f_cnt,
+                            int16_t* src, void* stream);
 
 /* The field's evaluation list of a marked fine stream: the slots s < *n_dev
  * with src[s] == -1, in slot order.  Three steps:

@@ -344,6 +344,27 @@ __global__ __launch_bounds__(kBlock) void compact_kernel(const int32_t* __restri
 // Bins of a hit ray taken from its sorted hit row (coarse pass).  Slots past
 // the hit count read as the reference's padding (-1, MAX_DEPTH, prob 0);
 // probs are seg/sum computed on the fly, as nsvf.py:70 computes them.
+// One 64-sample pass of a ray's compositing weights (renderer.py:119-121,195-197):
+// lane k of the pass holds fe_k = 7 max(sigma_k, 0) dist_k (0 past the
+// row); p_k = (1 - exp(-fe_k)) exp(-(carry + fe_0 + ... + fe_{k-1})) by an
+// inclusive wave scan, and carry grows by the pass's sum.  The coarse
+// composite and the fine sampler (which composites the coarse row itself)
+// both use it, so their weights are the same bits.
+__device__ __forceinline__ float composite_pass(float fe, bool in, float& carry) {
+  const int lane = threadIdx.x & 63;
+  float incl = fe;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float u = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += u;
+  }
+  const float excl = carry + (incl - fe);
+  const float a = 1.0f - expf(-fe);
+  const float tr = expf(-excl);
+  carry += __shfl(incl, 63, 64);
+  return in ? a * tr : 0.0f;
+}
+
 struct HitBins {
   const HitRec* row;
   int cnt;
@@ -357,13 +378,23 @@ struct HitBins {
   __device__ __forceinline__ float prob(int k) const {
     return k < cnt ? (row[k].hi - row[k].lo) / tot : 0.0f;
   }
+  __device__ __forceinline__ float staged_prob(int k) const { return prob(k); }
+  __device__ __forceinline__ void before_seq() const {}
   __device__ __forceinline__ int32_t next_first() const { return next; }
   __device__ __forceinline__ int32_t ray0_id(int k) const { return k < cnt0 ? row0[k].idx : -1; }
 };
 
 // Bins of a hit ray taken from its coarse samples (fine pass, nerf.py:64-79):
 // [depth - dist/2, depth + dist/2], probs (p + 1e-5) / sum over all P columns.
-struct SampleBins {
+constexpr int kRegPasses = 2;  // 64-bin passes of coarse weights a fine-sampler lane holds
+
+// kReg: the fine sampler composited the coarse row itself; prob_c holds all
+// the weights, each stored by the lane of its bin (bin 64 q + lane), and the
+// first kRegPasses * 64 are in registers too (lane's pr[q]).  A staged bin
+// is the lane's own: from pr, or read back from prob_c; lane 0's sequential
+// walk reads them after before_seq().
+template <bool kReg>
+struct SampleBinsT {
   const SampleRec* rec;
   const float* prob_c;
   int cnt;
@@ -371,6 +402,7 @@ struct SampleBins {
   const SampleRec* rec0;
   int cnt0;
   int32_t next;
+  float pr[kRegPasses];
   __device__ __forceinline__ int32_t id(int k) const { return k < cnt ? rec[k].idx : -1; }
   __device__ __forceinline__ float lo(int k) const {
     return k < cnt ? rec[k].depth - rec[k].dist * 0.5f : kMaxDepth;
@@ -381,10 +413,31 @@ struct SampleBins {
   __device__ __forceinline__ float prob(int k) const {
     return (k < cnt ? prob_c[k] + 1e-5f : 1e-5f) / den;
   }
+  // prob(k) of the lane's own bin k = 64 q + lane of a staging pass
+  __device__ __forceinline__ float staged_prob(int k) const {
+    if constexpr (kReg) {
+      const int q = k >> 6;
+      if (q >= kRegPasses) return prob(k);
+      float v = pr[0];
+#pragma unroll
+      for (int j = 1; j < kRegPasses; ++j) v = q == j ? pr[j] : v;
+      return (k < cnt ? v + 1e-5f : 1e-5f) / den;
+    } else {
+      return prob(k);
+    }
+  }
+  __device__ __forceinline__ void before_seq() const {
+    if constexpr (kReg) {
+      // the weights were stored by every lane of the wave (one work-group)
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
   __device__ __forceinline__ float depth(int k) const { return k < cnt ? rec[k].depth : 0.0f; }
   __device__ __forceinline__ int32_t next_first() const { return next; }
   __device__ __forceinline__ int32_t ray0_id(int k) const { return k < cnt0 ? rec0[k].idx : -1; }
 };
+using SampleBins = SampleBinsT<false>;
 
 __device__ __forceinline__ void grouped_neighbours(int64_t n, int64_t N, int64_t* ray0,
                                                    int64_t* next) {
@@ -471,6 +524,31 @@ __device__ __forceinline__ float fine_den(const float* prob, int cnt, int P) {
   return part;
 }
 
+// fine_den with the ray's first kRegPasses * 64 weights in registers (column
+// 64 q + lane in pr[q]); later columns from prob, stored by this same lane.
+__device__ __forceinline__ float fine_den_reg(const float (&pr)[kRegPasses], const float* prob,
+                                              int cnt, int P) {
+  const int lane = threadIdx.x & 63;
+  float part = 0.0f;
+  for (int k = lane; k < P; k += 64) {
+    const int q = k >> 6;
+    float v = 0.0f;
+    if (k < cnt) {
+      if (q < kRegPasses) {
+        v = pr[0];
+#pragma unroll
+        for (int j = 1; j < kRegPasses; ++j) v = q == j ? pr[j] : v;
+      } else {
+        v = prob[k];
+      }
+    }
+    part = part + (v + 1e-5f);
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) part = part + __shfl_xor(part, d, 64);
+  return part;
+}
+
 // Field reuse in the fine pass.  The fine pass evaluates the same field as
 // the coarse pass (nerf.py:58: field_fine only with --use-fine-model), and
 // the field of a sample depends only on its ray, voxel id and depth.  A
@@ -498,11 +576,17 @@ struct FieldReuse {
 // Fine pass: one wavefront per hit ray; bins are the ray's coarse samples and
 // the *c_max columns of the longest coarse row (nerf.py:64-79).  With src
 // (optional), the slots are marked for field reuse as FieldReuse describes.
+// kComposite: the wave first composites the ray's coarse row from c_field
+// (composite_pass, as the coarse composite does) and stores the weights to
+// c_prob, keeping the staged bins' weights in registers; otherwise c_prob
+// holds them.
+template <bool kComposite>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void fine_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_cnt,
     const int32_t* __restrict__ c_max_dev, const SampleRec* __restrict__ coarse,
-    const float* __restrict__ c_prob, const int32_t* __restrict__ f_bound,
+    const float4* __restrict__ c_field, float* __restrict__ c_prob,
+    const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
     int32_t* __restrict__ f_cnt, int16_t* __restrict__ src) {
   __shared__ float lds_c[kSampleWaves][kFineBins], lds_lo[kSampleWaves][kFineBins],
@@ -524,16 +608,39 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
       const int32_t r0 = hit_list[n0];
       const int64_t off = c_off[r];
       const int cnt = c_cnt[r];
-      const float den = fine_den(c_prob + off, cnt, P);
       int32_t next = -1;
       if (nn >= 0) {
         const int32_t rn = hit_list[nn];
         next = c_cnt[rn] > 0 ? coarse[c_off[rn]].idx : -1;
       }
-      SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0], next};
-      m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst,
-                    WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave], kFineBins},
-                    reuse);
+      const WaveBinsLds lds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave], kFineBins};
+      if constexpr (kComposite) {
+        SampleBinsT<true> bins{coarse + off, c_prob + off, cnt, 0.0f, coarse + c_off[r0],
+                               c_cnt[r0], next};
+#pragma unroll
+        for (int q = 0; q < kRegPasses; ++q) bins.pr[q] = 0.0f;
+        float carry = 0.0f;
+        for (int base = 0; base < cnt; base += 64) {
+          const int k = base + lane;
+          float fe = 0.0f;
+          if (k < cnt) {
+            const float sg = c_field[off + k].x;
+            fe = ((sg > 0.0f ? sg : 0.0f) * coarse[off + k].dist) * 7.0f;
+          }
+          const float p = composite_pass(fe, k < cnt, carry);
+          if (k < cnt) c_prob[off + k] = p;
+          const int q = base >> 6;
+#pragma unroll
+          for (int j = 0; j < kRegPasses; ++j) bins.pr[j] = q == j ? p : bins.pr[j];
+        }
+        bins.den = fine_den_reg(bins.pr, c_prob + off, cnt, P);
+        m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse);
+      } else {
+        const float den = fine_den(c_prob + off, cnt, P);
+        SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0],
+                        next};
+        m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse);
+      }
     } else {
       for (int k = lane; k < b; k += 64) {
         dst[k] = SampleRec{-1, 0.0f, 0.0f, r};
@@ -793,24 +900,13 @@ __global__ __launch_bounds__(kBlock) void composite_kernel(
       fe = (sg * sr.dist) * 7.0f;
       z = sr.depth;
     }
-    // inclusive scan of fe over the wave
-    float incl = fe;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const float u = __shfl_up(incl, d, 64);
-      if (lane >= d) incl += u;
-    }
-    const float excl = carry + (incl - fe);
-    const float a = 1.0f - expf(-fe);
-    const float tr = expf(-excl);
-    const float p = k < cnt ? a * tr : 0.0f;
+    const float p = composite_pass(fe, k < cnt, carry);
     if (k < cnt && prob_out) prob_out[off + k] = p;
     acc_p += p;
     acc_z += p * z;
     acc_r += p * f.y;
     acc_g += p * f.z;
     acc_b += p * f.w;
-    carry += __shfl(incl, 63, 64);
   }
   acc_p = wave_sum(acc_p);
   acc_z = wave_sum(acc_z);
@@ -1029,16 +1125,24 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
 
 int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev, int64_t n_hit_max,
                             const int32_t* c_off, const int32_t* c_cnt, const int32_t* c_max,
-                            const void* coarse, const float* c_prob, const int32_t* f_bound,
-                            const int32_t* f_off, float fine_steps, int compat, void* samples,
-                            int32_t* f_cnt, int16_t* src, void* stream) {
+                            const void* coarse, const void* c_field, float* c_prob,
+                            const int32_t* f_bound, const int32_t* f_off, float fine_steps,
+                            int compat, void* samples, int32_t* f_cnt, int16_t* src,
+                            void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
-  hipLaunchKernelGGL(fine_sample_kernel,
-                     dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
-                     dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, c_off, c_cnt, c_max,
-                     static_cast<const SampleRec*>(coarse), c_prob, f_bound, f_off, fine_steps,
-                     compat, static_cast<SampleRec*>(samples), f_cnt, src);
+  const dim3 grid(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192)));
+  if (c_field)
+    hipLaunchKernelGGL(fine_sample_kernel<true>, grid, dim3(kBlock), 0, as_stream(stream),
+                       hit_list, n_hit_dev, c_off, c_cnt, c_max,
+                       static_cast<const SampleRec*>(coarse), static_cast<const float4*>(c_field),
+                       c_prob, f_bound, f_off, fine_steps, compat,
+                       static_cast<SampleRec*>(samples), f_cnt, src);
+  else
+    hipLaunchKernelGGL(fine_sample_kernel<false>, grid, dim3(kBlock), 0, as_stream(stream),
+                       hit_list, n_hit_dev, c_off, c_cnt, c_max,
+                       static_cast<const SampleRec*>(coarse), nullptr, c_prob, f_bound, f_off,
+                       fine_steps, compat, static_cast<SampleRec*>(samples), f_cnt, src);
   return check_launch("render_sample_fine");
 }
 

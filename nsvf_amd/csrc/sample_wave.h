@@ -114,7 +114,7 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       const int k = t0 + lane;
       float pk = 0.0f;
       if (k < Pv) {
-        pk = bins.prob(k);
+        pk = bins.staged_prob(k);
         const float lk = bins.lo(k), hk = bins.hi(k);
         lds.lo[k] = lk;
         lds.hi[k] = hk;
@@ -140,6 +140,7 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
     seq = isnan(c);
   }
   if (seq) {
+    bins.before_seq();
     if (lane == 0) {
       StreamSink sink{out, ray, bound};
       ConstNoise noise{0.5f};

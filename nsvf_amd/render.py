@@ -181,8 +181,12 @@ class Renderer:
     grown as needed, so repeated views of the same size allocate nothing."""
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
-                 compat=True, split_field=None, reuse_coarse=True):
+                 compat=True, split_field=None, reuse_coarse=True, fold_composite=True):
         self.device = torch.device(device)
+        # hierarchical views: the fine sampler composites the coarse rows
+        # itself (the coarse composite's weights, bit for bit) instead of a
+        # coarse composite kernel whose colour the fine pass replaces
+        self.fold_composite = bool(fold_composite)
         # fine pass: copy the coarse field value of every fine sample that is
         # its bin's coarse sample again (same voxel and depth bits) and
         # evaluate the field on the other fine samples only (same results)
@@ -495,12 +499,19 @@ class Renderer:
         ray_rgb = self._buf("ray_rgb", R * 3, torch.float32)
         ray_depth = self._buf("ray_depth", R, torch.float32)
         ray_missed = self._buf("ray_missed", R, torch.float32)
-        check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), max(n_hit, 1), _p(c_off),
-                                      _p(c_cnt), _p(cs), _p(c_field), _p(c_prob), _p(ray_rgb),
-                                      _p(ray_depth), _p(ray_missed), st), "composite coarse")
-        self._track(sc, hit_list, scal, n_hit, c_off, c_cnt, cs, c_prob, st)
+        fine = self.hierarchical and n_hit > 0
+        fold = fine and self.fold_composite
+        if not fold:
+            check(L.nsvf_render_composite(_p(hit_list), _p(scal[0:1]), max(n_hit, 1), _p(c_off),
+                                          _p(c_cnt), _p(cs), _p(c_field), _p(c_prob),
+                                          _p(ray_rgb), _p(ray_depth), _p(ray_missed), st),
+                  "composite coarse")
+            self._track(sc, hit_list, scal, n_hit, c_off, c_cnt, cs, c_prob, st)
+        # in a hierarchical view the fine composite replaces the coarse colour
+        # of every hit ray: folded, the fine sampler composites the coarse rows
+        # itself (the same weights, into c_prob) and no coarse composite runs
         self._mark("coarse_composite")
-        if self.hierarchical and n_hit > 0:
+        if fine:
             f_off = self._buf("f_off", R, torch.int32)
             self._scan(f_bound, f_off, R, scal[3:4])
             fsm = self._stream("f_samples", slots_f)
@@ -511,9 +522,11 @@ class Renderer:
             if reuse:
                 src = self._buf("f_src", slots_f, torch.int16)
             check(L.nsvf_render_sample_fine(_p(hit_list), _p(scal[0:1]), R, _p(c_off), _p(c_cnt),
-                                            _p(scal[2:3]), _p(cs), _p(c_prob), _p(f_bound),
-                                            _p(f_off), fs, self.compat, _p(fsm), _p(f_cnt),
-                                            _p(src), st), "sample_fine")
+                                            _p(scal[2:3]), _p(cs), _p(c_field) if fold else None,
+                                            _p(c_prob), _p(f_bound), _p(f_off), fs, self.compat,
+                                            _p(fsm), _p(f_cnt), _p(src), st), "sample_fine")
+            if fold:
+                self._track(sc, hit_list, scal, n_hit, c_off, c_cnt, cs, c_prob, st)
             if reuse:
                 # the field's list of the fine slots to evaluate (src == -1),
                 # its length in scal[4]

@@ -178,3 +178,79 @@ def test_rows_past_the_staged_bins(device, cube, compat):
                                  np.float32(O.MAX_DEPTH)), ref2["f_depth"])
     assert np.abs(rgb - ref2["rgb"]).max() < 1e-5
     assert np.abs(missed - ref2["missed"]).max() < 1e-5
+
+
+def sampler_pair(r, device):
+    """The coarse weights and fine stream of the view r rendered last (kept),
+    and the same from the two-kernel sequence the fine sampler folds in:
+    nsvf_render_composite's weights, then nsvf_render_sample_fine without
+    c_field reading them."""
+    from nsvf_amd._lib import check, lib
+    from nsvf_amd.render import _p, _stream
+    L, ws, k = lib(), r._ws, r.kept
+    R = len(k["count"])
+    n_hit = len(k["hit_list"])
+    scal, st = ws["scalars"], _stream(device)
+    prob = torch.full_like(k["c_prob"], float("nan"))
+    tmp = torch.empty(R * 5, dtype=torch.float32, device=device)
+    check(L.nsvf_render_composite(_p(ws["hit_list"]), _p(scal[0:1]), n_hit, _p(ws["c_off"]),
+                                  _p(ws["c_cnt"]), _p(ws["c_samples"]), _p(ws["c_field"]),
+                                  _p(prob), _p(tmp[: R * 3]), _p(tmp[R * 3: R * 4]),
+                                  _p(tmp[R * 4:]), st), "composite")
+    fs = torch.full_like(ws["f_samples"], -7)
+    f_cnt = torch.full_like(ws["f_cnt"], -7)
+    src = torch.full_like(ws["f_src"], -7)
+    check(L.nsvf_render_sample_fine(_p(ws["hit_list"]), _p(scal[0:1]), R, _p(ws["c_off"]),
+                                    _p(ws["c_cnt"]), _p(scal[2:3]), _p(ws["c_samples"]), None,
+                                    _p(prob), _p(ws["f_bound"]), _p(ws["f_off"]), r.fine_steps,
+                                    r.compat, _p(fs), _p(f_cnt), _p(src), st), "sample_fine")
+    torch.cuda.synchronize()
+    hl = N(k["hit_list"]).astype(np.int64)
+    m = len(k["f_samples"][0])
+    return (N(prob), N(fs)[: m * 4].reshape(m, 4), N(f_cnt)[hl], N(src), N(ws["f_bound"])[hl],
+            N(ws["f_src"]))
+
+
+@pytest.mark.parametrize("scene", ["shell", "cube"])
+@pytest.mark.parametrize("compat", [False, True])
+def test_fine_sampler_composites_the_coarse_rows(device, cube, scene, compat):
+    """The fine sampler composites each ray's coarse row itself (no coarse
+    composite runs in a hierarchical view): its weights, fine stream, counts
+    and reuse marks equal the two-kernel sequence's bit for bit -- on the
+    shell and on the cube's rows of more than 128 coarse samples (weights
+    read back) and of more than 384 (the sequential walk)."""
+    if scene == "shell":
+        sc = shell()
+        rng = np.random.default_rng(6)
+        rs = np.tile(np.array([[0.2, -2.5, 1.1]], np.float32), (3000, 1))
+        tgt = rng.uniform(-0.8, 0.8, (3000, 3)).astype(np.float32)
+        rd = (tgt - rs).astype(np.float32)
+        rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    else:
+        sc, rs, rd = cube
+    dsc = DeviceScene(sc, device)
+    rs_d = torch.from_numpy(np.ascontiguousarray(rs)).to(device)
+    rd_d = torch.from_numpy(np.ascontiguousarray(rd)).to(device)
+    unfolded = Renderer(device, precision="bf16", hierarchical=True, compat=compat,
+                        fold_composite=False)
+    want = [N(t) for t in unfolded.render(dsc, rs_d, rd_d)]
+    r = Renderer(device, precision="bf16", hierarchical=True, compat=compat)
+    out = [N(t) for t in r.render(dsc, rs_d, rd_d, keep=True)]
+    for a, b in zip(out, want):  # the image, bit for bit
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    k = r.kept
+    prob, fs, f_cnt, src, f_bound, src_got = sampler_pair(r, device)
+    c_off, c_cnt = N(k["c_off"]), N(k["c_cnt"])
+    if scene == "cube":
+        assert (c_cnt > 384).sum() > 10
+    got_p = N(k["c_prob"])
+    for o, n in zip(c_off, c_cnt):
+        assert np.array_equal(got_p[o: o + n].view(np.int32), prob[o: o + n].view(np.int32))
+    f_off = N(k["f_off"])
+    assert np.array_equal(N(k["f_cnt"]), f_cnt)
+    got = N(k["f_samples"][0]), N(k["f_samples"][1]), N(k["f_samples"][2])
+    for o, n, b in zip(f_off, f_cnt, f_bound):
+        assert np.array_equal(got[0][o: o + n], fs[o: o + n, 0])
+        assert np.array_equal(got[1][o: o + n].view(np.int32), fs[o: o + n, 1])
+        assert np.array_equal(got[2][o: o + n].view(np.int32), fs[o: o + n, 2])
+        assert np.array_equal(src_got[o: o + b], src[o: o + b])
