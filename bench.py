@@ -4,8 +4,13 @@ voxel/8, 64-sample hierarchical pass, field 32 -> 3x128 -> 4) on one MI355X,
 with p50/p99 view latency, the plain FP32 GPU path and the CPU oracle timed
 beside it, and the roofline of the dominant kernel.
 
+The headline runs the field at the fastest precision whose per-pixel RGB
+error against the FP64 oracle stays within 1e-3 on this view: split BF16
+("bf16x3": three BF16 MFMAs per product on hi / lo operand parts, FP32
+accumulation).  Plain BF16 MFMA and FP32 MFMA are timed and checked beside it.
+
   python bench.py [--steps K] [--warmup W] [--gpus 1] [--plain-views V]
-                  [--cpu-rays C] [--no-plain] [--no-cpu]
+                  [--cpu-rays C] [--check-rays A] [--no-plain] [--no-cpu]
 
 Prints one JSON line.  A step is one view: rays already on the device to the
 RGB, depth and alpha image.
@@ -26,7 +31,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "rays/sec @800x800 (intersect+sample+render) vs plain FP32; p50/p99 view latency"
 PEAK_BF16_TFLOPS = 2500.0     # MI355X dense BF16 MFMA
+PEAK_FP32_TFLOPS = 157.3      # MI355X dense FP32 MFMA
 PEAK_HBM_GBS = 8000.0
+HEADLINE = "bf16x3"
+RGB_BAR = 1e-3                # per-pixel RGB error target against FP64
+GRAZE_EPS = 1e-6              # a hit shorter than this grazes a voxel face
 
 
 def parse():
@@ -37,16 +46,23 @@ def parse():
     ap.add_argument("--res", type=int, default=800)
     ap.add_argument("--voxels", type=int, default=100_000)
     ap.add_argument("--plain-views", type=int, default=2)
-    ap.add_argument("--cpu-rays", type=int, default=24000)
+    ap.add_argument("--cpu-rays", type=int, default=6000,
+                    help="rays of the view in each timed CPU-baseline run (median of --cpu-reps)")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--check-rays", type=int, default=3000,
+                    help="random rays of the view checked against the FP64 oracle (plus every "
+                         "ray whose hit row is full)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="time the headline precision only (for profiling runs)")
     ap.add_argument("--camera-seed", type=int, default=11)
     ap.add_argument("--field", choices=["auto", "split", "fused"], default="auto",
                     help="BF16 field as a gather kernel + an MLP kernel, one fused kernel, or "
                          "fused where the fused kernel supports the field's shape")
     ap.add_argument("--serve-requests", type=int, default=40,
                     help="requests per arrival rate in the serving measurement (0 = skip)")
-    ap.add_argument("--serve-rates", default="10,20,40",
+    ap.add_argument("--serve-rates", default="10,20,40,80,160",
                     help="Poisson arrival rates (requests/s) of the serving measurement")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the backward / training / pruning / splitting measurements")
@@ -70,6 +86,60 @@ def measured_traffic(kernel):
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def time_views(rnd, dsc, rs, rd, width, steps, warmup):
+    """Mean wall ms per view over `steps` views after `warmup`, the per-view
+    HIP-event latencies, and the field-kernel events of the timed views."""
+    import torch
+    for _ in range(warmup):
+        rnd.render(dsc, rs, rd, width=width)
+    torch.cuda.synchronize()
+    kev = rnd.record_kernels(True)
+    lat = []
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rnd.render(dsc, rs, rd, width=width)
+        e1.record()
+        lat.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    rnd.record_kernels(False)
+    lat_ms = np.array([a.elapsed_time(b) for a, b in lat])
+    return wall * 1000.0 / steps, lat_ms, kev
+
+
+def kernel_ms(kev, name):
+    ev = [(a, b) for k, _, a, b in kev if k == name]
+    return sum(a.elapsed_time(b) for a, b in ev), len(ev)
+
+
+def stage_rooflines(stages, R, n_hit, nodes, P, n_coarse_slots, n_fine_slots, n_fine_eval):
+    """Algorithmic HBM bytes of the bandwidth-bound stages of a view (padded
+    contract of the reference ops: R rays, P = max_hits, T octree nodes, M
+    samples of 16-byte records) over their traced times."""
+    rows = {
+        # rays in, node records once, sorted hit rows out (lo, hi, idx)
+        "intersect": R * 24 + nodes * 48 + R * P * 12,
+        # hit rows in, sample records out
+        "coarse_sample": n_hit * (16 * P + 4) + n_coarse_slots * 16,
+        # coarse samples and field in (bins + weights), fine records and marks out
+        "fine_sample": n_coarse_slots * (16 + 16) + n_hit * 4 + n_fine_slots * (16 + 2),
+        # samples and field in, one rgb / depth / missed per ray out
+        "fine_composite": n_fine_slots * (16 + 16 + 2) + R * 20,
+    }
+    out = {}
+    for k, b in rows.items():
+        ms = stages.get(k)
+        if ms:
+            gbs = b / (ms * 1e-3) / 1e9
+            out[k] = {"ms": ms, "algorithmic_bytes": int(b), "achieved_GBs": round(gbs, 1),
+                      "hbm_frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return out
 
 
 def main():
@@ -101,70 +171,58 @@ def main():
     rd = torch.from_numpy(rd_np).to(dev)
     log(f"scene {sc.num_voxels} voxels, {sc.num_keys} keys, {dsc.num_nodes} octree nodes "
         f"({time.time() - t0:.1f}s setup)")
+    split_arg = {"auto": None, "split": True, "fused": False}[args.field]
+    flops_per_sample = sc.mlp.flops_per_sample()
 
-    # ---- fused BF16 path
-    rnd = Renderer(dev, precision="bf16", hierarchical=True, fine_steps=64.0,
-                   split_field={"auto": None, "split": True, "fused": False}[args.field])
-    for _ in range(args.warmup):
-        rnd.render(dsc, rs, rd, width=args.res)
-    torch.cuda.synchronize()
-    kev = rnd.record_kernels(True)
-    lat = []
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        rnd.render(dsc, rs, rd, width=args.res)
-        e1.record()
-        lat.append((e0, e1))
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-    rnd.record_kernels(False)
-    lat_ms = np.array([a.elapsed_time(b) for a, b in lat])
-    ms_per_step = wall * 1000.0 / args.steps
-    value = R / (wall / args.steps)
+    # ---- the view at each field precision (headline first)
+    precs = [HEADLINE] if args.only_headline else [HEADLINE, "bf16", "fp32"]
+    res = {}
+    for prec in precs:
+        rnd = Renderer(dev, precision=prec, hierarchical=True, fine_steps=64.0,
+                       split_field=split_arg if prec == "bf16" else None)
+        steps = args.steps if prec != "fp32" else min(args.steps, 5)
+        warm = args.warmup if prec != "fp32" else min(args.warmup, 2)
+        ms, lat_ms, kev = time_views(rnd, dsc, rs, rd, args.res, steps, warm)
+        res[prec] = {"rnd": rnd, "ms": ms, "lat": lat_ms, "kev": kev, "steps": steps}
+        log(f"{prec}: {ms:.2f} ms/view, {R / ms / 1e3:.2f} Mrays/s")
+
+    h = res[HEADLINE]
+    rnd = h["rnd"]
+    ms_per_step = h["ms"]
+    value = R / (ms_per_step / 1000.0)
     st = rnd.last
     # exact sample counts of the view (identical for every repetition)
-    rnd.render(dsc, rs, rd, keep=True, width=args.res)
+    rgb_h, depth_h, missed_h = rnd.render(dsc, rs, rd, keep=True, width=args.res)
     n_coarse, n_fine = rnd.count_samples()
-    # fine samples the field evaluates (the rest reuse their coarse value)
     n_fine_eval = rnd.kept["f_evaluated"]
+    counts = rnd.kept["count"].cpu().numpy()
+    gpu_rows = rnd.kept["hits"].cpu().numpy()
     rnd.kept = None
-    log(f"fused bf16: {ms_per_step:.2f} ms/view, {value / 1e6:.2f} Mrays/s; hit rays {st.hit_rays}, "
-        f"coarse {n_coarse} samples ({st.coarse_slots} slots), fine {n_fine} samples "
-        f"({st.fine_slots} slots allocated, {n_fine_eval} evaluated)")
+    samples_per_view = n_coarse + n_fine_eval
 
-    # roofline of the dominant kernel: the BF16 field kernel (MFMA-bound) --
-    # the fused gather + MLP kernel, or the MLP kernel of the split path --
-    # with the MLP's algorithmic FLOPs of the samples it evaluates / its
-    # HIP-event time; a split path's gather kernel reported beside it
-    flops_per_sample = sc.mlp.flops_per_sample()
-    n_views = args.steps
-    split = rnd.uses_split_field(dsc)
-    mlp_ev = [(m, a, b) for k, m, a, b in kev if k == ("mlp" if split else "field")]
-    emb_ev = [(m, a, b) for k, m, a, b in kev if k == "embed"]
-    mlp_ms = sum(a.elapsed_time(b) for _, a, b in mlp_ev)
-    emb_ms = sum(a.elapsed_time(b) for _, a, b in emb_ev)
-    samples = (n_coarse + n_fine_eval) * n_views
-    achieved = flops_per_sample * samples / (mlp_ms * 1e-3) / 1e12
-    D = dsc.D
-    emb_bytes = samples * (16 + 2 * D) + n_views * 2 * (
-        sc.num_voxels * 44 + sc.num_keys * 4 * D)
-    kname = "mlp_bf16_kernel" if split else "field_fused_kernel"
-    roofline = {"bound": "mfma", "kernel": kname + "<32,128,3>",
+    # roofline of the dominant kernel: the headline's MLP kernel (MFMA-bound),
+    # MFMA work = 3 BF16 products per product of the model's FLOPs
+    mlp_ms, n_mlp = kernel_ms(h["kev"], "mlp")
+    emb_ms, n_emb = kernel_ms(h["kev"], "embed")
+    x3_flops = 3 * flops_per_sample
+    samples = samples_per_view * h["steps"]
+    achieved = x3_flops * samples / (mlp_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "mlp_bf16x3_kernel<32,128,3>",
                 "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
-                "traffic": measured_traffic(kname),
-                "avg_launch_ms": round(mlp_ms / max(len(mlp_ev), 1), 4),
-                "flops_per_sample": flops_per_sample,
-                "samples_per_launch": round(samples / max(len(mlp_ev), 1))}
-    kernels = {}
-    if emb_ev:
-        kernels["embed_bf16_kernel"] = {"avg_launch_ms": round(emb_ms / max(len(emb_ev), 1), 4),
-                                        "achieved_GBs": round(emb_bytes / (emb_ms * 1e-3) / 1e9, 1),
-                                        "bound": "hbm/l2"}
+                "traffic": measured_traffic("mlp_bf16x3_kernel"),
+                "avg_launch_ms": round(mlp_ms / max(n_mlp, 1), 4),
+                "flops_per_sample": x3_flops,
+                "model_flops_per_sample": flops_per_sample,
+                "model_TFLOPs": round(achieved / 3, 2),
+                "samples_per_launch": round(samples / max(n_mlp, 1))}
+    D = dsc.D
+    emb_bytes = samples * (16 + 4 * D) + n_emb * (sc.num_voxels * 44 + sc.num_keys * 4 * D)
+    kernels = {"embed_bf16x3_kernel": {
+        "avg_launch_ms": round(emb_ms / max(n_emb, 1), 4),
+        "algorithmic_bytes_per_launch": round(emb_bytes / max(n_emb, 1)),
+        "achieved_GBs": round(emb_bytes / (emb_ms * 1e-3) / 1e9, 1) if emb_ms else None,
+        "bound": "hbm", "traffic": measured_traffic("embed_bf16x3_kernel")}}
 
     # the same view from its camera: rays generated inside the traversal
     # kernel (outside the timed region; reported beside the metric)
@@ -200,24 +258,47 @@ def main():
         stages[n1] = round(e0.elapsed_time(e1), 3)
     rnd.trace = None
 
+    precisions = {}
+    for prec, r in res.items():
+        entry = {"ms_per_view": round(r["ms"], 3), "rays_per_sec": round(R / r["ms"] * 1e3, 1),
+                 "p50_ms": round(float(np.percentile(r["lat"], 50)), 3),
+                 "p99_ms": round(float(np.percentile(r["lat"], 99)), 3), "views": r["steps"]}
+        fk = "mlp" if prec == HEADLINE else "field"
+        fms, nf = kernel_ms(r["kev"], fk)
+        if prec == "bf16" and r["rnd"].uses_split_field(dsc):
+            fms, nf = kernel_ms(r["kev"], "mlp")
+        if nf:
+            mult = 3 if prec == HEADLINE else 1
+            peak = PEAK_FP32_TFLOPS if prec == "fp32" else PEAK_BF16_TFLOPS
+            tf = mult * flops_per_sample * samples_per_view * r["steps"] / (fms * 1e-3) / 1e12
+            entry["field_kernel"] = {"avg_launch_ms": round(fms / nf, 4),
+                                     "achieved_TFLOPs": round(tf, 1), "peak_TFLOPs": peak,
+                                     "frac": round(tf / peak, 4)}
+        precisions[prec] = entry
+
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "rays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
         "data": "synthetic",
         "config": {"workload": f"noise{128}: 128^3 grid, {sc.num_voxels} voxels, SVO max_hits 60, "
                                f"inverse-CDF step voxel/8 + 64 hierarchical samples, "
-                               f"{args.res}x{args.res} view, field 32->3x128->4 BF16 MFMA",
+                               f"{args.res}x{args.res} view, field 32->3x128->4 split-BF16 MFMA "
+                               f"(bf16x3, FP32 accumulation)",
                    "rays_per_view": R, "voxels": sc.num_voxels, "octree_nodes": dsc.num_nodes,
+                   "sampler": "reference mode (compat quirks reproduced)",
                    "parallelism": "single GPU"},
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
+        "p50_ms": round(float(np.percentile(h["lat"], 50)), 3),
+        "p99_ms": round(float(np.percentile(h["lat"], 99)), 3),
         "samples_per_view": {"hit_rays": st.hit_rays, "coarse": n_coarse, "fine": n_fine,
                              "fine_evaluated": n_fine_eval,
                              "coarse_slots": st.coarse_slots},
         "stages_ms": stages,
         "roofline": roofline,
-        **({"kernels": kernels} if kernels else {}),
+        "kernels": kernels,
+        "stage_rooflines": stage_rooflines(stages, R, st.hit_rays, dsc.num_nodes, sc.max_hits,
+                                           st.coarse_slots, st.fine_slots, n_fine_eval),
+        "precisions": precisions,
         "from_camera": {"ms_per_view": round(cam_ms, 3),
                         "note": "rays generated inside the traversal kernel from the camera"},
         "with_voxel_stats": {"ms_per_view": round(track_ms, 3), "voxels_seen": tracked_voxels,
@@ -235,33 +316,29 @@ def main():
             prgb, _, _ = pl.render(rs, rd)
         torch.cuda.synchronize()
         pms = (time.perf_counter() - t1) * 1000.0 / args.plain_views
-        frgb, _, _ = rnd.render(dsc, rs, rd, width=args.res)
-        mse = float(torch.mean((frgb.double() - prgb.double()) ** 2))
-        out["plain_fp32"] = {"rays_per_sec": round(R / (pms / 1000.0), 1), "ms_per_view": round(pms, 2),
-                             "views": args.plain_views,
-                             "psnr_fused_vs_plain_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None}
+        plain = {"rays_per_sec": round(R / (pms / 1000.0), 1), "ms_per_view": round(pms, 2),
+                 "views": args.plain_views}
+        for prec, r in res.items():
+            frgb, _, _ = r["rnd"].render(dsc, rs, rd, width=args.res)
+            d = frgb.double() - prgb.double()
+            mse = float(torch.mean(d * d))
+            plain[f"vs_{prec}"] = {"max_abs_rgb": float(d.abs().max()),
+                                   "psnr_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None}
+        out["plain_fp32"] = plain
         out["speedup_vs_plain_fp32"] = round(value / (R / (pms / 1000.0)), 2)
         log(f"plain fp32: {pms:.1f} ms/view; speedup {out['speedup_vs_plain_fp32']}x")
         del pl
         torch.cuda.empty_cache()
 
-    # ---- CPU baseline: the oracle on a bounded ray sample of the same view
-    if not args.no_cpu and args.cpu_rays > 0:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import nsvf_oracle as O
-        cores = len(os.sched_getaffinity(0))
-        rng = np.random.default_rng(0)
-        sel = np.sort(rng.choice(R, size=min(args.cpu_rays, R), replace=False))
-        t2 = time.perf_counter()
-        O.render_view(sc, rs_np[sel], rd_np[sel], dtype=np.float32)
-        cpu_s = time.perf_counter() - t2
-        out["cpu_baseline"] = {"value": round(len(sel) / cpu_s, 1), "unit": "rays/s", "cores": cores,
-                               "kind": "port",
-                               "sample": f"{len(sel)} seeded random rays of the {args.res}x{args.res} "
-                                         f"view, octree build + intersection + sampling (C++ "
-                                         f"restatement, 1 thread) + gather/MLP/compositing (NumPy "
-                                         f"FP32), {cpu_s:.1f} s"}
-        log(f"cpu oracle: {len(sel) / cpu_s:.0f} rays/s on {cores} cores")
+    # ---- CPU leg: the oracle on the host, timed as the CPU baseline and run
+    # in FP64 as the checker of the GPU outputs on a subset of the view
+    if not args.no_cpu:
+        views = {p: [t.cpu().numpy() for t in r["rnd"].render(dsc, rs, rd, width=args.res)]
+                 for p, r in res.items()}
+        cpu_base, accuracy = cpu_leg(args, sc, dsc, rs_np, rd_np, views, counts, gpu_rows)
+        if cpu_base:
+            out["cpu_baseline"] = cpu_base
+        out["accuracy_vs_fp64"] = accuracy
 
     if not args.no_rows:
         out["rows"] = rows(args, dev, sc, dsc)
@@ -270,6 +347,84 @@ def main():
         out["serving"] = serving(args, dev)
 
     print(json.dumps(out), flush=True)
+
+
+def cpu_leg(args, sc, dsc, rs_np, rd_np, views, counts, gpu_rows):
+    """The oracle on the host cores.  (1) CPU baseline: the reference path
+    restated on the CPU (octree build, intersection and sampling in C++, one
+    thread; gather, MLP and compositing in NumPy float32 on BLAS threads) on
+    a seeded random sample of the view's rays, the median of --cpu-reps
+    runs.  (2) Checker: the same path in float64 on a seeded subset of the
+    view (plus every ray with a full hit row) against each precision's GPU
+    view -- max abs error per output -- and the hit rows against the FP32
+    restatement (bit-exact) and an FP64 slab test (grazing rays)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import nsvf_oracle as O
+    from threadpoolctl import threadpool_limits
+
+    cores = len(os.sched_getaffinity(0))
+    R = rs_np.shape[0]
+    base = None
+    if args.cpu_rays > 0 and args.cpu_reps > 0:
+        rng = np.random.default_rng(0)
+        sel = np.sort(rng.choice(R, size=min(args.cpu_rays, R), replace=False))
+        times = []
+        with threadpool_limits(limits=cores):
+            t0 = time.perf_counter()
+            O.build_easy_octree(sc.points, np.float32(sc.voxel_size) * np.float32(0.5))
+            build_s = time.perf_counter() - t0
+            for _ in range(args.cpu_reps):
+                t1 = time.perf_counter()
+                O.render_view(sc, rs_np[sel], rd_np[sel], dtype=np.float32)
+                times.append(time.perf_counter() - t1)
+        med = float(np.median(times))
+        # render_view builds the octree once per call: the rate excludes it
+        per_view = max(med - build_s, 1e-9)
+        base = {"value": round(len(sel) / per_view, 1), "unit": "rays/s", "cores": cores,
+                "kind": "port",
+                "sample": f"{len(sel)} seeded random rays of the {args.res}x{args.res} view, median "
+                          f"of {len(times)} runs ({', '.join(f'{t:.2f}' for t in times)} s, each "
+                          f"with a {build_s:.2f} s octree build, excluded from the rate); "
+                          f"intersection + sampling in the C++ restatement (1 thread), gather / "
+                          f"MLP / compositing in NumPy float32 ({cores} BLAS threads)"}
+        log(f"cpu oracle: {base['value']:.0f} rays/s (median of {len(times)})")
+
+    # checker subset: random rays plus the rays whose hit row is full
+    rng = np.random.default_rng(1)
+    full = np.nonzero(counts >= sc.max_hits)[0]
+    sel = np.union1d(rng.choice(R, size=min(args.check_rays, R), replace=False), full)
+    t0 = time.perf_counter()
+    with threadpool_limits(limits=cores):
+        # geometry and sampling over the whole view (the reference sampler's
+        # quirks read neighbouring rays), the FP64 field on the subset
+        ref = O.render_view(sc, rs_np, rd_np, compat=True, dtype=np.float64, subset=sel)
+    acc = {"rays": int(len(sel)), "full_rows": int(len(full)), "bar_rgb": RGB_BAR,
+           "oracle_s": round(time.perf_counter() - t0, 1)}
+    for prec, (rgb, depth, missed) in views.items():
+        e_rgb = float(np.abs(rgb[sel] - ref["rgb"]).max())
+        mse = float(np.mean((rgb[sel].astype(np.float64) - ref["rgb"]) ** 2))
+        acc[prec] = {"max_abs_rgb": e_rgb,
+                     "max_abs_depth": float(np.abs(depth[sel] - ref["depth"]).max()),
+                     "max_abs_alpha": float(np.abs(missed[sel] - ref["missed"]).max()),
+                     "psnr_rgb_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None,
+                     "meets_rgb_bar": e_rgb <= RGB_BAR}
+    # hit rows: bit-exact against the FP32 restatement, and the rays whose
+    # hit set an FP64 slab test decides differently (grazing)
+    P = sc.max_hits
+    idx32, lo32, hi32 = ref["sorted_idx"][sel], ref["sorted_lo"][sel], ref["sorted_hi"][sel]
+    g = gpu_rows[sel]
+    cnt = counts[sel]
+    gi = np.where(np.arange(P)[None] < cnt[:, None], g[:, :, 2], -1)
+    glo = np.where(gi != -1, g[:, :, 0].view(np.float32), np.float32(O.MAX_DEPTH))
+    ghi = np.where(gi != -1, g[:, :, 1].view(np.float32), np.float32(O.MAX_DEPTH))
+    acc["hit_rows_bit_exact_vs_fp32_oracle"] = bool(
+        np.array_equal(gi, idx32) and np.array_equal(glo, lo32) and np.array_equal(ghi, hi32))
+    leaves = O.build_easy_octree(sc.points, np.float32(sc.voxel_size) * np.float32(0.5))[0][:sc.num_voxels]
+    acc["grazing"] = O.grazing_report(gi, glo, ghi, rs_np[sel], rd_np[sel], leaves,
+                                      sc.voxel_size, P, GRAZE_EPS)
+    log("accuracy vs fp64: " + ", ".join(f"{p} rgb {acc[p]['max_abs_rgb']:.2e}" for p in views)
+        + f"; grazing {acc['grazing']}")
+    return base, acc
 
 
 def _time_ms(fn, reps):
@@ -418,7 +573,7 @@ def serving(args, dev):
 
     t0 = time.time()
     scenes = {f"shell{r}": S.shell_scene(res=r, emb_seed=e, mlp_seed=m) for r, e, m in SERVE_SCENES}
-    srv = RenderServer(dev, slots=4, precision="bf16", hierarchical=True)
+    srv = RenderServer(dev, slots=4, precision=HEADLINE, hierarchical=True)
     for name, sc in scenes.items():
         srv.load_scene(name, sc)
     log(f"serving: {len(scenes)} scenes resident "
@@ -472,7 +627,7 @@ def serving(args, dev):
         del pl
         torch.cuda.empty_cache()
     return {"scenes": {n: sc.num_voxels for n, sc in scenes.items()}, "slots": 4,
-            "precision": "bf16", "sampler_compat": bool(srv.compat),
+            "precision": HEADLINE, "sampler_compat": bool(srv.compat),
             "resolutions": list(SERVE_RES), "runs": runs,
             "error_vs_plain_fp32_256": errs, "budget_peak_bytes": srv.budget.peak}
 

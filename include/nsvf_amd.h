@@ -365,7 +365,11 @@ int nsvf_render_reuse_fill(const int32_t* hit_list, const int32_t* n_hit_dev,
 /* Field evaluation of a sample stream: trilinear gather fused with the MLP
  * (L hidden ReLU layers of width H on a D-wide embedding, 4-wide head).
  *   precision 0: BF16 MFMA with FP32 accumulation, wfrag = packed bf16;
- *   precision 1: FP32 MFMA, wfrag = packed f32.
+ *   precision 1: FP32 MFMA, wfrag = packed f32;
+ *   precision 2: split BF16 ("bf16x3"): every product W X as
+ *     Wl Xh + Wh Xl + Wh Xh with W = Wh + Wl, X = Xh + Xl (bf16 parts),
+ *     FP32 accumulation; wfrag = packed bf16 hi / lo fragment pairs; needs
+ *     a workspace and nsvf_field_bf16x3_supported(D, H, L).
  *   samples: *m_dev records (m_max >= *m_dev); unused slots are skipped.
  *   list (optional): *m_dev slot indices -- then sample s is the record at
  *   slot list[s] and its result goes to out[list[s]]; other slots of out
@@ -380,6 +384,7 @@ int nsvf_render_reuse_fill(const int32_t* hit_list, const int32_t* n_hit_dev,
  *   Supported shapes: nsvf_field_supported(D, H, L).                        */
 int nsvf_field_supported(int D, int H, int L);
 int nsvf_field_fused_supported(int D, int H, int L);
+int nsvf_field_bf16x3_supported(int D, int H, int L);
 int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L,
                                    int64_t m_max);
 int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
@@ -405,6 +410,21 @@ int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace,
                         int64_t workspace_bytes, const int32_t* m_dev,
                         int64_t m_max, const void* wfrag, const float* bias,
                         float* out, const int32_t* list, void* stream);
+/* The two kernels of the split-BF16 path (precision 2), one by one: the
+ * gather writes hi and lo operand tiles, the MLP reads them.  Same
+ * arguments as the BF16 pair; workspace of nsvf_field_workspace_bytes(2,
+ * ...) bytes.                                                               */
+int nsvf_field_embed_bf16x3(int D, int H, int L, const float* centers,
+                            const int32_t* feats, const float* values,
+                            float voxel_size, const float* ray_start,
+                            const float* ray_dir, const void* samples,
+                            const int32_t* list, const int32_t* m_dev,
+                            int64_t m_max, void* workspace,
+                            int64_t workspace_bytes, void* stream);
+int nsvf_field_mlp_bf16x3(int D, int H, int L, const void* workspace,
+                          int64_t workspace_bytes, const int32_t* m_dev,
+                          int64_t m_max, const void* wfrag, const float* bias,
+                          float* out, const int32_t* list, void* stream);
 
 /* Front-to-back compositing of each hit ray's samples (one wavefront per
  * ray): free energy relu(sigma) * length * 7, probs, and the ray's rgb,

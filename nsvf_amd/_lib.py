@@ -73,6 +73,10 @@ SIGNATURES = {
     "nsvf_field_embed_bf16": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _L, _P],
                               _I),
     "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P, _P], _I),
+    "nsvf_field_bf16x3_supported": ([_I, _I, _I], _I),
+    "nsvf_field_embed_bf16x3": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _L,
+                                 _P], _I),
+    "nsvf_field_mlp_bf16x3": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P, _P], _I),
     "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_composite_backward": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_track_voxel_probs": ([_P, _P, _L, _P, _P, _P, _P, _P, _L, _P], _I),

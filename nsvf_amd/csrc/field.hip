@@ -158,6 +158,30 @@ __device__ __forceinline__ bf16x8 relu_pack8(const f32x4& lo, const f32x4& hi) {
   return __builtin_bit_cast(bf16x8, u);
 }
 
+// Split-BF16 operands: f = hi + lo with hi = bf16(f) and lo = bf16(f - hi)
+// (the difference is exact in FP32), so a product with hi and lo carries
+// about 16 significant bits of f.  Packs features {lo4[0..3], hi4[0..3]} in
+// the operand order of relu_pack8.
+__device__ __forceinline__ void split_pack2(float a, float b, unsigned int& hi,
+                                            unsigned int& lo) {
+  const bf16x2 h = __builtin_convertvector(f32x2{a, b}, bf16x2);
+  const f32x2 hf = __builtin_convertvector(h, f32x2);
+  const bf16x2 l = __builtin_convertvector(f32x2{a - hf[0], b - hf[1]}, bf16x2);
+  hi = __builtin_bit_cast(unsigned int, h);
+  lo = __builtin_bit_cast(unsigned int, l);
+}
+
+__device__ __forceinline__ void split_pack8(const f32x4& lo4, const f32x4& hi4, bf16x8& hi,
+                                            bf16x8& lo) {
+  unsigned int h0, h1, h2, h3, l0, l1, l2, l3;
+  split_pack2(lo4[0], lo4[1], h0, l0);
+  split_pack2(lo4[2], lo4[3], h1, l1);
+  split_pack2(hi4[0], hi4[1], h2, l2);
+  split_pack2(hi4[2], hi4[3], h3, l3);
+  hi = __builtin_bit_cast(bf16x8, u32x4{h0, h1, h2, h3});
+  lo = __builtin_bit_cast(bf16x8, u32x4{l0, l1, l2, l3});
+}
+
 // 1 / (1 + e^-x) from the hardware base-2 exponential and reciprocal (a few
 // ulp; the head's three sigmoids otherwise cost as much VALU as a hidden
 // layer's epilogue).  Saturates to 0 / 1 for large |x|.
@@ -533,15 +557,21 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
 // Output: the MLP kernel's tile layout -- for k-tile kt and lane group g,
 // features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3} as one 16-byte bf16x8
 // at ((t * NT + j) * KT0 + kt) * 64 + 16g + col, for sample
-// s = t * SAMPLES + 16j + col.
-template <int D, int H, int L>
+// s = t * 16 NT + 16j + col (NT n-tiles of 16 samples per MLP tile; 64 is a
+// multiple of 16 NT).  With kLo the split-BF16 MLP's second operand, the
+// rounding residual bf16(f - bf16(f)) of each feature, goes to the same
+// position lo_off elements further on.
+template <int D, int H, int L, int NT = Dims<D, H, L>::NT, bool kLo = false>
 __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRays rays,
                                                          const SampleRec* __restrict__ samples,
                                                          const int32_t* __restrict__ list,
                                                          const int32_t* m_dev, int64_t m_max,
-                                                         bf16x8* __restrict__ emb) {
+                                                         bf16x8* __restrict__ emb,
+                                                         int64_t lo_off = 0) {
   using Dm = Dims<D, H, L>;
   static_assert(D % 4 == 0, "embedding width must be a multiple of 4");
+  static_assert(64 % (16 * NT) == 0, "MLP tiles must divide the 64-slot gather tiles");
+  constexpr int SAMPLES = 16 * NT;
   constexpr int RUNS = 8;        // voxels staged per round
   constexpr int ROW4 = 2 * D;    // float4 per voxel: 8 corners x D / 4
   // per-run stride padded by one float4, so the same element of different
@@ -551,7 +581,7 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
   __shared__ int32_t run_vox[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t M = min(int64_t(*m_dev), m_max);
-  const int64_t total = ((M + Dm::SAMPLES - 1) / Dm::SAMPLES) * Dm::SAMPLES;
+  const int64_t total = ((M + 63) / 64) * 64;
   // persistent waves over 64-slot tiles; each tile's sample records are
   // loaded one tile ahead, so a tile's dependent chain starts at its keys
   const int64_t ntile = total / 64, stride = int64_t(gridDim.x) * 4;
@@ -647,10 +677,10 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
     }
     wave_sync();
   }
-  const int64_t t = s / Dm::SAMPLES;
-  const int within = int(s - t * Dm::SAMPLES);
+  const int64_t t = s / SAMPLES;
+  const int within = int(s - t * SAMPLES);
   const int j = within >> 4, col = within & 15;
-  bf16x8* base = emb + ((t * Dm::NT + j) * Dm::KT0) * 64 + col;
+  bf16x8* base = emb + ((t * NT + j) * Dm::KT0) * 64 + col;
 #pragma unroll
   for (int kt = 0; kt < Dm::KT0; ++kt) {
 #pragma unroll
@@ -663,13 +693,22 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
           const int feat = 32 * kt + 16 * h + 4 * g + i;
           f[4 * h + i] = feat < D ? acc[feat < D ? feat : 0] : 0.0f;
         }
-      const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
-      const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
-      const bf16x2 p2 = __builtin_convertvector(f32x2{f[4], f[5]}, bf16x2);
-      const bf16x2 p3 = __builtin_convertvector(f32x2{f[6], f[7]}, bf16x2);
-      const u32x4 u = {__builtin_bit_cast(unsigned int, p0), __builtin_bit_cast(unsigned int, p1),
-                       __builtin_bit_cast(unsigned int, p2), __builtin_bit_cast(unsigned int, p3)};
-      base[kt * 64 + 16 * g] = __builtin_bit_cast(bf16x8, u);
+      if constexpr (kLo) {
+        bf16x8 hi, lo;
+        split_pack8(f32x4{f[0], f[1], f[2], f[3]}, f32x4{f[4], f[5], f[6], f[7]}, hi, lo);
+        base[kt * 64 + 16 * g] = hi;
+        base[lo_off + kt * 64 + 16 * g] = lo;
+      } else {
+        const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
+        const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
+        const bf16x2 p2 = __builtin_convertvector(f32x2{f[4], f[5]}, bf16x2);
+        const bf16x2 p3 = __builtin_convertvector(f32x2{f[6], f[7]}, bf16x2);
+        const u32x4 u = {__builtin_bit_cast(unsigned int, p0),
+                         __builtin_bit_cast(unsigned int, p1),
+                         __builtin_bit_cast(unsigned int, p2),
+                         __builtin_bit_cast(unsigned int, p3)};
+        base[kt * 64 + 16 * g] = __builtin_bit_cast(bf16x8, u);
+      }
     }
   }
   }
@@ -713,6 +752,190 @@ __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_
     sink.begin_tile(t * Dm::SAMPLES, M, Dm::SAMPLES);
     mlp_tile<D, H, L>(b0, lw, lb, lane, g, col, t * Dm::SAMPLES, M, sink,
                       tn < tiles ? emb + tn * NT * Dm::KT0 * 64 : nullptr);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Split-BF16 MLP ("bf16x3"): each layer's product W X runs as
+// Wl Xh + Wh Xl + Wh Xh on v_mfma_f32_16x16x32_bf16 with FP32 accumulation,
+// where W = Wh + Wl and X = Xh + Xl are split as in split_pack2 (Henry, Tang
+// and Heinecke, "Leveraging the bfloat16 artificial intelligence datatype
+// for higher-precision computations", ARITH 2019).  The dropped Wl Xl term is
+// about 2^-16 of the product, so a layer keeps about 16 significant bits of
+// its operands where plain BF16 keeps 8; three MFMAs per product.  Weight
+// fragment f is stored as a pair: hi part at [2f][64 lanes], lo part at
+// [2f + 1][64 lanes].  For the shapes it supports both parts fill most of the
+// LDS (D = 32, H = 128, L = 3: 152 KB), so one block of eight waves -- two
+// per SIMD -- runs per CU, each wave on tiles of 32 samples (two n-tiles:
+// the hi and lo activations of four layers' worth of registers fit two waves
+// per SIMD); the layer-0 operands are the gather kernel's hi / lo tiles.
+template <int D, int H, int L>
+struct Dims3 : Dims<D, H, L> {
+  static constexpr int NT = 2;
+  static constexpr int SAMPLES = 32;
+  static constexpr size_t LDS_BYTES =
+      size_t(Dims<D, H, L>::FRAGS) * 2048 + size_t(Dims<D, H, L>::BIASES) * sizeof(float);
+};
+constexpr int kX3Waves = 8;
+
+template <int D, int H, int L>
+constexpr bool x3_supported() {
+  return H <= 128 && Dims3<D, H, L>::LDS_BYTES <= 160 * 1024;
+}
+
+// One split-BF16 layer, out = relu(W in + bias) split into hi / lo, over
+// m-tile pairs with the weight pairs read two ahead (as mfma_relu_layer).
+// Per output the accumulation order is k-tile 0, 1, ..., and within a
+// k-tile Wl Xh, Wh Xl, Wh Xh.
+template <int KT, int MT, int NT>
+__device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B, int lane, int g,
+                                                 const bf16x8 (&xh)[KT][NT],
+                                                 const bf16x8 (&xl)[KT][NT],
+                                                 bf16x8 (&oh)[MT / 2][NT],
+                                                 bf16x8 (&ol)[MT / 2][NT]) {
+  constexpr int kLead = 2, NF = MT * KT, NB = kLead + 1;
+  bf16x8 ah[NB], al[NB];
+#pragma unroll
+  for (int p = 0; p < kLead; ++p)
+    if (p < NF) {
+      ah[p] = A[(2 * p) * 64 + lane];
+      al[p] = A[(2 * p + 1) * 64 + lane];
+    }
+  f32x4 bias_next = *reinterpret_cast<const f32x4*>(B + 4 * g);
+#pragma unroll
+  for (int kt2 = 0; kt2 < MT / 2; ++kt2) {
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      const int mt = 2 * kt2 + half;
+      const f32x4 bias4 = bias_next;
+      if (mt + 1 < MT) bias_next = *reinterpret_cast<const f32x4*>(B + 16 * (mt + 1) + 4 * g);
+#pragma unroll
+      for (int kt = 0; kt < KT; ++kt) {
+        const int f = mt * KT + kt;
+        if (f + kLead < NF) {
+          ah[(f + kLead) % NB] = A[(2 * (f + kLead)) * 64 + lane];
+          al[(f + kLead) % NB] = A[(2 * (f + kLead) + 1) * 64 + lane];
+        }
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+              al[f % NB], xh[kt][j], kt == 0 ? bias4 : acc[half][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[f % NB], xl[kt][j],
+                                                                 acc[half][j], 0, 0, 0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[f % NB], xh[kt][j],
+                                                                 acc[half][j], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      f32x4 r0, r1;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        r0[i] = fmaxf(acc[0][j][i], 0.0f);
+        r1[i] = fmaxf(acc[1][j][i], 0.0f);
+      }
+      split_pack8(r0, r1, oh[kt2][j], ol[kt2][j]);
+    }
+  }
+}
+
+// MLP kernel of the split-BF16 path: operand tiles (hi at emb, lo at
+// emb + lo_off) from embed_bf16_kernel<D, H, L, 2, true>.
+template <int D, int H, int L>
+__global__ __launch_bounds__(64 * kX3Waves, 1) void mlp_bf16x3_kernel(
+    const bf16x8* __restrict__ emb, int64_t lo_off, const int32_t* m_dev, int64_t m_max,
+    const bf16x8* __restrict__ wfrag, const float* __restrict__ bias, float4* __restrict__ out,
+    const int32_t* __restrict__ list) {
+  using Dm = Dims3<D, H, L>;
+  constexpr int NT = Dm::NT, KT0 = Dm::KT0, KTH = Dm::KTH, MTH = Dm::MTH;
+  static_assert(MTH == 2 * KTH, "a layer's output m-tile pairs are the next layer's k-tiles");
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  bf16x8* lw = reinterpret_cast<bf16x8*>(smem_raw);
+  for (int i = threadIdx.x; i < Dm::FRAGS * 2 * 64; i += 64 * kX3Waves) lw[i] = wfrag[i];
+  float* lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 2048);
+  for (int i = threadIdx.x; i < Dm::BIASES; i += 64 * kX3Waves) lb[i] = bias[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
+  const int64_t stride = int64_t(gridDim.x) * kX3Waves;
+  int64_t t = int64_t(blockIdx.x) * kX3Waves + (threadIdx.x >> 6);
+  if (t >= tiles) return;
+  bf16x8 xh[KT0][NT], xl[KT0][NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int kt = 0; kt < KT0; ++kt) {
+      xh[kt][j] = emb[((t * NT + j) * KT0 + kt) * 64 + lane];
+      xl[kt][j] = emb[lo_off + ((t * NT + j) * KT0 + kt) * 64 + lane];
+    }
+  for (; t < tiles; t += stride) {
+    const int64_t tn = t + stride;
+    const int64_t base = t * Dm::SAMPLES;
+    StoreSink sink{out, list};
+    sink.begin_tile(base, M, Dm::SAMPLES);
+    bf16x8 bh[KTH][NT], bl[KTH][NT];
+    mfma3_relu_layer<KT0, MTH, NT>(lw, lb, lane, g, xh, xl, bh, bl);
+    // the layer-0 operands are dead: the next tile's load into them
+    if (tn < tiles) {
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int kt = 0; kt < KT0; ++kt) {
+          xh[kt][j] = emb[((tn * NT + j) * KT0 + kt) * 64 + lane];
+          xl[kt][j] = emb[lo_off + ((tn * NT + j) * KT0 + kt) * 64 + lane];
+        }
+    }
+    // hidden layers 1..L-1, two per trip so the activations alternate
+    // between two register sets without copies
+    bf16x8 nh[KTH][NT], nl[KTH][NT];
+#pragma unroll 1
+    for (int l = 1; l + 1 < L; l += 2) {
+      mfma3_relu_layer<KTH, MTH, NT>(lw + 2 * (Dm::F0 + (l - 1) * Dm::FH) * 64, lb + H * l, lane,
+                                     g, bh, bl, nh, nl);
+      mfma3_relu_layer<KTH, MTH, NT>(lw + 2 * (Dm::F0 + l * Dm::FH) * 64, lb + H * (l + 1), lane,
+                                     g, nh, nl, bh, bl);
+    }
+    if ((L - 1) % 2) {
+      mfma3_relu_layer<KTH, MTH, NT>(lw + 2 * (Dm::F0 + (L - 2) * Dm::FH) * 64, lb + H * (L - 1),
+                                     lane, g, bh, bl, nh, nl);
+#pragma unroll
+      for (int kt = 0; kt < KTH; ++kt)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          bh[kt][j] = nh[kt][j];
+          bl[kt][j] = nl[kt][j];
+        }
+    }
+    // head: 4 outputs in rows 0..3 of one m-tile
+    const bf16x8* A = lw + 2 * (Dm::F0 + (L - 1) * Dm::FH) * 64;
+    const f32x4 bias4 = *reinterpret_cast<const f32x4*>(lb + H * L + 4 * g);
+    f32x4 acc[NT];
+#pragma unroll
+    for (int kt = 0; kt < KTH; ++kt) {
+      const bf16x8 ah = A[(2 * kt) * 64 + lane], al = A[(2 * kt + 1) * 64 + lane];
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[kt][j], kt == 0 ? bias4 : acc[j],
+                                                         0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[kt][j], acc[j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[kt][j], acc[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      const int64_t s = base + 16 * j + col;
+      sink.put(j, s, g == 0 && s < M, acc[j][0], acc[j][1], acc[j][2], acc[j][3]);
+    }
   }
 }
 
@@ -1188,10 +1411,13 @@ int is_pow2(float v) {
   return (m == 0.5f && v > 0.0f && std::isnormal(v) && std::isnormal(1.0f / v)) ? 1 : 0;
 }
 
+// Operand tiles of the split BF16 path: the gather kernel writes every slot
+// of its 64-slot tiles, KT0 bf16x8 per lane and 16-sample column (64 bytes
+// per sample and k-tile).
 template <int D, int H, int L>
 int64_t bf16_workspace_bytes(int64_t m_max) {
   using Dm = Dims<D, H, L>;
-  return ceil_div(m_max, Dm::SAMPLES) * Dm::NT * Dm::KT0 * 64 * 16;
+  return ceil_div(m_max, 64) * 64 * Dm::KT0 * 64;
 }
 
 // Blocks of `kernel` (`threads` per block, `lds` bytes of dynamic LDS) that
@@ -1215,10 +1441,11 @@ int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
   // persistent waves: one resident round of blocks, each wave looping over
   // 64-slot tiles
   static const int64_t resident = resident_blocks(embed_bf16_kernel<D, H, L>);
-  const int64_t tiles64 = ceil_div(m_max, Dm::SAMPLES) * (Dm::SAMPLES / 64);
+  const int64_t tiles64 = ceil_div(m_max, 64);
   const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles64, 4), resident));
   hipLaunchKernelGGL((embed_bf16_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(256), 0, stream,
-                     sc, rays, samples, list, m_dev, m_max, static_cast<bf16x8*>(workspace));
+                     sc, rays, samples, list, m_dev, m_max, static_cast<bf16x8*>(workspace),
+                     int64_t(0));
   return check_launch("field_embed_bf16");
 }
 
@@ -1244,6 +1471,56 @@ int launch_bf16_split(FieldScene sc, FieldRays rays, const SampleRec* samples,
       launch_embed_bf16<D, H, L>(sc, rays, samples, list, m_dev, m_max, workspace, stream);
   if (rc) return rc;
   return launch_mlp_bf16<D, H, L>(m_dev, m_max, wfrag, bias, out, list, workspace, stream);
+}
+
+// Split-BF16 path: hi and lo operand tiles of 32-sample MLP tiles, each
+// region covering every slot of the gather kernel's 64-slot tiles.
+template <int D, int H, int L>
+int64_t x3_lo_offset(int64_t m_max) {  // in bf16x8 elements
+  return ceil_div(m_max, 64) * 64 * Dims<D, H, L>::KT0 * 4;
+}
+
+template <int D, int H, int L>
+int64_t bf16x3_workspace_bytes(int64_t m_max) {
+  return 2 * x3_lo_offset<D, H, L>(m_max) * 16;
+}
+
+template <int D, int H, int L>
+int launch_embed_bf16x3(FieldScene sc, FieldRays rays, const SampleRec* samples,
+                        const int32_t* list, const int32_t* m_dev, int64_t m_max,
+                        void* workspace, hipStream_t stream) {
+  if constexpr (x3_supported<D, H, L>()) {
+    static const int64_t resident = resident_blocks(embed_bf16_kernel<D, H, L, 2, true>);
+    const int64_t blocks =
+        std::max<int64_t>(1, std::min(ceil_div(ceil_div(m_max, 64), 4), resident));
+    hipLaunchKernelGGL((embed_bf16_kernel<D, H, L, 2, true>), dim3(unsigned(blocks)), dim3(256),
+                       0, stream, sc, rays, samples, list, m_dev, m_max,
+                       static_cast<bf16x8*>(workspace), x3_lo_offset<D, H, L>(m_max));
+    return check_launch("field_embed_bf16x3");
+  } else {
+    return set_error(NSVF_EINVAL, "field bf16x3: shape D=%d H=%d L=%d not supported", D, H, L);
+  }
+}
+
+template <int D, int H, int L>
+int launch_mlp_bf16x3(const int32_t* m_dev, int64_t m_max, const void* wfrag, const float* bias,
+                      float* out, const int32_t* list, const void* workspace,
+                      hipStream_t stream) {
+  if constexpr (x3_supported<D, H, L>()) {
+    using Dm = Dims3<D, H, L>;
+    static const int64_t resident =
+        resident_blocks(mlp_bf16x3_kernel<D, H, L>, 64 * kX3Waves, Dm::LDS_BYTES);
+    const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
+    const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles, kX3Waves), resident));
+    hipLaunchKernelGGL((mlp_bf16x3_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(64 * kX3Waves),
+                       Dm::LDS_BYTES, stream, static_cast<const bf16x8*>(workspace),
+                       x3_lo_offset<D, H, L>(m_max), m_dev, m_max,
+                       static_cast<const bf16x8*>(wfrag), bias, reinterpret_cast<float4*>(out),
+                       list);
+    return check_launch("field_mlp_bf16x3");
+  } else {
+    return set_error(NSVF_EINVAL, "field bf16x3: shape D=%d H=%d L=%d not supported", D, H, L);
+  }
 }
 
 // The fused BF16 field over a sample stream: the run-staged gather + MLP
@@ -1343,10 +1620,23 @@ int nsvf_field_fused_supported(int D, int H, int L) {
   return 0;
 }
 
+int nsvf_field_bf16x3_supported(int D, int H, int L) {
+#define NSVF_X3_CASE(d, h, l) \
+  if (D == d && H == h && L == l) return x3_supported<d, h, l>() ? 1 : 0;
+  NSVF_X3_CASE(32, 128, 3)
+  NSVF_X3_CASE(48, 256, 4)
+  NSVF_X3_CASE(32, 256, 4)
+#undef NSVF_X3_CASE
+  return 0;
+}
+
 int64_t nsvf_field_workspace_bytes(int precision, int D, int H, int L, int64_t m_max) {
-  if (precision != 0 || m_max <= 0) return 0;
-#define NSVF_WS_CASE(d, h, l) \
-  if (D == d && H == h && L == l) return bf16_workspace_bytes<d, h, l>(m_max);
+  if ((precision != 0 && precision != 2) || m_max <= 0) return 0;
+#define NSVF_WS_CASE(d, h, l)                                                  \
+  if (D == d && H == h && L == l)                                              \
+    return precision == 2 ? (x3_supported<d, h, l>() ? bf16x3_workspace_bytes<d, h, l>(m_max) \
+                                                     : 0)                      \
+                          : bf16_workspace_bytes<d, h, l>(m_max);
   NSVF_WS_CASE(32, 128, 3)
   NSVF_WS_CASE(48, 256, 4)
   NSVF_WS_CASE(32, 256, 4)
@@ -1370,8 +1660,25 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
   hipStream_t st = as_stream(stream);
   const StreamSource src{sc, rays, static_cast<const SampleRec*>(samples), list};
   const StoreSink sink{reinterpret_cast<float4*>(out), list};
+  if (precision < 0 || precision > 2)
+    return set_error(NSVF_EINVAL, "field_eval: precision %d (0 bf16, 1 fp32, 2 bf16x3)",
+                     precision);
+  if (precision == 2) {
+    if (!nsvf_field_bf16x3_supported(D, H, L))
+      return set_error(NSVF_EINVAL, "field_eval: bf16x3 does not support D=%d H=%d L=%d", D, H,
+                       L);
+    if (!workspace || workspace_bytes < nsvf_field_workspace_bytes(2, D, H, L, m_max))
+      return set_error(NSVF_EINVAL, "field_eval: bf16x3 needs a workspace of %lld bytes",
+                       (long long)nsvf_field_workspace_bytes(2, D, H, L, m_max));
+  }
 #define NSVF_FIELD_CASE(d, h, l)                                                               \
   if (D == d && H == h && L == l) {                                                            \
+    if (precision == 2) {                                                                      \
+      const int rc = launch_embed_bf16x3<d, h, l>(                                             \
+          sc, rays, static_cast<const SampleRec*>(samples), list, m_dev, m_max, workspace, st); \
+      if (rc) return rc;                                                                       \
+      return launch_mlp_bf16x3<d, h, l>(m_dev, m_max, wfrag, bias, out, list, workspace, st);  \
+    }                                                                                          \
     if (precision == 0 && workspace &&                                                        \
         workspace_bytes >= bf16_workspace_bytes<d, h, l>(m_max))                               \
       return launch_bf16_split<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples),      \
@@ -1430,6 +1737,55 @@ int nsvf_field_mlp_bf16(int D, int H, int L, const void* workspace, int64_t work
   NSVF_MLP_CASE(32, 256, 4)
 #undef NSVF_MLP_CASE
   return set_error(NSVF_EINVAL, "field_mlp_bf16: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_field_embed_bf16x3(int D, int H, int L, const float* centers, const int32_t* feats,
+                            const float* values, float voxel_size, const float* ray_start,
+                            const float* ray_dir, const void* samples, const int32_t* list,
+                            const int32_t* m_dev, int64_t m_max, void* workspace,
+                            int64_t workspace_bytes, void* stream) {
+  clear_error();
+  if (m_max <= 0) return NSVF_OK;
+  if (!centers || !feats || !values || !ray_start || !ray_dir || !samples || !m_dev || !workspace)
+    return set_error(NSVF_EINVAL, "field_embed_bf16x3: null pointer");
+  if (!nsvf_field_bf16x3_supported(D, H, L))
+    return set_error(NSVF_EINVAL, "field_embed_bf16x3: unsupported shape D=%d H=%d L=%d", D, H,
+                     L);
+  if (workspace_bytes < nsvf_field_workspace_bytes(2, D, H, L, m_max))
+    return set_error(NSVF_EINVAL, "field_embed_bf16x3: workspace too small");
+  FieldScene sc{centers, feats, values, voxel_size, 1.0f / voxel_size, is_pow2(voxel_size)};
+  FieldRays rays{ray_start, ray_dir};
+#define NSVF_EMB3_CASE(d, h, l)                                                                 \
+  if (D == d && H == h && L == l)                                                               \
+    return launch_embed_bf16x3<d, h, l>(sc, rays, static_cast<const SampleRec*>(samples), list, \
+                                        m_dev, m_max, workspace, as_stream(stream));
+  NSVF_EMB3_CASE(32, 128, 3)
+  NSVF_EMB3_CASE(48, 256, 4)
+  NSVF_EMB3_CASE(32, 256, 4)
+#undef NSVF_EMB3_CASE
+  return set_error(NSVF_EINVAL, "field_embed_bf16x3: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_field_mlp_bf16x3(int D, int H, int L, const void* workspace, int64_t workspace_bytes,
+                          const int32_t* m_dev, int64_t m_max, const void* wfrag,
+                          const float* bias, float* out, const int32_t* list, void* stream) {
+  clear_error();
+  if (m_max <= 0) return NSVF_OK;
+  if (!workspace || !m_dev || !wfrag || !bias || !out)
+    return set_error(NSVF_EINVAL, "field_mlp_bf16x3: null pointer");
+  if (!nsvf_field_bf16x3_supported(D, H, L))
+    return set_error(NSVF_EINVAL, "field_mlp_bf16x3: unsupported shape D=%d H=%d L=%d", D, H, L);
+  if (workspace_bytes < nsvf_field_workspace_bytes(2, D, H, L, m_max))
+    return set_error(NSVF_EINVAL, "field_mlp_bf16x3: workspace too small");
+#define NSVF_MLP3_CASE(d, h, l) \
+  if (D == d && H == h && L == l) \
+    return launch_mlp_bf16x3<d, h, l>(m_dev, m_max, wfrag, bias, out, list, workspace, \
+                                      as_stream(stream));
+  NSVF_MLP3_CASE(32, 128, 3)
+  NSVF_MLP3_CASE(48, 256, 4)
+  NSVF_MLP3_CASE(32, 256, 4)
+#undef NSVF_MLP3_CASE
+  return set_error(NSVF_EINVAL, "field_mlp_bf16x3: unsupported shape D=%d H=%d L=%d", D, H, L);
 }
 
 int nsvf_field_prune(int precision, int D, int H, int L, const float* centers,

@@ -72,9 +72,19 @@ def pack_layer_f32(W, kpad):
     return Wp[rows, cols]                                   # [MT,KT,64]
 
 
+def split_bf16(x):
+    """float32 x as bf16 bit patterns (hi, lo) with hi = bf16(x) and
+    lo = bf16(x - hi) (the difference is exact in float32)."""
+    x = np.ascontiguousarray(x, np.float32)
+    hi = bf16_round(x)
+    return bf16_bits(hi), bf16_bits((x - hi).astype(np.float32))
+
+
 def pack_mlp(mlp: FieldMLP, device=None):
-    """Returns dict with 'bf16' (uint16 fragments as int16 tensor), 'f32'
-    (float fragments), 'bias' (float), and the shape (D, H, L)."""
+    """Returns dict with 'bf16' (uint16 fragments as int16 tensor), 'bf16x3'
+    (the split-BF16 kernel's fragment pairs: per fragment its hi parts, then
+    its lo parts), 'f32' (float fragments), 'bias' (float), and the shape
+    (D, H, L)."""
     D = mlp.weights[0].shape[1]
     H = mlp.weights[0].shape[0]
     L = len(mlp.weights) - 1
@@ -87,15 +97,19 @@ def pack_mlp(mlp: FieldMLP, device=None):
         frags16.append(pack_layer_bf16(W, ((K + 31) // 32) * 32).reshape(-1))
         frags32.append(pack_layer_f32(W, ((K + 15) // 16) * 16).reshape(-1))
     bias = np.concatenate([b for b in mlp.biases[:-1]] + [_pad(mlp.biases[-1][None], 1, 16)[0]])
-    w16 = bf16_bits(np.concatenate(frags16)).view(np.int16)
+    all16 = np.concatenate(frags16)
+    w16 = bf16_bits(all16).view(np.int16)
+    hi, lo = split_bf16(all16)
+    pairs = np.stack([hi.reshape(-1, 64 * 8), lo.reshape(-1, 64 * 8)], 1)   # [frag, 2, 512]
     w32 = np.concatenate(frags32).astype(np.float32)
     out = {
         "bf16": torch.from_numpy(w16.copy()),
+        "bf16x3": torch.from_numpy(pairs.reshape(-1).view(np.int16).copy()),
         "f32": torch.from_numpy(w32),
         "bias": torch.from_numpy(bias.astype(np.float32)),
         "shape": (D, H, L),
     }
     if device is not None:
-        for k in ("bf16", "f32", "bias"):
+        for k in ("bf16", "bf16x3", "f32", "bias"):
             out[k] = out[k].to(device)
     return out

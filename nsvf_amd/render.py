@@ -30,7 +30,10 @@ from .scene import VoxelScene
 
 BG_DEPTH = 5.0              # fairnr/modules/implicit.py:19
 BG_COLOR = (1.0, 1.0, 1.0)  # fairnr/modules/field.py:69 (transparent background)
-PRECISIONS = {"bf16": 0, "fp32": 1}
+# field MLP arithmetic: BF16 MFMA, FP32 MFMA, or split BF16 (three BF16 MFMAs
+# per product on hi / lo operand parts, about 16 significant bits)
+PRECISIONS = {"bf16": 0, "fp32": 1, "bf16x3": 2}
+_WEIGHTS = {0: "bf16", 1: "f32", 2: "bf16x3"}
 
 
 def _p(t):
@@ -123,7 +126,7 @@ class DeviceScene:
     def nbytes(self):
         ts = [self.centers, self.tree_records,
               self.feats, self.values,
-              self.mlp["bf16"], self.mlp["f32"], self.mlp["bias"]]
+              self.mlp["bf16"], self.mlp["bf16x3"], self.mlp["f32"], self.mlp["bias"]]
         return sum(t.numel() * t.element_size() for t in ts)
 
 
@@ -195,6 +198,8 @@ class Renderer:
         # fused kernel (False), or per scene (None): fused where the fused
         # kernel stages the field's weights in LDS, else split
         self.split_field = split_field
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision {precision!r}: one of {sorted(PRECISIONS)}")
         self.precision = precision
         self.hierarchical = hierarchical
         self.fine_steps = float(fine_steps)
@@ -255,7 +260,7 @@ class Renderer:
         list of stream slots; then only those slots of out are written)."""
         L = lib()
         prec = PRECISIONS[self.precision]
-        w = sc.mlp["bf16"] if prec == 0 else sc.mlp["f32"]
+        w = sc.mlp[_WEIGHTS[prec]]
         st = _stream(self.device)
         ev = self._kernel_events is not None and m > 0
 
@@ -264,18 +269,18 @@ class Renderer:
             e.record()
             return e
 
-        if prec == 0 and self.uses_split_field(sc):
-            ws_bytes = int(L.nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
+        if prec == 2 or (prec == 0 and self.uses_split_field(sc)):
+            embed, mlp = ((L.nsvf_field_embed_bf16x3, L.nsvf_field_mlp_bf16x3) if prec == 2 else
+                          (L.nsvf_field_embed_bf16, L.nsvf_field_mlp_bf16))
+            ws_bytes = int(L.nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, m))
             ws = self._buf("field_ws", (ws_bytes + 15) // 16 * 4, torch.int32)
             e0 = event() if ev else None
-            check(L.nsvf_field_embed_bf16(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats),
-                                          _p(sc.values), sc.voxel_size, _p(ray_start), _p(ray_dir),
-                                          _p(samples), _p(slots), _p(m_dev), m, _p(ws), ws_bytes,
-                                          st), "field_embed_bf16")
+            check(embed(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
+                        sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples), _p(slots),
+                        _p(m_dev), m, _p(ws), ws_bytes, st), "field_embed")
             e1 = event() if ev else None
-            check(L.nsvf_field_mlp_bf16(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(m_dev), m, _p(w),
-                                        _p(sc.mlp["bias"]), _p(out), _p(slots), st),
-                  "field_mlp_bf16")
+            check(mlp(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(m_dev), m, _p(w),
+                      _p(sc.mlp["bias"]), _p(out), _p(slots), st), "field_mlp")
             if ev:
                 e2 = event()
                 self._kernel_events.append(("embed", m, e0, e1))
@@ -339,8 +344,9 @@ class Renderer:
                               "e_boff": (nb, i32), "e_list": (slots_f, i32),
                               "scan_ws": (int(lib().nsvf_scan_workspace_ints(max(R, nb))), i32)})
             m = max(m, slots_f)
-        if PRECISIONS[self.precision] == 0 and self.uses_split_field(sc) and m > 0:
-            ws_bytes = int(lib().nsvf_field_workspace_bytes(0, sc.D, sc.H, sc.L, m))
+        prec = PRECISIONS[self.precision]
+        if (prec == 2 or (prec == 0 and self.uses_split_field(sc))) and m > 0:
+            ws_bytes = int(lib().nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, m))
             sizes["field_ws"] = ((ws_bytes + 15) // 16 * 4, i32)
         return sizes
 

@@ -289,13 +289,15 @@ class RenderServer:
     """Resident scenes + a Scheduler over `slots` RenderSlots.
 
     budget_bytes: device bytes for scenes and request workspaces (default:
-    90% of the free device memory at construction).  compat: the sampler's
+    90% of the free device memory at construction).  precision: the field's
+    arithmetic (default split BF16, whose per-pixel error against FP64 is
+    about 1e-5).  compat: the sampler's
     reference mode (True) or its evident intent (False, the default here): in
     the reference mode a ray whose hit row is full gets one extra sample
     carrying the next ray's voxel id (sample_gpu.cu:188), so a pixel would
     depend on which other pixels share its request."""
 
-    def __init__(self, device, slots=4, budget_bytes=None, precision="bf16", hierarchical=True,
+    def __init__(self, device, slots=4, budget_bytes=None, precision="bf16x3", hierarchical=True,
                  fine_steps=64.0, compat=False):
         import torch
         self.device = torch.device(device)

@@ -47,15 +47,17 @@ namespace {
 // Slab test of one ray against one axis-aligned box.  Same comparisons,
 // ternaries and NaN behaviour as intersect_gpu.cu:73-122: an axis whose
 // bounds come out NaN fails every comparison and is skipped.
-inline bool ray_aabb(const float o[3], const float inv[3], const float c[3],
-                     float h, float* out_lo, float* out_hi) {
-  float f_low = 0.0f;
-  float f_high = 100000.0f;
+// (Templated on the arithmetic type: float is the restatement, double the
+// FP64 check of which rays graze a face.)
+template <class T>
+inline bool ray_aabb(const T o[3], const T inv[3], const T c[3], T h, T* out_lo, T* out_hi) {
+  T f_low = T(0);
+  T f_high = T(100000);
   for (int a = 0; a < 3; ++a) {
-    float lo = (c[a] - h - o[a]) * inv[a];
-    float hi = (c[a] + h - o[a]) * inv[a];
+    T lo = (c[a] - h - o[a]) * inv[a];
+    T hi = (c[a] + h - o[a]) * inv[a];
     if (hi < lo) {
-      float t = lo;
+      T t = lo;
       lo = hi;
       hi = t;
     }
@@ -171,6 +173,40 @@ extern "C" {
 // aabb_intersect: brute force over all voxels, first n_max hits in voxel-index
 // order.  Outputs are [B,M,n_max]; idx is -1 padded and the depths keep the
 // caller's zero fill (intersect.cpp:64-69, intersect_gpu.cu:148-150).
+// FP64 brute-force slab test of float32 rays against float32 box centres
+// (the same comparisons in double): the first n_max hits of each ray in
+// point order, for stating which FP32 hit sets differ from exact
+// arithmetic.  Returns the largest hit count of a ray (may exceed n_max).
+int orc_aabb_intersect_f64(int M, int N, const float* ray_start, const float* ray_dir,
+                           const float* points, double half_voxel, int n_max, int32_t* idx,
+                           double* min_depth, double* max_depth) {
+  int most = 0;
+  for (int r = 0; r < M; ++r) {
+    const double o[3] = {ray_start[3 * r], ray_start[3 * r + 1], ray_start[3 * r + 2]};
+    const double inv[3] = {1.0 / double(ray_dir[3 * r]), 1.0 / double(ray_dir[3 * r + 1]),
+                           1.0 / double(ray_dir[3 * r + 2])};
+    int k = 0;
+    for (int n = 0; n < N; ++n) {
+      const double c[3] = {points[3 * n], points[3 * n + 1], points[3 * n + 2]};
+      double lo, hi;
+      if (!ray_aabb<double>(o, inv, c, half_voxel, &lo, &hi)) continue;
+      if (k < n_max) {
+        idx[size_t(r) * n_max + k] = n;
+        min_depth[size_t(r) * n_max + k] = lo;
+        max_depth[size_t(r) * n_max + k] = hi;
+      }
+      ++k;
+    }
+    for (int j = k; j < n_max; ++j) {
+      idx[size_t(r) * n_max + j] = -1;
+      min_depth[size_t(r) * n_max + j] = 0.0;
+      max_depth[size_t(r) * n_max + j] = 0.0;
+    }
+    if (k > most) most = k;
+  }
+  return most;
+}
+
 void orc_aabb_intersect(int B, int M, int N, const float* ray_start,
                         const float* ray_dir, const float* points,
                         float voxelsize, int n_max, int32_t* idx,

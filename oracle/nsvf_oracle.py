@@ -72,6 +72,8 @@ def lib():
     L.orc_inverse_cdf_sampling.restype = None
     L.orc_uniform_ray_sampling.argtypes = [i, i, i, i, f, P, P, P, P, P, P, P, P]
     L.orc_uniform_ray_sampling.restype = None
+    L.orc_aabb_intersect_f64.argtypes = [i, i, P, P, P, ctypes.c_double, i, P, P, P]
+    L.orc_aabb_intersect_f64.restype = i
     _LIB = L
     return L
 
@@ -670,6 +672,68 @@ def aabb_intersect_f64(ray_start, ray_dir, points, voxelsize, n_max):
     return idx, lo_out, hi_out
 
 
+def aabb_hits_f64(ray_start, ray_dir, points, voxelsize, n_max):
+    """FP64 brute-force slab test (C++, the same comparisons in double) of
+    float32 rays [R,3] against float32 box centres [N,3] of half-size
+    voxelsize / 2: idx [R,n_max] i32 in point order (-1 pad), entry / exit
+    depths float64, and the largest hit count of a ray."""
+    rs, rd, pts = _c(ray_start, np.float32), _c(ray_dir, np.float32), _c(points, np.float32)
+    R, N = rs.shape[0], pts.shape[0]
+    idx = np.zeros((R, n_max), np.int32)
+    lo = np.zeros((R, n_max), np.float64)
+    hi = np.zeros((R, n_max), np.float64)
+    most = lib().orc_aabb_intersect_f64(R, N, _p(rs), _p(rd), _p(pts),
+                                        float(np.float32(voxelsize)) * 0.5, n_max, _p(idx),
+                                        _p(lo), _p(hi))
+    return idx, lo, hi, int(most)
+
+
+def grazing_report(idx, lo, hi, ray_start, ray_dir, leaf_points, voxel_size, n_max, eps=1e-6):
+    """FP32 hit rows (idx [R,P] with -1 pads; lo / hi entry and exit depths)
+    of rays against the voxels with centres `leaf_points`, compared with
+    FP64 slab tests of the same rays and boxes.  A ray differs when its FP32
+    hit set is not the FP64 one (for a full row of P hits, when it holds a
+    voxel the FP64 test misses); a difference is a graze when every voxel
+    in it is crossed over less than `eps` (its FP64 interval if the FP64
+    test hits it, else its FP32 one).  Also: rays with an FP32 hit shorter
+    than `eps`, and the largest entry / exit depth difference over the hits
+    both agree on."""
+    idx = np.asarray(idx)
+    R, P = idx.shape
+    cap = max(4 * int(n_max), 256)
+    i64, lo64, hi64, most = aabb_hits_f64(ray_start, ray_dir, leaf_points, voxel_size, cap)
+    if most > cap:
+        raise RuntimeError(f"grazing_report: a ray hits {most} boxes (> {cap})")
+    differing = grazing = short = 0
+    depth_err = 0.0
+    for r in range(R):
+        m32 = idx[r] != -1
+        k32 = idx[r][m32]
+        l32, h32 = np.asarray(lo[r])[m32], np.asarray(hi[r])[m32]
+        m64 = i64[r] != -1
+        pos64 = {int(v): j for j, v in enumerate(i64[r][m64])}
+        s32, s64 = set(int(v) for v in k32), set(pos64)
+        if ((h32 - l32) < eps).any():
+            short += 1
+        full = len(s32) >= n_max
+        diff = (s32 - s64) if full else (s32 ^ s64)
+        if diff:
+            differing += 1
+            pos32 = {int(v): j for j, v in enumerate(k32)}
+            ok = all((hi64[r, pos64[v]] - lo64[r, pos64[v]] < eps) if v in pos64 else
+                     (h32[pos32[v]] - l32[pos32[v]] < eps) for v in diff)
+            grazing += int(ok)
+        both = [(j, pos64[int(v)]) for j, v in enumerate(k32) if int(v) in pos64]
+        if both:
+            a = np.array(both)
+            depth_err = max(depth_err,
+                            float(np.abs(l32[a[:, 0]] - lo64[r, a[:, 1]]).max()),
+                            float(np.abs(h32[a[:, 0]] - hi64[r, a[:, 1]]).max()))
+    return {"rays": int(R), "eps": eps, "rays_differing_from_fp64": differing,
+            "differences_that_graze": grazing, "rays_with_hits_shorter_than_eps": short,
+            "max_abs_hit_depth_vs_fp64": depth_err}
+
+
 # ---------------------------------------------------------------------------
 # One view, end to end (fairnr_model.py:142-186 for the NSVF model with
 # hierarchical sampling; nsvf.py:43-110; nerf.py:64-79; renderer.py:77-252)
@@ -720,13 +784,23 @@ def prepare_fine_bins(si, sd, sx, probs, fine_steps):
 
 
 def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, compat=True,
-                dtype=np.float64, coarse_probs=None, use_octree=True):
+                dtype=np.float64, coarse_probs=None, use_octree=True, subset=None):
     """Render rays [R,3] of ``scene`` (a VoxelScene).  Integer and sampling
     decisions are the FP32 restatement's; gather, MLP and compositing run in
     ``dtype``.  ``coarse_probs`` ([N_hit, S_c] float32) replaces the coarse
     compositing probabilities that drive the fine sampler, so the fine
     samples can be compared with another implementation's for the same
-    probabilities.  Returns a dict of the outputs and intermediates."""
+    probabilities.
+
+    ``subset`` (sorted ray ids): intersection and both sampling passes still
+    run over every ray -- in the reference's sampler mode a ray's samples
+    depend on its neighbours in the wrapper's grouping -- but the field and
+    compositing only for the hit rays in the subset; the other hit rays get
+    zero coarse probabilities (their fine pass then samples uniformly, which
+    touches only their own outputs).  The image outputs (rgb, depth, missed)
+    are then the subset's, in its order; ``eval_rows`` gives the positions
+    of the evaluated rays among the hit rays.  Returns a dict of the
+    outputs and intermediates."""
     rs = np.asarray(ray_start, np.float32)
     rd = np.asarray(ray_dir, np.float32)
     vs = np.float32(scene.voxel_size)
@@ -738,29 +812,42 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
         idx, lo, hi = aabb_intersect(rs[None], rd[None], pts_shift[None], vs, scene.max_hits)
     idx, lo, hi, hits = sort_hits(idx[0], lo[0], hi[0])
     out = {"hits": hits, "sorted_idx": idx, "sorted_lo": lo, "sorted_hi": hi}
+    hit_rays = np.nonzero(hits)[0]
+    if subset is None:
+        ev = np.arange(len(hit_rays))
+        frame_hits = hits
+    else:
+        subset = np.asarray(subset, np.int64)
+        ev = np.nonzero(np.isin(hit_rays, subset))[0]
+        frame_hits = hits[subset]
+    out["eval_rows"] = ev
     idx, lo, hi = idx[hits], lo[hits], hi[hits]
     probs, steps = probs_steps(idx, lo, hi, scene.step_size)
     out.update(probs=probs, steps=steps)
     si, sd, sx, S, _ = inverse_cdf_sampling(idx, lo, hi, probs, steps, -1.0, True, compat=compat)
     sd, sx = sample_tail(si, sd, sx)
     out.update(c_idx=si, c_depth=sd, c_dist=sx)
-    rsh, rdh = rs[hits], rd[hits]
-    sig, col = _field_eval(si, sd, sx, rsh, rdh, pts_shift, scene.feats, scene.values, vs,
-                           scene.mlp, dtype)
-    rgb, depth, missed, p = composite(sig, col, sd, sx, si, dtype=dtype)
+    rsh, rdh = rs[hits][ev], rd[hits][ev]
+    sig, col = _field_eval(si[ev], sd[ev], sx[ev], rsh, rdh, pts_shift, scene.feats, scene.values,
+                           vs, scene.mlp, dtype)
+    rgb, depth, missed, p = composite(sig, col, sd[ev], sx[ev], si[ev], dtype=dtype)
     out.update(c_sigma=sig, c_rgb=col, c_probs=p, coarse_rgb=rgb, coarse_depth=depth,
                coarse_missed=missed)
     if hierarchical and si.shape[0] > 0:
-        p_use = np.asarray(p, np.float32) if coarse_probs is None else np.asarray(coarse_probs, np.float32)
+        if coarse_probs is None:
+            p_use = np.zeros(si.shape, np.float32)
+            p_use[ev] = np.asarray(p, np.float32)
+        else:
+            p_use = np.asarray(coarse_probs, np.float32)
         fi, flo, fhi, fp, fst = prepare_fine_bins(si, sd, sx, p_use, fine_steps)
         si2, sd2, sx2, S2, _ = inverse_cdf_sampling(fi, flo, fhi, fp, fst, -1.0, True, compat=compat)
         sd2, sx2 = sample_tail(si2, sd2, sx2)
         out.update(f_idx=si2, f_depth=sd2, f_dist=sx2)
-        sig2, col2 = _field_eval(si2, sd2, sx2, rsh, rdh, pts_shift, scene.feats, scene.values, vs,
-                                 scene.mlp, dtype)
-        rgb, depth, missed, p2 = composite(sig2, col2, sd2, sx2, si2, dtype=dtype)
+        sig2, col2 = _field_eval(si2[ev], sd2[ev], sx2[ev], rsh, rdh, pts_shift, scene.feats,
+                                 scene.values, vs, scene.mlp, dtype)
+        rgb, depth, missed, p2 = composite(sig2, col2, sd2[ev], sx2[ev], si2[ev], dtype=dtype)
         out.update(f_sigma=sig2, f_probs=p2)
-    full_rgb, full_depth, full_missed = fill_background(hits, rgb, depth, missed, dtype=dtype)
+    full_rgb, full_depth, full_missed = fill_background(frame_hits, rgb, depth, missed, dtype=dtype)
     out.update(rgb=full_rgb, depth=full_depth, missed=full_missed)
     return out
 

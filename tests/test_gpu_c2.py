@@ -1,0 +1,227 @@
+"""The headline workload against the oracle: the 128^3 value-noise scene
+(100,000 voxels, 226,263 octree nodes), one 800x800 view from the benchmark's
+camera (seed 11), SVO intersection with max_hits 60, inverse-CDF sampling at
+step voxel/8, the 64-sample hierarchical pass.
+
+The oracle intersects and samples the whole view (in the reference's sampler
+mode a ray's samples depend on its neighbours in the wrapper's grouping) and
+evaluates the FP64 field on a seeded subset: 20,000 random rays plus every
+ray whose hit row is full or whose hit row has equal entry depths (the rows
+where traversal order decides the result).
+
+Bars:
+  * hit flags, counts and sorted hit rows bit-exact on every ray of the view;
+  * coarse sample streams bit-exact on every hit ray; fine streams bit-exact
+    given the GPU's coarse probabilities; sampler mode on and off;
+  * FP32 field: RGB, depth and alpha within 1e-5 of FP64 with the same
+    samples, on the subset;
+  * split-BF16 field (the served precision): per-pixel RGB within 1e-3 of
+    FP64 end to end, on the subset;
+  * grazing rays (hit sets an FP64 slab test decides differently) counted;
+    each difference is a crossing shorter than 1e-6;
+  * prune scores of 256 voxels within 1e-4 of FP64; the x8 split bit-exact
+    (values within 1e-6)."""
+import numpy as np
+import pytest
+import torch
+
+import nsvf_oracle as O
+from nsvf_amd import scene as S
+from nsvf_amd.render import DeviceScene, Renderer
+from nsvf_amd.voxels import prune_scores, split_scene
+
+pytestmark = pytest.mark.gpu
+
+RES = 800
+SUBSET = 20_000
+
+
+def N(t):
+    return t.detach().cpu().numpy()
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def csr_to_padded(cnt, off, arr, width, fill):
+    """Rows off[n] .. off[n] + cnt[n] of arr as a [len(cnt), width] array
+    (vectorised)."""
+    cnt = np.asarray(cnt, np.int64)
+    off = np.asarray(off, np.int64)
+    out = np.full((len(cnt), width), fill, dtype=arr.dtype)
+    col = np.arange(width)[None]
+    m = col < cnt[:, None]
+    out[m] = arr[(off[:, None] + col)[m]]
+    return out
+
+
+@pytest.fixture(scope="module")
+def c2(device):
+    sc = S.noise_scene(res=128, num_voxels=100_000)
+    rs, rd = S.camera_rays(RES, RES, S.sphere_camera(2.5, 11))
+    dsc = DeviceScene(sc, device)
+    log(f"c2: {sc.num_voxels} voxels, {dsc.num_nodes} nodes, {rs.shape[0]} rays")
+    return sc, rs, rd, dsc
+
+
+@pytest.fixture(scope="module")
+def subset(c2):
+    sc, rs, rd, dsc = c2
+    sel, n_full, n_tied = subset_rays(sc, rs, rd)
+    log(f"c2: subset {len(sel)} rays ({n_full} full rows, {n_tied} tied rows)")
+    return sel, n_full
+
+
+def gpu_view(dsc, rs, rd, device, precision, compat, rows=True):
+    r = Renderer(device, precision=precision, hierarchical=True, compat=compat)
+    out = r.render(dsc, torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device),
+                   keep=True, width=RES)
+    torch.cuda.synchronize()
+    k = r.kept
+    hl = N(k["hit_list"])
+    got = {"rgb": N(out[0]), "depth": N(out[1]), "missed": N(out[2]),
+           "flag": N(k["flag"]).astype(bool), "count": N(k["count"]),
+           "hits": N(k["hits"]) if rows else None, "tot": N(k["tot"]), "hit_list": hl}
+    c_cnt, c_off = N(k["c_cnt"]), N(k["c_off"])
+    got["c_cnt"], got["c_off"] = c_cnt, c_off
+    got["c_cols"] = [N(c) for c in k["c_samples"][:3]]
+    got["c_prob"] = N(k["c_prob"])
+    got["f_cnt"], got["f_off"] = N(k["f_cnt"]), N(k["f_off"])
+    got["f_cols"] = [N(c) for c in k["f_samples"][:3]]
+    return got
+
+
+def subset_rays(sc, rs, rd):
+    """20k seeded random rays plus the rows whose order matters (from the
+    oracle's sorted hit rows)."""
+    cen, ch = O.build_easy_octree(sc.points, np.float32(sc.voxel_size) * np.float32(0.5))
+    idx, lo, hi = O.svo_intersect(rs[None], rd[None], cen[None], ch[None], sc.voxel_size,
+                                  sc.max_hits)
+    idx, lo, _, _ = O.sort_hits(idx[0], lo[0], hi[0])
+    R = rs.shape[0]
+    valid = idx != -1
+    full = valid.all(1)
+    tied = ((lo[:, 1:] == lo[:, :-1]) & valid[:, 1:]).any(1)
+    rng = np.random.default_rng(21)
+    base = rng.choice(R, SUBSET, replace=False)
+    return np.union1d(base, np.nonzero(full | tied)[0]), int(full.sum()), int(tied.sum())
+
+
+@pytest.fixture(scope="module", params=[True, False], ids=["compat", "intent"])
+def views(request, c2, subset, device):
+    """Per sampler mode: GPU FP32 and split-BF16 views, the oracle's view
+    (FP64 field on the subset) and the oracle fed the GPU FP32 coarse
+    probabilities (same samples)."""
+    compat = request.param
+    sc, rs, rd, dsc = c2
+    g32 = gpu_view(dsc, rs, rd, device, "fp32", compat)
+    g3 = gpu_view(dsc, rs, rd, device, "bf16x3", compat, rows=False)
+    log(f"c2 compat={compat}: GPU views done, {int(g32['flag'].sum())} hit rays")
+    sel, n_full = subset
+    ref = O.render_view(sc, rs, rd, compat=compat, subset=sel)
+    log(f"c2 compat={compat}: oracle end to end done")
+    W = ref["c_idx"].shape[1]
+    gp = csr_to_padded(g32["c_cnt"], g32["c_off"], g32["c_prob"], W, np.float32(0))
+    ref_s = O.render_view(sc, rs, rd, compat=compat, subset=sel, coarse_probs=gp)
+    log(f"c2 compat={compat}: oracle same-samples done")
+    return {"compat": compat, "sel": sel, "n_full": n_full, "g32": g32, "g3": g3, "ref": ref,
+            "ref_s": ref_s}
+
+
+def test_c2_hit_rows_bit_exact(c2, views):
+    sc, rs, rd, dsc = c2
+    g, ref = views["g32"], views["ref"]
+    P = sc.max_hits
+    assert np.array_equal(g["flag"], ref["hits"])
+    ridx = ref["sorted_idx"]
+    cnt = g["count"]
+    assert np.array_equal(cnt, (ridx != -1).sum(1))
+    m = np.arange(P)[None] < cnt[:, None]
+    gi = np.where(m, g["hits"][:, :, 2], -1)
+    glo = np.where(m, g["hits"][:, :, 0].view(np.float32), np.float32(O.MAX_DEPTH))
+    ghi = np.where(m, g["hits"][:, :, 1].view(np.float32), np.float32(O.MAX_DEPTH))
+    assert np.array_equal(gi, ridx)
+    assert np.array_equal(glo, ref["sorted_lo"])
+    assert np.array_equal(ghi, ref["sorted_hi"])
+    steps = g["tot"][g["flag"]] / np.float32(sc.step_size)
+    assert np.array_equal(steps.astype(np.float32), ref["steps"])
+    log(f"hit rows bit-exact on {rs.shape[0]} rays ({int((cnt == P).sum())} full rows)")
+
+
+def _stream_equal(cnt, off, cols, ri, rd_, rx):
+    W = ri.shape[1]
+    assert np.array_equal(cnt, (ri != -1).sum(1))
+    assert np.array_equal(csr_to_padded(cnt, off, cols[0], W, -1), ri)
+    assert np.array_equal(csr_to_padded(cnt, off, cols[1], W, np.float32(O.MAX_DEPTH)), rd_)
+    assert np.array_equal(csr_to_padded(cnt, off, cols[2], W, np.float32(0)), rx)
+
+
+def test_c2_coarse_streams_bit_exact(views):
+    ref = views["ref"]
+    for g in (views["g32"], views["g3"]):
+        _stream_equal(g["c_cnt"], g["c_off"], g["c_cols"], ref["c_idx"], ref["c_depth"],
+                      ref["c_dist"])
+    log(f"coarse streams bit-exact ({int(views['g32']['c_cnt'].sum())} samples)")
+
+
+def test_c2_fine_streams_bit_exact_given_coarse_probs(views):
+    g, ref = views["g32"], views["ref_s"]
+    _stream_equal(g["f_cnt"], g["f_off"], g["f_cols"], ref["f_idx"], ref["f_depth"], ref["f_dist"])
+    log(f"fine streams bit-exact ({int(g['f_cnt'].sum())} samples)")
+
+
+def test_c2_fp32_within_1e5_of_fp64_same_samples(views):
+    sel, g, ref = views["sel"], views["g32"], views["ref_s"]
+    e = {k: float(np.abs(g[k][sel] - ref[k]).max()) for k in ("rgb", "depth", "missed")}
+    log(f"fp32 vs fp64 (same samples, {len(sel)} rays): rgb {e['rgb']:.2e} "
+        f"depth {e['depth']:.2e} alpha {e['missed']:.2e}")
+    assert e["rgb"] < 1e-5 and e["missed"] < 1e-5 and e["depth"] < 1e-5
+
+
+def test_c2_bf16x3_within_1e3_of_fp64_end_to_end(views):
+    sel, g, ref = views["sel"], views["g3"], views["ref"]
+    e = {k: float(np.abs(g[k][sel] - ref[k]).max()) for k in ("rgb", "depth", "missed")}
+    d = g["rgb"][sel].astype(np.float64) - ref["rgb"]
+    psnr = 10 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
+    log(f"bf16x3 vs fp64 (end to end, {len(sel)} rays, {views['n_full']} full rows): "
+        f"rgb {e['rgb']:.2e} depth {e['depth']:.2e} alpha {e['missed']:.2e} PSNR {psnr:.1f} dB")
+    assert e["rgb"] <= 1e-3 and psnr >= 45.0
+
+
+def test_c2_grazing_rays_counted(c2, views):
+    if not views["compat"]:
+        pytest.skip("hit rows do not depend on the sampler mode")
+    sc, rs, rd, dsc = c2
+    sel, ref = views["sel"], views["ref"]
+    leaves = O.build_easy_octree(sc.points, np.float32(sc.voxel_size) * np.float32(0.5))[0]
+    rep = O.grazing_report(ref["sorted_idx"][sel], ref["sorted_lo"][sel], ref["sorted_hi"][sel],
+                           rs[sel], rd[sel], leaves[: sc.num_voxels], sc.voxel_size, sc.max_hits)
+    log(f"grazing: {rep}")
+    assert rep["differences_that_graze"] == rep["rays_differing_from_fp64"]
+    assert rep["max_abs_hit_depth_vs_fp64"] < 1e-5
+
+
+def test_c2_prune_scores_256_voxels(c2):
+    sc, rs, rd, dsc = c2
+    s32 = N(prune_scores(dsc, "fp32"))
+    sel = np.sort(np.random.default_rng(5).choice(sc.num_voxels, 256, replace=False))
+    ref = O.prune_max_sigma(O.shift_centers(sc.points, sc.voxel_size), sc.feats, sc.values,
+                            sc.voxel_size, sc.mlp, sel)
+    scale = np.maximum(1.0, np.abs(ref))
+    err = np.abs(s32[sel] - ref) / scale
+    log(f"prune scores: max relative error {err.max():.2e} on 256 voxels")
+    assert (err <= 1e-4).all()
+
+
+def test_c2_split_matches_oracle(c2, device):
+    sc, rs, rd, dsc = c2
+    new = split_scene(sc, device)
+    rp, rf, rv, rk = O.split_voxels(O.shift_centers(sc.points, sc.voxel_size), sc.feats,
+                                    sc.values, sc.voxel_size)
+    assert new.num_voxels == 800_000
+    assert np.array_equal(new.points, rp.astype(np.float32))
+    assert np.array_equal(new.keys, rk)
+    assert np.array_equal(new.feats, rf)
+    assert np.abs(new.values - rv).max() < 1e-6
+    log(f"split: {new.num_voxels} voxels, {new.num_keys} keys bit-exact")

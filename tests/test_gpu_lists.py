@@ -41,7 +41,8 @@ def test_eval_list_matches_numpy(device, n):
     assert (got[E:] == -9).all()
 
 
-@pytest.mark.parametrize("precision,split", [("fp32", None), ("bf16", True), ("bf16", False)])
+@pytest.mark.parametrize("precision,split", [("fp32", None), ("bf16", True), ("bf16", False),
+                                             ("bf16x3", None)])
 def test_field_over_a_list(device, precision, split):
     sc = shell()
     rs, rd = shell_rays(width=64)
