@@ -249,12 +249,15 @@ int nsvf_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n,
  *   children[k][8].  16-byte aligned.  Rays whose result depends on
  *   traversal order (n_max reached, equal entry depths) are traversed again
  *   in the reference's child order.
+ *   depth: the octree's depth (the root's children[8] = 1 << (depth + 1));
+ *   the traversal holds trees of depth <= 22 and returns NSVF_ESTACK for
+ *   deeper ones (build_octree's trees are at most 19 deep).
  *   hits: [R, n_max] records {float lo, float hi, int32 idx, int32 pad};
  *   only the first count[r] records of a row are written.                  */
 int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
-                     int width, const void* records, int T, int n_max, float step_size,
-                     void* hits, int32_t* count, float* tot, int32_t* hit_flag,
-                     int32_t* bound, void* stream);
+                     int width, const void* records, int T, int depth, int n_max,
+                     float step_size, void* hits, int32_t* count, float* tot,
+                     int32_t* hit_flag, int32_t* bound, void* stream);
 
 /* One view's pinhole camera, as the reference builds its rays
  * (fairnr/renderer.py:91-112, data/geometry.py:68-122, data_utils.py:57-65):
@@ -282,10 +285,10 @@ int nsvf_generate_rays(const nsvf_camera* cam, float* ray_start,
  * ray_dir [h * w, 3] for the later stages; lanes take 8 x 8 pixel tiles.
  * Outputs as nsvf_render_hits with R = h * w.                             */
 int nsvf_render_hits_camera(const nsvf_camera* cam, const void* records, int T,
-                            int n_max, float step_size, float* ray_start,
-                            float* ray_dir, void* hits, int32_t* count,
-                            float* tot, int32_t* hit_flag, int32_t* bound,
-                            void* stream);
+                            int depth, int n_max, float step_size,
+                            float* ray_start, float* ray_dir, void* hits,
+                            int32_t* count, float* tot, int32_t* hit_flag,
+                            int32_t* bound, void* stream);
 
 /* list[rank[r]] = r for every ray with hit_flag[r] != 0.                  */
 int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,

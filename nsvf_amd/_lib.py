@@ -51,9 +51,9 @@ SIGNATURES = {
     "nsvf_split_values": ([_P, _P, _P, _L, _I, _F, _P, _L, _L, _P, _P, _P], _I),
     "nsvf_scan_workspace_ints": ([_L], _L),
     "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
-    "nsvf_render_hits": ([_P, _P, _L, _I, _P, _I, _I, _F, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_hits": ([_P, _P, _L, _I, _P, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_generate_rays": ([_CAM, _P, _P, _P], _I),
-    "nsvf_render_hits_camera": ([_CAM, _P, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_hits_camera": ([_CAM, _P, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_sample_coarse": ([_P, _P, _L, _P, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P,
                                    _P, _P], _I),

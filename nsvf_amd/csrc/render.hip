@@ -1032,12 +1032,25 @@ using namespace nsvf;
 
 extern "C" {
 
+// The traversal keeps one LDS record per level: level 0 holds the root, a
+// node of depth d (children[8] = 1 << (d + 1)) is expanded at level
+// depth - d + 1, and depth-0 nodes push their leaves without expanding
+// them, so a tree of depth `depth` needs depth + 2 levels.
+static int check_tree_depth(int depth, const char* what) {
+  if (depth < 0) return set_error(NSVF_EINVAL, "%s: tree depth %d", what, depth);
+  if (depth + 2 > kMaxLevels)
+    return set_error(NSVF_ESTACK, "%s: octree depth %d needs %d traversal levels (at most %d)",
+                     what, depth, depth + 2, kMaxLevels);
+  return NSVF_OK;
+}
+
 int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R, int width,
-                     const void* records, int T, int n_max, float step_size, void* hits,
-                     int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
+                     const void* records, int T, int depth, int n_max, float step_size,
+                     void* hits, int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
                      void* stream) {
   clear_error();
   if (R < 0 || T <= 0 || n_max <= 0) return set_error(NSVF_EINVAL, "render_hits: bad size");
+  if (const int rc = check_tree_depth(depth, "render_hits")) return rc;
   if (width < 0 || (width > 0 && R % width != 0))
     return set_error(NSVF_EINVAL, "render_hits: %lld rays are not rows of width %d",
                      static_cast<long long>(R), width);
@@ -1077,13 +1090,14 @@ int nsvf_generate_rays(const nsvf_camera* cam, float* ray_start, float* ray_dir,
   return check_launch("generate_rays");
 }
 
-int nsvf_render_hits_camera(const nsvf_camera* cam, const void* records, int T, int n_max,
-                            float step_size, float* ray_start, float* ray_dir, void* hits,
-                            int32_t* count, float* tot, int32_t* hit_flag, int32_t* bound,
-                            void* stream) {
+int nsvf_render_hits_camera(const nsvf_camera* cam, const void* records, int T, int depth,
+                            int n_max, float step_size, float* ray_start, float* ray_dir,
+                            void* hits, int32_t* count, float* tot, int32_t* hit_flag,
+                            int32_t* bound, void* stream) {
   clear_error();
   if (const int rc = check_camera(cam, "render_hits_camera")) return rc;
   if (T <= 0 || n_max <= 0) return set_error(NSVF_EINVAL, "render_hits_camera: bad size");
+  if (const int rc = check_tree_depth(depth, "render_hits_camera")) return rc;
   if (!records || !ray_start || !ray_dir || !hits || !count || !tot || !hit_flag || !bound)
     return set_error(NSVF_EINVAL, "render_hits_camera: null pointer");
   if (T >= (1 << 27)) return set_error(NSVF_EINVAL, "render_hits_camera: more than 2^27 nodes");

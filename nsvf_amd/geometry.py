@@ -1,7 +1,6 @@
-"""Voxel geometry helpers (fairnr/data/geometry.py), in torch.
-
-Only the helpers the sparse-voxel path uses: lattice offsets, discretisation,
-the octree wrapper, trilinear interpolation and image fill-in.
+"""Voxel geometry helpers (fairnr/data/geometry.py), in torch: the
+discretisation of voxel centres and the octree wrapper the device scene is
+built with.
 """
 from __future__ import annotations
 
@@ -11,19 +10,6 @@ import numpy as np
 import torch
 
 from . import _ext
-
-
-def offset_points(point_xyz, quarter_voxel=1.0, offset_only=False, bits=2):
-    """geometry.py:229-238.  bits^3 lattice offsets (2i+1-bits)/(bits-1) *
-    quarter_voxel around each point, x slowest and z fastest (meshgrid 'ij');
-    with bits=2 these are the 8 corners in the order 4x + 2y + z."""
-    c = torch.arange(1, 2 * bits, 2, device=point_xyz.device)
-    ox, oy, oz = torch.meshgrid([c, c, c], indexing="ij")
-    offset = (torch.cat([ox.reshape(-1, 1), oy.reshape(-1, 1), oz.reshape(-1, 1)], 1)
-              .type_as(point_xyz) - bits) / float(bits - 1)
-    if not offset_only:
-        return point_xyz.unsqueeze(1) + offset.unsqueeze(0).type_as(point_xyz) * quarter_voxel
-    return offset.type_as(point_xyz) * quarter_voxel
 
 
 def discretize_points(voxel_points, voxel_size):
@@ -58,34 +44,3 @@ def build_easy_octree(points, half_voxel):
     centers, children = _ext.build_octree(center.cpu(), coords.cpu(), depth)
     centers = centers.to(points.device).float() * half_voxel + residual
     return centers, children.to(points.device)
-
-
-def trilinear_interp(p, q, point_feats):
-    """geometry.py:195-200: weights prod(p*q + (1-p)(1-q)) over the 8 corners."""
-    weights = (p * q + (1 - p) * (1 - q)).prod(dim=-1, keepdim=True)
-    if point_feats.dim() == 2:
-        point_feats = point_feats.view(point_feats.size(0), 8, -1)
-    return (weights * point_feats).sum(1)
-
-
-def fill_in(shape, hits, input, initial=1.0):
-    """geometry.py:303-317: scatter the hit rays' values back to the frame."""
-    input_sizes = list(input.size())
-    if len(input_sizes) == len(shape) and all(shape[i] == input_sizes[i] for i in range(len(shape))):
-        return input
-    if isinstance(initial, torch.Tensor):
-        output = initial.expand(*shape)
-    else:
-        output = input.new_ones(*shape) * initial
-    if len(shape) == 1:
-        return output.masked_scatter(hits, input)
-    return output.masked_scatter(hits.unsqueeze(-1).expand(*shape), input)
-
-
-def voxel_corner_keys(points, voxel_size):
-    """encoder.py:269-276: unique sorted corner keys and each voxel's 8 corner
-    ids (half-voxel integer units)."""
-    coords, _ = discretize_points(points, voxel_size * 0.5)
-    keys0 = offset_points(coords, 1.0).reshape(-1, 3)
-    keys, feats = torch.unique(keys0, dim=0, sorted=True, return_inverse=True)
-    return keys.long(), feats.reshape(-1, 8).long()

@@ -48,6 +48,44 @@ def _stream(dev):
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
+# Geometry helpers of the reference's torch orchestration
+# (fairnr/data/geometry.py), used only by this comparator.
+
+def offset_points(point_xyz, quarter_voxel=1.0, offset_only=False, bits=2):
+    """geometry.py:229-238.  bits^3 lattice offsets (2i+1-bits)/(bits-1) *
+    quarter_voxel around each point, x slowest and z fastest (meshgrid 'ij');
+    with bits=2 these are the 8 corners in the order 4x + 2y + z."""
+    c = torch.arange(1, 2 * bits, 2, device=point_xyz.device)
+    ox, oy, oz = torch.meshgrid([c, c, c], indexing="ij")
+    offset = (torch.cat([ox.reshape(-1, 1), oy.reshape(-1, 1), oz.reshape(-1, 1)], 1)
+              .type_as(point_xyz) - bits) / float(bits - 1)
+    if not offset_only:
+        return point_xyz.unsqueeze(1) + offset.unsqueeze(0).type_as(point_xyz) * quarter_voxel
+    return offset.type_as(point_xyz) * quarter_voxel
+
+
+def trilinear_interp(p, q, point_feats):
+    """geometry.py:195-200: weights prod(p*q + (1-p)(1-q)) over the 8 corners."""
+    weights = (p * q + (1 - p) * (1 - q)).prod(dim=-1, keepdim=True)
+    if point_feats.dim() == 2:
+        point_feats = point_feats.view(point_feats.size(0), 8, -1)
+    return (weights * point_feats).sum(1)
+
+
+def fill_in(shape, hits, input, initial=1.0):
+    """geometry.py:303-317: scatter the hit rays' values back to the frame."""
+    input_sizes = list(input.size())
+    if len(input_sizes) == len(shape) and all(shape[i] == input_sizes[i] for i in range(len(shape))):
+        return input
+    if isinstance(initial, torch.Tensor):
+        output = initial.expand(*shape)
+    else:
+        output = input.new_ones(*shape) * initial
+    if len(shape) == 1:
+        return output.masked_scatter(hits, input)
+    return output.masked_scatter(hits.unsqueeze(-1).expand(*shape), input)
+
+
 def _svo_ray_intersect(voxelsize, n_max, points, children, ray_start, ray_dir):
     """clib/__init__.py:98-135 with the naive kernel."""
     G = min(2048, int(2 * 10 ** 9 / (points.numel() + children.numel())))
@@ -195,8 +233,8 @@ class PlainNSVF:
         point_feats = F.embedding(F.embedding(idx, st["voxel_vertex_idx"]),
                                   st["voxel_vertex_emb"]).view(point_xyz.size(0), -1)
         p = ((xyz - point_xyz) / self.voxel_size + .5).unsqueeze(1)
-        q = geometry.offset_points(p, .5, offset_only=True).unsqueeze(0) + .5
-        return {"emb": geometry.trilinear_interp(p, q, point_feats),
+        q = offset_points(p, .5, offset_only=True).unsqueeze(0) + .5
+        return {"emb": trilinear_interp(p, q, point_feats),
                 "dists": samples["sampled_point_distance"]}
 
     def field(self, emb):
@@ -301,9 +339,9 @@ class PlainNSVF:
                 io["probs"] = safe / safe.sum(-1, keepdim=True)
                 io["steps"] = safe.new_ones(*safe.size()[:-1]) * self.fine_num_samples
                 samples, res = self.raymarch(rs, rd, io, st)
-            rgb = geometry.fill_in((R, 3), hits, res["colors"], 0.0)
-            depth = geometry.fill_in((R,), hits, res["depths"], 0.0)
-            missed = geometry.fill_in((R,), hits, res["missed"], 1.0)
+            rgb = fill_in((R, 3), hits, res["colors"], 0.0)
+            depth = fill_in((R,), hits, res["depths"], 0.0)
+            missed = fill_in((R,), hits, res["missed"], 1.0)
         rgb = rgb + missed.unsqueeze(-1) * torch.ones(3, device=self.device)
         depth = depth + missed * BG_DEPTH
         return rgb, depth, missed

@@ -70,6 +70,15 @@ def node_boxes(centers, children, half_voxel):
     return mn, mx, scale
 
 
+def tree_depth(children):
+    """Depth of the octree whose root is the last node: its children[8] is
+    1 << (depth + 1) (octree.cpp:119); a tree of one leaf has depth -1."""
+    scale = int(children[-1, 8])
+    if scale < 1 or scale & (scale - 1):
+        raise ValueError(f"octree root scale {scale} is not a power of two")
+    return scale.bit_length() - 2
+
+
 def pack_tree(centers, children, half_voxel):
     """The octree packed as node records for the traversal: record k holds,
     for child slot s, (min xyz, scale bits) and (max xyz, id bits) of child
@@ -109,6 +118,7 @@ class DeviceScene:
         # the octree packed for the traversal (boxes and child ids)
         rec = pack_tree(cen.float().numpy(), ch.int().numpy(), np.float32(vs) * np.float32(0.5))
         self.tree_records = torch.from_numpy(rec).to(self.device)
+        self.tree_depth = tree_depth(ch.int().numpy())
         self.feats = torch.from_numpy(scene.feats.astype(np.int32)).to(self.device)
         self.values = torch.from_numpy(scene.values).to(self.device)
         self.voxel_size = float(vs)
@@ -380,7 +390,8 @@ class Renderer:
 
         def intersect(sc, P, hits, count, tot, flag, c_bound, st):
             check(lib().nsvf_render_hits(_p(ray_start), _p(ray_dir), R, int(width),
-                                         _p(sc.tree_records), sc.num_nodes, P, sc.step_size,
+                                         _p(sc.tree_records), sc.num_nodes, sc.tree_depth, P,
+                                         sc.step_size,
                                          _p(hits), _p(count), _p(tot), _p(flag), _p(c_bound), st),
                   "render_hits")
         self._begin(sc, R, ray_start, ray_dir, intersect)
@@ -397,7 +408,8 @@ class Renderer:
 
         def intersect(sc, P, hits, count, tot, flag, c_bound, st):
             check(lib().nsvf_render_hits_camera(ctypes.byref(cam), _p(sc.tree_records),
-                                                sc.num_nodes, P, sc.step_size, _p(rs), _p(rd),
+                                                sc.num_nodes, sc.tree_depth, P, sc.step_size,
+                                                _p(rs), _p(rd),
                                                 _p(hits), _p(count), _p(tot), _p(flag),
                                                 _p(c_bound), st), "render_hits_camera")
         self._begin(sc, R, rs, rd, intersect)

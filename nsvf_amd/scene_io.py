@@ -188,7 +188,11 @@ def load_voxels(path, voxel_size=None, kind=None):
     file (encoder.py:215-267); kind "ply", "txt" or "bbox" (default: by
     extension, ``bbox.txt`` for the box).  Returns (points [N,3] f32, voxel
     size, voxel index or None) -- the index is the ``quality`` property the
-    exporter writes, used to reorder checkpoint buffers (encoder.py:320-323)."""
+    exporter writes, which a caller may use to reorder checkpoint buffers
+    (encoder.py:320-323).  The reference tests ``'quality'`` among the file's
+    element names (encoder.py:257), where it never is, so it never reorders;
+    the index here is an explicit opt-in.  A mesh-form file gives no index:
+    its points are the unique corner vertices, not one row per voxel."""
     if kind is None:
         kind = ("ply" if path.endswith(".ply") else
                 "bbox" if os.path.basename(path) == "bbox.txt" else "txt")
@@ -206,7 +210,7 @@ def load_voxels(path, voxel_size=None, kind=None):
             pts = np.unique(pts, axis=0)
         elif voxel_size is None:
             raise ValueError(f"{path}: a vertex-only voxel file needs the voxel size")
-        if "quality" in v:
+        elif "quality" in v:
             index = v["quality"].astype(np.int64)
         return np.ascontiguousarray(pts), float(voxel_size), index
     if kind == "bbox":

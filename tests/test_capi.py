@@ -83,3 +83,42 @@ def test_cpu_tensors_are_rejected_like_the_reference():
         _ext.svo_intersect(z, z, z, torch.zeros(1, 2, 9, dtype=torch.int32), 0.1, 4)
     with pytest.raises(NotImplementedError):
         _ext.ball_intersect(z, z, z, 0.1, 4)
+
+
+def test_render_hits_rejects_trees_deeper_than_the_traversal():
+    """The fused traversal holds one record per level (depth + 2 levels, at
+    most 24): a deeper tree is refused with NSVF_ESTACK before any launch
+    instead of losing the hits below the last level."""
+    L = _lib.lib()
+    buf = (ctypes.c_float * 64)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    args = lambda depth: (p, p, 64, 0, p, 3, depth, 60, 0.01, p, p, p, p, p, None)
+    assert L.nsvf_render_hits(*args(23)) == -3
+    assert b"depth 23" in L.nsvf_last_error()
+    assert L.nsvf_render_hits(*args(-2)) == -1
+    cam = _lib.Camera()
+    cam.H = cam.W = cam.h = cam.w = 8
+    rc = L.nsvf_render_hits_camera(ctypes.byref(cam), p, 3, 25, 60, 0.01, p, p, p, p, p, p, p,
+                                   None)
+    assert rc == -3
+
+
+def test_tree_depth_of_packed_trees():
+    """DeviceScene passes the depth the root's scale gives (octree.cpp:119):
+    the shell scene's tree is 5 deep, a one-voxel tree 0, and a synthetic
+    chain 25 deep is beyond the traversal."""
+    from nsvf_amd.render import tree_depth
+    sc = shell()
+    coords, _ = O.discretize_points(sc.points, np.float32(sc.voxel_size) / 2)
+    (_, rch), _ = _octree_both(coords)
+    assert tree_depth(rch) == 5
+    ch = np.full((2, 9), -1, np.int32)
+    ch[0, 8], ch[1, :] = 1, [0, -1, -1, -1, -1, -1, -1, -1, 2]
+    assert tree_depth(ch) == 0
+    deep = np.full((27, 9), -1, np.int32)
+    deep[0, 8] = 1
+    for k in range(1, 27):
+        deep[k, 0], deep[k, 8] = k - 1, 1 << k
+    assert tree_depth(deep) == 25
+    with pytest.raises(ValueError):
+        tree_depth(np.array([[0, -1, -1, -1, -1, -1, -1, -1, 3]], np.int32))

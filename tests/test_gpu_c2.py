@@ -15,8 +15,15 @@ Bars:
     given the GPU's coarse probabilities; sampler mode on and off;
   * FP32 field: RGB, depth and alpha within 1e-5 of FP64 with the same
     samples, on the subset;
-  * split-BF16 field (the served precision): per-pixel RGB within 1e-3 of
-    FP64 end to end, on the subset;
+  * split-BF16 field (the served precision): per-pixel RGB, depth and alpha
+    within 5e-5 / 5e-4 / 5e-5 of FP64 with the same samples; end to end
+    within 1e-3 on every ray except sampling flips -- rays where the FP32
+    path, too, is more than 1e-4 from FP64 end to end.  A flip is a sampler
+    decision that FP32 rounding of the coarse probabilities changes: in the
+    reference's mode the fine pass's trailing loop stops on ray 0's slot of
+    the batch row (sample_gpu.cu:194), so where a fine step ends near a bin
+    boundary the ray gains or loses its last bins' tails.  At most 10 flips
+    in the subset;
   * grazing rays (hit sets an FP64 slab test decides differently) counted;
     each difference is a crossing shorter than 1e-6;
   * prune scores of 256 voxels within 1e-4 of FP64; the x8 split bit-exact
@@ -124,9 +131,11 @@ def views(request, c2, subset, device):
     W = ref["c_idx"].shape[1]
     gp = csr_to_padded(g32["c_cnt"], g32["c_off"], g32["c_prob"], W, np.float32(0))
     ref_s = O.render_view(sc, rs, rd, compat=compat, subset=sel, coarse_probs=gp)
+    gp3 = csr_to_padded(g3["c_cnt"], g3["c_off"], g3["c_prob"], W, np.float32(0))
+    ref_s3 = O.render_view(sc, rs, rd, compat=compat, subset=sel, coarse_probs=gp3)
     log(f"c2 compat={compat}: oracle same-samples done")
     return {"compat": compat, "sel": sel, "n_full": n_full, "g32": g32, "g3": g3, "ref": ref,
-            "ref_s": ref_s}
+            "ref_s": ref_s, "ref_s3": ref_s3}
 
 
 def test_c2_hit_rows_bit_exact(c2, views):
@@ -179,14 +188,27 @@ def test_c2_fp32_within_1e5_of_fp64_same_samples(views):
     assert e["rgb"] < 1e-5 and e["missed"] < 1e-5 and e["depth"] < 1e-5
 
 
-def test_c2_bf16x3_within_1e3_of_fp64_end_to_end(views):
-    sel, g, ref = views["sel"], views["g3"], views["ref"]
+def test_c2_bf16x3_same_samples(views):
+    sel, g, ref = views["sel"], views["g3"], views["ref_s3"]
     e = {k: float(np.abs(g[k][sel] - ref[k]).max()) for k in ("rgb", "depth", "missed")}
-    d = g["rgb"][sel].astype(np.float64) - ref["rgb"]
+    log(f"bf16x3 vs fp64 (same samples, {len(sel)} rays): rgb {e['rgb']:.2e} "
+        f"depth {e['depth']:.2e} alpha {e['missed']:.2e}")
+    assert e["rgb"] <= 5e-5 and e["missed"] <= 5e-5 and e["depth"] <= 5e-4
+
+
+def test_c2_bf16x3_within_1e3_of_fp64_end_to_end(views):
+    sel, g3, g32, ref = views["sel"], views["g3"], views["g32"], views["ref"]
+    e3 = np.abs(g3["rgb"][sel] - ref["rgb"]).max(1)
+    e32 = np.abs(g32["rgb"][sel] - ref["rgb"]).max(1)
+    flips = e32 > 1e-4
+    d = g3["rgb"][sel].astype(np.float64) - ref["rgb"]
     psnr = 10 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
-    log(f"bf16x3 vs fp64 (end to end, {len(sel)} rays, {views['n_full']} full rows): "
-        f"rgb {e['rgb']:.2e} depth {e['depth']:.2e} alpha {e['missed']:.2e} PSNR {psnr:.1f} dB")
-    assert e["rgb"] <= 1e-3 and psnr >= 45.0
+    rest = float(e3[~flips].max()) if (~flips).any() else 0.0
+    log(f"end to end vs fp64 ({len(sel)} rays, {views['n_full']} full rows): bf16x3 rgb max "
+        f"{e3.max():.2e}, fp32 rgb max {e32.max():.2e}; {int(flips.sum())} sampling flips "
+        f"(fp32 also > 1e-4: {np.round(e32[flips], 5).tolist()}); bf16x3 elsewhere {rest:.2e}; "
+        f"bf16x3 PSNR {psnr:.1f} dB")
+    assert rest <= 1e-3 and int(flips.sum()) <= 10 and psnr >= 45.0
 
 
 def test_c2_grazing_rays_counted(c2, views):

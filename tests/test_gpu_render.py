@@ -3,9 +3,9 @@ on the spherical-shell scene (64x64 view).
 
 Bars: hit flags, sorted hit rows, sample counts and sample streams bit-exact
 (both sides use the same FP32 operation order); field outputs and images
-within the stated tolerances (FP32 MFMA: 1e-4 on sigma, 1e-5 on the image
-when both sides sample the same points; BF16 MFMA: error reported, PSNR
-against FP64 >= 40 dB)."""
+within the stated tolerances (FP32 MFMA: 2e-5 on sigma, 1e-5 on the image
+when both sides sample the same points; split BF16: 1e-4 on the image; BF16
+MFMA: bars just above its measured error, PSNR against FP64 > 60 dB)."""
 import math
 
 import numpy as np
@@ -81,7 +81,7 @@ def test_hits_and_coarse_samples_bit_exact(setup, device):
     assert np.array_equal(sx, ref["c_dist"])
 
 
-@pytest.mark.parametrize("precision,tol_sigma,tol_rgb", [("fp32", 2e-4, 2e-5), ("bf16", 0.15, 0.03),
+@pytest.mark.parametrize("precision,tol_sigma,tol_rgb", [("fp32", 2e-5, 2e-6), ("bf16", 0.08, 0.02),
                                                          ("bf16x3", 4e-4, 1e-4)])
 def test_field_matches_fp64(setup, device, precision, tol_sigma, tol_rgb):
     sc, rs, rd, dsc, ref = setup
@@ -125,8 +125,7 @@ def psnr(a, b):
 
 def test_image_fp32_within_1e5_of_fp64(setup, device):
     """Same sample points on both sides (the oracle's fine sampler is fed the
-    GPU's coarse probabilities): RGB, depth and alpha within 1e-5 (depth
-    within 1e-5 relative to the 5.0 background depth scale)."""
+    GPU's coarse probabilities): RGB, depth and alpha within 1e-5."""
     sc, rs, rd, dsc, ref = setup
     r, rgb, depth, missed = run(dsc, rs, rd, device, precision="fp32")
     k = r.kept
@@ -138,7 +137,7 @@ def test_image_fp32_within_1e5_of_fp64(setup, device):
     e_alpha = np.abs(missed - ref2["missed"]).max()
     e_depth = np.abs(depth - ref2["depth"]).max()
     print(f"fp32 vs fp64: rgb {e_rgb:.2e} alpha {e_alpha:.2e} depth {e_depth:.2e}")
-    assert e_rgb < 1e-5 and e_alpha < 1e-5 and e_depth < 5e-5
+    assert e_rgb < 1e-5 and e_alpha < 1e-5 and e_depth < 1e-5
 
 
 def test_image_bf16_against_fp64(setup, device):
@@ -146,9 +145,11 @@ def test_image_bf16_against_fp64(setup, device):
     r, rgb, depth, missed = run(dsc, rs, rd, device, precision="bf16")
     p = psnr(rgb, ref["rgb"])
     e = np.abs(rgb - ref["rgb"]).max()
-    print(f"bf16 vs fp64: PSNR {p:.2f} dB, max |d rgb| {e:.3e}, "
-          f"max |d alpha| {np.abs(missed - ref['missed']).max():.3e}")
-    assert p > 40.0
+    ea = np.abs(missed - ref['missed']).max()
+    print(f"bf16 vs fp64: PSNR {p:.2f} dB, max |d rgb| {e:.3e}, max |d alpha| {ea:.3e}")
+    # measured: PSNR 64.6 dB, rgb 6.8e-3, alpha 9.4e-3 (BF16 operands carry 8
+    # significant bits; the split-BF16 precision is the one within 1e-3)
+    assert p > 60.0 and e < 1e-2 and ea < 1.5e-2
 
 
 def test_image_bf16x3_against_fp64(setup, device):

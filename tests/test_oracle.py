@@ -692,3 +692,241 @@ def test_track_voxel_probs_matches_scatter_add():
         np.add.at(col, np.where(idx[r] < 0, N, idx[r]), p[r])
         ref = np.maximum(ref, col[:N])
     assert np.array_equal(O.track_voxel_probs(idx, p, N), ref)
+
+
+# ---- the inverse-CDF wrapper's grouping (clib/__init__.py:234-270): which
+# ray is "the next ray" (sample_gpu.cu:188) and "ray 0" (:194) in compat mode
+
+def _wrapper_rows(N, P, counts, probs, steps):
+    """N rays; ray r's slot k holds voxel id 1000 r + k for k < counts[r]
+    over unit bins [k, k + 1], probabilities probs[r] per slot."""
+    idx = np.full((N, P), -1, np.int32)
+    lo = np.full((N, P), 1e4, np.float32)
+    hi = np.full((N, P), 1e4, np.float32)
+    pr = np.zeros((N, P), np.float32)
+    for r in range(N):
+        c = counts[r]
+        idx[r, :c] = 1000 * r + np.arange(c)
+        lo[r, :c] = np.arange(c)
+        hi[r, :c] = np.arange(c) + 1
+        pr[r, :c] = probs[r]
+    return idx, lo, hi, pr, np.full(N, np.float32(steps), np.float32)
+
+
+def _next_ray_by_hand(N):
+    """The next ray each ray reads, derived from the wrapper: rows padded to
+    H = 200 ceil(N / 200) with copies of row 0 (a pad row's ids are row
+    0's), regrouped to [200, H / 200, P] (batch row b holds rays
+    (H / 200) b ...), cut into chunks of 800 columns, each a contiguous
+    [200, cols, P] buffer: the next ray of (b, c) is (b, c + 1) inside the
+    chunk, the first column of the chunk in batch row b + 1 after its last
+    column, and past the end of the buffer (-1) for the last ray of the
+    chunk.  Returns the ray whose ids the next slot holds (-1: none)."""
+    G = 200
+    H = -(-N // G) * G
+    K = H // G
+    out = []
+    for r in range(N):
+        b, c = divmod(r, K)
+        c0 = (c // 800) * 800
+        width = min(800, K - c0)
+        if c + 1 < c0 + width:
+            nb, nc = b, c + 1
+        elif b + 1 < G:
+            nb, nc = b + 1, c0
+        else:
+            out.append(-1)
+            continue
+        nr = nb * K + nc
+        out.append(nr if nr < N else 0)     # pad rows copy row 0
+    return np.array(out)
+
+
+def _ray0_by_hand(N):
+    """Ray 0 of each ray's batch row inside its chunk: (b, first column of
+    the chunk)."""
+    K = -(-N // 200)
+    r = np.arange(N)
+    b, c = r // K, r % K
+    return b * K + (c // 800) * 800
+
+
+@pytest.mark.parametrize("N", [401, 200 * 801])
+def test_icdf_wrapper_next_ray_is_hand_derived(N):
+    """Full rows whose probabilities sum to 0.8 with 4 steps: the last CDF
+    step (0.875) runs past the last bin, and in compat mode the trailing
+    loop then adds one sample carrying the next slot's voxel id -- the next
+    ray's first id, or none (-1) past the buffer."""
+    P = 4
+    idx, lo, hi, pr, st = _wrapper_rows(N, P, [P] * N, [0.2] * N, 4.0)
+    si, _, _, _, _ = O.inverse_cdf_sampling(idx, lo, hi, pr, st, -1.0, True, compat=True)
+    nxt = _next_ray_by_hand(N)
+    # steps at CDF 0.125 (bin 0), 0.375 (bin 1), 0.625 (bin 3), 0.875 (past
+    # bin 3) with the tails of bins 0..3 between them: 7 samples of the ray's
+    # own ids, then the extra one
+    last = si[:, 7]
+    assert (si[:, :7] // 1000 == np.arange(N)[:, None]).all()
+    assert np.array_equal(si[0, :7] % 1000, [0, 0, 1, 1, 2, 3, 3])
+    want = np.where(nxt >= 0, 1000 * np.maximum(nxt, 0), -1)
+    assert np.array_equal(last, want)
+    if N == 401:        # the last real ray reads the first pad row, a copy of row 0
+        assert last[400] == 0 and last[399] == 400 * 1000
+    else:               # a chunk's last column: the next batch row, not the next ray
+        assert last[799] == 801 * 1000 and last[800] == 1601 * 1000
+    si2, _, _, _, _ = O.inverse_cdf_sampling(idx, lo, hi, pr, st, -1.0, True, compat=False)
+    assert (si2[:, 7:] == -1).all()
+
+
+@pytest.mark.parametrize("N", [401, 200 * 801])
+def test_icdf_wrapper_ray0_is_hand_derived(N):
+    """Rays of 6 bins whose first bin holds all the probability: the 4 steps
+    land in bin 0 and the trailing loop emits bin tails 0, 1, ... until the
+    slot after the current one is empty in ray 0 of the batch row (compat
+    mode), so a ray gets 4 + min(6, count of its ray 0) samples."""
+    P = 8
+    counts = np.full(N, 6)
+    r0 = _ray0_by_hand(N)
+    heads = np.unique(r0)
+    counts[heads] = 1 + (np.arange(len(heads)) % 5)       # ray 0s: 1 .. 5 bins
+    probs = [[1.0] + [0.0] * 7 for _ in range(N)]
+    idx, lo, hi, pr, st = _wrapper_rows(N, P, counts, [np.array(p[:c] + [0] * (P - c))[: c]
+                                                        for p, c in zip(probs, counts)], 4.0)
+    pr[:, 0] = 1.0
+    si, _, _, _, _ = O.inverse_cdf_sampling(idx, lo, hi, pr, st, -1.0, True, compat=True)
+    got = (si != -1).sum(1)
+    want = 4 + np.minimum(counts, counts[r0])
+    assert np.array_equal(got, want)
+    si2, _, _, _, _ = O.inverse_cdf_sampling(idx, lo, hi, pr, st, -1.0, True, compat=False)
+    assert np.array_equal((si2 != -1).sum(1), 4 + counts)
+
+
+# ---- hierarchical bins (nerf.py:64-79): [z - d/2, z + d/2], probs
+# (p + 1e-5) / sum over every column, pads included
+
+def _two_equal_bins():
+    si = np.array([[7, 8, -1, -1]], np.int32)
+    sd = np.array([[1.0, 1.5, 1e4, 1e4]], np.float32)
+    sx = np.array([[0.5, 0.5, 0.0, 0.0]], np.float32)
+    pr = np.array([[0.25, 0.25, 0.0, 0.0]], np.float32)
+    return si, sd, sx, pr
+
+
+def test_fine_bins_hand_computed():
+    si, sd, sx, pr = _two_equal_bins()
+    fi, flo, fhi, fp, fst = O.prepare_fine_bins(si, sd, sx, pr, 64.0)
+    assert np.array_equal(fi, si)
+    assert list(flo[0]) == [0.75, 1.25, 1e4, 1e4] and list(fhi[0]) == [1.25, 1.75, 1e4, 1e4]
+    den = 0.5 + 4e-5                                 # two samples + 1e-5 on all four columns
+    want = np.array([0.25001, 0.25001, 1e-5, 1e-5]) / den
+    assert np.abs(fp[0] - want).max() < 2e-7 * want.max()
+    assert fst[0] == 64.0
+    # the alternatives a slip would give differ by ~2e-5 in p0
+    for alt in (0.25001 / (0.5 + 4e-8 * 0 + 2e-5),    # pads left out of the sum
+                0.25000001 / 0.50000004):             # the commented-out 1e-8
+        assert abs(fp[0, 0] - alt) > 1e-5
+
+
+def test_fine_samples_hand_computed():
+    """64 steps over the two bins: steps k < 32 land in bin 0 at
+    z = 0.75 + 0.5 (k + 1/2) / 64 / p0, then bin 0's tail, steps 32..63 in
+    bin 1 at z = 1.25 + 0.5 ((k + 1/2) / 64 - p0) / p1, then bin 1's tail:
+    66 samples, each (midpoint of z and the previous z, their distance)."""
+    si, sd, sx, pr = _two_equal_bins()
+    fi, flo, fhi, fp, fst = O.prepare_fine_bins(si, sd, sx, pr, 64.0)
+    for compat in (True, False):
+        oi, od, ox, _, _ = O.inverse_cdf_sampling(fi, flo, fhi, fp, fst, -1.0, True, compat=compat)
+        valid = oi[0] != -1
+        assert valid.sum() == 66
+        p0 = 0.25001 / 0.50004
+        z = [0.75 + 0.5 * (k + 0.5) / 64 / p0 for k in range(32)] + [1.25]
+        z += [1.25 + 0.5 * ((k + 0.5) / 64 - p0) / p0 for k in range(32, 64)] + [1.75]
+        prev = [0.75] + z[:-1]
+        want_d = (np.array(z) + np.array(prev)) / 2
+        want_x = np.array(z) - np.array(prev)
+        assert list(oi[0][valid]) == [7] * 33 + [8] * 33
+        assert np.abs(od[0][valid] - want_d).max() < 1e-6
+        assert np.abs(ox[0][valid] - want_x).max() < 1e-6
+        # with p0 = 0.5 (1e-8, or pads left out of the sum) step 31 moves by ~2e-5
+        z31_alt = 0.75 + 0.5 * 31.5 / 64 / 0.5
+        assert abs(z[31] - z31_alt) > 1e-5
+
+
+def test_fine_bins_reject_swapped_halves():
+    si, sd, sx, pr = _two_equal_bins()
+    _, flo, fhi, _, _ = O.prepare_fine_bins(si, sd, sx, pr, 64.0)
+    assert (flo[0, :2] < fhi[0, :2]).all() and (flo[0, :2] == sd[0, :2] - sx[0, :2] / 2).all()
+
+
+def test_fill_background_hand_computed():
+    """nsvf.py:89-101: hit rays keep their colour plus missed x background
+    and their depth plus missed x 5; rays that hit nothing are background
+    (rgb 1, depth 5, missed 1)."""
+    hits = np.array([False, True, False, True])
+    rgb = np.array([[0.1, 0.2, 0.3], [0.0, 0.5, 1.0]])
+    depth = np.array([2.0, 3.0])
+    missed = np.array([0.25, 0.0])
+    frgb, fd, fm = O.fill_background(hits, rgb, depth, missed, bg=(1.0, 0.5, 0.0))
+    assert np.allclose(frgb, [[1.0, 0.5, 0.0], [0.35, 0.325, 0.3], [1.0, 0.5, 0.0], [0.0, 0.5, 1.0]],
+                       rtol=0, atol=1e-15)
+    assert np.allclose(fd, [5.0, 2.0 + 0.25 * 5.0, 5.0, 3.0], rtol=0, atol=1e-15)
+    assert np.array_equal(fm, [1.0, 0.25, 1.0, 0.0])
+
+
+# ---- FP64 slab test and grazing report
+
+def test_aabb_hits_f64_matches_numpy_restatement():
+    sc = shell()
+    rs, rd = shell_rays(width=16)
+    pts = O.shift_centers(sc.points, sc.voxel_size)
+    i_np, lo_np, hi_np = O.aabb_intersect_f64(rs, rd, pts, sc.voxel_size, 60)
+    i_c, lo_c, hi_c, most = O.aabb_hits_f64(rs, rd, pts, sc.voxel_size, 60)
+    assert most <= 60
+    assert np.array_equal(i_np, i_c)
+    assert np.array_equal(lo_np, lo_c) and np.array_equal(hi_np, hi_c)
+
+
+def test_grazing_report_counts_a_constructed_graze():
+    """Rows that equal the FP64 test are no difference; an FP32 hit the FP64
+    test misses counts as a graze when the FP32 crossing is shorter than
+    eps, and not when it is long; an FP64 hit missing from the FP32 row is
+    judged by its FP64 crossing."""
+    pts = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0], [0.0, 20.0, 0.0]], np.float32)
+    rs = np.array([[-2.0, 0.0, 0.0], [-2.0, 5.0, 0.0]], np.float32)
+    rd = np.array([[1.0, 0.0, 0.0], [1.0, 0.0, 0.0]], np.float32)
+    idx, lo, hi = O.aabb_intersect(rs[None], rd[None], pts[None], 1.0, 4)
+    idx, lo, hi = idx[0], lo[0], hi[0]
+    assert list(idx[0, :3]) == [0, 1, -1] and (idx[1] == -1).all()
+    rep = O.grazing_report(idx, lo, hi, rs, rd, pts, 1.0, 4)
+    assert rep["rays"] == 2 and rep["rays_differing_from_fp64"] == 0
+    assert rep["max_abs_hit_depth_vs_fp64"] == 0.0
+
+    def case(k, a, b):
+        i2, l2, h2 = idx.copy(), lo.copy(), hi.copy()
+        i2[0, 2], l2[0, 2], h2[0, 2] = k, np.float32(a), np.float32(b)
+        return O.grazing_report(i2, l2, h2, rs, rd, pts, 1.0, 4)
+
+    rep = case(2, 4.5, 4.5)             # FP32 touches box 2 over length 0
+    assert rep["rays_differing_from_fp64"] == 1 and rep["differences_that_graze"] == 1
+    assert rep["rays_with_hits_shorter_than_eps"] == 1
+    rep = case(2, 1.0, 2.0)             # ... over length 1: not a graze
+    assert rep["rays_differing_from_fp64"] == 1 and rep["differences_that_graze"] == 0
+    i2 = idx.copy()
+    i2[0, 1] = -1                       # FP32 drops box 1, which FP64 crosses over 1
+    rep = O.grazing_report(i2, lo, hi, rs, rd, pts, 1.0, 4)
+    assert rep["rays_differing_from_fp64"] == 1 and rep["differences_that_graze"] == 0
+
+
+def test_render_view_subset_equals_full_view():
+    """Field and compositing on a subset of rays (geometry and sampling over
+    all of them) give the full view's outputs at those rays, in both
+    sampler modes."""
+    sc = shell()
+    rs, rd = shell_rays(width=32)
+    sel = np.sort(np.random.default_rng(3).choice(rs.shape[0], 300, replace=False))
+    for compat in (True, False):
+        full = O.render_view(sc, rs, rd, compat=compat)
+        sub = O.render_view(sc, rs, rd, compat=compat, subset=sel)
+        for k in ("rgb", "depth", "missed"):
+            assert np.abs(sub[k] - full[k][sel]).max() < 1e-12
+        assert np.array_equal(sub["c_idx"], full["c_idx"]) and np.array_equal(sub["f_idx"][sub["eval_rows"]],
+                                                                            full["f_idx"][sub["eval_rows"]])

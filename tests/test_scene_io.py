@@ -69,8 +69,9 @@ def test_export_mesh_two_voxels(tmp_path):
     IO.export_voxels(sc, str(p), return_mesh=True)
     d = IO.read_ply(str(p))
     assert d["vertex"]["x"].shape == (12,) and d["face"]["vertex_indices"].shape == (11, 4)
-    pts, got_vs, _ = IO.load_voxels(str(p))
+    pts, got_vs, index = IO.load_voxels(str(p))
     assert got_vs == vs and pts.shape == (12, 3)
+    assert index is None   # the mesh's points are its corners, not one row per voxel
     assert set(map(tuple, pts.tolist())) == {(x, y, z) for x in (0.0, 0.5, 1.0)
                                              for y in (0.0, 0.5) for z in (0.0, 0.5)}
 
