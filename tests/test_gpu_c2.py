@@ -17,13 +17,13 @@ Bars:
     samples, on the subset;
   * split-BF16 field (the served precision): per-pixel RGB, depth and alpha
     within 5e-5 / 5e-4 / 5e-5 of FP64 with the same samples; end to end
-    within 1e-3 on every ray except sampling flips -- rays where the FP32
-    path, too, is more than 1e-4 from FP64 end to end.  A flip is a sampler
-    decision that FP32 rounding of the coarse probabilities changes: in the
-    reference's mode the fine pass's trailing loop stops on ray 0's slot of
-    the batch row (sample_gpu.cu:194), so where a fine step ends near a bin
-    boundary the ray gains or loses its last bins' tails.  At most 10 flips
-    in the subset;
+    within 1e-3 on every ray whose fine samples carry the oracle's voxel ids
+    -- the others are sampling flips, counted: a sampler decision that the
+    rounding of the coarse probabilities changes (in the reference's mode,
+    for instance, the fine pass's trailing loop stops on ray 0's slot of the
+    batch row, sample_gpu.cu:194, so where the fine steps end near a bin
+    boundary a ray gains or loses its last bins' tails).  At most 0.1% of
+    the subset flips, for FP32 and split BF16 alike;
   * grazing rays (hit sets an FP64 slab test decides differently) counted;
     each difference is a crossing shorter than 1e-6;
   * prune scores of 256 voxels within 1e-4 of FP64; the x8 split bit-exact
@@ -196,19 +196,40 @@ def test_c2_bf16x3_same_samples(views):
     assert e["rgb"] <= 5e-5 and e["missed"] <= 5e-5 and e["depth"] <= 5e-4
 
 
+def fine_flips(g, ref, sel):
+    """Subset rays whose GPU fine samples do not carry the oracle's voxel ids
+    (count or sequence), the oracle sampling with its own (FP64) coarse
+    probabilities."""
+    ev = ref["eval_rows"]
+    ri = ref["f_idx"][ev]
+    cnt = g["f_cnt"][ev]
+    W = max(ri.shape[1], int(cnt.max()) if len(cnt) else 0)
+    ri = np.pad(ri, ((0, 0), (0, W - ri.shape[1])), constant_values=-1)
+    gi = csr_to_padded(cnt, g["f_off"][ev], g["f_cols"][0], W, -1)
+    flip_rows = (gi != ri).any(1)
+    hit_pos = np.nonzero(g["flag"][sel])[0]       # subset positions of its hit rays, in order
+    out = np.zeros(len(sel), bool)
+    out[hit_pos[flip_rows]] = True
+    return out
+
+
 def test_c2_bf16x3_within_1e3_of_fp64_end_to_end(views):
     sel, g3, g32, ref = views["sel"], views["g3"], views["g32"], views["ref"]
     e3 = np.abs(g3["rgb"][sel] - ref["rgb"]).max(1)
     e32 = np.abs(g32["rgb"][sel] - ref["rgb"]).max(1)
-    flips = e32 > 1e-4
+    f3, f32 = fine_flips(g3, ref, sel), fine_flips(g32, ref, sel)
     d = g3["rgb"][sel].astype(np.float64) - ref["rgb"]
     psnr = 10 * np.log10(1.0 / max(float(np.mean(d * d)), 1e-30))
-    rest = float(e3[~flips].max()) if (~flips).any() else 0.0
+    rest3 = float(e3[~f3].max()) if (~f3).any() else 0.0
+    rest32 = float(e32[~f32].max()) if (~f32).any() else 0.0
     log(f"end to end vs fp64 ({len(sel)} rays, {views['n_full']} full rows): bf16x3 rgb max "
-        f"{e3.max():.2e}, fp32 rgb max {e32.max():.2e}; {int(flips.sum())} sampling flips "
-        f"(fp32 also > 1e-4: {np.round(e32[flips], 5).tolist()}); bf16x3 elsewhere {rest:.2e}; "
+        f"{e3.max():.2e}, {int(f3.sum())} flips, {rest3:.2e} elsewhere; fp32 rgb max "
+        f"{e32.max():.2e}, {int(f32.sum())} flips, {rest32:.2e} elsewhere; flip errors bf16x3 "
+        f"{np.round(e3[f3], 5).tolist()}, fp32 {np.round(e32[f32], 5).tolist()}; "
         f"bf16x3 PSNR {psnr:.1f} dB")
-    assert rest <= 1e-3 and int(flips.sum()) <= 10 and psnr >= 45.0
+    assert rest3 <= 1e-3 and rest32 <= 1e-4
+    assert f3.sum() <= len(sel) // 1000 and f32.sum() <= len(sel) // 1000
+    assert psnr >= 45.0
 
 
 def test_c2_grazing_rays_counted(c2, views):
