@@ -398,6 +398,27 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
                     const float* bias, float* out, void* workspace,
                     int64_t workspace_bytes, void* stream);
 
+/* nsvf_field_eval without a caller workspace and without allocating: the
+ * BF16 (0) and split-BF16 (2) paths stage their operand tiles in a static
+ * device array of nsvf_field_static_scratch_bytes() bytes (64 MiB) that
+ * every GPU the library runs on holds until the process ends; FP32 (1)
+ * needs none.  Larger m_max is refused with NSVF_EINVAL (about 524,288
+ * split-BF16 or 1,048,576 BF16 samples at D = 32).  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid it).
+ * Calls on one GPU share the array: calls that overlap on that GPU (two
+ * streams, or two host threads) race and must be ordered by the caller.
+ * Same arguments and results as nsvf_field_eval otherwise.                 */
+int64_t nsvf_field_static_scratch_bytes(void);
+int nsvf_field_eval_static(int precision, int D, int H, int L,
+                           const float* centers, const int32_t* feats,
+                           const float* values, float voxel_size,
+                           const float* ray_start, const float* ray_dir,
+                           const void* samples, const int32_t* list,
+                           const int32_t* m_dev, int64_t m_max,
+                           const void* wfrag, const float* bias, float* out,
+                           void* stream);
+
 /* The two kernels of the workspace BF16 path, callable one by one (so a
  * caller can time or overlap them): the gather writes the MLP's operand
  * tiles into the workspace, the MLP reads them.  Same arguments as

@@ -74,6 +74,9 @@ SIGNATURES = {
                               _I),
     "nsvf_field_mlp_bf16": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P, _P], _I),
     "nsvf_field_bf16x3_supported": ([_I, _I, _I], _I),
+    "nsvf_field_static_scratch_bytes": ([], _L),
+    "nsvf_field_eval_static": ([_I, _I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _P,
+                                _P, _P], _I),
     "nsvf_field_embed_bf16x3": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _L,
                                  _P], _I),
     "nsvf_field_mlp_bf16x3": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P, _P], _I),

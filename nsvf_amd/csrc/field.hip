@@ -26,6 +26,7 @@
 // biases fp32 [layer][rows padded to 16].  BF16 fragments are staged in LDS.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "render.h"
 
@@ -1599,6 +1600,44 @@ int launch_prune(int precision, const FieldScene& sc, int64_t v0, int64_t n, int
 }  // namespace
 }  // namespace nsvf
 
+namespace nsvf {
+
+// Scratch of the allocation-free field entry point (nsvf_field_eval_static):
+// the split paths' operand tiles, 64 MiB on every GPU the library runs on,
+// held until the process ends.  Host code never uses the array's name as an
+// address: each device's own copy is found once with hipGetSymbolAddress.
+constexpr int64_t kStaticScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The current device's address of `scratch`, looked up on the first call
+// on that device (with it current) and kept.
+static int scratch_address(void** out) {
+  int dev = 0;
+  const hipError_t de = hipGetDevice(&dev);
+  if (de != hipSuccess)
+    return set_error(NSVF_ELAUNCH, "field_eval_static: hipGetDevice: %s", hipGetErrorString(de));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return set_error(NSVF_EINVAL, "field_eval_static: device %d beyond the first %d", dev,
+                     kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_error(NSVF_ELAUNCH, "field_eval_static: hipGetSymbolAddress: %s",
+                       hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NSVF_OK;
+}
+
+}  // namespace nsvf
+
 using namespace nsvf;
 
 extern "C" {
@@ -1693,6 +1732,41 @@ int nsvf_field_eval(int precision, int D, int H, int L, const float* centers,
   NSVF_FIELD_CASE(32, 256, 4)
 #undef NSVF_FIELD_CASE
   return set_error(NSVF_EINVAL, "field_eval: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int64_t nsvf_field_static_scratch_bytes(void) { return kStaticScratchBytes; }
+
+int nsvf_field_eval_static(int precision, int D, int H, int L, const float* centers,
+                           const int32_t* feats, const float* values, float voxel_size,
+                           const float* ray_start, const float* ray_dir, const void* samples,
+                           const int32_t* list, const int32_t* m_dev, int64_t m_max,
+                           const void* wfrag, const float* bias, float* out, void* stream) {
+  clear_error();
+  if (m_max <= 0) return NSVF_OK;
+  if (precision < 0 || precision > 2)
+    return set_error(NSVF_EINVAL, "field_eval_static: precision %d (0 bf16, 1 fp32, 2 bf16x3)",
+                     precision);
+  if (precision == 2 && !nsvf_field_bf16x3_supported(D, H, L))
+    return set_error(NSVF_EINVAL, "field_eval_static: bf16x3 does not support D=%d H=%d L=%d", D,
+                     H, L);
+  // FP32 runs without scratch; BF16 and split BF16 stage their operand tiles
+  const int64_t need = precision == 1 ? 0 : nsvf_field_workspace_bytes(precision, D, H, L, m_max);
+  if (precision != 1 && need <= 0)
+    return set_error(NSVF_EINVAL, "field_eval_static: unsupported shape D=%d H=%d L=%d", D, H, L);
+  if (need > kStaticScratchBytes) {
+    const int64_t per = need / m_max > 0 ? need / m_max : 1;
+    return set_error(NSVF_EINVAL,
+                     "field_eval_static: %lld samples need %lld bytes of scratch; the static "
+                     "scratch holds %lld (about %lld samples at this precision and shape)",
+                     (long long)m_max, (long long)need, (long long)kStaticScratchBytes,
+                     (long long)(kStaticScratchBytes / per));
+  }
+  void* ws = nullptr;
+  if (need > 0)
+    if (const int rc = scratch_address(&ws)) return rc;
+  return nsvf_field_eval(precision, D, H, L, centers, feats, values, voxel_size, ray_start,
+                         ray_dir, samples, list, m_dev, m_max, wfrag, bias, out, ws,
+                         need > 0 ? kStaticScratchBytes : 0, stream);
 }
 
 int nsvf_field_embed_bf16(int D, int H, int L, const float* centers, const int32_t* feats,
