@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-headline", action="store_true",
                     help="time the headline precision only (for profiling runs)")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the from-camera and voxel-statistics views (so that a kernel "
+                         "trace holds the headline's identical launches only)")
     ap.add_argument("--camera-seed", type=int, default=11)
     ap.add_argument("--field", choices=["auto", "split", "fused"], default="auto",
                     help="BF16 field as a gather kernel + an MLP kernel, one fused kernel, or "
@@ -226,30 +229,38 @@ def main():
         "achieved_GBs": round(emb_bytes / (emb_ms * 1e-3) / 1e9, 1) if emb_ms else None,
         "bound": "hbm", "traffic": measured_traffic("embed_bf16x3_kernel")}}
 
-    # the same view from its camera: rays generated inside the traversal
-    # kernel (outside the timed region; reported beside the metric)
-    from nsvf_amd.rays import ViewCamera
-    vcam = ViewCamera.looking_at(cam, args.res, args.res)
-    rnd.render_camera(dsc, vcam)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
+    variants = {}
+    if not args.no_variants:
+        # the same view from its camera: rays generated inside the traversal
+        # kernel (outside the timed region; reported beside the metric)
+        from nsvf_amd.rays import ViewCamera
+        vcam = ViewCamera.looking_at(cam, args.res, args.res)
         rnd.render_camera(dsc, vcam)
-    torch.cuda.synchronize()
-    cam_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            rnd.render_camera(dsc, vcam)
+        torch.cuda.synchronize()
+        cam_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
 
-    # the same view with train-set voxel statistics on (both passes fold
-    # their sample probabilities into a per-voxel running max)
-    rnd.track = torch.zeros(sc.num_voxels, dtype=torch.float32, device=dev)
-    rnd.render(dsc, rs, rd, width=args.res)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
+        # the same view with train-set voxel statistics on (both passes fold
+        # their sample probabilities into a per-voxel running max)
+        rnd.track = torch.zeros(sc.num_voxels, dtype=torch.float32, device=dev)
         rnd.render(dsc, rs, rd, width=args.res)
-    torch.cuda.synchronize()
-    track_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
-    tracked_voxels = int((rnd.track > 0).sum())
-    rnd.track = None
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            rnd.render(dsc, rs, rd, width=args.res)
+        torch.cuda.synchronize()
+        track_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
+        tracked_voxels = int((rnd.track > 0).sum())
+        rnd.track = None
+        variants = {
+            "from_camera": {"ms_per_view": round(cam_ms, 3),
+                            "note": "rays generated inside the traversal kernel from the camera"},
+            "with_voxel_stats": {"ms_per_view": round(track_ms, 3),
+                                 "voxels_seen": tracked_voxels,
+                                 "note": "track_voxel_probs after both passes"}}
 
     # per-stage breakdown of one traced view (outside the timed region)
     rnd.trace = []
@@ -301,10 +312,7 @@ def main():
         "stage_rooflines": stage_rooflines(stages, R, st.hit_rays, dsc.num_nodes, sc.max_hits,
                                            st.coarse_slots, st.fine_slots, n_fine_eval),
         "precisions": precisions,
-        "from_camera": {"ms_per_view": round(cam_ms, 3),
-                        "note": "rays generated inside the traversal kernel from the camera"},
-        "with_voxel_stats": {"ms_per_view": round(track_ms, 3), "voxels_seen": tracked_voxels,
-                             "note": "track_voxel_probs after both passes"},
+        **variants,
     }
 
     # ---- plain FP32 path (the comparator)
