@@ -64,9 +64,12 @@ def parse():
                     help="BF16 field as a gather kernel + an MLP kernel, one fused kernel, or "
                          "fused where the fused kernel supports the field's shape")
     ap.add_argument("--serve-requests", type=int, default=40,
-                    help="requests per arrival rate in the serving measurement (0 = skip)")
-    ap.add_argument("--serve-rates", default="10,20,40,80,160",
-                    help="Poisson arrival rates (requests/s) of the serving measurement")
+                    help="least requests per arrival rate in the serving measurement (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="arrivals per rate span at least this long (more requests at high rates)")
+    ap.add_argument("--serve-rates", default="10,20,40,80,120,160,inf",
+                    help="Poisson arrival rates (requests/s) of the serving measurement; inf: "
+                         "every request arrives at once (the saturation throughput)")
     ap.add_argument("--no-rows", action="store_true",
                     help="skip the backward / training / pruning / splitting measurements")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -667,25 +670,67 @@ def serving(args, dev):
             reqs.append((name, torch.from_numpy(rs).to(dev), torch.from_numpy(rd).to(dev), res))
         return reqs
 
-    # warm-up: one request of each scene at each size
-    warm = [(n, *make_requests(1)[0][1:]) for n in names]
-    for w in warm:
-        srv.submit(*w)
+    # warm-up: every slot sees the largest job (so the measured runs
+    # allocate nothing) and every scene renders once
+    big = names[-1]
+    for _ in range(2 * len(srv.scheduler.slots)):
+        rs, rd = S.camera_rays(800, 800, S.sphere_camera(2.5, int(rng.integers(1 << 30))))
+        srv.submit(big, torch.from_numpy(rs).to(dev), torch.from_numpy(rd).to(dev), width=800)
+    srv.drain()
+    for n in names:
+        srv.submit(n, *make_requests(1)[0][1:])
     srv.drain()
     torch.cuda.synchronize()
+    # each (scene, resolution) alone: submitted to the idle server, arrival
+    # to completion (the second of two)
+    solo = {}
+    for n in names:
+        for res in SERVE_RES:
+            rs, rd = S.camera_rays(res, res, S.sphere_camera(2.5, 99))
+            rs, rd = torch.from_numpy(rs).to(dev), torch.from_numpy(rd).to(dev)
+            for _ in range(2):
+                req = srv.submit(n, rs, rd, width=res)
+                srv.drain()
+            solo[(n, res * res)] = req.latency
+    log("serving: alone (ms) " + ", ".join(f"{n}@{int(math.sqrt(k))}: {v * 1e3:.1f}"
+                                           for (n, k), v in solo.items()))
+    allocs0 = torch.cuda.memory_stats(dev).get("num_alloc_retries", 0), \
+        torch.cuda.memory_stats(dev).get("segment.all.allocated", 0)
     runs = []
-    for rate in [float(x) for x in args.serve_rates.split(",") if x]:
-        items = make_requests(args.serve_requests)
-        gaps = rng.exponential(1.0 / rate, len(items))
-        arrivals = [(float(t), *it) for t, it in zip(np.cumsum(gaps) - gaps[0], items)]
+    for rate_s in [x for x in args.serve_rates.split(",") if x]:
+        rate = float(rate_s)
+        n = args.serve_requests if math.isinf(rate) else \
+            max(args.serve_requests, int(rate * args.serve_seconds))
+        items = make_requests(n)
+        if math.isinf(rate):
+            arrivals = [(0.0, *it) for it in items]
+        else:
+            gaps = rng.exponential(1.0 / rate, len(items))
+            arrivals = [(float(t), *it) for t, it in zip(np.cumsum(gaps) - gaps[0], items)]
         torch.cuda.synchronize()
         reqs = srv.run_arrivals(arrivals)
         torch.cuda.synchronize()
         summ = latency_summary(reqs)
-        summ["rate_req_per_s"] = rate
+        summ["rate_req_per_s"] = rate_s
+        done = [r for r in reqs if r.state == "done"]
+        slow = np.array([r.latency / solo[(r.scene, r.rays)] for r in done])
+        span = max(r.finished for r in done) - min(r.arrival for r in done)
+        summ["slowdown_vs_alone"] = {"p50": round(float(np.percentile(slow, 50)), 2),
+                                     "p99": round(float(np.percentile(slow, 99)), 2)}
+        # the requests' time alone, summed, over the run's span: above 1 when
+        # overlapping them beats running them one after another
+        summ["alone_time_per_span"] = round(sum(solo[(r.scene, r.rays)] for r in done) / span, 3)
         runs.append(summ)
-        log(f"serving @ {rate:g} req/s: {summ['rays_per_sec'] / 1e6:.1f} Mrays/s, "
-            f"p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms")
+        ph = summ["phases_ms"]
+        log(f"serving @ {rate_s} req/s ({n} requests): {summ['rays_per_sec'] / 1e6:.1f} Mrays/s, "
+            f"p50 {summ['p50_ms']:.1f} ms, p99 {summ['p99_ms']:.1f} ms; queued p50/p99 "
+            f"{ph['queued']['p50']}/{ph['queued']['p99']}, geometry {ph['geometry']['p50']}/"
+            f"{ph['geometry']['p99']}, shading {ph['shading']['p50']}/{ph['shading']['p99']} ms; "
+            f"mean batch {summ['mean_batch']}; slowdown vs alone p50/p99 "
+            f"{summ['slowdown_vs_alone']['p50']}/{summ['slowdown_vs_alone']['p99']}, "
+            f"alone time / span {summ['alone_time_per_span']}")
+    allocs1 = torch.cuda.memory_stats(dev).get("num_alloc_retries", 0), \
+        torch.cuda.memory_stats(dev).get("segment.all.allocated", 0)
     # error against the plain FP32 path, one 256^2 view per scene
     errs = {}
     for name in names:
@@ -701,9 +746,15 @@ def serving(args, dev):
                       "psnr_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None}
         del pl
         torch.cuda.empty_cache()
+    sat = max(runs, key=lambda r: r["rays_per_sec"] or 0)
     return {"scenes": {n: sc.num_voxels for n, sc in scenes.items()}, "slots": 4,
             "precision": HEADLINE, "sampler_compat": bool(srv.compat),
+            "max_batch": srv.scheduler.max_batch, "max_batch_rays": srv.scheduler.max_batch_rays,
             "resolutions": list(SERVE_RES), "runs": runs,
+            "saturation": {"rays_per_sec": sat["rays_per_sec"], "at_rate": sat["rate_req_per_s"],
+                           "p50_ms": sat["p50_ms"], "p99_ms": sat["p99_ms"]},
+            "alone_ms": {f"{n}@{int(math.sqrt(k))}": round(v * 1e3, 2) for (n, k), v in solo.items()},
+            "device_segments_allocated_during_runs": int(allocs1[1] - allocs0[1]),
             "error_vs_plain_fp32_256": errs, "budget_peak_bytes": srv.budget.peak}
 
 

@@ -302,7 +302,10 @@ int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,
  * steps = sum of segment lengths / step_size, probs = length / sum).  Writes
  * c_cnt[r], the fine-pass bound f_bound[r] = ceil(fine_steps) + c_cnt[r] + 2
  * and *max_count = the longest row.  *n_hit_dev (device) is the number of
- * hit rays, n_hit_max >= it.                                               */
+ * hit rays, n_hit_max >= it.  Segments (n_seg in 1..16, else 0 and NULLs):
+ * several views rendered as one batch, segment s holding the frame rays
+ * [seg_bounds[s], seg_bounds[s + 1]); seg_max[s] (zeroed by the caller)
+ * receives the longest row of segment s.                                  */
 int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
                               int64_t n_hit_max, const void* hits,
                               const int32_t* count, const float* tot,
@@ -310,7 +313,8 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
                               int n_max, float step_size, int compat,
                               float fine_steps, void* samples, int32_t* c_cnt,
                               int32_t* f_bound, int32_t* max_count,
-                              void* stream);
+                              const int32_t* seg_bounds, int n_seg,
+                              int32_t* seg_max, void* stream);
 
 /* Fine (hierarchical) inverse-CDF sampling: bins are each ray's coarse
  * samples [depth -+ length/2] with probs (p + 1e-5) / column sum over the
@@ -325,6 +329,9 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
  * -1: to evaluate; -2: unused slot.  The field list of the -1 slots comes
  * from nsvf_render_eval_list; nsvf_render_composite_reuse reads the reused
  * values from the coarse field.
+ * Segments as for nsvf_render_sample_coarse: a ray of segment s normalises
+ * its bins over seg_max[s] columns (its own view's longest coarse row)
+ * instead of *c_max, so each view of a batch gets its own result.
  * c_prob: the coarse samples' compositing weights, slot-aligned with the
  * coarse stream.  Without c_field it holds them (nsvf_render_composite's
  * prob_out).  With c_field (float4 per coarse slot, the coarse field) the
@@ -339,7 +346,8 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev,
                             float* c_prob, const int32_t* f_bound,
                             const int32_t* f_off, float fine_steps,
                             int compat, void* samples, int32_t* f_cnt,
-                            int16_t* src, void* stream);
+                            int16_t* src, const int32_t* seg_bounds, int n_seg,
+                            const int32_t* seg_max, void* stream);
 
 /* The field's evaluation list of a marked fine stream: the slots s < *n_dev
  * with src[s] == -1, in slot order.  Three steps:

@@ -1109,7 +1109,7 @@ __global__ __launch_bounds__(64 * kFusedWaves, 1) void field_fused_kernel(
   static_assert(Dm::SAMPLES == 64, "one MLP tile per wave");
   static_assert(weights_in_lds<D, H, L>(), "weights staged in LDS");
   constexpr int NT = Dm::NT, KT0 = Dm::KT0, Q = D / 4;
-  constexpr int ROW4 = PF::ROW4, RUN_STRIDE = PF::RUN_STRIDE, RUNS = PF::RUNS, SLOTS = PF::SLOTS;
+  constexpr int RUN_STRIDE = PF::RUN_STRIDE, RUNS = PF::RUNS;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   bf16x8* lw = reinterpret_cast<bf16x8*>(smem_raw);
   float* lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 1024);
@@ -1438,7 +1438,6 @@ template <int D, int H, int L>
 int launch_embed_bf16(FieldScene sc, FieldRays rays, const SampleRec* samples,
                       const int32_t* list, const int32_t* m_dev, int64_t m_max, void* workspace,
                       hipStream_t stream) {
-  using Dm = Dims<D, H, L>;
   // persistent waves: one resident round of blocks, each wave looping over
   // 64-slot tiles
   static const int64_t resident = resident_blocks(embed_bf16_kernel<D, H, L>);
@@ -1486,6 +1485,13 @@ int64_t bf16x3_workspace_bytes(int64_t m_max) {
   return 2 * x3_lo_offset<D, H, L>(m_max) * 16;
 }
 
+// Tiles each wave of the split-BF16 kernels takes before its block retires.
+// The grid holds as many blocks as that needs (not one resident round), so a
+// kernel of another stream -- a concurrent request's -- gets compute units
+// as blocks retire instead of waiting for the whole launch; a block's
+// weight staging is amortised over its waves' 32 tiles.
+constexpr int64_t kX3TilesPerWave = 32;
+
 template <int D, int H, int L>
 int launch_embed_bf16x3(FieldScene sc, FieldRays rays, const SampleRec* samples,
                         const int32_t* list, const int32_t* m_dev, int64_t m_max,
@@ -1493,7 +1499,8 @@ int launch_embed_bf16x3(FieldScene sc, FieldRays rays, const SampleRec* samples,
   if constexpr (x3_supported<D, H, L>()) {
     static const int64_t resident = resident_blocks(embed_bf16_kernel<D, H, L, 2, true>);
     const int64_t blocks =
-        std::max<int64_t>(1, std::min(ceil_div(ceil_div(m_max, 64), 4), resident));
+        std::max<int64_t>(std::min<int64_t>(resident, ceil_div(ceil_div(m_max, 64), 4)),
+                          ceil_div(ceil_div(m_max, 64), 4 * kX3TilesPerWave));
     hipLaunchKernelGGL((embed_bf16_kernel<D, H, L, 2, true>), dim3(unsigned(blocks)), dim3(256),
                        0, stream, sc, rays, samples, list, m_dev, m_max,
                        static_cast<bf16x8*>(workspace), x3_lo_offset<D, H, L>(m_max));
@@ -1512,7 +1519,9 @@ int launch_mlp_bf16x3(const int32_t* m_dev, int64_t m_max, const void* wfrag, co
     static const int64_t resident =
         resident_blocks(mlp_bf16x3_kernel<D, H, L>, 64 * kX3Waves, Dm::LDS_BYTES);
     const int64_t tiles = ceil_div(m_max, Dm::SAMPLES);
-    const int64_t blocks = std::max<int64_t>(1, std::min(ceil_div(tiles, kX3Waves), resident));
+    const int64_t blocks =
+        std::max<int64_t>(std::min<int64_t>(resident, ceil_div(tiles, kX3Waves)),
+                          ceil_div(tiles, kX3Waves * kX3TilesPerWave));
     hipLaunchKernelGGL((mlp_bf16x3_kernel<D, H, L>), dim3(unsigned(blocks)), dim3(64 * kX3Waves),
                        Dm::LDS_BYTES, stream, static_cast<const bf16x8*>(workspace),
                        x3_lo_offset<D, H, L>(m_max), m_dev, m_max,

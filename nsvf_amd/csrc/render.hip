@@ -468,23 +468,41 @@ constexpr int kSampleWaves = kBlock / 64;
 // bins and registers for 7 or 8 it spills and runs slower).
 constexpr int kCoarseBins = 128, kFineBins = 384;
 
+// Ray segments of a batch of several views rendered together: segment s
+// holds the frame rays [bounds[s], bounds[s + 1]).  The hierarchical pass
+// normalises a ray's bins over the columns of the longest coarse row of its
+// own view (nerf.py:64-79 pads every row to the longest one of the batch it
+// is rendered in), so each view of a batch gets the result it gets alone.
+constexpr int kMaxSegments = 16;
+
+__device__ __forceinline__ int segment_of(const int32_t* __restrict__ bounds, int n_seg,
+                                          int32_t r) {
+  int s = 0;
+  while (s + 1 < n_seg && r >= bounds[s + 1]) ++s;
+  return s;
+}
+
 // Coarse pass: one wavefront per hit ray n (ray r = hit_list[n]) writes the
 // ray's samples into its slot range [c_off[r], c_off[r] + bound[r]); count,
-// fine-pass bound and the longest row are recorded for the fine pass.
+// fine-pass bound and the longest row -- of the frame, and with segments
+// (n_seg > 0) of each segment -- are recorded for the fine pass.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void coarse_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const HitRec* __restrict__ hits, const int32_t* __restrict__ count,
     const float* __restrict__ tot, const int32_t* __restrict__ bound,
     const int32_t* __restrict__ c_off, int n_max, float step_size, int compat, float fine_steps,
     SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
-    int32_t* __restrict__ max_count) {
+    int32_t* __restrict__ max_count, const int32_t* __restrict__ seg_bounds, int n_seg,
+    int32_t* __restrict__ seg_max) {
   __shared__ float lds_c[kSampleWaves][kCoarseBins], lds_lo[kSampleWaves][kCoarseBins],
       lds_hi[kSampleWaves][kCoarseBins];
   __shared__ int32_t lds_id[kSampleWaves][kCoarseBins];
   __shared__ int block_max;
+  __shared__ int block_seg_max[kMaxSegments];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
   if (threadIdx.x == 0) block_max = 0;
+  if (threadIdx.x < kMaxSegments) block_seg_max[threadIdx.x] = 0;
   __syncthreads();
   int wave_max = 0;  // longest row of this wave's rays (one atomic per block at the end)
   for (int64_t n = int64_t(blockIdx.x) * kSampleWaves + wave; n < N;
@@ -503,12 +521,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     if (lane == 0) {
       c_cnt[r] = m;
       f_bound[r] = int(ceilf(fine_steps)) + m + 2;
+      if (n_seg > 0 && m > 0) atomicMax(&block_seg_max[segment_of(seg_bounds, n_seg, r)], m);
     }
     wave_max = m > wave_max ? m : wave_max;
   }
   if (lane == 0) atomicMax(&block_max, wave_max);
   __syncthreads();
   if (threadIdx.x == 0 && block_max > 0) atomicMax(max_count, block_max);
+  if (int(threadIdx.x) < n_seg && block_seg_max[threadIdx.x] > 0)
+    atomicMax(&seg_max[threadIdx.x], block_seg_max[threadIdx.x]);
 }
 
 // Column sum of (p + 1e-5) over the P bin columns of a fine-pass ray
@@ -574,7 +595,9 @@ struct FieldReuse {
 };
 
 // Fine pass: one wavefront per hit ray; bins are the ray's coarse samples and
-// the *c_max columns of the longest coarse row (nerf.py:64-79).  With src
+// the *c_max columns of the longest coarse row (nerf.py:64-79) -- with
+// segments (n_seg > 0), the seg_max[s] columns of the longest row of the
+// ray's segment s.  With src
 // (optional), the slots are marked for field reuse as FieldReuse describes.
 // kComposite: the wave first composites the ray's coarse row from c_field
 // (composite_pass, as the coarse composite does) and stores the weights to
@@ -588,16 +611,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     const float4* __restrict__ c_field, float* __restrict__ c_prob,
     const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
-    int32_t* __restrict__ f_cnt, int16_t* __restrict__ src) {
+    int32_t* __restrict__ f_cnt, int16_t* __restrict__ src, const int32_t* __restrict__ seg_bounds,
+    int n_seg, const int32_t* __restrict__ seg_max) {
   __shared__ float lds_c[kSampleWaves][kFineBins], lds_lo[kSampleWaves][kFineBins],
       lds_hi[kSampleWaves][kFineBins];
   __shared__ int32_t lds_id[kSampleWaves][kFineBins];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t N = *n_hit_dev;
-  const int P = *c_max_dev;  // bin columns = longest coarse row
+  const int P_frame = *c_max_dev;  // bin columns = longest coarse row
   for (int64_t n = int64_t(blockIdx.x) * kSampleWaves + wave; n < N;
        n += int64_t(gridDim.x) * kSampleWaves) {
     const int32_t r = hit_list[n];
+    const int P = n_seg > 0 ? seg_max[segment_of(seg_bounds, n_seg, r)] : P_frame;
     const int b = f_bound[r];
     SampleRec* dst = out + f_off[r];
     int m = 0;
@@ -1125,15 +1150,19 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
                               const int32_t* bound, const int32_t* c_off, int n_max,
                               float step_size, int compat, float fine_steps, void* samples,
                               int32_t* c_cnt, int32_t* f_bound, int32_t* max_count,
+                              const int32_t* seg_bounds, int n_seg, int32_t* seg_max,
                               void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
+  if (n_seg < 0 || n_seg > kMaxSegments || (n_seg > 0 && (!seg_bounds || !seg_max)))
+    return set_error(NSVF_EINVAL, "render_sample_coarse: %d segments (at most %d, with bounds "
+                     "and maxima)", n_seg, kMaxSegments);
   hipLaunchKernelGGL(coarse_sample_kernel,
                      dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
                      dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev,
                      static_cast<const HitRec*>(hits), count, tot, bound, c_off, n_max, step_size,
                      compat, fine_steps, static_cast<SampleRec*>(samples), c_cnt, f_bound,
-                     max_count);
+                     max_count, seg_bounds, n_seg, seg_max);
   return check_launch("render_sample_coarse");
 }
 
@@ -1142,21 +1171,26 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev, i
                             const void* coarse, const void* c_field, float* c_prob,
                             const int32_t* f_bound, const int32_t* f_off, float fine_steps,
                             int compat, void* samples, int32_t* f_cnt, int16_t* src,
+                            const int32_t* seg_bounds, int n_seg, const int32_t* seg_max,
                             void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
+  if (n_seg < 0 || n_seg > kMaxSegments || (n_seg > 0 && (!seg_bounds || !seg_max)))
+    return set_error(NSVF_EINVAL, "render_sample_fine: %d segments (at most %d, with bounds "
+                     "and maxima)", n_seg, kMaxSegments);
   const dim3 grid(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192)));
   if (c_field)
     hipLaunchKernelGGL(fine_sample_kernel<true>, grid, dim3(kBlock), 0, as_stream(stream),
                        hit_list, n_hit_dev, c_off, c_cnt, c_max,
                        static_cast<const SampleRec*>(coarse), static_cast<const float4*>(c_field),
                        c_prob, f_bound, f_off, fine_steps, compat,
-                       static_cast<SampleRec*>(samples), f_cnt, src);
+                       static_cast<SampleRec*>(samples), f_cnt, src, seg_bounds, n_seg, seg_max);
   else
     hipLaunchKernelGGL(fine_sample_kernel<false>, grid, dim3(kBlock), 0, as_stream(stream),
                        hit_list, n_hit_dev, c_off, c_cnt, c_max,
                        static_cast<const SampleRec*>(coarse), nullptr, c_prob, f_bound, f_off,
-                       fine_steps, compat, static_cast<SampleRec*>(samples), f_cnt, src);
+                       fine_steps, compat, static_cast<SampleRec*>(samples), f_cnt, src,
+                       seg_bounds, n_seg, seg_max);
   return check_launch("render_sample_fine");
 }
 

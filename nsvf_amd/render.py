@@ -216,6 +216,7 @@ class Renderer:
         self.compat = int(bool(compat))
         self._ws = {}
         self._host = None      # pinned [n_hit, slots] of the view in flight
+        self._seg_host = None  # pinned segment bounds of a batch of views
         self._pending = None   # the view between begin() and finish()
         self.last = None
         self.trace = None   # optional list of (stage, event) marks
@@ -339,7 +340,7 @@ class Renderer:
                 "scan_ws": (int(lib().nsvf_scan_workspace_ints(R)), i32),
                 "ray_rgb": (R * 3, f32), "ray_depth": (R, f32), "ray_missed": (R, f32),
                 "c_cnt": (R, i32), "f_off": (R, i32), "f_cnt": (R, i32),
-                "out": (R * 5, f32)}
+                "out": (R * 5, f32), "seg_bounds": (17, i32), "seg_max": (16, i32)}
 
     def _shading_sizes(self, sc, R, n_hit, slots_c, slots_f):
         i32, f32 = torch.int32, torch.float32
@@ -374,19 +375,29 @@ class Renderer:
     # ---- phase 1: intersection, compaction, slot offsets
 
     @torch.no_grad()
-    def begin(self, sc: DeviceScene, ray_start, ray_dir, width=0):
+    def begin(self, sc: DeviceScene, ray_start, ray_dir, width=0, segments=None):
         """Enqueues the geometry stages of a view on the current stream:
         intersection, hit compaction and the coarse slot offsets
         (encoder.py:498-535), then an asynchronous copy of the hit-ray count
         and slot total to pinned host memory.  geometry_ready() tells when
         finish() can run without waiting.  width: 0, or the pixel width of
         the image(s) whose rows the rays are (a traversal-order hint; the
-        results do not depend on it)."""
+        results do not depend on it).  segments: for several views rendered
+        as one batch, their ray counts in order (at most 16; the rays are
+        their concatenation); each view's hierarchical pass then normalises
+        over its own longest coarse row, so every view gets the bits it
+        gets alone (with the sampler's evident-intent mode)."""
         ray_start = ray_start.contiguous()
         ray_dir = ray_dir.contiguous()
         R = ray_start.shape[0]
         if width and R % width:
             raise ValueError(f"{R} rays are not rows of width {width}")
+        if segments is not None:
+            segments = [int(n) for n in segments]
+            if sum(segments) != R or len(segments) > 16 or min(segments) < 0:
+                raise ValueError(f"segments {segments} do not split {R} rays into <= 16 views")
+            if len(segments) == 1:
+                segments = None
 
         def intersect(sc, P, hits, count, tot, flag, c_bound, st):
             check(lib().nsvf_render_hits(_p(ray_start), _p(ray_dir), R, int(width),
@@ -394,7 +405,7 @@ class Renderer:
                                          sc.step_size,
                                          _p(hits), _p(count), _p(tot), _p(flag), _p(c_bound), st),
                   "render_hits")
-        self._begin(sc, R, ray_start, ray_dir, intersect)
+        self._begin(sc, R, ray_start, ray_dir, intersect, segments)
 
     @torch.no_grad()
     def begin_camera(self, sc: DeviceScene, camera):
@@ -414,13 +425,22 @@ class Renderer:
                                                 _p(c_bound), st), "render_hits_camera")
         self._begin(sc, R, rs, rd, intersect)
 
-    def _begin(self, sc, R, ray_start, ray_dir, intersect):
+    def _begin(self, sc, R, ray_start, ray_dir, intersect, segments=None):
         L = lib()
         dev = self.device
         st = _stream(dev)
         P = sc.max_hits
         for name, (n, dtype) in self._geometry_sizes(R, P).items():
             self._buf(name, n, dtype)
+        if segments is not None:
+            # segment bounds to the device through pinned memory (the previous
+            # view's copy has completed: a renderer runs one view at a time)
+            if self._seg_host is None:
+                self._seg_host = torch.zeros(17, dtype=torch.int32, pin_memory=True)
+            self._seg_host[: len(segments) + 1] = torch.from_numpy(
+                np.concatenate([[0], np.cumsum(segments)]).astype(np.int32))
+            self._buf("seg_bounds", 17, torch.int32).copy_(self._seg_host, non_blocking=True)
+            self._buf("seg_max", 16, torch.int32).zero_()
         self._mark("start")
         hits = self._buf("hits", R * P * 4, torch.int32)
         count = self._buf("count", R, torch.int32)
@@ -445,7 +465,8 @@ class Renderer:
         self._host.copy_(scal[:2], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record()
-        self._pending = {"sc": sc, "rs": ray_start, "rd": ray_dir, "R": R, "event": ev}
+        self._pending = {"sc": sc, "rs": ray_start, "rd": ray_dir, "R": R, "event": ev,
+                         "segments": segments}
         self._mark("compact")
 
     def geometry_ready(self):
@@ -456,6 +477,13 @@ class Renderer:
         n_hit, slots_c = int(self._host[0]), int(self._host[1])
         slots_f = slots_c + n_hit * (int(math.ceil(self.fine_steps)) + 2)   # fine-slot bound
         return n_hit, slots_c, slots_f
+
+    def _segment_args(self):
+        segs = self._pending.get("segments")
+        if not segs:
+            return (None, 0, None)
+        return (_p(self._buf("seg_bounds", 17, torch.int32)), len(segs),
+                _p(self._buf("seg_max", 16, torch.int32)))
 
     def _sample_coarse(self):
         """Coarse sample stream of the view begun last (its geometry done)."""
@@ -475,7 +503,7 @@ class Renderer:
             _p(self._buf("tot", R, torch.float32)), _p(self._buf("c_bound", R, torch.int32)),
             _p(self._buf("c_off", R, torch.int32)), P, sc.step_size, self.compat,
             self.fine_steps, _p(cs), _p(c_cnt), _p(self._buf("f_bound", R, torch.int32)),
-            _p(scal[2:3]), _stream(self.device)), "sample_coarse")
+            _p(scal[2:3]), *self._segment_args(), _stream(self.device)), "sample_coarse")
         self._mark("coarse_sample")
         return CoarseStream(R, n_hit, slots_c, self._buf("hits", R * P * 4, torch.int32),
                             self._buf("count", R, torch.int32), self._buf("tot", R, torch.float32),
@@ -542,7 +570,8 @@ class Renderer:
             check(L.nsvf_render_sample_fine(_p(hit_list), _p(scal[0:1]), R, _p(c_off), _p(c_cnt),
                                             _p(scal[2:3]), _p(cs), _p(c_field) if fold else None,
                                             _p(c_prob), _p(f_bound), _p(f_off), fs, self.compat,
-                                            _p(fsm), _p(f_cnt), _p(src), st), "sample_fine")
+                                            _p(fsm), _p(f_cnt), _p(src), *self._segment_args(),
+                                            st), "sample_fine")
             if fold:
                 self._track(sc, hit_list, scal, n_hit, c_off, c_cnt, cs, c_prob, st)
             if reuse:

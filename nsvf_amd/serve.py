@@ -43,6 +43,11 @@ class RequestTooLarge(RuntimeError):
     """The request's workspace does not fit the budget even when it runs alone."""
 
 
+class SlotOutOfMemory(RuntimeError):
+    """A slot could not allocate its workspace although the budget admitted
+    it (the allocator's cached blocks did not fit the request's shape)."""
+
+
 class HbmBudget:
     """Byte accounting of device memory: resident scenes plus the workspace
     of the requests in flight may not exceed `total`."""
@@ -74,12 +79,16 @@ class Request:
     ray_dir: Any
     arrival: float
     state: str = "queued"     # queued, geometry, shading, done, rejected
-    admitted: Optional[float] = None
+    admitted: Optional[float] = None    # its geometry started (first time)
+    shaded: Optional[float] = None      # its shading started
     finished: Optional[float] = None
     result: Any = None
     error: Optional[str] = None
     preemptions: int = 0
     width: int = 0            # image width of the rays (a traversal-order hint), 0 = none
+    batch: int = 1            # requests in the job it ran in
+    solo: bool = False        # run alone (its batch did not fit)
+    oom: int = 0              # allocation failures of its jobs
 
     @property
     def rays(self) -> int:
@@ -90,39 +99,75 @@ class Request:
         return None if self.finished is None else self.finished - self.arrival
 
 
-class Slot:
-    """What the scheduler needs from a slot (the GPU one is RenderSlot)."""
+class Job:
+    """Requests of one scene rendered as one batch of rays (their rays
+    concatenated in order) in one slot.  Every kernel of the path computes
+    each ray from its own inputs (the sampler in its evident-intent mode),
+    and the hierarchical pass normalises each request's rays over its own
+    longest coarse row (the batch's segments), so each request's pixels are
+    the same bits as when it runs alone."""
 
-    def geometry_bytes(self, req: Request) -> int: ...
-    def start_geometry(self, req: Request): ...
+    def __init__(self, requests):
+        self.requests = list(requests)
+        self.scene = self.requests[0].scene
+        self.rid = self.requests[0].rid
+        widths = {r.width for r in self.requests}
+        # one common image width keeps the traversal's pixel tiles (the
+        # concatenation is an image of the same width); mixed widths run the
+        # traversal over row strips, with the same results
+        self.width = widths.pop() if len(widths) == 1 else 0
+
+    @property
+    def rays(self) -> int:
+        return sum(r.rays for r in self.requests)
+
+
+class Slot:
+    """What the scheduler needs from a slot (the GPU one is RenderSlot).
+    Byte counts are what the slot would newly allocate (its workspace is
+    kept between jobs, so a job that fits what it holds needs 0)."""
+
+    def geometry_bytes(self, job: Job) -> int: ...
+    def start_geometry(self, job: Job): ...
     def geometry_ready(self) -> bool: ...
     def shading_bytes(self) -> int: ...
     def start_shading(self): ...
     def shading_done(self) -> bool: ...
-    def result(self) -> Any: ...
-    def reset(self): ...
+    def result(self) -> list: ...          # one result per request of the job
+    def reset(self): ...                   # drop the job, keep the workspace
+    def workspace_bytes(self) -> int: ...  # bytes the slot holds now
+    def release_workspace(self): ...
 
 
 @dataclass
 class _SlotState:
     slot: Slot
-    req: Optional[Request] = None
+    job: Optional[Job] = None
     phase: str = "idle"       # idle, geometry, shading
-    held: int = 0             # bytes reserved for the request in this slot
+    held: int = 0             # bytes of the budget the slot holds
 
 
 class Scheduler:
     """Continuous batching over a fixed set of slots under an HbmBudget.
 
     poll() advances every slot without blocking: finished shading completes
-    its request and frees the slot and its bytes; finished geometry starts
-    shading when its bytes fit; free slots take queued requests (first come,
-    first served) when their geometry bytes fit."""
+    its job's requests and frees the slot; finished geometry starts shading
+    when its bytes fit; free slots take jobs -- the oldest queued request
+    plus up to max_batch - 1 later requests of the same scene among the
+    next `window` queued, within max_batch_rays and the budget.  A slot
+    keeps its workspace between jobs (no allocation once it has seen the
+    largest job); idle slots give theirs back when a job needs the memory."""
 
-    def __init__(self, slots, budget: HbmBudget, clock=time.perf_counter):
+    def __init__(self, slots, budget: HbmBudget, clock=time.perf_counter, max_batch=1,
+                 max_batch_rays=None, window=16, on_release=None, max_oom_retries=2):
         self.slots = [_SlotState(s) for s in slots]
         self.budget = budget
         self.clock = clock
+        self.on_release = on_release    # called after idle slots gave memory back
+        self.max_oom_retries = int(max_oom_retries)
+        self.max_batch = int(max_batch)
+        self.max_batch_rays = max_batch_rays
+        self.window = int(window)
         self.queue = collections.deque()
         self.done = []
         self._ids = itertools.count()
@@ -140,31 +185,115 @@ class Scheduler:
     def idle(self):
         return not self.queue and self.in_flight == 0
 
-    def _finish(self, st: _SlotState, state, error=None):
-        req = st.req
-        req.state = state
-        req.error = error
-        req.finished = self.clock()
-        if state == "done":
-            req.result = st.slot.result()
-        self.budget.release(st.held)
-        st.slot.reset()
-        st.req, st.phase, st.held = None, "idle", 0
-        self.done.append(req)
+    def held_bytes(self):
+        return sum(s.held for s in self.slots)
 
-    def _preempt(self, st: _SlotState):
-        req = st.req
-        req.state = "queued"
-        req.preemptions += 1
-        self.budget.release(st.held)
+    def _true_up(self, st: _SlotState):
+        """The budget holds what the slot actually holds."""
+        now = int(st.slot.workspace_bytes())
+        if now < st.held:
+            self.budget.release(st.held - now)
+        elif now > st.held:
+            self.budget.reserve(now - st.held)
+        st.held = now
+
+    def _free_slot(self, st: _SlotState):
         st.slot.reset()
-        st.req, st.phase, st.held = None, "idle", 0
-        self.queue.appendleft(req)
+        st.job, st.phase = None, "idle"
+        self._true_up(st)
+
+    def _finish(self, st: _SlotState, state, error=None):
+        job = st.job
+        now = self.clock()
+        results = st.slot.result() if state == "done" else [None] * len(job.requests)
+        for req, res in zip(job.requests, results):
+            req.state, req.error, req.finished = state, error, now
+            req.result = res
+            req.batch = len(job.requests)
+            self.done.append(req)
+        self._free_slot(st)
+
+    def _requeue(self, st: _SlotState, solo=False):
+        """Puts the slot's job back at the head of the queue (in order); the
+        slot gives its workspace back (memory is short when this happens)."""
+        for req in reversed(st.job.requests):
+            req.state = "queued"
+            req.preemptions += 1
+            req.solo = req.solo or solo
+            self.queue.appendleft(req)
+        self._free_slot(st)
+        st.slot.release_workspace()
+        self._true_up(st)
+
+    def release_idle(self, keep=None):
+        """Idle slots (other than `keep`) give their workspace back."""
+        freed = False
+        for st in self.slots:
+            if st is not keep and st.phase == "idle" and st.held > 0:
+                st.slot.release_workspace()
+                self._true_up(st)
+                freed = True
+        if freed and self.on_release is not None:
+            self.on_release()
+        return freed
+
+    def _out_of_memory(self, st: _SlotState, err):
+        """A phase failed to allocate: memory goes back to the device and
+        the job is retried (its requests one by one if it was a batch);
+        a single request that keeps failing with nothing else in flight is
+        rejected."""
+        retries = max(r.oom for r in st.job.requests) + 1
+        for r in st.job.requests:
+            r.oom = retries
+        others = any(s.phase != "idle" for s in self.slots if s is not st)
+        if len(st.job.requests) == 1 and retries > self.max_oom_retries and not others:
+            self._finish(st, "rejected", f"out of device memory: {err}")
+            self.release_idle()
+            return
+        self._requeue(st, solo=len(st.job.requests) > 1)
+        self.release_idle()
+        if self.on_release is not None:
+            self.on_release()
+
+    def _take_job(self, st: _SlotState):
+        """A job for idle slot st from the queue head, or None if the head
+        does not fit now."""
+        head = self.queue[0]
+        job = Job([head])
+        need = st.slot.geometry_bytes(job)
+        if not self.budget.fits(need) and self.release_idle(keep=st):
+            need = st.slot.geometry_bytes(job)
+        if not self.budget.fits(need) and st.held > 0:
+            # what this slot keeps may be the wrong shape for the job
+            st.slot.release_workspace()
+            self._true_up(st)
+            need = st.slot.geometry_bytes(job)
+        if not self.budget.fits(need):
+            return None, need
+        taken = [0]
+        if not head.solo and self.max_batch > 1:
+            for i in range(1, min(len(self.queue), self.window)):
+                if len(taken) >= self.max_batch:
+                    break
+                r = self.queue[i]
+                if r.scene != head.scene or r.solo:
+                    continue
+                if self.max_batch_rays is not None and job.rays + r.rays > self.max_batch_rays:
+                    continue
+                trial = Job(job.requests + [r])
+                n = st.slot.geometry_bytes(trial)
+                if not self.budget.fits(n):
+                    break
+                job, need = trial, n
+                taken.append(i)
+        for i in reversed(taken):
+            del self.queue[i]
+        return job, need
 
     def poll(self) -> bool:
         """One scheduling pass; True if anything changed."""
         changed = False
-        # 1. completions free slots and bytes
+        # 1. completions free slots
         for st in self.slots:
             if st.phase == "shading" and st.slot.shading_done():
                 self._finish(st, "done")
@@ -173,49 +302,71 @@ class Scheduler:
         for st in self.slots:
             if st.phase == "geometry" and st.slot.geometry_ready():
                 need = st.slot.shading_bytes()
+                if not self.budget.fits(need):
+                    self.release_idle()
                 if not self.budget.fits(need) and not any(s.phase == "shading" for s in self.slots):
                     # nothing in flight will free memory: preempt the other
-                    # requests that hold only geometry memory
+                    # jobs that hold only geometry memory
                     for other in self.slots:
                         if other is not st and other.phase == "geometry":
-                            self._preempt(other)
+                            self._requeue(other)
                             changed = True
+                    self.release_idle()
                     if not self.budget.fits(need):
-                        self._finish(st, "rejected",
-                                     f"shading needs {need} bytes; {self.budget.total - self.budget.used} "
-                                     f"free with nothing else in flight")
+                        if len(st.job.requests) > 1:
+                            self._requeue(st, solo=True)    # retry its requests one by one
+                        else:
+                            self._finish(st, "rejected",
+                                         f"shading needs {need} bytes; "
+                                         f"{self.budget.total - self.budget.used} free with "
+                                         f"nothing else in flight")
                         changed = True
                         continue
                 if self.budget.fits(need):
                     self.budget.reserve(need)
                     st.held += need
-                    st.slot.start_shading()
+                    now = self.clock()
+                    for req in st.job.requests:
+                        req.state, req.shaded = "shading", now
                     st.phase = "shading"
+                    try:
+                        st.slot.start_shading()
+                    except SlotOutOfMemory as e:
+                        self._out_of_memory(st, e)
+                        changed = True
+                        continue
+                    self._true_up(st)
                     changed = True
-        # 3. free slots take queued requests
-        for st in self.slots:
+        # 3. free slots take jobs (the slot holding the most workspace first)
+        idle = sorted((s for s in self.slots if s.phase == "idle"), key=lambda s: -s.held)
+        for st in idle:
             if not self.queue:
                 break
-            if st.phase != "idle":
-                continue
-            req = self.queue[0]
-            need = st.slot.geometry_bytes(req)
-            if not self.budget.fits(need):
+            job, need = self._take_job(st)
+            if job is None:
                 if self.in_flight == 0:
-                    self.queue.popleft()
-                    st.req = req
+                    req = self.queue.popleft()
+                    st.job = Job([req])
                     self._finish(st, "rejected",
-                                 f"geometry needs {need} bytes; {self.budget.total - self.budget.used} free")
+                                 f"geometry needs {need} bytes; "
+                                 f"{self.budget.total - self.budget.used} free")
                     changed = True
                 break   # first come, first served: wait for memory
-            self.queue.popleft()
             self.budget.reserve(need)
-            st.req, st.phase, st.held = req, "geometry", need
-            req.state = "geometry"
-            if req.admitted is None:
-                req.admitted = self.clock()
-            st.slot.start_geometry(req)
+            st.held += need
+            st.job, st.phase = job, "geometry"
+            now = self.clock()
+            for req in job.requests:
+                req.state = "geometry"
+                if req.admitted is None:
+                    req.admitted = now
             changed = True
+            try:
+                st.slot.start_geometry(job)
+            except SlotOutOfMemory as e:
+                self._out_of_memory(st, e)
+                break
+            self._true_up(st)
         return changed
 
     def drain(self, sleep=0.0):
@@ -229,7 +380,7 @@ class Scheduler:
 
 
 class RenderSlot(Slot):
-    """A Renderer on its own stream; its workspace lives for one request."""
+    """A Renderer on its own stream; its workspace is kept between jobs."""
 
     def __init__(self, server, index):
         import torch
@@ -243,18 +394,34 @@ class RenderSlot(Slot):
         self.index = index
         self._done = None
         self._out = None
+        self._job = None
+        self._cat_bytes = 0
 
-    def geometry_bytes(self, req):
-        R = req.rays
-        return self.renderer.geometry_bytes(self.server.scenes[req.scene], R)
+    def geometry_bytes(self, job):
+        sc = self.server.scenes[job.scene]
+        R = job.rays
+        cat = R * 24 if len(job.requests) > 1 else 0     # the batch's concatenated rays
+        return self.renderer.geometry_bytes(sc, R) + cat
 
-    def start_geometry(self, req):
+    def start_geometry(self, job):
         import torch
-        sc = self.server.scenes[req.scene]
-        # the request's rays were written on the submitting stream
+        sc = self.server.scenes[job.scene]
+        self._job = job
+        # the requests' rays were written on the submitting stream
         self.stream.wait_stream(torch.cuda.current_stream(self.server.device))
-        with torch.cuda.stream(self.stream):
-            self.renderer.begin(sc, req.ray_start, req.ray_dir, req.width)
+        try:
+            with torch.cuda.stream(self.stream):
+                segments = None
+                if len(job.requests) == 1:
+                    rs, rd = job.requests[0].ray_start, job.requests[0].ray_dir
+                else:
+                    rs = torch.cat([r.ray_start for r in job.requests])
+                    rd = torch.cat([r.ray_dir for r in job.requests])
+                    self._cat_bytes = job.rays * 24
+                    segments = [r.rays for r in job.requests]
+                self.renderer.begin(sc, rs, rd, job.width, segments=segments)
+        except torch.OutOfMemoryError as e:
+            raise SlotOutOfMemory(str(e).splitlines()[0]) from None
 
     def geometry_ready(self):
         return self.renderer.geometry_ready()
@@ -264,32 +431,48 @@ class RenderSlot(Slot):
 
     def start_shading(self):
         import torch
-        with torch.cuda.stream(self.stream):
-            self._out = self.renderer.finish()
-            self._done = torch.cuda.Event()
-            self._done.record(self.stream)
+        try:
+            with torch.cuda.stream(self.stream):
+                self._out = self.renderer.finish()
+                self._done = torch.cuda.Event()
+                self._done.record(self.stream)
+        except torch.OutOfMemoryError as e:
+            raise SlotOutOfMemory(str(e).splitlines()[0]) from None
 
     def shading_done(self):
         return self._done is not None and self._done.query()
 
     def result(self):
-        return self._out
+        rgb, depth, missed = self._out
+        out, a = [], 0
+        for req in self._job.requests:
+            b = a + req.rays
+            out.append((rgb[a:b], depth[a:b], missed[a:b]))
+            a = b
+        return out
 
     def reset(self):
         pd = self.renderer._pending
         if pd is not None:          # preempted: let its geometry kernels end first
             pd["event"].synchronize()
-        self.renderer.release_workspace()
         self.renderer._pending = None
         self._done = None
         self._out = None
+        self._job = None
+        self._cat_bytes = 0
+
+    def workspace_bytes(self):
+        return self.renderer.workspace_bytes() + self._cat_bytes
+
+    def release_workspace(self):
+        self.renderer.release_workspace()
 
 
 class RenderServer:
     """Resident scenes + a Scheduler over `slots` RenderSlots.
 
     budget_bytes: device bytes for scenes and request workspaces (default:
-    90% of the free device memory at construction).  precision: the field's
+    85% of the free device memory at construction).  precision: the field's
     arithmetic (default split BF16, whose per-pixel error against FP64 is
     about 1e-5).  compat: the sampler's
     reference mode (True) or its evident intent (False, the default here): in
@@ -298,7 +481,7 @@ class RenderServer:
     depend on which other pixels share its request."""
 
     def __init__(self, device, slots=4, budget_bytes=None, precision="bf16x3", hierarchical=True,
-                 fine_steps=64.0, compat=False):
+                 fine_steps=64.0, compat=False, max_batch=8, max_batch_rays=640_000):
         import torch
         self.device = torch.device(device)
         self.precision = precision
@@ -307,11 +490,23 @@ class RenderServer:
         self.compat = compat
         if budget_bytes is None:
             free, _ = torch.cuda.mem_get_info(self.device)
-            budget_bytes = int(free * 0.9)
+            budget_bytes = int(free * 0.85)
         self.budget = HbmBudget(budget_bytes)
         self.scenes = {}
         self._scene_bytes = {}
-        self.scheduler = Scheduler([RenderSlot(self, i) for i in range(slots)], self.budget)
+        if compat and max_batch > 1:
+            raise ValueError("batching needs the sampler's evident-intent mode (compat=False): "
+                             "in the reference's mode a ray reads its neighbours")
+        self.scheduler = Scheduler([RenderSlot(self, i) for i in range(slots)], self.budget,
+                                   max_batch=max_batch, max_batch_rays=max_batch_rays,
+                                   on_release=self._return_cached)
+
+    def _return_cached(self):
+        """Workspace the slots gave back goes back to the device (it would
+        otherwise stay in the allocator's cache in blocks of the old shapes)."""
+        import torch
+        torch.cuda.synchronize(self.device)
+        torch.cuda.empty_cache()
 
     def load_scene(self, name, scene):
         """Uploads a VoxelScene (or takes a DeviceScene) and keeps it resident."""
@@ -326,7 +521,7 @@ class RenderServer:
         return dsc
 
     def unload_scene(self, name):
-        if any(st.req is not None and st.req.scene == name for st in self.scheduler.slots):
+        if any(st.job is not None and st.job.scene == name for st in self.scheduler.slots):
             raise RuntimeError(f"scene {name} has requests in flight")
         del self.scenes[name]
         self.budget.release(self._scene_bytes.pop(name))
@@ -340,6 +535,10 @@ class RenderServer:
 
     def poll(self):
         return self.scheduler.poll()
+
+    def release_idle(self):
+        """Idle slots give their kept workspace back to the allocator."""
+        return self.scheduler.release_idle()
 
     def drain(self):
         self.scheduler.drain()
@@ -365,13 +564,25 @@ class RenderServer:
 
 def latency_summary(reqs):
     """rays/s over the span of the run, p50/p99 latency (ms) of the done
-    requests, and counts."""
+    requests and of its phases -- queued (arrival to its geometry start),
+    geometry (to its shading start: intersection, compaction and the wait
+    for shading memory) and shading (to completion) -- and counts."""
     done = [r for r in reqs if r.state == "done"]
     lat = np.array([r.latency for r in done]) * 1e3 if done else np.zeros(1)
     span = (max(r.finished for r in done) - min(r.arrival for r in done)) if done else 0.0
     rays = sum(r.rays for r in done)
+
+    def pct(vals):
+        v = np.array(vals) * 1e3 if vals else np.zeros(1)
+        return {"p50": round(float(np.percentile(v, 50)), 3),
+                "p99": round(float(np.percentile(v, 99)), 3)}
+
     return {"requests": len(reqs), "done": len(done),
             "rejected": sum(r.state == "rejected" for r in reqs),
             "preemptions": sum(r.preemptions for r in reqs),
             "rays_per_sec": rays / span if span > 0 else None,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "phases_ms": {"queued": pct([r.admitted - r.arrival for r in done]),
+                          "geometry": pct([r.shaded - r.admitted for r in done]),
+                          "shading": pct([r.finished - r.shaded for r in done])},
+            "mean_batch": round(float(np.mean([r.batch for r in done])), 2) if done else 0.0}

@@ -49,12 +49,19 @@ def check_same(srv, reqs, items):
 
 
 def test_mixed_batch_matches_alone(server):
+    """Submitted together, requests of one scene run as batches of
+    concatenated rays (mixed resolutions: the traversal over row strips);
+    each request's pixels equal its render alone."""
     items = requests(9, 1, server.device)
     reqs = [server.submit(*it) for it in items]
     server.drain()
     torch.cuda.synchronize()
     check_same(server, reqs, items)
-    assert server.budget.used == sum(server._scene_bytes.values())
+    assert max(r.batch for r in reqs) > 1
+    scenes = sum(server._scene_bytes.values())
+    assert server.budget.used == scenes + server.scheduler.held_bytes()
+    server.release_idle()
+    assert server.budget.used == scenes
 
 
 def test_staggered_arrivals_match_alone(server):
@@ -96,3 +103,30 @@ def test_tight_budget_queues_and_matches_alone(device):
     torch.cuda.synchronize()
     assert shading == 1, shading
     check_same(srv, reqs, items)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_segmented_batch_equals_views_alone(device, precision):
+    """Four views of one scene rendered as one batch of concatenated rays
+    with their segments: each view's image equals its render alone bit for
+    bit (each segment's fine pass normalises over its own longest coarse
+    row, as the view does alone)."""
+    sc = S.shell_scene(res=32)
+    from nsvf_amd.render import DeviceScene
+    dsc = DeviceScene(sc, device)
+    items = requests(4, 3, device)
+    rs = torch.cat([it[1] for it in items])
+    rd = torch.cat([it[2] for it in items])
+    segs = [it[1].shape[0] for it in items]
+    r = Renderer(device, precision=precision, hierarchical=True, compat=False)
+    r.begin(dsc, rs, rd, segments=segs)
+    batch = [t.clone() for t in r.finish()]
+    torch.cuda.synchronize()
+    a = 0
+    for (scene, rs_i, rd_i), n in zip(items, segs):
+        alone_r = Renderer(device, precision=precision, hierarchical=True, compat=False)
+        ref = alone_r.render(dsc, rs_i, rd_i)
+        torch.cuda.synchronize()
+        for got, want in zip(batch, ref):
+            assert torch.equal(got[a:a + n], want)
+        a += n
