@@ -297,10 +297,19 @@ int nsvf_render_compact(const int32_t* hit_flag, const int32_t* rank,
 /* Sample streams: records {int32 voxel id (-1 = unused slot), float depth,
  * float length, int32 ray id}; ray r owns slots [off[r], off[r] + bound[r])
  * (off = exclusive scan of the bounds), the first cnt[r] hold its samples.
+ *
+ * Noise (both samplers; fairnr/clib/__init__.py:250-256): step i of a ray
+ * sits at cdf (i + u_i) / steps.  noise_seed < 0: u_i = 0.5 (evaluation, or
+ * --deterministic-step).  noise_seed in [0, 2^32): training-mode noise,
+ * u_i = clamp((pcg(i + pcg(r + pcg(seed))) >> 8) / 2^24, 0.001, 0.999) for
+ * frame ray r, with pcg the PCG hash of Jarzynski and Olano (JCGT 2020):
+ * counter-based, so any implementation can reproduce the draws, where the
+ * reference draws torch's uniform_() into a [rays, max_steps] tensor.
 
- * Coarse inverse-CDF sampling of the hit rays (deterministic noise 0.5,
- * steps = sum of segment lengths / step_size, probs = length / sum).  Writes
- * c_cnt[r], the fine-pass bound f_bound[r] = ceil(fine_steps) + c_cnt[r] + 2
+ * Coarse inverse-CDF sampling of the hit rays (fairnr/modules/encoder.py:
+ * 538-547; steps = sum of segment lengths / step_size, probs = length /
+ * sum).  Writes c_cnt[r], the fine-pass bound f_bound[r] =
+ * ceil(fine_steps) + c_cnt[r] + 2
  * and *max_count = the longest row.  *n_hit_dev (device) is the number of
  * hit rays, n_hit_max >= it.  Segments (n_seg in 1..16, else 0 and NULLs):
  * several views rendered as one batch, segment s holding the frame rays
@@ -314,12 +323,13 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
                               float fine_steps, void* samples, int32_t* c_cnt,
                               int32_t* f_bound, int32_t* max_count,
                               const int32_t* seg_bounds, int n_seg,
-                              int32_t* seg_max, void* stream);
+                              int32_t* seg_max, int64_t noise_seed,
+                              void* stream);
 
 /* Fine (hierarchical) inverse-CDF sampling: bins are each ray's coarse
  * samples [depth -+ length/2] with probs (p + 1e-5) / column sum over the
  * *c_max (device) columns; `fine_steps` CDF steps (fairnr/models/nerf.py:
- * 64-79).  Writes the fine stream and f_cnt[r].
+ * 64-79); noise as above.  Writes the fine stream and f_cnt[r].
  * Field reuse (optional src, int16, fine-slot aligned): the fine pass
  * evaluates the coarse pass's field (nerf.py:58), whose value depends only
  * on ray, voxel and depth, and a coarse bin that no fine step splits comes
@@ -347,7 +357,8 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev,
                             const int32_t* f_off, float fine_steps,
                             int compat, void* samples, int32_t* f_cnt,
                             int16_t* src, const int32_t* seg_bounds, int n_seg,
-                            const int32_t* seg_max, void* stream);
+                            const int32_t* seg_max, int64_t noise_seed,
+                            void* stream);
 
 /* The field's evaluation list of a marked fine stream: the slots s < *n_dev
  * with src[s] == -1, in slot order.  Three steps:

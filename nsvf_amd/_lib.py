@@ -56,9 +56,9 @@ SIGNATURES = {
     "nsvf_render_hits_camera": ([_CAM, _P, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_sample_coarse": ([_P, _P, _L, _P, _P, _P, _P, _P, _I, _F, _I, _F, _P, _P, _P,
-                                   _P, _P, _I, _P, _P], _I),
+                                   _P, _P, _I, _P, _L, _P], _I),
     "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P,
-                                 _P, _P, _P, _I, _P, _P], _I),
+                                 _P, _P, _P, _I, _P, _L, _P], _I),
     "nsvf_render_eval_list_blocks": ([_L], _L),
     "nsvf_render_eval_list_count": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_eval_list": ([_P, _P, _L, _P, _P, _P], _I),

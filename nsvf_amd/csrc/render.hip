@@ -485,7 +485,9 @@ __device__ __forceinline__ int segment_of(const int32_t* __restrict__ bounds, in
 // Coarse pass: one wavefront per hit ray n (ray r = hit_list[n]) writes the
 // ray's samples into its slot range [c_off[r], c_off[r] + bound[r]); count,
 // fine-pass bound and the longest row -- of the frame, and with segments
-// (n_seg > 0) of each segment -- are recorded for the fine pass.
+// (n_seg > 0) of each segment -- are recorded for the fine pass.  kNoise:
+// training-mode steps, u = HashNoise(seed, r) (else 0.5).
+template <bool kNoise>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void coarse_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const HitRec* __restrict__ hits, const int32_t* __restrict__ count,
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     const int32_t* __restrict__ c_off, int n_max, float step_size, int compat, float fine_steps,
     SampleRec* __restrict__ out, int32_t* __restrict__ c_cnt, int32_t* __restrict__ f_bound,
     int32_t* __restrict__ max_count, const int32_t* __restrict__ seg_bounds, int n_seg,
-    int32_t* __restrict__ seg_max) {
+    int32_t* __restrict__ seg_max, uint32_t seed) {
   __shared__ float lds_c[kSampleWaves][kCoarseBins], lds_lo[kSampleWaves][kCoarseBins],
       lds_hi[kSampleWaves][kCoarseBins];
   __shared__ int32_t lds_id[kSampleWaves][kCoarseBins];
@@ -515,9 +517,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) voi
     HitBins bins{hits + int64_t(r) * n_max, cnt, tot[r], hits + int64_t(r0) * n_max, count[r0],
                  nn < 0 ? -1 : hits[int64_t(hit_list[nn]) * n_max].idx};
     const float steps = tot[r] / step_size;
-    const int m = icdf_wave(bins, n_max, cnt, steps, compat, bound[r], r, out + c_off[r],
-                            WaveBinsLds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave],
-                                        kCoarseBins});
+    const WaveBinsLds lds{lds_c[wave], lds_lo[wave], lds_hi[wave], lds_id[wave], kCoarseBins};
+    int m;
+    if constexpr (kNoise)
+      m = icdf_wave(bins, n_max, cnt, steps, compat, bound[r], r, out + c_off[r], lds,
+                    NoSlotHook{}, HashNoise::of(seed, uint32_t(r)));
+    else
+      m = icdf_wave(bins, n_max, cnt, steps, compat, bound[r], r, out + c_off[r], lds);
     if (lane == 0) {
       c_cnt[r] = m;
       f_bound[r] = int(ceilf(fine_steps)) + m + 2;
@@ -602,8 +608,8 @@ struct FieldReuse {
 // kComposite: the wave first composites the ray's coarse row from c_field
 // (composite_pass, as the coarse composite does) and stores the weights to
 // c_prob, keeping the staged bins' weights in registers; otherwise c_prob
-// holds them.
-template <bool kComposite>
+// holds them.  kNoise: training-mode steps, u = HashNoise(seed, r) (else 0.5).
+template <bool kComposite, bool kNoise>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void fine_sample_kernel(
     const int32_t* __restrict__ hit_list, const int32_t* __restrict__ n_hit_dev,
     const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_cnt,
@@ -612,7 +618,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     const int32_t* __restrict__ f_bound,
     const int32_t* __restrict__ f_off, float fine_steps, int compat, SampleRec* __restrict__ out,
     int32_t* __restrict__ f_cnt, int16_t* __restrict__ src, const int32_t* __restrict__ seg_bounds,
-    int n_seg, const int32_t* __restrict__ seg_max) {
+    int n_seg, const int32_t* __restrict__ seg_max, uint32_t seed) {
   __shared__ float lds_c[kSampleWaves][kFineBins], lds_lo[kSampleWaves][kFineBins],
       lds_hi[kSampleWaves][kFineBins];
   __shared__ int32_t lds_id[kSampleWaves][kFineBins];
@@ -659,12 +665,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
           for (int j = 0; j < kRegPasses; ++j) bins.pr[j] = q == j ? p : bins.pr[j];
         }
         bins.den = fine_den_reg(bins.pr, c_prob + off, cnt, P);
-        m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse);
+        if constexpr (kNoise)
+          m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse,
+                        HashNoise::of(seed, uint32_t(r)));
+        else
+          m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse);
       } else {
         const float den = fine_den(c_prob + off, cnt, P);
         SampleBins bins{coarse + off, c_prob + off, cnt, den, coarse + c_off[r0], c_cnt[r0],
                         next};
-        m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse);
+        if constexpr (kNoise)
+          m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse,
+                        HashNoise::of(seed, uint32_t(r)));
+        else
+          m = icdf_wave(bins, P, cnt, fine_steps, compat, b, r, dst, lds, reuse);
       }
     } else {
       for (int k = lane; k < b; k += 64) {
@@ -1151,18 +1165,23 @@ int nsvf_render_sample_coarse(const int32_t* hit_list, const int32_t* n_hit_dev,
                               float step_size, int compat, float fine_steps, void* samples,
                               int32_t* c_cnt, int32_t* f_bound, int32_t* max_count,
                               const int32_t* seg_bounds, int n_seg, int32_t* seg_max,
-                              void* stream) {
+                              int64_t noise_seed, void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
   if (n_seg < 0 || n_seg > kMaxSegments || (n_seg > 0 && (!seg_bounds || !seg_max)))
     return set_error(NSVF_EINVAL, "render_sample_coarse: %d segments (at most %d, with bounds "
                      "and maxima)", n_seg, kMaxSegments);
-  hipLaunchKernelGGL(coarse_sample_kernel,
+  if (noise_seed > int64_t(UINT32_MAX))
+    return set_error(NSVF_EINVAL, "render_sample_coarse: noise seed %lld beyond 32 bits",
+                     static_cast<long long>(noise_seed));
+  auto kernel = noise_seed >= 0 ? coarse_sample_kernel<true> : coarse_sample_kernel<false>;
+  hipLaunchKernelGGL(kernel,
                      dim3(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192))),
                      dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev,
                      static_cast<const HitRec*>(hits), count, tot, bound, c_off, n_max, step_size,
                      compat, fine_steps, static_cast<SampleRec*>(samples), c_cnt, f_bound,
-                     max_count, seg_bounds, n_seg, seg_max);
+                     max_count, seg_bounds, n_seg, seg_max,
+                     uint32_t(noise_seed >= 0 ? noise_seed : 0));
   return check_launch("render_sample_coarse");
 }
 
@@ -1172,25 +1191,25 @@ int nsvf_render_sample_fine(const int32_t* hit_list, const int32_t* n_hit_dev, i
                             const int32_t* f_bound, const int32_t* f_off, float fine_steps,
                             int compat, void* samples, int32_t* f_cnt, int16_t* src,
                             const int32_t* seg_bounds, int n_seg, const int32_t* seg_max,
-                            void* stream) {
+                            int64_t noise_seed, void* stream) {
   clear_error();
   if (n_hit_max <= 0) return NSVF_OK;
   if (n_seg < 0 || n_seg > kMaxSegments || (n_seg > 0 && (!seg_bounds || !seg_max)))
     return set_error(NSVF_EINVAL, "render_sample_fine: %d segments (at most %d, with bounds "
                      "and maxima)", n_seg, kMaxSegments);
+  if (noise_seed > int64_t(UINT32_MAX))
+    return set_error(NSVF_EINVAL, "render_sample_fine: noise seed %lld beyond 32 bits",
+                     static_cast<long long>(noise_seed));
   const dim3 grid(unsigned(std::min<int64_t>(ceil_div(n_hit_max, kSampleWaves), 8192)));
-  if (c_field)
-    hipLaunchKernelGGL(fine_sample_kernel<true>, grid, dim3(kBlock), 0, as_stream(stream),
-                       hit_list, n_hit_dev, c_off, c_cnt, c_max,
-                       static_cast<const SampleRec*>(coarse), static_cast<const float4*>(c_field),
-                       c_prob, f_bound, f_off, fine_steps, compat,
-                       static_cast<SampleRec*>(samples), f_cnt, src, seg_bounds, n_seg, seg_max);
-  else
-    hipLaunchKernelGGL(fine_sample_kernel<false>, grid, dim3(kBlock), 0, as_stream(stream),
-                       hit_list, n_hit_dev, c_off, c_cnt, c_max,
-                       static_cast<const SampleRec*>(coarse), nullptr, c_prob, f_bound, f_off,
-                       fine_steps, compat, static_cast<SampleRec*>(samples), f_cnt, src,
-                       seg_bounds, n_seg, seg_max);
+  const bool noisy = noise_seed >= 0;
+  const uint32_t seed = uint32_t(noisy ? noise_seed : 0);
+  auto kernel = c_field ? (noisy ? fine_sample_kernel<true, true> : fine_sample_kernel<true, false>)
+                        : (noisy ? fine_sample_kernel<false, true> : fine_sample_kernel<false, false>);
+  hipLaunchKernelGGL(kernel, grid, dim3(kBlock), 0, as_stream(stream), hit_list, n_hit_dev, c_off,
+                     c_cnt, c_max, static_cast<const SampleRec*>(coarse),
+                     static_cast<const float4*>(c_field), c_prob, f_bound, f_off, fine_steps,
+                     compat, static_cast<SampleRec*>(samples), f_cnt, src, seg_bounds, n_seg,
+                     seg_max, seed);
   return check_launch("render_sample_fine");
 }
 

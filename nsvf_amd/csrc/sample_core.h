@@ -117,4 +117,28 @@ struct ConstNoise {
   __device__ __forceinline__ float at(int) const { return v; }
 };
 
+// Training-mode noise (clib/__init__.py:252-256 draws noise.uniform_()
+// clamped to [0.001, 0.999] when the sampler is not deterministic): a
+// counter-based uniform value per (seed, ray row, step) from the PCG hash
+// (Jarzynski and Olano, "Hash functions for GPU rendering", JCGT 2020), so a
+// kernel needs no random state and any implementation can reproduce the
+// draws: key = hash(row + hash(seed)), u = (hash(step + key) >> 8) / 2^24,
+// clamped.
+__host__ __device__ __forceinline__ uint32_t pcg_hash(uint32_t v) {
+  const uint32_t state = v * 747796405u + 2891336453u;
+  const uint32_t word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+  return (word >> 22u) ^ word;
+}
+
+struct HashNoise {
+  uint32_t key;  // pcg_hash(row + pcg_hash(seed))
+  __device__ __forceinline__ static HashNoise of(uint32_t seed, uint32_t row) {
+    return HashNoise{pcg_hash(row + pcg_hash(seed))};
+  }
+  __device__ __forceinline__ float at(int k) const {
+    const float u = float(pcg_hash(uint32_t(k) + key) >> 8) * (1.0f / 16777216.0f);
+    return fminf(fmaxf(u, 0.001f), 0.999f);
+  }
+};
+
 }  // namespace nsvf

@@ -1,8 +1,9 @@
 // Inverse-CDF sampling of one ray by one wavefront, with the same outputs,
-// bit for bit, as the sequential walk of sample_core.h (icdf_ray with noise
-// 0.5 into a StreamSink).  The sequential walk interleaves two sorted
-// sequences -- the steps' cdf values (i + 0.5) / steps and the bins'
-// cumulative probabilities C_k -- so every output has a closed-form place:
+// bit for bit, as the sequential walk of sample_core.h (icdf_ray into a
+// StreamSink, with the same noise).  The sequential walk interleaves two
+// sorted sequences -- the steps' cdf values (i + u_i) / steps, u_i in (0, 1)
+// (0.5 when deterministic), and the bins' cumulative probabilities C_k -- so
+// every output has a closed-form place:
 //
 //   b_i   = the first bin k with cdf_i <= C_k (binary search; C in LDS);
 //   step i's sample goes to position i + b_i (i steps and b_i bin tails
@@ -86,11 +87,14 @@ struct NoSlotHook {
 // Samples ray `ray` with bins `bins` (P columns, the first Pv valid) into
 // out[0, bound); slots past the samples are marked unused.  Returns the
 // sample count (same value in every lane).  lds: this wave's staging arrays;
-// hook: called beside every slot written (the caller's lane decides nothing).
-template <class Bins, class Hook = NoSlotHook>
+// hook: called beside every slot written (the caller's lane decides nothing);
+// noise: u_i of step i (its cdf is (i + u_i) / steps; 0.5 when
+// deterministic).  Any u_i in (0, 1) keeps the steps' cdf increasing, so the
+// closed-form places hold for training-mode noise too.
+template <class Bins, class Hook = NoSlotHook, class Noise = ConstNoise>
 __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compat, int bound,
                          int32_t ray, SampleRec* out, const WaveBinsLds& lds,
-                         Hook&& hook = NoSlotHook{}) {
+                         Hook&& hook = NoSlotHook{}, const Noise& noise = ConstNoise{0.5f}) {
   const int lane = threadIdx.x & 63;
   // the row's sizes are the same in every lane; scalar copies keep the
   // loops below uniform (no per-lane exit masks)
@@ -143,7 +147,6 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
     bins.before_seq();
     if (lane == 0) {
       StreamSink sink{out, ray, bound};
-      ConstNoise noise{0.5f};
       icdf_ray(bins, P, steps, -1.0f, noise, compat, bound, sink);
       n = sink.n;
     }
@@ -164,7 +167,7 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
       int b = Pv;
       float z = 0.0f;
       if (act) {
-        const float cdf = (float(i) + 0.5f) * ss;
+        const float cdf = (float(i) + noise.at(i)) * ss;
         int lo = 0, hi = Pv;
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
@@ -220,13 +223,13 @@ __device__ int icdf_wave(const Bins& bins, int P, int Pv, float steps, int compa
         float e = floorf(ck * steps - 0.5f) + 1.0f;
         e = fminf(fmaxf(e, 0.0f), float(T));
         int ik = int(e);
-        while (ik > 0 && (float(ik - 1) + 0.5f) * ss > ck) --ik;
-        while (ik < T && (float(ik) + 0.5f) * ss <= ck) ++ik;
+        while (ik > 0 && (float(ik - 1) + noise.at(ik - 1)) * ss > ck) --ik;
+        while (ik < T && (float(ik) + noise.at(ik)) * ss <= ck) ++ik;
         if (ik < T) {
           const float lk = lds.lo[k], hk = lds.hi[k];
           float zk = lk;
           if (ik > 0) {
-            const float cdfp = (float(ik - 1) + 0.5f) * ss;
+            const float cdfp = (float(ik - 1) + noise.at(ik - 1)) * ss;
             const float cmin = k > 0 ? C[k - 1] : 0.0f;
             if (k == 0 || cdfp > cmin) zk = __fmaf_rn((cdfp - cmin) / (ck - cmin), hk - lk, lk);
           }

@@ -70,6 +70,18 @@ def node_boxes(centers, children, half_voxel):
     return mn, mx, scale
 
 
+def pass_seed(seed, k):
+    """Noise seed of sampling pass k (0 coarse, 1 fine) of a view rendered
+    with `seed` (None: -1, no noise): seed + k * 0x9E3779B9 mod 2^32, so the
+    two passes draw unrelated streams."""
+    if seed is None:
+        return -1
+    seed = int(seed)
+    if not 0 <= seed < 1 << 32:
+        raise ValueError(f"noise seed {seed}: not in [0, 2^32)")
+    return (seed + k * 0x9E3779B9) & 0xFFFFFFFF
+
+
 def tree_depth(children):
     """Depth of the octree whose root is the last node: its children[8] is
     1 << (depth + 1) (octree.cpp:119); a tree of one leaf has depth -1."""
@@ -194,8 +206,14 @@ class Renderer:
     grown as needed, so repeated views of the same size allocate nothing."""
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
-                 compat=True, split_field=None, reuse_coarse=True, fold_composite=True):
+                 compat=True, split_field=None, reuse_coarse=True, fold_composite=True,
+                 noise_seed=None):
         self.device = torch.device(device)
+        # None: every step at the middle of its CDF interval (noise 0.5, the
+        # evaluation sampler); an int in [0, 2^32): training-mode noise of the
+        # samplers (clib/__init__.py:250-256), counter-based, a different
+        # stream per seed and pass
+        self.noise_seed = noise_seed
         # hierarchical views: the fine sampler composites the coarse rows
         # itself (the coarse composite's weights, bit for bit) instead of a
         # coarse composite kernel whose colour the fine pass replaces
@@ -485,6 +503,10 @@ class Renderer:
         return (_p(self._buf("seg_bounds", 17, torch.int32)), len(segs),
                 _p(self._buf("seg_max", 16, torch.int32)))
 
+    def _pass_seed(self, k):
+        """The sampler seed of pass k (0 coarse, 1 fine): -1 without noise."""
+        return pass_seed(self.noise_seed, k)
+
     def _sample_coarse(self):
         """Coarse sample stream of the view begun last (its geometry done)."""
         L = lib()
@@ -503,7 +525,8 @@ class Renderer:
             _p(self._buf("tot", R, torch.float32)), _p(self._buf("c_bound", R, torch.int32)),
             _p(self._buf("c_off", R, torch.int32)), P, sc.step_size, self.compat,
             self.fine_steps, _p(cs), _p(c_cnt), _p(self._buf("f_bound", R, torch.int32)),
-            _p(scal[2:3]), *self._segment_args(), _stream(self.device)), "sample_coarse")
+            _p(scal[2:3]), *self._segment_args(), self._pass_seed(0), _stream(self.device)),
+            "sample_coarse")
         self._mark("coarse_sample")
         return CoarseStream(R, n_hit, slots_c, self._buf("hits", R * P * 4, torch.int32),
                             self._buf("count", R, torch.int32), self._buf("tot", R, torch.float32),
@@ -571,7 +594,7 @@ class Renderer:
                                             _p(scal[2:3]), _p(cs), _p(c_field) if fold else None,
                                             _p(c_prob), _p(f_bound), _p(f_off), fs, self.compat,
                                             _p(fsm), _p(f_cnt), _p(src), *self._segment_args(),
-                                            st), "sample_fine")
+                                            self._pass_seed(1), st), "sample_fine")
             if fold:
                 self._track(sc, hit_list, scal, n_hit, c_off, c_cnt, cs, c_prob, st)
             if reuse:

@@ -182,17 +182,33 @@ class TrainRenderer:
     gather, MLP and compositing with autograd, then the background fill
     (nsvf.py:94-101)."""
 
-    def __init__(self, device, compat=True):
+    def __init__(self, device, compat=True, deterministic_step=False, seed=0):
         self.device = torch.device(device)
         self.geometry = Renderer(device, precision="fp32", hierarchical=False, compat=compat)
+        # training samples with noise unless --deterministic-step
+        # (encoder.py:546: deterministic = deterministic_step or not training);
+        # forward() k of this renderer uses noise seed `seed + k` (mod 2^32)
+        self.deterministic_step = bool(deterministic_step)
+        self.seed = int(seed)
+        self.steps = 0
         self.last = None
+        self.last_seed = None
 
-    def forward(self, sc: DeviceScene, field: TrainableField, ray_start, ray_dir):
+    def forward(self, sc: DeviceScene, field: TrainableField, ray_start, ray_dir, seed=None):
         """ray_start, ray_dir [R,3] float32 on the device.  Returns (rgb [R,3],
-        depth [R], missed [R]) connected to `field`'s parameters."""
+        depth [R], missed [R]) connected to `field`'s parameters.  seed: this
+        step's noise seed (default: the renderer's next one); ignored with
+        deterministic_step."""
         _need_cuda(ray_start, "ray_start")
         _need_cuda(ray_dir, "ray_dir")
         rs, rd = ray_start.contiguous().float(), ray_dir.contiguous().float()
+        if self.deterministic_step:
+            seed = None
+        elif seed is None:
+            seed = (self.seed + self.steps) & 0xFFFFFFFF
+        self.steps += 1
+        self.last_seed = seed
+        self.geometry.noise_seed = seed
         g = self.geometry.coarse_stream(sc, rs, rd).owned()
         self.last = g
         rec = g.samples

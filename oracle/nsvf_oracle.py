@@ -251,6 +251,36 @@ def inverse_cdf_sampling(pts_idx, min_depth, max_depth, probs, steps, fixed_step
     return si[:, :max_len], sd[:, :max_len], sx[:, :max_len], max_steps, flags
 
 
+def pcg_hash(v):
+    """PCG hash (Jarzynski and Olano, "Hash functions for GPU rendering",
+    JCGT 9(3) 2020) of uint32 values, modulo 2^32."""
+    v = np.asarray(v, np.uint32)
+    with np.errstate(over="ignore"):
+        state = v * np.uint32(747796405) + np.uint32(2891336453)
+        word = ((state >> ((state >> np.uint32(28)) + np.uint32(4))) ^ state) * np.uint32(277803737)
+    return (word >> np.uint32(22)) ^ word
+
+
+def hash_noise(seed, rows, n_steps):
+    """The samplers' training-mode noise [len(rows), n_steps] float32: step k
+    of frame ray r draws u = (pcg(k + pcg(r + pcg(seed))) >> 8) / 2^24,
+    clamped to [0.001, 0.999] as clib/__init__.py:256 clamps its uniform_()
+    draws (the reference's torch generator cannot be reproduced; this
+    counter-based stream is the library's own, for any seed in [0, 2^32))."""
+    with np.errstate(over="ignore"):
+        key = pcg_hash(np.asarray(rows, np.int64).astype(np.uint32) + pcg_hash(np.uint32(seed)))
+        k = np.arange(n_steps, dtype=np.uint32)
+        h = pcg_hash(k[None, :] + key[:, None])
+    u = (h >> np.uint32(8)).astype(np.float32) * np.float32(2.0 ** -24)
+    return np.minimum(np.maximum(u, np.float32(0.001)), np.float32(0.999))
+
+
+def pass_seed(seed, k):
+    """Seed of sampling pass k (0 coarse, 1 fine) of a view rendered with
+    noise seed `seed`: seed + k * 0x9E3779B9 modulo 2^32."""
+    return (int(seed) + k * 0x9E3779B9) & 0xFFFFFFFF
+
+
 def uniform_ray_sampling(pts_idx, min_depth, max_depth, step_size, max_ray_length,
                          deterministic=True, noise=None):
     """``uniform_ray_sampling``: clib/__init__.py:178-228.
@@ -784,7 +814,8 @@ def prepare_fine_bins(si, sd, sx, probs, fine_steps):
 
 
 def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, compat=True,
-                dtype=np.float64, coarse_probs=None, use_octree=True, subset=None):
+                dtype=np.float64, coarse_probs=None, use_octree=True, subset=None,
+                noise_seed=None):
     """Render rays [R,3] of ``scene`` (a VoxelScene).  Integer and sampling
     decisions are the FP32 restatement's; gather, MLP and compositing run in
     ``dtype``.  ``coarse_probs`` ([N_hit, S_c] float32) replaces the coarse
@@ -799,8 +830,13 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
     zero coarse probabilities (their fine pass then samples uniformly, which
     touches only their own outputs).  The image outputs (rgb, depth, missed)
     are then the subset's, in its order; ``eval_rows`` gives the positions
-    of the evaluated rays among the hit rays.  Returns a dict of the
-    outputs and intermediates."""
+    of the evaluated rays among the hit rays.
+
+    ``noise_seed``: training-mode sampling (encoder.py:546 with the model in
+    training): both passes draw hash_noise rows of the hit rays' frame ids,
+    with seeds pass_seed(noise_seed, 0) and pass_seed(noise_seed, 1); None
+    samples at noise 0.5.  Returns a dict of the outputs and
+    intermediates."""
     rs = np.asarray(ray_start, np.float32)
     rd = np.asarray(ray_dir, np.float32)
     vs = np.float32(scene.voxel_size)
@@ -824,7 +860,15 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
     idx, lo, hi = idx[hits], lo[hits], hi[hits]
     probs, steps = probs_steps(idx, lo, hi, scene.step_size)
     out.update(probs=probs, steps=steps)
-    si, sd, sx, S, _ = inverse_cdf_sampling(idx, lo, hi, probs, steps, -1.0, True, compat=compat)
+    def noise(k, n_cols):
+        if noise_seed is None:
+            return None
+        return hash_noise(pass_seed(noise_seed, k), hit_rays, n_cols)
+
+    det = noise_seed is None
+    nz = None if det else noise(0, int(np.ceil(steps).astype(np.int64).max(initial=0)) + idx.shape[1])
+    si, sd, sx, S, _ = inverse_cdf_sampling(idx, lo, hi, probs, steps, -1.0, det, noise=nz,
+                                            compat=compat)
     sd, sx = sample_tail(si, sd, sx)
     out.update(c_idx=si, c_depth=sd, c_dist=sx)
     rsh, rdh = rs[hits][ev], rd[hits][ev]
@@ -840,7 +884,9 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
         else:
             p_use = np.asarray(coarse_probs, np.float32)
         fi, flo, fhi, fp, fst = prepare_fine_bins(si, sd, sx, p_use, fine_steps)
-        si2, sd2, sx2, S2, _ = inverse_cdf_sampling(fi, flo, fhi, fp, fst, -1.0, True, compat=compat)
+        nz = None if det else noise(1, int(math.ceil(fine_steps)) + fi.shape[1])
+        si2, sd2, sx2, S2, _ = inverse_cdf_sampling(fi, flo, fhi, fp, fst, -1.0, det, noise=nz,
+                                                    compat=compat)
         sd2, sx2 = sample_tail(si2, sd2, sx2)
         out.update(f_idx=si2, f_depth=sd2, f_dist=sx2)
         sig2, col2 = _field_eval(si2[ev], sd2[ev], sx2[ev], rsh, rdh, pts_shift, scene.feats,

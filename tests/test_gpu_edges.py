@@ -203,7 +203,8 @@ def sampler_pair(r, device):
     check(L.nsvf_render_sample_fine(_p(ws["hit_list"]), _p(scal[0:1]), R, _p(ws["c_off"]),
                                     _p(ws["c_cnt"]), _p(scal[2:3]), _p(ws["c_samples"]), None,
                                     _p(prob), _p(ws["f_bound"]), _p(ws["f_off"]), r.fine_steps,
-                                    r.compat, _p(fs), _p(f_cnt), _p(src), st), "sample_fine")
+                                    r.compat, _p(fs), _p(f_cnt), _p(src), None, 0, None, -1,
+                                    st), "sample_fine")
     torch.cuda.synchronize()
     hl = N(k["hit_list"]).astype(np.int64)
     m = len(k["f_samples"][0])

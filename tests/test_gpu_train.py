@@ -170,7 +170,7 @@ def c3(scene, device):
     rs, rd, target = c3_batch()
     t = lambda a: torch.from_numpy(a).to(device)
     field = TrainableField(sc, device)
-    tr = TrainRenderer(device)
+    tr = TrainRenderer(device, deterministic_step=True)
     rgb, _, _ = tr.forward(dsc, field, t(rs), t(rd))
     loss = mse_loss(rgb, t(target))
     loss.backward()

@@ -239,7 +239,12 @@ def icdf_one(bins, probs, steps, noise=0.5, compat=True, P=None, extra_rows=None
             pr[0, r, k] = p[k]
         st[0, r] = s
     S = int(np.ceil(st).max()) + P
-    nz = np.full((1, R, S), noise, np.float32)
+    if np.ndim(noise):
+        # per-step noise of the first ray
+        nz = np.full((1, R, S), 0.5, np.float32)
+        nz[0, 0, :len(noise)] = noise
+    else:
+        nz = np.full((1, R, S), noise, np.float32)
     oi, od, ox, fl = O.inverse_cdf_sampling_op(idx, lo, hi, nz, pr, st, -1.0, compat)
     n = (oi[0, 0] != -1).sum()
     return oi[0, 0], od[0, 0], ox[0, 0]
@@ -251,6 +256,66 @@ def test_icdf_single_bin_closed_form():
     assert list(oi[:n]) == [5] * 5 and (oi[n:] == -1).all()
     assert list(od[:n]) == [1.0625, 1.25, 1.5, 1.75, 1.9375]
     assert list(ox[:n]) == [0.125, 0.25, 0.25, 0.25, 0.125]
+
+
+def test_icdf_single_bin_with_noise_closed_form():
+    """Step i sits at cdf (i + u_i) / steps (clib/__init__.py:250-256 with
+    training-mode noise): z_i = 1 + (i + u_i) / 4 on the bin [1, 2]."""
+    u = [0.25, 0.5, 0.75, 0.125]
+    oi, od, ox = icdf_one([(5, 1.0, 2.0)], [1.0], 4.0, noise=u)
+    z = [1.0] + [1.0 + (i + u[i]) / 4 for i in range(4)] + [2.0]
+    assert list(oi[:5]) == [5] * 5 and (oi[5:] == -1).all()
+    assert list(od[:5]) == [(z[j] + z[j + 1]) / 2 for j in range(5)]
+    assert list(ox[:5]) == [z[j + 1] - z[j] for j in range(5)]
+
+
+def test_hash_noise_is_uniform_and_keyed():
+    """The counter-based noise: the PCG hash is a bijection of uint32 (no
+    collisions over 2^20 consecutive inputs); draws are uniform on
+    [0.001, 0.999] (mean, quartiles, clamp mass 0.001 per side) and
+    decorrelated across steps, rays and seeds; the same key gives the same
+    draws."""
+    h = O.pcg_hash(np.arange(1 << 20, dtype=np.uint32))
+    assert len(np.unique(h)) == 1 << 20
+    u = O.hash_noise(3, np.arange(2000), 500)
+    assert u.dtype == np.float32 and u.min() >= np.float32(0.001) and u.max() <= np.float32(0.999)
+    n = u.size
+    assert abs(u.mean() - 0.5) < 5 / np.sqrt(12 * n)
+    for q in (0.25, 0.5, 0.75):
+        assert abs((u < q).mean() - q) < 5 * np.sqrt(q * (1 - q) / n)
+    lo = (u == np.float32(0.001)).mean()
+    assert abs(lo - 0.001) < 5 * np.sqrt(0.001 / n)
+    c = lambda a, b: abs(np.corrcoef(a.ravel(), b.ravel())[0, 1])
+    assert c(u[:, :-1], u[:, 1:]) < 0.01
+    assert c(u[:-1], u[1:]) < 0.01
+    assert c(u, O.hash_noise(4, np.arange(2000), 500)) < 0.01
+    assert np.array_equal(u[7:9], O.hash_noise(3, [7, 8], 500))
+    assert O.pass_seed(5, 0) == 5 and O.pass_seed(2**32 - 1, 1) == (2**32 - 1 + 0x9E3779B9) % 2**32
+
+
+def test_render_view_with_noise_stays_in_bins():
+    """With noise the coarse samples still tile each ray's hit segments
+    (every sample inside its voxel's segment, the lengths summing to the
+    segments' total) and differ from noise 0.5.  (Depths need not increase:
+    a zero-length hit sorted after a longer one with the same entry depth
+    gets its sample after that one's.)"""
+    from _cases import shell, shell_rays
+    sc = shell()
+    rs, rd = shell_rays(width=16, seed=3)
+    a = O.render_view(sc, rs, rd, compat=False, noise_seed=9, hierarchical=False)
+    b = O.render_view(sc, rs, rd, compat=False, hierarchical=False)
+    assert a["c_idx"].shape[0] == b["c_idx"].shape[0] > 0
+    assert not np.array_equal(a["c_depth"], b["c_depth"])
+    idx, lo, hi = a["sorted_idx"][a["hits"]], a["sorted_lo"][a["hits"]], a["sorted_hi"][a["hits"]]
+    for n in range(idx.shape[0]):
+        m = a["c_idx"][n] != -1
+        d, x, v = a["c_depth"][n][m], a["c_dist"][n][m], a["c_idx"][n][m]
+        seg = {int(i): (l, h) for i, l, h in zip(idx[n], lo[n], hi[n]) if i != -1}
+        for dd, xx, vv in zip(d, x, v):
+            l, h = seg[int(vv)]
+            assert l - 1e-5 <= dd - xx / 2 and dd + xx / 2 <= h + 1e-5
+        tot = sum(h - l for l, h in seg.values())
+        assert abs(x.astype(np.float64).sum() - tot) < 1e-4 * max(1.0, tot)
 
 
 def test_icdf_two_bins_with_gap():
