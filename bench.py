@@ -567,27 +567,52 @@ def rows(args, dev, sc, dsc):
     res["composite_backward"] = {"ms": round(ms, 3), "samples": M, "algorithmic_bytes": nbytes,
                                  "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
                                  "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-    # one training step: forward, MSE against U(0,1) targets, backward
-    tf = TrainableField(sc, dev)
-    tr = TrainRenderer(dev)
+    # one training step: forward, MSE against U(0,1) targets, backward --
+    # with the HIP training MLP, and with torch's Linear layers for reference
     target = torch.rand((R, 3), device=dev)
+    for key, impl in (("train_step", "hip"), ("train_step_torch_mlp", "torch")):
+        tf = TrainableField(sc, dev, mlp=impl)
+        tr = TrainRenderer(dev)
 
-    def step():
-        tf.zero_grad(set_to_none=True)
-        rgb, _, _ = tr.forward(dsc, tf, rs, rd)
-        mse_loss(rgb, target).backward()
+        def step():
+            tf.zero_grad(set_to_none=True)
+            rgb, _, _ = tr.forward(dsc, tf, rs, rd)
+            mse_loss(rgb, target).backward()
 
-    torch.cuda.synchronize()
-    step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(5):
+        torch.cuda.synchronize()
         step()
-    torch.cuda.synchronize()
-    res["train_step"] = {"ms": round((time.perf_counter() - t0) * 1e3 / 5, 2), "rays": R,
-                         "samples": M, "mlp": "torch FP32 nn.Linear",
-                         "kernels": "HIP gather / composite forward and backward"}
-    del tf, tr
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            step()
+        torch.cuda.synchronize()
+        res[key] = {"ms": round((time.perf_counter() - t0) * 1e3 / 5, 2), "rays": R,
+                    "samples": M,
+                    "mlp": "HIP FP32 MFMA" if impl == "hip" else "torch FP32 nn.Linear",
+                    "kernels": "HIP gather / composite forward and backward"}
+        del tf, tr
+        torch.cuda.empty_cache()
+    # the training MLP kernels alone on the batch's samples
+    from nsvf_amd.train import FieldMLP
+    tf = TrainableField(sc, dev)
+    prm = []
+    for w_, b_ in zip(tf.weights, tf.biases):
+        prm += [w_.detach().requires_grad_(True), b_.detach().requires_grad_(True)]
+    emb = torch.randn((M, D), device=dev).mul_(0.1).requires_grad_(True)
+    dout = torch.randn((M, 4), device=dev)
+    fl = sc.mlp.flops_per_sample()
+    ms_f = _time_ms(lambda: FieldMLP.apply(emb, *prm), 5)
+    ms_fb = _time_ms(lambda: FieldMLP.apply(emb, *prm).backward(dout), 5)
+    # backward: recomputed forward + input gradients + weight gradients, plus
+    # the first layer once more (about 3 forward FLOPs and one layer)
+    ms_b = ms_fb - ms_f
+    bw_flops = M * (3 * fl + 2 * dsc.D * dsc.H)
+    res["mlp_train"] = {"forward_ms": round(ms_f, 3), "backward_ms": round(ms_b, 3),
+                        "samples": M,
+                        "forward_TFLOPs": round(M * fl / (ms_f * 1e-3) / 1e12, 1),
+                        "backward_TFLOPs": round(bw_flops / (ms_b * 1e-3) / 1e12, 1),
+                        "peak_TFLOPs": 157.3}
+    del tf, prm, emb, dout
     torch.cuda.empty_cache()
     # pruning: 16^3 lattice per voxel through the field MLP, max relu(sigma)
     flops = sc.num_voxels * 4096 * sc.mlp.flops_per_sample()

@@ -518,6 +518,34 @@ int nsvf_render_composite_backward(const int32_t* hit_list,
                                    const float* grad_missed, float* grad_field,
                                    void* stream);
 
+/* The field MLP for training (fairnr/modules/field.py:217-279: L Linear +
+ * ReLU layers of width H over D inputs, a 4-wide head; output (sigma,
+ * sigmoid(r, g, b)), which the reference differentiates with torch
+ * autograd), forward and backward on FP32 MFMA.  Built for D=32, H=128,
+ * L=3 (nsvf_mlp_train_supported); other shapes return NSVF_EINVAL and the
+ * size queries -1.
+ * Parameters in torch layout: weights[l] [out][in] and biases[l] [out] for
+ * l = 0..L (device pointers, host arrays of L + 1).  nsvf_mlp_train_pack
+ * writes the kernels' operand fragments into `packed` (packed_floats).
+ * Forward: out [M][4] from emb [M][D].  Backward: from dout [M][4] (the
+ * gradient of the output) to demb [M][D] and dparams (param_floats, the
+ * gradients of W_0, b_0, ..., W_L, b_L in that order and layout); the
+ * activations are recomputed from emb, and the parameter gradients summed
+ * over samples in a fixed order through `partial` (partial_floats(M)), so
+ * they are deterministic.                                                  */
+int nsvf_mlp_train_supported(int D, int H, int L);
+int64_t nsvf_mlp_train_packed_floats(int D, int H, int L);
+int64_t nsvf_mlp_train_param_floats(int D, int H, int L);
+int64_t nsvf_mlp_train_partial_floats(int D, int H, int L, int64_t M);
+int nsvf_mlp_train_pack(int D, int H, int L, const float* const* weights,
+                        const float* const* biases, float* packed, void* stream);
+int nsvf_mlp_train_forward(int D, int H, int L, const float* packed,
+                           const float* emb, int64_t M, float* out, void* stream);
+int nsvf_mlp_train_backward(int D, int H, int L, const float* packed,
+                            const float* emb, int64_t M, const float* dout,
+                            float* demb, float* partial, int64_t partial_floats,
+                            float* dparams, void* stream);
+
 /* Frame outputs for all R rays: hit rays' composite plus missed * background
  * (fairnr/models/nsvf.py:89-101); rays that hit nothing get the background. */
 int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const float* ray_rgb,

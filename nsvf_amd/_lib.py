@@ -60,6 +60,13 @@ SIGNATURES = {
     "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P,
                                  _P, _P, _P, _I, _P, _L, _P], _I),
     "nsvf_render_eval_list_blocks": ([_L], _L),
+    "nsvf_mlp_train_supported": ([_I, _I, _I], _I),
+    "nsvf_mlp_train_packed_floats": ([_I, _I, _I], _L),
+    "nsvf_mlp_train_param_floats": ([_I, _I, _I], _L),
+    "nsvf_mlp_train_partial_floats": ([_I, _I, _I, _L], _L),
+    "nsvf_mlp_train_pack": ([_I, _I, _I, _P, _P, _P, _P], _I),
+    "nsvf_mlp_train_forward": ([_I, _I, _I, _P, _P, _L, _P, _P], _I),
+    "nsvf_mlp_train_backward": ([_I, _I, _I, _P, _P, _L, _P, _P, _P, _L, _P, _P], _I),
     "nsvf_render_eval_list_count": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_eval_list": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_render_reuse_fill": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P], _I),

@@ -13,10 +13,10 @@ through three parts:
   compositing        renderer.py:119-121,195-218 -- relu, exp, cumsum.
 
 Here the geometry stages are the fused renderer's kernels (one coarse sample
-stream per batch of rays), the gather and the compositing are autograd
-Functions over hand-written HIP forward and backward kernels (the gather's
-backward deterministic and atomic-free), and the MLP is FP32 torch.nn.Linear
-on the GPU's BLAS, as in the reference.
+stream per batch of rays), and the gather, the MLP and the compositing are
+autograd Functions over hand-written HIP forward and backward kernels, all
+deterministic and atomic-free (the MLP on FP32 MFMA; torch.nn.Linear with
+torch autograd, as in the reference, on request).
 """
 from __future__ import annotations
 
@@ -149,13 +149,102 @@ class StreamComposite(torch.autograd.Function):
         return grad_field, None
 
 
+def _mlp_shape(weights):
+    return int(weights[0].shape[1]), int(weights[0].shape[0]), len(weights) - 1
+
+
+def mlp_train_supported(D, H, L):
+    """Whether the HIP training MLP is built for input width D, L hidden
+    layers of width H (and the 4-wide head)."""
+    return bool(lib().nsvf_mlp_train_supported(int(D), int(H), int(L)))
+
+
+def pack_mlp_train(weights, biases):
+    """The packed operand fragments and biases of the training MLP kernels
+    (a flat float32 device tensor), from torch-layout weights [out, in]."""
+    L_ = lib()
+    D, H, L = _mlp_shape(weights)
+    n = int(L_.nsvf_mlp_train_packed_floats(D, H, L))
+    if n < 0:
+        raise RuntimeError(f"training MLP: shape D={D} H={H} L={L} is not built")
+    dev = weights[0].device
+    ws = [w.detach().contiguous().float() for w in weights]
+    bs = [b.detach().contiguous().float() for b in biases]
+    for w, b in zip(ws, bs):
+        _need_cuda(w, "weight")
+        _need_cuda(b, "bias")
+    packed = torch.empty(n, dtype=torch.float32, device=dev)
+    wp = (ctypes.c_void_p * len(ws))(*[w.data_ptr() for w in ws])
+    bp = (ctypes.c_void_p * len(bs))(*[b.data_ptr() for b in bs])
+    check(L_.nsvf_mlp_train_pack(D, H, L, wp, bp, _p(packed), _stream(dev)), "mlp_train_pack")
+    return packed
+
+
+class FieldMLP(torch.autograd.Function):
+    """The field MLP (field.py:217-279): L Linear + ReLU layers, then a
+    4-wide head, output (sigma, sigmoid(r, g, b)).  Forward and backward are
+    the hand-written FP32-MFMA kernels: the backward recomputes the
+    activations from the input rather than storing them, and sums the
+    weight gradients over samples in a fixed order (deterministic).
+    Arguments: emb [M,D], then W_0, b_0, ..., W_L, b_L in torch layout."""
+
+    @staticmethod
+    def forward(ctx, emb, *params):
+        weights, biases = list(params[0::2]), list(params[1::2])
+        D, H, L = _mlp_shape(weights)
+        _need_cuda(emb, "emb")
+        emb = emb.contiguous()
+        if emb.dtype != torch.float32 or emb.dim() != 2 or emb.shape[1] != D:
+            raise RuntimeError(f"emb must be float32 [M, {D}]")
+        packed = pack_mlp_train(weights, biases)
+        M = emb.shape[0]
+        out = torch.empty((M, 4), dtype=torch.float32, device=emb.device)
+        check(lib().nsvf_mlp_train_forward(D, H, L, _p(packed), _p(emb), M, _p(out),
+                                           _stream(emb.device)), "mlp_train_forward")
+        ctx.save_for_backward(emb, packed)
+        ctx.shape = (D, H, L)
+        ctx.param_shapes = [tuple(t.shape) for t in params]
+        return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        emb, packed = ctx.saved_tensors
+        D, H, L = ctx.shape
+        L_ = lib()
+        dev = emb.device
+        M = emb.shape[0]
+        dout = dout.contiguous().float()
+        demb = torch.empty((M, D), dtype=torch.float32, device=dev)
+        n_part = int(L_.nsvf_mlp_train_partial_floats(D, H, L, M))
+        n_par = int(L_.nsvf_mlp_train_param_floats(D, H, L))
+        partial = torch.empty(n_part, dtype=torch.float32, device=dev)
+        dparams = torch.empty(n_par, dtype=torch.float32, device=dev)
+        check(L_.nsvf_mlp_train_backward(D, H, L, _p(packed), _p(emb), M, _p(dout), _p(demb),
+                                         _p(partial), n_part, _p(dparams), _stream(dev)),
+              "mlp_train_backward")
+        grads, o = [], 0
+        for shp in ctx.param_shapes:
+            n = int(np.prod(shp))
+            grads.append(dparams[o:o + n].view(shp))
+            o += n
+        return (demb, *grads)
+
+
 class TrainableField(nn.Module):
     """The trainable state of a voxel scene: corner embeddings [K,D] and the
-    field MLP (L hidden ReLU layers, a 4-wide head: sigma, sigmoid RGB)."""
+    field MLP (L hidden ReLU layers, a 4-wide head: sigma, sigmoid RGB).
 
-    def __init__(self, scene: VoxelScene, device):
+    mlp="hip": the MLP runs forward and backward on the hand-written
+    FP32-MFMA kernels (FieldMLP; built for D=32, H=128, L=3 -- other shapes
+    raise); mlp="torch": torch.nn.functional.linear with torch autograd, as
+    in the reference."""
+
+    def __init__(self, scene: VoxelScene, device, mlp="hip"):
         super().__init__()
         dev = torch.device(device)
+        if mlp not in ("hip", "torch"):
+            raise ValueError(f"mlp={mlp!r}: 'hip' or 'torch'")
+        self.mlp_impl = mlp
         self.values = nn.Parameter(torch.from_numpy(np.array(scene.values, np.float32)).to(dev))
         self.weights = nn.ParameterList(
             [nn.Parameter(torch.from_numpy(np.array(w, np.float32)).to(dev))
@@ -166,6 +255,15 @@ class TrainableField(nn.Module):
 
     def mlp(self, emb):
         """[M,D] -> [M,4] = (sigma, r, g, b)."""
+        if self.mlp_impl == "hip":
+            D, H, L = _mlp_shape(list(self.weights))
+            if not mlp_train_supported(D, H, L):
+                raise RuntimeError(f"the HIP training MLP is not built for D={D} H={H} L={L}; "
+                                   "construct the field with mlp='torch'")
+            params = []
+            for w, b in zip(self.weights, self.biases):
+                params += [w, b]
+            return FieldMLP.apply(emb, *params)
         h = emb
         n = len(self.weights)
         for i in range(n):
