@@ -130,3 +130,41 @@ def test_segmented_batch_equals_views_alone(device, precision):
         for got, want in zip(batch, ref):
             assert torch.equal(got[a:a + n], want)
         a += n
+
+
+def test_serving_sized_scenes_mix_matches_alone(device):
+    """The serving configuration: four resident shells of about 20k, 100k,
+    250k and 400k voxels at the headline precision, twelve requests at 256^2,
+    400^2, 512^2 and 800^2 over all four scenes, half submitted together and
+    half arriving while those run (same-scene requests share launches); every
+    image equals its render alone bit for bit."""
+    scenes = {f"shell{r}": S.shell_scene(res=r, emb_seed=e, mlp_seed=m)
+              for r, e, m in [(46, 11, 21), (79, 12, 22), (107, 13, 23), (125, 14, 24)]}
+    srv = RenderServer(device, slots=4, precision="bf16x3", hierarchical=True)
+    for name, sc in scenes.items():
+        srv.load_scene(name, sc)
+    names = list(scenes)
+    rng = np.random.default_rng(17)
+    items = []
+    for i in range(12):
+        name = names[i % 4] if i < 8 else names[int(rng.integers(4))]
+        res = [256, 400, 512, 800][int(rng.integers(4))]
+        cam = S.sphere_camera(float(rng.uniform(2.0, 3.0)), 500 + i)
+        rs, rd = S.camera_rays(res, res, cam)
+        items.append((name, torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device), res))
+    print("requests:", [(n, w) for n, _, _, w in items], flush=True)
+    reqs = [srv.submit(n, rs, rd, width=w) for n, rs, rd, w in items[:6]]
+    for _ in range(3):
+        srv.poll()
+    reqs += [srv.submit(n, rs, rd, width=w) for n, rs, rd, w in items[6:]]
+    srv.drain()
+    torch.cuda.synchronize()
+    assert max(r.batch for r in reqs) > 1
+    for req, (name, rs, rd, w) in zip(reqs, items):
+        assert req.state == "done", (req.rid, req.state, req.error)
+        r = Renderer(device, precision="bf16x3", hierarchical=True, compat=srv.compat)
+        ref = r.render(srv.scenes[name], rs, rd, width=w)
+        torch.cuda.synchronize()
+        for got, want in zip(req.result, ref):
+            assert torch.equal(got, want), f"request {req.rid} ({name}, {w}^2) differs from alone"
+        del r
