@@ -207,8 +207,18 @@ class Renderer:
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
                  compat=True, split_field=None, reuse_coarse=True, fold_composite=True,
-                 noise_seed=None):
+                 noise_seed=None, tolerance=0.0, chunk_size=65536):
         self.device = torch.device(device)
+        # early ray termination (--raymarching-tolerance, renderer.py:136-174):
+        # > 0 evaluates each pass's field in the reference's chunks of at most
+        # chunk_size rows and hits and stops rays whose free energy passes
+        # -log(tolerance); 0 (the reference's default) evaluates every sample
+        if not tolerance >= 0.0 or tolerance >= 1.0:
+            raise ValueError(f"tolerance {tolerance}: in [0, 1)")
+        if int(chunk_size) <= 0 or int(chunk_size) >= 1 << 31:
+            raise ValueError(f"chunk_size {chunk_size}: in [1, 2^31)")
+        self.tolerance = float(tolerance)
+        self.chunk_size = int(chunk_size)
         # None: every step at the middle of its CDF interval (noise 0.5, the
         # evaluation sampler); an int in [0, 2^32): training-mode noise of the
         # samplers (clib/__init__.py:250-256), counter-based, a different
@@ -323,6 +333,58 @@ class Renderer:
         if ev:
             self._kernel_events.append(("field", m, e0, event()))
 
+    def _reuse(self):
+        """Fine-pass field reuse (off under early termination: the marched
+        chunks decide which fine samples are evaluated)."""
+        return self.reuse_coarse and self.tolerance <= 0.0
+
+    def _march(self, sc, ray_start, ray_dir, samples, slots, hit_list, n_hit, off, cnt, S, out,
+               tag):
+        """A pass's field with early ray termination: every slot of `out`
+        zeroed, then the chunks of the reference's marching evaluated in
+        steps (one chunk per row group each; csrc/march.hip).  off / cnt:
+        the pass's frame-indexed slot offsets and counts; S: its longest
+        row.  Marks the evaluated slots in the `{tag}_eval` buffer and the
+        rows still marching in `{tag}_active`."""
+        L = lib()
+        st = _stream(self.device)
+        i32 = torch.int32
+        out.zero_()
+        mark = self._buf(f"{tag}_eval", max(slots, 1), torch.int8)
+        mark.zero_()
+        active = self._buf(f"{tag}_active", max(n_hit, 1), i32)
+        active.fill_(1)
+        if n_hit <= 0 or S <= 0:
+            return
+        chunk = self.chunk_size
+        G = -(-n_hit // chunk)
+        hist = self._buf("m_hist", G * (S + 1), i32)
+        hist.zero_()
+        span = self._buf("m_span", 2 * G, i32)
+        nxt = self._buf("m_next", G, i32)
+        nxt.zero_()
+        total = self._buf("m_total_fe", n_hit, torch.float32)
+        total.zero_()
+        ecnt = self._buf("m_ecnt", n_hit, i32)
+        eoff = self._buf("m_eoff", n_hit, i32)
+        m_dev = self._buf("m_count", 1, i32)
+        bound = min(G * chunk, slots)
+        lst = self._buf("m_list", bound, i32)
+        thr = float(np.float32(-math.log(self.tolerance)))
+        for step in range(S):
+            check(L.nsvf_march_step_begin(_p(hit_list), n_hit, _p(cnt), _p(active), chunk, S,
+                                          _p(hist), _p(span), _p(nxt), _p(ecnt), st),
+                  "march_step_begin")
+            self._scan(ecnt, eoff, n_hit, m_dev)
+            check(L.nsvf_march_step_list(_p(hit_list), n_hit, _p(off), chunk, _p(span), _p(ecnt),
+                                         _p(eoff), _p(lst), _p(mark), st), "march_step_list")
+            self._field(sc, ray_start, ray_dir, samples, m_dev, bound, out, lst)
+            check(L.nsvf_march_step_end(_p(hit_list), n_hit, _p(off), _p(ecnt), chunk, _p(span),
+                                        _p(samples), _p(out), _p(total), _p(active), thr, st),
+                  "march_step_end")
+            if step % 16 == 15 and int(nxt[:G].min()) >= S:
+                break
+
     def _scan(self, counts, offsets, n, total):
         ws = self._buf("scan_ws", int(lib().nsvf_scan_workspace_ints(n)), torch.int32)
         check(lib().nsvf_scan_exclusive_i32(_p(counts), _p(offsets), n, _p(ws), _p(total),
@@ -367,7 +429,7 @@ class Renderer:
         m = slots_c
         if self.hierarchical and n_hit > 0:
             sizes.update({"f_samples": (slots_f * 4, i32), "f_field": (slots_f * 4, f32)})
-            if self.reuse_coarse:
+            if self._reuse():
                 nb = int(lib().nsvf_render_eval_list_blocks(slots_f))
                 sizes.update({"f_src": (slots_f, torch.int16), "e_bcnt": (nb, i32),
                               "e_boff": (nb, i32), "e_list": (slots_f, i32),
@@ -562,7 +624,11 @@ class Renderer:
         flag, rank, c_off, c_cnt, cs, f_bound = g.flag, g.rank, g.off, g.cnt, g.samples, g.f_bound
         slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)
         c_field = self._buf("c_field", slots_c * 4, torch.float32)
-        self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
+        if self.tolerance > 0:
+            self._march(sc, ray_start, ray_dir, cs, slots_c, hit_list, n_hit, c_off, c_cnt,
+                        int(scal[2]), c_field, "c")
+        else:
+            self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
         self._mark("coarse_field")
         c_prob = self._buf("c_prob", slots_c, torch.float32)
         ray_rgb = self._buf("ray_rgb", R * 3, torch.float32)
@@ -586,7 +652,7 @@ class Renderer:
             fsm = self._stream("f_samples", slots_f)
             f_cnt = self._buf("f_cnt", R, torch.int32)
             f_field = self._buf("f_field", slots_f * 4, torch.float32)
-            reuse = self.reuse_coarse
+            reuse = self._reuse()
             src = None
             if reuse:
                 src = self._buf("f_src", slots_f, torch.int16)
@@ -610,7 +676,11 @@ class Renderer:
                 check(L.nsvf_render_eval_list(_p(src), _p(scal[3:4]), slots_f, _p(boff),
                                               _p(e_list), st), "eval_list")
             self._mark("fine_sample")
-            if reuse:
+            if self.tolerance > 0:
+                S_f = int(f_cnt.index_select(0, hit_list[:n_hit].long()).max())
+                self._march(sc, ray_start, ray_dir, fsm, slots_f, hit_list, n_hit, f_off, f_cnt,
+                            S_f, f_field, "f")
+            elif reuse:
                 self._field(sc, ray_start, ray_dir, fsm, scal[4:5], slots_f, f_field, e_list)
             else:
                 self._field(sc, ray_start, ray_dir, fsm, scal[3:4], slots_f, f_field)
@@ -647,6 +717,9 @@ class Renderer:
                 "c_samples": _columns(cs), "c_field": c_field.view(-1, 4), "c_prob": c_prob,
                 "c_max": int(scal[2]),
             }
+            if self.tolerance > 0:
+                self.kept.update({"c_eval": self._ws["c_eval"][:slots_c],
+                                  "c_active": self._ws["c_active"][:n_hit]})
             if self.hierarchical and n_hit > 0:
                 if reuse:   # the kept fine field holds every sample's value
                     check(L.nsvf_render_reuse_fill(_p(hit_list), _p(scal[0:1]), n_hit, _p(c_off),
@@ -655,6 +728,9 @@ class Renderer:
                 self.kept.update({"f_cnt": f_cnt[hl], "f_off": f_off[hl],
                                   "f_samples": _columns(fsm), "f_field": f_field.view(-1, 4),
                                   "f_evaluated": int(scal[4]) if reuse else int(f_cnt[hl].sum())})
+                if self.tolerance > 0:
+                    self.kept.update({"f_eval": self._ws["f_eval"][:slots_f],
+                                      "f_active": self._ws["f_active"][:n_hit]})
                 if self.track is not None:
                     self.kept["f_prob"] = f_prob
         return rgb, depth, missed

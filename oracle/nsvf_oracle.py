@@ -496,6 +496,62 @@ def composite(sigma, rgb, depth, dists, idx, dtype=np.float64):
     return col, out_depth, missed, probs
 
 
+def march_chunks(idx, sigma, dists, tolerance, chunk_size):
+    """Which samples the reference's chunked ray marching evaluates
+    (renderer.py:136-174, with forward():234-247 splitting the rows).
+
+    Rows (hit rays) go in groups of chunk_size; in a group, the columns of
+    the padded [N,S] samples go in chunks: a chunk ends before the column
+    whose count of hits (idx != -1, rows not yet stopped) would take the
+    chunk's total past chunk_size (a chunk holds at least one column).
+    After each chunk that evaluated something, every row adds the float32
+    sum (left to right) of its chunk's free energies relu(sigma) * dist * 7
+    (0 where not evaluated) to its running total, and rows whose total
+    exceeds float32(-log(tolerance)) stop: their later samples are not
+    evaluated (their field is 0) and no longer count toward chunk sizes.
+
+    sigma: [N,S] field sigma (cast to float32 for the decisions).  Returns
+    (evaluated [N,S] bool, stopped [N] bool, margin [N] float64: the least
+    |total - threshold| / threshold over the row's chunk ends, to tell rows
+    whose decision another implementation's rounding could flip)."""
+    idx = np.asarray(idx)
+    N, S = idx.shape
+    sig = np.asarray(sigma, np.float64).astype(np.float32)
+    dist = np.asarray(dists, np.float32)
+    fe = ((np.maximum(sig, np.float32(0)) * dist).astype(np.float32) *
+          np.float32(FREE_ENERGY_SCALE)).astype(np.float32)
+    tol = np.float32(-math.log(tolerance)) if tolerance > 0 else None
+    evaluated = np.zeros((N, S), bool)
+    stopped = np.zeros(N, bool)
+    margin = np.full(N, np.inf)
+    for r0 in range(0, N, chunk_size):
+        rows = slice(r0, min(N, r0 + chunk_size))
+        hits = (idx[rows] != -1).astype(np.int64)
+        acc = np.zeros(hits.shape[0], np.float32)
+        early = np.zeros(hits.shape[0], bool)
+        size, start = 0, 0
+        for i in range(S + 1):
+            if (i == S or size + hits[:, i].sum() > chunk_size) and i > start:
+                mask = (idx[rows, start:i] != -1) & ~early[:, None]
+                if mask.any():
+                    evaluated[rows, start:i] = mask
+                    part = np.zeros(hits.shape[0], np.float32)
+                    for j in range(i - start):
+                        part = (part + np.where(mask[:, j], fe[rows, start + j], 0)).astype(np.float32)
+                    acc = (acc + part).astype(np.float32)
+                    if tol is not None:
+                        live = ~early
+                        m = np.abs(acc.astype(np.float64) - float(tol)) / float(tol)
+                        margin[rows][live] = np.minimum(margin[rows][live], m[live])
+                        early = acc > tol
+                        hits[early] = 0
+                start, size = i, 0
+            if i < S:
+                size += hits[:, i].sum()
+        stopped[rows] = early
+    return evaluated, stopped, margin
+
+
 def composite_backward(sigma, rgb, depth, dists, idx, g_rgb, g_depth, g_missed, dtype=np.float64):
     """Gradients of composite()'s (rgb, depth, missed) with respect to sigma
     [R,S] and rgb [R,S,3], as autograd forms them through renderer.py:
@@ -815,7 +871,7 @@ def prepare_fine_bins(si, sd, sx, probs, fine_steps):
 
 def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, compat=True,
                 dtype=np.float64, coarse_probs=None, use_octree=True, subset=None,
-                noise_seed=None):
+                noise_seed=None, tolerance=0.0, chunk_size=65536):
     """Render rays [R,3] of ``scene`` (a VoxelScene).  Integer and sampling
     decisions are the FP32 restatement's; gather, MLP and compositing run in
     ``dtype``.  ``coarse_probs`` ([N_hit, S_c] float32) replaces the coarse
@@ -836,7 +892,16 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
     training): both passes draw hash_noise rows of the hit rays' frame ids,
     with seeds pass_seed(noise_seed, 0) and pass_seed(noise_seed, 1); None
     samples at noise 0.5.  Returns a dict of the outputs and
-    intermediates."""
+    intermediates.
+
+    ``tolerance`` > 0: the reference's early ray termination
+    (``--raymarching-tolerance``) with ``chunk_size`` as its
+    ``--valid-chunk-size`` in rows and samples: each pass evaluates only the
+    samples march_chunks() selects (the others have field 0); the
+    selections are returned as ``c_evaluated`` / ``f_evaluated`` with the
+    stopped rows and their decision margins."""
+    if tolerance > 0 and subset is not None:
+        raise ValueError("early termination couples the rays of a row group: no subset")
     rs = np.asarray(ray_start, np.float32)
     rd = np.asarray(ray_dir, np.float32)
     vs = np.float32(scene.voxel_size)
@@ -874,6 +939,10 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
     rsh, rdh = rs[hits][ev], rd[hits][ev]
     sig, col = _field_eval(si[ev], sd[ev], sx[ev], rsh, rdh, pts_shift, scene.feats, scene.values,
                            vs, scene.mlp, dtype)
+    if tolerance > 0:
+        e, stop, margin = march_chunks(si, sig, sx, tolerance, chunk_size)
+        sig, col = np.where(e, sig, 0), np.where(e[..., None], col, 0)
+        out.update(c_evaluated=e, c_stopped=stop, c_margin=margin)
     rgb, depth, missed, p = composite(sig, col, sd[ev], sx[ev], si[ev], dtype=dtype)
     out.update(c_sigma=sig, c_rgb=col, c_probs=p, coarse_rgb=rgb, coarse_depth=depth,
                coarse_missed=missed)
@@ -891,6 +960,10 @@ def render_view(scene, ray_start, ray_dir, hierarchical=True, fine_steps=64.0, c
         out.update(f_idx=si2, f_depth=sd2, f_dist=sx2)
         sig2, col2 = _field_eval(si2[ev], sd2[ev], sx2[ev], rsh, rdh, pts_shift, scene.feats,
                                  scene.values, vs, scene.mlp, dtype)
+        if tolerance > 0:
+            e, stop, margin = march_chunks(si2, sig2, sx2, tolerance, chunk_size)
+            sig2, col2 = np.where(e, sig2, 0), np.where(e[..., None], col2, 0)
+            out.update(f_evaluated=e, f_stopped=stop, f_margin=margin)
         rgb, depth, missed, p2 = composite(sig2, col2, sd2[ev], sx2[ev], si2[ev], dtype=dtype)
         out.update(f_sigma=sig2, f_probs=p2)
     full_rgb, full_depth, full_missed = fill_background(frame_hits, rgb, depth, missed, dtype=dtype)

@@ -995,3 +995,74 @@ def test_render_view_subset_equals_full_view():
             assert np.abs(sub[k] - full[k][sel]).max() < 1e-12
         assert np.array_equal(sub["c_idx"], full["c_idx"]) and np.array_equal(sub["f_idx"][sub["eval_rows"]],
                                                                             full["f_idx"][sub["eval_rows"]])
+
+
+# ---------------------------------------------------------------------------
+# Early ray termination (renderer.py:136-174): chunked marching
+
+def _march_case(counts, hot, S=None):
+    """idx [N,S] with counts[n] leading samples, sigma 1 at (row, col) in
+    `hot` (else 0), dists 1/7: free energy = sigma."""
+    S = S or max(counts)
+    idx = -np.ones((len(counts), S), np.int32)
+    for n, c in enumerate(counts):
+        idx[n, :c] = 1
+    sig = np.zeros(idx.shape)
+    for n, c in hot:
+        sig[n, c] = 1.0
+    return idx, sig, np.full(idx.shape, np.float32(1 / 7))
+
+
+def test_march_stops_at_chunk_ends():
+    """Two rows of 6 samples, row 0 past the threshold (-log 0.5) at column
+    2: it is evaluated to the end of the chunk holding column 2 -- chunks of
+    2 columns at chunk_size 4, of 3 at 6, of 1 at 2."""
+    idx, sig, dx = _march_case([6, 6], [(0, 2)])
+    for chunk, last in ((4, 3), (6, 2), (2, 2)):
+        e, stop, _ = O.march_chunks(idx, sig, dx, 0.5, chunk)
+        assert list(e[0]) == [c <= last for c in range(6)], chunk
+        assert e[1].all() and list(stop) == [True, False]
+
+
+def test_march_stopped_rows_leave_the_chunk_counts():
+    """Three rows of 4 samples at chunk_size 4: the first chunk is column 0
+    (3 hits; two columns would be 6).  Row 0 stops there, so the next chunk
+    holds columns 1-2 (2 + 2 hits): row 1, past the threshold at column 1,
+    is still evaluated at column 2.  Were stopped rows still counted, the
+    chunk would be column 1 alone and row 1 would stop before column 2."""
+    idx, sig, dx = _march_case([4, 4, 4], [(0, 0), (1, 1)])
+    e, stop, _ = O.march_chunks(idx, sig, dx, 0.5, 4)
+    assert list(e[0]) == [True, False, False, False]
+    assert list(e[1]) == [True, True, True, False]
+    assert e[2].all() and list(stop) == [True, True, False]
+
+
+def test_march_row_groups_and_no_tolerance():
+    """chunk_size 1: every row is its own group (forward():236-244), so a
+    stop never depends on another row; tolerance 0 evaluates everything."""
+    idx, sig, dx = _march_case([3, 2, 3], [(0, 0), (2, 1)])
+    e, stop, _ = O.march_chunks(idx, sig, dx, 0.5, 1)
+    assert list(e[0]) == [True, False, False] and list(e[1]) == [True, True, False]
+    assert list(e[2]) == [True, True, False] and list(stop) == [True, False, True]
+    e, stop, _ = O.march_chunks(idx, sig, dx, 0.0, 1)
+    assert (e == (idx != -1)).all() and not stop.any()
+
+
+def test_render_view_with_tolerance_and_one_chunk_is_unchanged():
+    """A chunk holding every sample ends only after the whole row, so early
+    termination changes nothing; a small chunk skips samples behind opaque
+    ones and changes the image by less than the tolerance allows."""
+    from _cases import shell, shell_rays
+    sc = shell()
+    rs, rd = shell_rays(width=16, seed=3)
+    a = O.render_view(sc, rs, rd, compat=False)
+    b = O.render_view(sc, rs, rd, compat=False, tolerance=1e-3, chunk_size=1 << 30)
+    for k in ("rgb", "depth", "missed"):
+        assert np.array_equal(a[k], b[k])
+    c = O.render_view(sc, rs, rd, compat=False, hierarchical=False, tolerance=1e-2, chunk_size=64)
+    d = O.render_view(sc, rs, rd, compat=False, hierarchical=False)
+    skipped = (c["c_idx"] != -1) & ~c["c_evaluated"]
+    assert skipped.sum() > 0 and c["c_stopped"].sum() > 0
+    # a stopped ray's remaining transmittance is below the tolerance, and
+    # skipped samples could only have added that much
+    assert np.abs(c["rgb"] - d["rgb"]).max() <= 1e-2 + 1e-6
