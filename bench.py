@@ -258,7 +258,30 @@ def main():
         track_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
         tracked_voxels = int((rnd.track > 0).sum())
         rnd.track = None
+        # early ray termination (--raymarching-tolerance 1e-3 at the
+        # reference's default chunk of 64 x 1024 rows / hits): the reference's
+        # chunked marching, off by default
+        ref_img = rnd.render(dsc, rs, rd, width=args.res)[0].clone()
+        et = Renderer(dev, precision=HEADLINE, hierarchical=True, compat=True, tolerance=1e-3)
+        img = et.render(dsc, rs, rd, width=args.res, keep=True)[0]
+        torch.cuda.synchronize()
+        ev = (int(et.kept["c_eval"].sum()), int(et.kept["f_eval"].sum()))
+        stopped = int((et.kept["f_active"] == 0).sum())
+        t1 = time.perf_counter()
+        for _ in range(max(1, args.steps // 4)):
+            et.render(dsc, rs, rd, width=args.res)
+        torch.cuda.synchronize()
+        et_ms = (time.perf_counter() - t1) * 1000.0 / max(1, args.steps // 4)
+        et_diff = float((img - ref_img).abs().max())
+        del et, img, ref_img
+        torch.cuda.empty_cache()
         variants = {
+            "early_termination": {"ms_per_view": round(et_ms, 3), "tolerance": 1e-3,
+                                  "chunk": 65536, "coarse_evaluated": ev[0],
+                                  "fine_evaluated": ev[1], "rays_stopped_fine": stopped,
+                                  "max_rgb_change": et_diff,
+                                  "note": "the reference's chunked marching (renderer.py:"
+                                          "136-174); off by default"},
             "from_camera": {"ms_per_view": round(cam_ms, 3),
                             "note": "rays generated inside the traversal kernel from the camera"},
             "with_voxel_stats": {"ms_per_view": round(track_ms, 3),
