@@ -788,6 +788,35 @@ constexpr bool x3_supported() {
 // m-tile pairs with the weight pairs read two ahead (as mfma_relu_layer).
 // Per output the accumulation order is k-tile 0, 1, ..., and within a
 // k-tile Wl Xh, Wh Xl, Wh Xh.
+// ReLU on the float's bits: a non-negative float is a non-negative int32 and
+// a negative one a negative int32, so max with 0 clears exactly the negative
+// values -- one integer max, where fmaxf costs a NaN-quieting max more (a NaN
+// with the sign bit clear passes through, as torch.relu passes NaN).
+__device__ __forceinline__ float relu_bits(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
+}
+
+// The epilogue of one m-tile pair and sample n-tile j: ReLU, split into hi /
+// lo BF16 operands of the next layer.
+template <int NT>
+__device__ __forceinline__ void x3_epilogue(const f32x4 (&acc)[2][NT], int j, bf16x8& oh,
+                                            bf16x8& ol) {
+  f32x4 r0, r1;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r0[i] = relu_bits(acc[0][j][i]);
+    r1[i] = relu_bits(acc[1][j][i]);
+  }
+  split_pack8(r0, r1, oh, ol);
+}
+
+// One split-BF16 layer, out = relu(W in + bias) split into hi / lo, over
+// m-tile pairs with the weight pairs read two ahead (as mfma_relu_layer).
+// Per output the accumulation order is k-tile 0, 1, ..., and within a
+// k-tile Wl Xh, Wh Xl, Wh Xh.  A pair's epilogue runs one n-tile at a time
+// between the next pair's MFMAs, so its VALU work issues while they execute;
+// the wave raises its priority for the layer so that the SIMD's other wave,
+// if it is between layers, does not take its issue slots.
 template <int KT, int MT, int NT>
 __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B, int lane, int g,
                                                  const bf16x8 (&xh)[KT][NT],
@@ -795,6 +824,7 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
                                                  bf16x8 (&oh)[MT / 2][NT],
                                                  bf16x8 (&ol)[MT / 2][NT]) {
   constexpr int kLead = 2, NF = MT * KT, NB = kLead + 1;
+  static_assert(2 * KT >= NT, "an epilogue slot per n-tile");
   bf16x8 ah[NB], al[NB];
 #pragma unroll
   for (int p = 0; p < kLead; ++p)
@@ -803,6 +833,8 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
       al[p] = A[(2 * p + 1) * 64 + lane];
     }
   f32x4 bias_next = *reinterpret_cast<const f32x4*>(B + 4 * g);
+  __builtin_amdgcn_s_setprio(1);
+  f32x4 prev[2][NT];  // the previous pair's accumulators, epilogue pending
 #pragma unroll
   for (int kt2 = 0; kt2 < MT / 2; ++kt2) {
     f32x4 acc[2][NT];
@@ -830,20 +862,19 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
         for (int j = 0; j < NT; ++j)
           acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[f % NB], xh[kt][j],
                                                                  acc[half][j], 0, 0, 0);
+        const int slot = half * KT + kt;
+        if (kt2 > 0 && slot < NT) x3_epilogue<NT>(prev, slot, oh[kt2 - 1][slot], ol[kt2 - 1][slot]);
         __builtin_amdgcn_sched_barrier(0);
       }
     }
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      f32x4 r0, r1;
+    for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        r0[i] = fmaxf(acc[0][j][i], 0.0f);
-        r1[i] = fmaxf(acc[1][j][i], 0.0f);
-      }
-      split_pack8(r0, r1, oh[kt2][j], ol[kt2][j]);
-    }
+      for (int j = 0; j < NT; ++j) prev[h][j] = acc[h][j];
   }
+#pragma unroll
+  for (int j = 0; j < NT; ++j) x3_epilogue<NT>(prev, j, oh[MT / 2 - 1][j], ol[MT / 2 - 1][j]);
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // MLP kernel of the split-BF16 path: operand tiles (hi at emb, lo at
