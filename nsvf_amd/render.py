@@ -207,8 +207,15 @@ class Renderer:
 
     def __init__(self, device, precision="bf16", hierarchical=True, fine_steps=64.0,
                  compat=True, split_field=None, reuse_coarse=True, fold_composite=True,
-                 noise_seed=None, tolerance=0.0, chunk_size=65536):
+                 noise_seed=None, tolerance=0.0, chunk_size=65536, field_chunk=None):
         self.device = torch.device(device)
+        # the split field paths stage operand tiles for every sample of a
+        # launch (128 B per sample for split BF16); field_chunk (samples)
+        # bounds that workspace by evaluating a pass in launches of at most
+        # that many samples -- same values, fewer bytes held (serving)
+        if field_chunk is not None and int(field_chunk) < 1024:
+            raise ValueError(f"field_chunk {field_chunk}: at least 1024 samples")
+        self.field_chunk = None if field_chunk is None else int(field_chunk)
         # early ray termination (--raymarching-tolerance, renderer.py:136-174):
         # > 0 evaluates each pass's field in the reference's chunks of at most
         # chunk_size rows and hits and stops rays whose free energy passes
@@ -311,19 +318,34 @@ class Renderer:
         if prec == 2 or (prec == 0 and self.uses_split_field(sc)):
             embed, mlp = ((L.nsvf_field_embed_bf16x3, L.nsvf_field_mlp_bf16x3) if prec == 2 else
                           (L.nsvf_field_embed_bf16, L.nsvf_field_mlp_bf16))
-            ws_bytes = int(L.nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, m))
+            C = m if self.field_chunk is None else min(m, self.field_chunk)
+            ws_bytes = int(L.nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, C))
             ws = self._buf("field_ws", (ws_bytes + 15) // 16 * 4, torch.int32)
-            e0 = event() if ev else None
-            check(embed(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
-                        sc.voxel_size, _p(ray_start), _p(ray_dir), _p(samples), _p(slots),
-                        _p(m_dev), m, _p(ws), ws_bytes, st), "field_embed")
-            e1 = event() if ev else None
-            check(mlp(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(m_dev), m, _p(w),
-                      _p(sc.mlp["bias"]), _p(out), _p(slots), st), "field_mlp")
-            if ev:
-                e2 = event()
-                self._kernel_events.append(("embed", m, e0, e1))
-                self._kernel_events.append(("mlp", m, e1, e2))
+            if C < m:
+                # launch k takes samples [kC, kC + C) of the m_dev: its count
+                # clamp(m_dev - kC, 0, C) computed on the device
+                K = -(-m // C)
+                ar = torch.arange(K, dtype=torch.int64, device=self.device) * C
+                cnts = (m_dev[:1].to(torch.int64) - ar).clamp_(0, C).to(torch.int32)
+            else:
+                K, cnts = 1, None
+            for k in range(K):
+                mk = m_dev if cnts is None else cnts[k:k + 1]
+                if slots is not None:
+                    lst, smp, o = slots[k * C:], samples, out
+                else:
+                    lst, smp, o = None, samples[k * C:], out[k * C * 4:]
+                e0 = event() if ev else None
+                check(embed(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
+                            sc.voxel_size, _p(ray_start), _p(ray_dir), _p(smp), _p(lst),
+                            _p(mk), C, _p(ws), ws_bytes, st), "field_embed")
+                e1 = event() if ev else None
+                check(mlp(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(mk), C, _p(w),
+                          _p(sc.mlp["bias"]), _p(o), _p(lst), st), "field_mlp")
+                if ev:
+                    e2 = event()
+                    self._kernel_events.append(("embed", C, e0, e1))
+                    self._kernel_events.append(("mlp", C, e1, e2))
             return
         e0 = event() if ev else None
         check(L.nsvf_field_eval(prec, sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats), _p(sc.values),
@@ -437,6 +459,8 @@ class Renderer:
             m = max(m, slots_f)
         prec = PRECISIONS[self.precision]
         if (prec == 2 or (prec == 0 and self.uses_split_field(sc))) and m > 0:
+            if self.field_chunk is not None:
+                m = min(m, self.field_chunk)
             ws_bytes = int(lib().nsvf_field_workspace_bytes(prec, sc.D, sc.H, sc.L, m))
             sizes["field_ws"] = ((ws_bytes + 15) // 16 * 4, i32)
         return sizes

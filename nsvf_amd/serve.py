@@ -390,7 +390,7 @@ class RenderSlot(Slot):
         self.stream = torch.cuda.Stream(device=server.device)   # default priority
         self.renderer = Renderer(server.device, precision=server.precision,
                                  hierarchical=server.hierarchical, fine_steps=server.fine_steps,
-                                 compat=server.compat)
+                                 compat=server.compat, field_chunk=server.field_chunk)
         self.index = index
         self._done = None
         self._out = None
@@ -474,16 +474,21 @@ class RenderServer:
     budget_bytes: device bytes for scenes and request workspaces (default:
     85% of the free device memory at construction).  precision: the field's
     arithmetic (default split BF16, whose per-pixel error against FP64 is
-    about 1e-5).  compat: the sampler's
+    about 1e-5).  field_chunk: samples per field launch (the split paths'
+    workspace grows with it; None = a pass in one launch).  compat: the sampler's
     reference mode (True) or its evident intent (False, the default here): in
     the reference mode a ray whose hit row is full gets one extra sample
     carrying the next ray's voxel id (sample_gpu.cu:188), so a pixel would
     depend on which other pixels share its request."""
 
     def __init__(self, device, slots=4, budget_bytes=None, precision="bf16x3", hierarchical=True,
-                 fine_steps=64.0, compat=False, max_batch=8, max_batch_rays=640_000):
+                 fine_steps=64.0, compat=False, max_batch=8, max_batch_rays=640_000,
+                 field_chunk=1 << 24):
         import torch
         self.device = torch.device(device)
+        # samples per field launch: bounds each slot's operand-tile workspace
+        # (2 GiB at split BF16 and 32-wide embeddings) whatever the request
+        self.field_chunk = field_chunk
         self.precision = precision
         self.hierarchical = hierarchical
         self.fine_steps = float(fine_steps)

@@ -343,3 +343,22 @@ def test_tiled_traversal_order_is_invisible(device):
         assert np.array_equal(a, b)
     with pytest.raises(ValueError):
         Renderer(device).render(dsc, rs, rd, width=7)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
+def test_chunked_field_launches_are_invisible(setup, device, precision):
+    """A pass evaluated in field launches of at most 3,000 samples (the
+    coarse stream by offsets, the fine pass's evaluation list by slices)
+    equals one launch per pass bit for bit, with a smaller workspace."""
+    sc, rs, rd, dsc, ref = setup
+    t = lambda a: torch.from_numpy(a).to(device)
+    outs, ws = [], []
+    for chunk in (None, 3000):
+        r = Renderer(device, precision=precision, hierarchical=True, split_field=True,
+                     field_chunk=chunk)
+        outs.append([N(x) for x in r.render(dsc, t(rs), t(rd))])
+        ws.append(r._ws["field_ws"].numel())
+        torch.cuda.synchronize()
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert ws[1] < ws[0]
