@@ -38,7 +38,16 @@ constexpr int kMaxLevels = 24;
 
 // Slab test against a box given by its corners; with mn = fp32(c - h) and
 // mx = fp32(c + h) this is the reference's test (intersect_gpu.cu:93-94
-// evaluates (c - h) - o and (c + h) - o) bit for bit.
+// evaluates (c - h) - o and (c + h) - o) bit for bit.  The reference returns
+// early (per axis: hi < f_low, lo > f_high, then f_low > f_high); here every
+// axis runs and only the final f_low > f_high decides, which is the same
+// test: f_low only grows and f_high only shrinks, so any early exit leaves
+// f_low > f_high at the end, and without one the running values are the
+// reference's.  The swap and the updates are compare-and-selects, as the
+// reference's ternaries, so an axis whose bounds come out NaN (direction 0
+// and origin on the slab plane) fails every comparison and changes nothing
+// (a min / max pair would not: it drops the NaN operand).  No branches: the
+// traversal runs one test per child and lane without divergence.
 __device__ __forceinline__ bool ray_box(const float o[3], const float inv[3], const float4& mn,
                                         const float4& mx, float* out_lo, float* out_hi) {
   float f_low = 0.0f;
@@ -47,22 +56,17 @@ __device__ __forceinline__ bool ray_box(const float o[3], const float inv[3], co
   const float a1[3] = {mx.x, mx.y, mx.z};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    float lo = (a0[a] - o[a]) * inv[a];
-    float hi = (a1[a] - o[a]) * inv[a];
-    if (hi < lo) {
-      float t = lo;
-      lo = hi;
-      hi = t;
-    }
-    if (hi < f_low) return false;
-    if (lo > f_high) return false;
+    const float l0 = (a0[a] - o[a]) * inv[a];
+    const float h0 = (a1[a] - o[a]) * inv[a];
+    const bool sw = h0 < l0;
+    const float lo = sw ? h0 : l0;
+    const float hi = sw ? l0 : h0;
     f_low = (lo > f_low) ? lo : f_low;
     f_high = (hi < f_high) ? hi : f_high;
-    if (f_low > f_high) return false;
   }
   *out_lo = f_low;
   *out_hi = f_high;
-  return true;
+  return !(f_low > f_high);
 }
 
 // A ray's hit row as the traversal fills it, with what the kernel needs
