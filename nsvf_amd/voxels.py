@@ -54,17 +54,24 @@ def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536):
 
 
 def pruning(sc: DeviceScene, th=0.5, precision="fp32", bits=16, train_stats=False,
-            max_voxel_probs=None):
+            max_voxel_probs=None, delta=None):
     """keep [N] bool (encoder.py:605-615).  From the field: 1 - min(exp(-relu
     sigma)) > th over each voxel's lattice, computed from the max as
-    1 - exp(-max) > th (exp is monotone).  With train_stats: max_voxel_probs
-    > th, the statistics a Renderer gathers in its ``track`` tensor while
-    rendering the training views (encoder.py:594-603)."""
+    1 - exp(-max) > th (exp is monotone).  delta: the opacity over an
+    interval of that length instead, 1 - exp(-max * delta) > th (the rule
+    "prune where max(1 - exp(-sigma delta)) < th"; the reference's own rule
+    has no interval, i.e. delta = 1).  With train_stats: max_voxel_probs > th,
+    the statistics a Renderer gathers in its ``track`` tensor while rendering
+    the training views (encoder.py:594-603)."""
     if train_stats:
         if max_voxel_probs is None or max_voxel_probs.numel() != sc.scene.num_voxels:
             raise ValueError("train_stats pruning needs max_voxel_probs [num_voxels]")
         return max_voxel_probs > th
     vmax = prune_scores(sc, precision, bits)
+    if delta is not None:
+        if not float(delta) > 0.0:
+            raise ValueError(f"delta {delta}: positive")
+        vmax = vmax * float(delta)
     return (1 - torch.exp(-vmax)) > th
 
 

@@ -96,3 +96,17 @@ def test_split_small_and_isolated_voxels(device):
                                         sc.voxel_size)
         assert np.array_equal(new.keys, rk) and np.array_equal(new.feats, rf)
         assert np.abs(new.values - rv).max() < 1e-6
+
+
+def test_pruning_over_an_interval(dscene):
+    """delta: keep where 1 - exp(-max relu(sigma) * delta) > th; delta = 1 is
+    the reference's rule, a short interval keeps only the densest voxels."""
+    sc, dsc = dscene
+    s = prune_scores(dsc, "fp32")
+    assert torch.equal(pruning(dsc, th=0.5, delta=1.0), pruning(dsc, th=0.5))
+    step = sc.step_size
+    k = pruning(dsc, th=0.5, delta=step)
+    assert torch.equal(k, (1 - torch.exp(-s * step)) > 0.5)
+    assert int(k.sum()) <= int(pruning(dsc, th=0.5).sum())
+    with pytest.raises(ValueError):
+        pruning(dsc, delta=0.0)
