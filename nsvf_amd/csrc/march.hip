@@ -141,6 +141,8 @@ int nsvf_march_step_begin(const int32_t* hit_list, int64_t n_hit, const int32_t*
   if (n_hit <= 0) return NSVF_OK;
   if (chunk <= 0 || S < 0)
     return set_error(NSVF_EINVAL, "march_step_begin: chunk %d, S %d", chunk, S);
+  if (!hit_list || !cnt || !active || !hist || !span || !next || !ecnt)
+    return set_error(NSVF_EINVAL, "march_step_begin: null pointer");
   const int groups = int(ceil_div(n_hit, chunk));
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(march_hist_kernel, dim3(blocks_of(n_hit)), dim3(256), 0, st, hit_list,
@@ -158,6 +160,8 @@ int nsvf_march_step_list(const int32_t* hit_list, int64_t n_hit, const int32_t* 
   clear_error();
   if (n_hit <= 0) return NSVF_OK;
   if (chunk <= 0) return set_error(NSVF_EINVAL, "march_step_list: chunk %d", chunk);
+  if (!hit_list || !off || !span || !ecnt || !eoff || !list)
+    return set_error(NSVF_EINVAL, "march_step_list: null pointer");
   hipLaunchKernelGGL(march_list_kernel, dim3(blocks_of(n_hit)), dim3(256), 0, as_stream(stream),
                      hit_list, n_hit, off, chunk, span, ecnt, eoff, list, mark);
   return check_launch("march_step_list");
@@ -170,6 +174,8 @@ int nsvf_march_step_end(const int32_t* hit_list, int64_t n_hit, const int32_t* o
   clear_error();
   if (n_hit <= 0) return NSVF_OK;
   if (chunk <= 0) return set_error(NSVF_EINVAL, "march_step_end: chunk %d", chunk);
+  if (!hit_list || !off || !ecnt || !span || !samples || !field || !total || !active)
+    return set_error(NSVF_EINVAL, "march_step_end: null pointer");
   hipLaunchKernelGGL(march_accumulate_kernel, dim3(blocks_of(n_hit)), dim3(256), 0,
                      as_stream(stream), hit_list, n_hit, off, ecnt, chunk, span,
                      static_cast<const MarchRec*>(samples), reinterpret_cast<const float4*>(field),

@@ -466,6 +466,7 @@ int nsvf_mlp_train_forward(int D, int H, int L, const float* packed, const float
                      D, H, L);
   if (M < 0) return set_error(NSVF_EINVAL, "mlp_train_forward: M=%lld", static_cast<long long>(M));
   if (M == 0) return NSVF_OK;
+  if (!packed || !emb || !out) return set_error(NSVF_EINVAL, "mlp_train_forward: null pointer");
   const int64_t tiles = ceil_div(M, 16);
   const unsigned grid = unsigned(std::min<int64_t>(ceil_div(tiles, kTrainWaves), 4096));
   hipLaunchKernelGGL((mlp_train_forward_kernel<32, 128, 3>), dim3(grid), dim3(kTrainBlock), 0,
@@ -481,6 +482,8 @@ int nsvf_mlp_train_backward(int D, int H, int L, const float* packed, const floa
     return set_error(NSVF_EINVAL, "mlp_train_backward: shape D=%d H=%d L=%d (built: 32, 128, 3)",
                      D, H, L);
   if (M < 0) return set_error(NSVF_EINVAL, "mlp_train_backward: M=%lld", static_cast<long long>(M));
+  if (!packed || !partial || !dparams || (M > 0 && (!emb || !dout || !demb)))
+    return set_error(NSVF_EINVAL, "mlp_train_backward: null pointer");
   using T = TrainDims<32, 128, 3>;
   const int nb = backward_blocks(M);
   if (partial_floats < int64_t(nb) * T::PARAMS)
