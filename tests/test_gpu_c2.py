@@ -80,8 +80,9 @@ def subset(c2):
     return sel, n_full
 
 
-def gpu_view(dsc, rs, rd, device, precision, compat, rows=True):
-    r = Renderer(device, precision=precision, hierarchical=True, compat=compat)
+def gpu_view(dsc, rs, rd, device, precision, compat, rows=True, noise_seed=None):
+    r = Renderer(device, precision=precision, hierarchical=True, compat=compat,
+                 noise_seed=noise_seed)
     out = r.render(dsc, torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device),
                    keep=True, width=RES)
     torch.cuda.synchronize()
@@ -178,6 +179,23 @@ def test_c2_fine_streams_bit_exact_given_coarse_probs(views):
     g, ref = views["g32"], views["ref_s"]
     _stream_equal(g["f_cnt"], g["f_off"], g["f_cols"], ref["f_idx"], ref["f_depth"], ref["f_dist"])
     log(f"fine streams bit-exact ({int(g['f_cnt'].sum())} samples)")
+
+
+def test_c2_noisy_streams_bit_exact(c2, subset, device):
+    """Training-mode noise on the headline view in the reference's sampler
+    mode: the coarse stream, and the fine stream given the GPU's coarse
+    weights, equal the oracle's sampler fed the same counter-based draws."""
+    sc, rs, rd, dsc = c2
+    sel, _ = subset
+    g = gpu_view(dsc, rs, rd, device, "fp32", True, rows=False, noise_seed=7)
+    W = int(g["c_cnt"].max())
+    gp = csr_to_padded(g["c_cnt"], g["c_off"], g["c_prob"], W, np.float32(0))
+    ref = O.render_view(sc, rs, rd, compat=True, subset=sel, coarse_probs=gp, noise_seed=7)
+    _stream_equal(g["c_cnt"], g["c_off"], g["c_cols"], ref["c_idx"], ref["c_depth"],
+                  ref["c_dist"])
+    _stream_equal(g["f_cnt"], g["f_off"], g["f_cols"], ref["f_idx"], ref["f_depth"], ref["f_dist"])
+    log(f"noisy streams bit-exact ({int(g['c_cnt'].sum())} coarse, {int(g['f_cnt'].sum())} fine "
+        f"samples)")
 
 
 def test_c2_fp32_within_1e5_of_fp64_same_samples(views):
