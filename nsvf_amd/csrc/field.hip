@@ -726,6 +726,9 @@ __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_
   constexpr int NT = Dm::NT;
   constexpr bool kLds = weights_in_lds<D, H, L>();
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
+  if (int64_t(blockIdx.x) * 4 >= tiles) return;  // no tile: no weight staging
   const bf16x8* lw = wfrag;
   float* lb = reinterpret_cast<float*>(smem_raw);
   if constexpr (kLds) {
@@ -737,8 +740,6 @@ __global__ __launch_bounds__(kFieldBlock, (Dims<D, H, L>::MIN_BLOCKS)) void mlp_
   for (int i = threadIdx.x; i < Dm::BIASES; i += kFieldBlock) lb[i] = bias[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-  const int64_t M = min(int64_t(*m_dev), m_max);
-  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
   const int64_t stride = int64_t(gridDim.x) * 4;
   int64_t t = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (t >= tiles) return;
@@ -888,14 +889,17 @@ __global__ __launch_bounds__(64 * kX3Waves, 1) void mlp_bf16x3_kernel(
   constexpr int NT = Dm::NT, KT0 = Dm::KT0, KTH = Dm::KTH, MTH = Dm::MTH;
   static_assert(MTH == 2 * KTH, "a layer's output m-tile pairs are the next layer's k-tiles");
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int64_t M = min(int64_t(*m_dev), m_max);
+  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
+  // a block without tiles (the device count below this launch's bound)
+  // leaves before staging the weights
+  if (int64_t(blockIdx.x) * kX3Waves >= tiles) return;
   bf16x8* lw = reinterpret_cast<bf16x8*>(smem_raw);
   for (int i = threadIdx.x; i < Dm::FRAGS * 2 * 64; i += 64 * kX3Waves) lw[i] = wfrag[i];
   float* lb = reinterpret_cast<float*>(smem_raw + size_t(Dm::FRAGS) * 2048);
   for (int i = threadIdx.x; i < Dm::BIASES; i += 64 * kX3Waves) lb[i] = bias[i];
   __syncthreads();
   const int lane = threadIdx.x & 63, g = lane >> 4, col = lane & 15;
-  const int64_t M = min(int64_t(*m_dev), m_max);
-  const int64_t tiles = (M + Dm::SAMPLES - 1) / Dm::SAMPLES;
   const int64_t stride = int64_t(gridDim.x) * kX3Waves;
   int64_t t = int64_t(blockIdx.x) * kX3Waves + (threadIdx.x >> 6);
   if (t >= tiles) return;

@@ -483,11 +483,13 @@ class RenderServer:
 
     def __init__(self, device, slots=4, budget_bytes=None, precision="bf16x3", hierarchical=True,
                  fine_steps=64.0, compat=False, max_batch=8, max_batch_rays=640_000,
-                 field_chunk=1 << 24):
+                 field_chunk=1 << 23):
         import torch
         self.device = torch.device(device)
         # samples per field launch: bounds each slot's operand-tile workspace
-        # (2 GiB at split BF16 and 32-wide embeddings) whatever the request
+        # (1 GiB at split BF16 and 32-wide embeddings) whatever the request,
+        # and keeps a launch's tiles in the last-level cache between the
+        # gather and the MLP
         self.field_chunk = field_chunk
         self.precision = precision
         self.hierarchical = hierarchical
