@@ -571,12 +571,18 @@ def rows(args, dev, sc, dsc):
     xyz = (rs.index_select(0, ray) + rd.index_select(0, ray) * rec[:, 1].view(torch.float32)[:, None]).contiguous()
     grad_emb = torch.randn((M, D), device=dev)
     K, N = dsc.values.shape[0], dsc.centers.shape[0]
+    from nsvf_amd.train import gather_backward_plan
+    plan = gather_backward_plan(dsc.feats, K)
+    ms_np = _time_ms(lambda: trilinear_gather_backward(s_idx, xyz, dsc.centers, dsc.feats, K,
+                                                       grad_emb, dsc.voxel_size), 10)
     ms = _time_ms(lambda: trilinear_gather_backward(s_idx, xyz, dsc.centers, dsc.feats, K, grad_emb,
-                                                    dsc.voxel_size), 10)
+                                                    dsc.voxel_size, plan), 10)
     nbytes = M * 4 * D + M * (4 + 12) + N * 44 + K * 4 * D
     res["gather_backward"] = {"ms": round(ms, 3), "samples": M, "algorithmic_bytes": nbytes,
                               "achieved_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
-                              "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+                              "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                              "ms_sorting_incidences_per_call": round(ms_np, 3),
+                              "note": "the scene's corner-incidence order computed once"}
     field = torch.rand((M, 4), device=dev)
     field[:, 0] = field[:, 0] * 20 - 5
     gr, gd, gm = torch.randn((R, 3), device=dev), torch.randn(R, device=dev), torch.randn(R, device=dev)

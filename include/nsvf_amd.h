@@ -213,6 +213,28 @@ int nsvf_trilinear_gather_backward(const int32_t* s_idx, const float* xyz,
                                    void* workspace, int64_t workspace_bytes,
                                    void* stream);
 
+/* The per-key order of the (voxel, corner) incidences depends only on the
+ * scene's corner keys: nsvf_trilinear_gather_backward_plan computes it once
+ * into `plan` (plan_bytes(N, K) bytes; the workspace as for the backward)
+ * and nsvf_trilinear_gather_backward_planned takes it instead of sorting
+ * the incidences on every call -- the same gradients, bit for bit.  A plan
+ * stays valid while feats, N and K do.                                     */
+int64_t nsvf_trilinear_gather_backward_plan_bytes(int64_t N, int64_t K);
+int nsvf_trilinear_gather_backward_plan(const int32_t* feats, int64_t N,
+                                        int64_t K, void* plan,
+                                        int64_t plan_bytes, void* workspace,
+                                        int64_t workspace_bytes, void* stream);
+int nsvf_trilinear_gather_backward_planned(const int32_t* s_idx,
+                                           const float* xyz, int64_t M,
+                                           const float* centers,
+                                           const int32_t* feats, int64_t N,
+                                           int64_t K, int D, float voxel_size,
+                                           const float* grad_out,
+                                           float* grad_values, const void* plan,
+                                           void* workspace,
+                                           int64_t workspace_bytes,
+                                           void* stream);
+
 /* ------------------------------------------------------------------------
  * Fused render pipeline.  No counterpart in the reference's native module:
  * these replace the Python stages of one view (fairnr/models/

@@ -45,6 +45,10 @@ SIGNATURES = {
     "nsvf_trilinear_gather_backward_workspace": ([_L, _L, _L, _I, _P], _L),
     "nsvf_trilinear_gather_backward": ([_P, _P, _L, _P, _P, _L, _L, _I, _F, _P, _P, _P, _L, _P],
                                        _I),
+    "nsvf_trilinear_gather_backward_plan_bytes": ([_L, _L], _L),
+    "nsvf_trilinear_gather_backward_plan": ([_P, _L, _L, _P, _L, _P, _L, _P], _I),
+    "nsvf_trilinear_gather_backward_planned": ([_P, _P, _L, _P, _P, _L, _L, _I, _F, _P, _P, _P,
+                                                _P, _L, _P], _I),
     "nsvf_field_prune": ([_I, _I, _I, _I, _P, _P, _P, _L, _F, _L, _L, _I, _P, _P, _P, _P], _I),
     "nsvf_split_workspace_bytes": ([_L, _P], _L),
     "nsvf_split_voxels": ([_P, _L, _F, _P, _L, _P, _P, _P, _P], _I),

@@ -15,7 +15,10 @@
 //      over the voxel's samples for corner c and features 4j.. (+32 ...),
 //      in sample order, into G[v][c] -- the per-(voxel, corner) partials;
 //   3. sort the (voxel, corner) incidences by corner key the same way, and
-//      sum each key's <= 8 partials G[v][c] in incidence order.
+//      sum each key's <= 8 partials G[v][c] in incidence order.  The
+//      incidence order depends only on the scene's corner keys, so a caller
+//      that differentiates the same scene repeatedly computes it once (a
+//      "plan": the sorted incidences and the key segments) and passes it.
 //
 // The corner weights are the forward gather's, computed with the same
 // operations, so the backward is the exact adjoint of the forward's weights.
@@ -206,12 +209,70 @@ __global__ __launch_bounds__(kBlock) void key_sum_kernel(const int32_t* __restri
   reinterpret_cast<float4*>(grad_values + k * D)[q] = acc;
 }
 
+// Plan layout: [8N] sorted incidence ids, then [K + 1] key segment starts.
+inline int64_t plan_bytes(int64_t N, int64_t K) { return align256(8 * N * 4) + (K + 1) * 4; }
+
+// Step 3's incidence order into plan (ivals: 8N ids, kseg: K + 1 starts),
+// using the workspace's incidence buffers and sort space.
+int build_plan(const int32_t* feats, int64_t N, int64_t K, const GradWorkspace& w,
+               int32_t* ivals, int32_t* kseg, hipStream_t st) {
+  hipLaunchKernelGGL(incidence_keys_kernel, dim3(unsigned(ceil_div(8 * N, kBlock))), dim3(kBlock),
+                     0, st, feats, 8 * N, int32_t(K), w.ikeys_in, w.ivals_in);
+  size_t tb = w.sort_tmp_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tb, w.ikeys_in, w.ikeys_out,
+                                                    w.ivals_in, ivals, int(8 * N), 0,
+                                                    key_bits(K), st);
+  if (e != hipSuccess)
+    return set_error(NSVF_ELAUNCH, "gather backward: incidence sort: %s", hipGetErrorString(e));
+  hipLaunchKernelGGL(segment_starts_kernel, dim3(unsigned(ceil_div(K + 1, kBlock))), dim3(kBlock),
+                     0, st, w.ikeys_out, 8 * N, K + 1, kseg);
+  return check_launch("gather backward plan");
+}
+
+int check_sizes(const char* what, int64_t M, int64_t N, int64_t K, int D) {
+  if (M < 0 || N <= 0 || K <= 0 || D <= 0) return set_error(NSVF_EINVAL, "%s: bad size", what);
+  if (D % 4 != 0 || D > 128)
+    return set_error(NSVF_EINVAL, "%s: D=%d (a multiple of 4, <= 128)", what, D);
+  if (M >= (int64_t(1) << 31) || 8 * N >= (int64_t(1) << 31) || K >= (int64_t(1) << 30))
+    return set_error(NSVF_EINVAL, "%s: size beyond 32-bit indexing", what);
+  return NSVF_OK;
+}
+
 }  // namespace
 }  // namespace nsvf
 
 using namespace nsvf;
 
 extern "C" {
+
+int64_t nsvf_trilinear_gather_backward_plan_bytes(int64_t N, int64_t K) {
+  clear_error();
+  if (N <= 0 || K <= 0) {
+    set_error(NSVF_EINVAL, "trilinear_gather_backward_plan_bytes: bad size");
+    return -1;
+  }
+  return plan_bytes(N, K);
+}
+
+int nsvf_trilinear_gather_backward_plan(const int32_t* feats, int64_t N, int64_t K, void* plan,
+                                        int64_t plan_bytes_given, void* workspace,
+                                        int64_t workspace_bytes, void* stream) {
+  clear_error();
+  if (int rc = check_sizes("trilinear_gather_backward_plan", 0, N, K, 4)) return rc;
+  if (!feats || !plan || !workspace)
+    return set_error(NSVF_EINVAL, "trilinear_gather_backward_plan: null pointer");
+  if (plan_bytes_given < plan_bytes(N, K))
+    return set_error(NSVF_EINVAL, "trilinear_gather_backward_plan: plan buffer too small");
+  hipStream_t st = as_stream(stream);
+  GradWorkspace w;
+  // the incidence buffers and sort space do not depend on M or D
+  if (int rc = layout(0, N, K, 4, st, static_cast<char*>(workspace), &w)) return rc;
+  if (workspace_bytes < w.total)
+    return set_error(NSVF_EINVAL, "trilinear_gather_backward_plan: workspace too small");
+  char* pb = static_cast<char*>(plan);
+  return build_plan(feats, N, K, w, reinterpret_cast<int32_t*>(pb),
+                    reinterpret_cast<int32_t*>(pb + align256(8 * N * 4)), st);
+}
 
 int64_t nsvf_trilinear_gather_backward_workspace(int64_t M, int64_t N, int64_t K, int D,
                                                  void* stream) {
@@ -225,18 +286,14 @@ int64_t nsvf_trilinear_gather_backward_workspace(int64_t M, int64_t N, int64_t K
   return w.total;
 }
 
-int nsvf_trilinear_gather_backward(const int32_t* s_idx, const float* xyz, int64_t M,
-                                   const float* centers, const int32_t* feats, int64_t N,
-                                   int64_t K, int D, float voxel_size, const float* grad_out,
-                                   float* grad_values, void* workspace, int64_t workspace_bytes,
-                                   void* stream) {
+int nsvf_trilinear_gather_backward_planned(const int32_t* s_idx, const float* xyz, int64_t M,
+                                           const float* centers, const int32_t* feats, int64_t N,
+                                           int64_t K, int D, float voxel_size,
+                                           const float* grad_out, float* grad_values,
+                                           const void* plan, void* workspace,
+                                           int64_t workspace_bytes, void* stream) {
   clear_error();
-  if (M < 0 || N <= 0 || K <= 0 || D <= 0)
-    return set_error(NSVF_EINVAL, "trilinear_gather_backward: bad size");
-  if (D % 4 != 0 || D > 128)
-    return set_error(NSVF_EINVAL, "trilinear_gather_backward: D=%d (a multiple of 4, <= 128)", D);
-  if (M >= (int64_t(1) << 31) || 8 * N >= (int64_t(1) << 31) || K >= (int64_t(1) << 30))
-    return set_error(NSVF_EINVAL, "trilinear_gather_backward: size beyond 32-bit indexing");
+  if (int rc = check_sizes("trilinear_gather_backward", M, N, K, D)) return rc;
   if ((M > 0 && (!s_idx || !xyz || !grad_out)) || !centers || !feats || !grad_values ||
       !workspace)
     return set_error(NSVF_EINVAL, "trilinear_gather_backward: null pointer");
@@ -266,20 +323,29 @@ int nsvf_trilinear_gather_backward(const int32_t* s_idx, const float* xyz, int64
   hipLaunchKernelGGL(voxel_corner_kernel, dim3(unsigned(ceil_div(N, kBlock / 64))), dim3(kBlock),
                      0, st, w.seg, w.vals_out, xyz, centers, voxel_size, grad_out, D, N,
                      w.partial);
-  // 3. incidences sorted by corner key, then one sum per key
-  hipLaunchKernelGGL(incidence_keys_kernel, dim3(unsigned(ceil_div(8 * N, kBlock))), dim3(kBlock),
-                     0, st, feats, 8 * N, int32_t(K), w.ikeys_in, w.ivals_in);
-  size_t tb = w.sort_tmp_bytes;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(w.sort_tmp, tb, w.ikeys_in, w.ikeys_out,
-                                                    w.ivals_in, w.ivals_out, int(8 * N), 0,
-                                                    key_bits(K), st);
-  if (e != hipSuccess)
-    return set_error(NSVF_ELAUNCH, "gather backward: incidence sort: %s", hipGetErrorString(e));
-  hipLaunchKernelGGL(segment_starts_kernel, dim3(unsigned(ceil_div(K + 1, kBlock))), dim3(kBlock),
-                     0, st, w.ikeys_out, 8 * N, K + 1, w.kseg);
+  // 3. incidences in corner-key order (the plan, or computed here), then
+  // one sum per key
+  const int32_t* ivals = w.ivals_out;
+  const int32_t* kseg = w.kseg;
+  if (plan) {
+    ivals = static_cast<const int32_t*>(plan);
+    kseg = reinterpret_cast<const int32_t*>(static_cast<const char*>(plan) + align256(8 * N * 4));
+  } else if (int rc2 = build_plan(feats, N, K, w, w.ivals_out, w.kseg, st)) {
+    return rc2;
+  }
   hipLaunchKernelGGL(key_sum_kernel, dim3(unsigned(ceil_div(K * (D / 4), kBlock))), dim3(kBlock),
-                     0, st, w.kseg, w.ivals_out, w.partial, K, D, grad_values);
+                     0, st, kseg, ivals, w.partial, K, D, grad_values);
   return check_launch("trilinear_gather_backward");
+}
+
+int nsvf_trilinear_gather_backward(const int32_t* s_idx, const float* xyz, int64_t M,
+                                   const float* centers, const int32_t* feats, int64_t N,
+                                   int64_t K, int D, float voxel_size, const float* grad_out,
+                                   float* grad_values, void* workspace, int64_t workspace_bytes,
+                                   void* stream) {
+  return nsvf_trilinear_gather_backward_planned(s_idx, xyz, M, centers, feats, N, K, D, voxel_size,
+                                                grad_out, grad_values, nullptr, workspace,
+                                                workspace_bytes, stream);
 }
 
 }  // extern "C"

@@ -138,6 +138,9 @@ class DeviceScene:
         self.max_hits = int(scene.max_hits)
         self.mlp = pack_mlp(scene.mlp, self.device)
         self.D, self.H, self.L = self.mlp["shape"]
+        # the gather backward's corner-incidence order (training), made on
+        # first use
+        self.gather_plan = None
         if not lib().nsvf_field_supported(self.D, self.H, self.L):
             raise RuntimeError(f"field shape D={self.D} H={self.H} L={self.L} not supported")
 

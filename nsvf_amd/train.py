@@ -33,7 +33,7 @@ from .scene import VoxelScene
 
 
 def _p(t):
-    return ctypes.c_void_p(t.data_ptr())
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
 
 
 def _stream(dev):
@@ -51,7 +51,7 @@ class TrilinearGather(torch.autograd.Function):
     reference (sample points come from non-differentiable sampling)."""
 
     @staticmethod
-    def forward(ctx, values, s_idx, xyz, centers, feats, voxel_size):
+    def forward(ctx, values, s_idx, xyz, centers, feats, voxel_size, plan=None):
         for t, n in ((values, "values"), (s_idx, "s_idx"), (xyz, "xyz"), (centers, "centers"),
                      (feats, "feats")):
             _need_cuda(t, n)
@@ -70,6 +70,7 @@ class TrilinearGather(torch.autograd.Function):
         ctx.save_for_backward(s_idx, xyz, centers, feats)
         ctx.shape = (K, D)
         ctx.voxel_size = float(voxel_size)
+        ctx.plan = plan
         return out
 
     @staticmethod
@@ -79,16 +80,41 @@ class TrilinearGather(torch.autograd.Function):
         grad_values = None
         if ctx.needs_input_grad[0]:
             grad_values = trilinear_gather_backward(s_idx, xyz, centers, feats, K, grad_out,
-                                                    ctx.voxel_size)
-        return grad_values, None, None, None, None, None
+                                                    ctx.voxel_size, ctx.plan)
+        return grad_values, None, None, None, None, None, None
 
 
-def trilinear_gather(values, s_idx, xyz, centers, feats, voxel_size):
-    return TrilinearGather.apply(values, s_idx, xyz, centers, feats, voxel_size)
+def trilinear_gather(values, s_idx, xyz, centers, feats, voxel_size, plan=None):
+    """plan: optional gather_backward_plan(feats, K) of the scene, reused
+    by the backward instead of sorting the corner incidences again."""
+    return TrilinearGather.apply(values, s_idx, xyz, centers, feats, voxel_size, plan)
 
 
-def trilinear_gather_backward(s_idx, xyz, centers, feats, num_keys, grad_out, voxel_size):
-    """grad_values [K,D] of the gather (deterministic; see the HIP source)."""
+def gather_backward_plan(feats, num_keys):
+    """The corner-key order of a scene's (voxel, corner) incidences (a
+    uint8 device buffer): what every backward of the gather over this scene
+    would sort again."""
+    L = lib()
+    dev = feats.device
+    N = feats.shape[0]
+    st = _stream(dev)
+    nb = int(L.nsvf_trilinear_gather_backward_plan_bytes(N, num_keys))
+    ws_bytes = int(L.nsvf_trilinear_gather_backward_workspace(0, N, num_keys, 4, st))
+    if nb < 0 or ws_bytes < 0:
+        check(-1, "trilinear_gather_backward_plan_bytes")
+    plan = torch.empty(nb, dtype=torch.uint8, device=dev)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    check(L.nsvf_trilinear_gather_backward_plan(_p(feats.contiguous()), N, num_keys, _p(plan), nb,
+                                                _p(ws), ws_bytes, st),
+          "trilinear_gather_backward_plan")
+    return plan
+
+
+def trilinear_gather_backward(s_idx, xyz, centers, feats, num_keys, grad_out, voxel_size,
+                              plan=None):
+    """grad_values [K,D] of the gather (deterministic; see the HIP source).
+    plan: gather_backward_plan(feats, num_keys), or None to sort the corner
+    incidences in this call (same result)."""
     L = lib()
     dev = grad_out.device
     grad_out = grad_out.contiguous().float()
@@ -100,9 +126,10 @@ def trilinear_gather_backward(s_idx, xyz, centers, feats, num_keys, grad_out, vo
         check(-1, "trilinear_gather_backward_workspace")
     ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=dev)
     grad_values = torch.empty((num_keys, D), dtype=torch.float32, device=dev)
-    check(L.nsvf_trilinear_gather_backward(_p(s_idx), _p(xyz), M, _p(centers), _p(feats), N,
-                                           num_keys, D, float(voxel_size), _p(grad_out),
-                                           _p(grad_values), _p(ws), nbytes, st),
+    check(L.nsvf_trilinear_gather_backward_planned(_p(s_idx), _p(xyz), M, _p(centers), _p(feats),
+                                                   N, num_keys, D, float(voxel_size),
+                                                   _p(grad_out), _p(grad_values), _p(plan),
+                                                   _p(ws), nbytes, st),
           "trilinear_gather_backward")
     return grad_values
 
@@ -315,7 +342,10 @@ class TrainRenderer:
         ray = rec[:, 3].long()
         # sample points x = o + d * z, each operation rounded (geometry.py:160-161)
         xyz = rs.index_select(0, ray) + rd.index_select(0, ray) * depth[:, None]
-        emb = trilinear_gather(field.values, s_idx, xyz, sc.centers, sc.feats, sc.voxel_size)
+        if sc.gather_plan is None:  # the scene's incidence order, once per scene
+            sc.gather_plan = gather_backward_plan(sc.feats, field.values.shape[0])
+        emb = trilinear_gather(field.values, s_idx, xyz, sc.centers, sc.feats, sc.voxel_size,
+                               sc.gather_plan)
         out = field.mlp(emb)
         rgb, dep, missed = StreamComposite.apply(out, g)
         bg = torch.tensor(BG_COLOR, dtype=torch.float32, device=rgb.device)

@@ -18,8 +18,8 @@ import nsvf_oracle as O
 from _cases import shell, shell_rays
 from nsvf_amd import scene as S
 from nsvf_amd.render import DeviceScene, Renderer
-from nsvf_amd.train import (StreamComposite, TrainableField, TrainRenderer, mse_loss,
-                            trilinear_gather, trilinear_gather_backward)
+from nsvf_amd.train import (StreamComposite, TrainableField, TrainRenderer, gather_backward_plan,
+                            mse_loss, trilinear_gather, trilinear_gather_backward)
 
 pytestmark = pytest.mark.gpu
 
@@ -71,8 +71,11 @@ def test_gather_backward_matches_oracle_and_is_deterministic(scene, device):
     args = (t(vi), t(x), dsc.centers, dsc.feats, sc.num_keys, t(G), sc.voxel_size)
     g1 = trilinear_gather_backward(*args)
     g2 = trilinear_gather_backward(*args)
+    # with the scene's precomputed incidence order: the same bits
+    g3 = trilinear_gather_backward(*args, gather_backward_plan(dsc.feats, sc.num_keys))
     torch.cuda.synchronize()
     assert torch.equal(g1, g2), "backward is not deterministic"
+    assert torch.equal(g1, g3), "the planned backward differs"
     ref = O.trilinear_backward(vi, x, cen, sc.feats, sc.num_keys, G, sc.voxel_size)
     err = np.abs(N(g1) - ref).max() / np.abs(ref).max()
     assert err < 1e-5, err
