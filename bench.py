@@ -679,8 +679,12 @@ def rows(args, dev, sc, dsc):
     res["scene_files"] = {"ms": round((t3 - t1) * 1e3, 1), "save_ms": round((t1 - t0) * 1e3, 1),
                           "load_ms": round((t2 - t1) * 1e3, 1),
                           "resident_ms": round((t3 - t2) * 1e3, 1), "voxels": loaded.num_voxels}
-    log("rows: " + ", ".join(f"{k} {v.get('ms', v.get('ms_including_transfers'))} ms"
-                              for k, v in res.items()))
+    def row_ms(v):
+        if "forward_ms" in v:
+            return f"{v['forward_ms']} + {v['backward_ms']}"
+        return v.get("ms", v.get("ms_including_transfers"))
+
+    log("rows: " + ", ".join(f"{k} {row_ms(v)} ms" for k, v in res.items()))
     return res
 
 
