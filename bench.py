@@ -645,12 +645,15 @@ def rows(args, dev, sc, dsc):
     torch.cuda.empty_cache()
     # pruning: 16^3 lattice per voxel through the field MLP, max relu(sigma)
     flops = sc.num_voxels * 4096 * sc.mlp.flops_per_sample()
-    for prec, peak in (("bf16", PEAK_BF16_TFLOPS), ("fp32", 157.3)):
+    for prec, peak in (("bf16", PEAK_BF16_TFLOPS), ("bf16x3", PEAK_BF16_TFLOPS),
+                       ("fp32", 157.3)):
         ms = _time_ms(lambda: prune_scores(dsc, prec), 2)
         tf_s = flops / (ms * 1e-3) / 1e12
+        # split BF16 runs three BF16 MFMAs per product
+        mfma_tf = tf_s * (3 if prec == "bf16x3" else 1)
         res[f"prune_{prec}"] = {"ms": round(ms, 2), "lattice_points": sc.num_voxels * 4096,
                                 "achieved_TFLOPs": round(tf_s, 1), "peak_TFLOPs": peak,
-                                "frac": round(tf_s / peak, 4)}
+                                "frac": round(mfma_tf / peak, 4)}
     # splitting (device key rebuild + interpolation; includes the uploads and
     # the readback of the new scene)
     torch.cuda.synchronize()

@@ -110,3 +110,26 @@ def test_pruning_over_an_interval(dscene):
     assert int(k.sum()) <= int(pruning(dsc, th=0.5).sum())
     with pytest.raises(ValueError):
         pruning(dsc, delta=0.0)
+
+
+def test_prune_scores_split_bf16_match_oracle(dscene):
+    """Split-BF16 pruning scores (the lattice as a sample stream through the
+    renderer's split-BF16 field kernels): within 1e-4 of the FP64 lattice
+    evaluation like the FP32 path, and the same keep decisions wherever the
+    score is farther than that from the threshold."""
+    sc, dsc = dscene
+    s3 = prune_scores(dsc, "bf16x3").cpu().numpy()
+    s32 = prune_scores(dsc, "fp32").cpu().numpy()
+    assert s3.shape == (sc.num_voxels,) and (s3 >= 0).all()
+    scale_all = np.maximum(1.0, np.abs(s32))
+    assert (np.abs(s3 - s32) <= 1e-4 * scale_all).all()
+    rng = np.random.default_rng(4)
+    sel = np.sort(rng.choice(sc.num_voxels, 96, replace=False))
+    ref = O.prune_max_sigma(O.shift_centers(sc.points, sc.voxel_size), sc.feats, sc.values,
+                            sc.voxel_size, sc.mlp, sel)
+    scale = np.maximum(1.0, np.abs(ref))
+    assert (np.abs(s3[sel] - ref) <= 1e-4 * scale).all()
+    clear = np.abs(ref - LN2) > 1e-4 * scale
+    keep_ref = (1 - np.exp(-ref)) > 0.5
+    keep = (1 - np.exp(-s3[sel].astype(np.float64))) > 0.5
+    assert np.array_equal(keep[clear], keep_ref[clear])
