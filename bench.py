@@ -258,11 +258,12 @@ def main():
         track_ms = (time.perf_counter() - t1) * 1000.0 / args.steps
         tracked_voxels = int((rnd.track > 0).sum())
         rnd.track = None
-        # early ray termination (--raymarching-tolerance 1e-3 at the
-        # reference's default chunk of 64 x 1024 rows / hits): the reference's
-        # chunked marching, off by default
+        # early ray termination as the reference's README renders with it
+        # (raymarching_tolerance 0.01, chunk_size 512: chunks of 512 x 1024
+        # rows / hits); off by default
         ref_img = rnd.render(dsc, rs, rd, width=args.res)[0].clone()
-        et = Renderer(dev, precision=HEADLINE, hierarchical=True, compat=True, tolerance=1e-3)
+        et = Renderer(dev, precision=HEADLINE, hierarchical=True, compat=True, tolerance=0.01,
+                      chunk_size=512 * 1024)
         img = et.render(dsc, rs, rd, width=args.res, keep=True)[0]
         torch.cuda.synchronize()
         ev = (int(et.kept["c_eval"].sum()), int(et.kept["f_eval"].sum()))
@@ -276,8 +277,8 @@ def main():
         del et, img, ref_img
         torch.cuda.empty_cache()
         variants = {
-            "early_termination": {"ms_per_view": round(et_ms, 3), "tolerance": 1e-3,
-                                  "chunk": 65536, "coarse_evaluated": ev[0],
+            "early_termination": {"ms_per_view": round(et_ms, 3), "tolerance": 0.01,
+                                  "chunk": 512 * 1024, "coarse_evaluated": ev[0],
                                   "fine_evaluated": ev[1], "rays_stopped_fine": stopped,
                                   "max_rgb_change": et_diff,
                                   "note": "the reference's chunked marching (renderer.py:"

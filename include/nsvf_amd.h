@@ -548,33 +548,36 @@ int nsvf_render_composite_backward(const int32_t* hit_list,
  * at most `chunk` hits of rows still marching (a chunk holds at least one
  * column); after each chunk a row's float32 total of free energy
  * relu(sigma) * dist * 7 grows by its chunk's sum (left to right), and a row
- * whose total exceeds `threshold` (= float32(-log(tolerance))) stops.
- * One step = one chunk of every group:
- *   nsvf_march_step_begin  plans the chunk: ecnt[n] = the row's samples in
- *                          it (0 for stopped rows and finished groups);
- *   (caller)               eoff = exclusive scan of ecnt (nsvf_scan_*);
- *   nsvf_march_step_list   the slots to evaluate -> list[eoff[n] + t]
- *                          (mark[slot] = 1 if mark is not NULL);
- *   (caller)               the field over the list (nsvf_field_* with it);
- *   nsvf_march_step_end    totals and stop flags from the field (float4
- *                          per slot, sigma first).
- * State on the device: active[n_hit] (1 at the start), total[n_hit] (0),
- * hist[groups * (S + 1)] (0; S = the pass's longest row), span[2 * groups],
- * next[groups] (0); groups = ceil(n_hit / chunk).  At most S steps; a group
- * is done when next[g] == S.                                               */
-int nsvf_march_step_begin(const int32_t* hit_list, int64_t n_hit,
-                          const int32_t* cnt, const int32_t* active, int chunk,
-                          int S, int32_t* hist, int32_t* span, int32_t* next,
-                          int32_t* ecnt, void* stream);
-int nsvf_march_step_list(const int32_t* hit_list, int64_t n_hit,
-                         const int32_t* off, int chunk, const int32_t* span,
-                         const int32_t* ecnt, const int32_t* eoff,
-                         int32_t* list, int8_t* mark, void* stream);
-int nsvf_march_step_end(const int32_t* hit_list, int64_t n_hit,
-                        const int32_t* off, const int32_t* ecnt, int chunk,
-                        const int32_t* span, const void* samples,
-                        const float* field, float* total, int32_t* active,
-                        float threshold, void* stream);
+ * whose total exceeds `threshold` (= float32(-log(tolerance))) stops: the
+ * reference evaluates none of its later samples.
+ * The entry points take the pass's field with every sample evaluated
+ * (float4 per slot, sigma first; a sample's value does not depend on the
+ * others) and replay the marching:
+ *   nsvf_march_begin  active[n] = 1, total[n] = 0, hist[g][c] = rows of
+ *                     group g with c samples (hist zeroed by the caller,
+ *                     nsvf_march_hist_ints ints; groups = ceil(n_hit /
+ *                     chunk), S = the pass's longest row);
+ *   nsvf_march_step   one chunk of every group (span[2 groups] and next
+ *                     [groups], next zeroed before the first step): the
+ *                     chunk's end column from the histogram, then each
+ *                     marching row's chunk sum, its samples marked
+ *                     (mark[slot] = 1, mark zeroed by the caller) and the
+ *                     stop test; at most S steps, a group is done once
+ *                     next[g] == S;
+ *   nsvf_march_end    zeroes the field of the unmarked samples, which is
+ *                     what the reference's compositing then sees.       */
+int64_t nsvf_march_hist_ints(int64_t n_hit, int chunk, int S);
+int nsvf_march_begin(const int32_t* hit_list, int64_t n_hit, const int32_t* cnt,
+                     int chunk, int S, int32_t* active, float* total,
+                     int32_t* hist, void* stream);
+int nsvf_march_step(const int32_t* hit_list, int64_t n_hit, const int32_t* off,
+                    const int32_t* cnt, const void* samples, const float* field,
+                    int chunk, int S, float threshold, int32_t* active,
+                    float* total, int32_t* hist, int32_t* span, int32_t* next,
+                    int8_t* mark, void* stream);
+int nsvf_march_end(const int32_t* hit_list, int64_t n_hit, const int32_t* off,
+                   const int32_t* cnt, const int8_t* mark, float* field,
+                   void* stream);
 
 /* The field MLP for training (fairnr/modules/field.py:217-279: L Linear +
  * ReLU layers of width H over D inputs, a 4-wide head; output (sigma,

@@ -64,9 +64,10 @@ SIGNATURES = {
     "nsvf_render_sample_fine": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _F, _I, _P,
                                  _P, _P, _P, _I, _P, _L, _P], _I),
     "nsvf_render_eval_list_blocks": ([_L], _L),
-    "nsvf_march_step_begin": ([_P, _L, _P, _P, _I, _I, _P, _P, _P, _P, _P], _I),
-    "nsvf_march_step_list": ([_P, _L, _P, _I, _P, _P, _P, _P, _P, _P], _I),
-    "nsvf_march_step_end": ([_P, _L, _P, _P, _I, _P, _P, _P, _P, _P, _F, _P], _I),
+    "nsvf_march_hist_ints": ([_L, _I, _I], _L),
+    "nsvf_march_begin": ([_P, _L, _P, _I, _I, _P, _P, _P, _P], _I),
+    "nsvf_march_step": ([_P, _L, _P, _P, _P, _P, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_march_end": ([_P, _L, _P, _P, _P, _P, _P], _I),
     "nsvf_mlp_train_supported": ([_I, _I, _I], _I),
     "nsvf_mlp_train_packed_floats": ([_I, _I, _I], _L),
     "nsvf_mlp_train_param_floats": ([_I, _I, _I], _L),

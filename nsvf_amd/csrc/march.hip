@@ -1,19 +1,23 @@
-// Early ray termination as the reference's chunked ray marching does it
+// Early ray termination as the reference's chunked ray marching decides it
 // (fairnr/modules/renderer.py:136-174, with forward():234-247 splitting the
 // rows): the hit rays of a pass go in groups of `chunk` rows; within a
-// group the sample columns go in chunks of at most `chunk` hits; after each
-// chunk every row adds its chunk's free energy to a running total, and rows
-// past -log(tolerance) stop -- their later samples are not evaluated and no
-// longer count toward chunk sizes.
+// group the sample columns go in chunks of at most `chunk` hits of rows
+// still marching; after each chunk every row adds its chunk's free energy
+// to a running total, and rows past -log(tolerance) stop -- their later
+// samples are not evaluated (their field is 0) and no longer count toward
+// chunk sizes.
 //
-// One step advances every group by one chunk, in six launches and with no
-// host synchronisation: a histogram of the active rows' sample counts per
-// group, a plan (one thread per group: the chunk's end column from the
-// histogram), the chunk's per-row sample counts, their scan (scan.hip), the
-// evaluation list, and -- after the field kernels ran over the list -- the
-// free-energy totals and stop flags.  Row n of a pass is hit ray
-// hit_list[n]; its samples are stream slots off[r], off[r] + 1, ... (column
-// j = slot off[r] + j).
+// The outputs depend only on which samples are evaluated and on their
+// values, and a sample's field value does not depend on the others.  So the
+// field runs once over every sample of the pass (one launch at full
+// occupancy, where the reference runs a forward pass per chunk), then the
+// marching is replayed in steps of two small launches -- each group's next
+// chunk end from its histogram of marching rows (one thread per group),
+// then every marching row's chunk sum and stop test (one thread per row,
+// stopped rows leaving the histogram) -- marking the samples the reference
+// evaluates; finally the field of the others is zeroed.  Row n of a pass is
+// hit ray hit_list[n]; its samples are stream slots off[r], off[r] + 1, ...
+// (column j = slot off[r] + j).
 #include <algorithm>
 
 #include "common.h"
@@ -28,31 +32,36 @@ struct MarchRec {  // a stream record {voxel id, depth, length, ray}
   int32_t ray;
 };
 
-__global__ __launch_bounds__(256) void march_hist_kernel(const int32_t* __restrict__ hit_list,
+// Every row marching, total 0; hist[g][c] = rows of group g with c samples
+// (hist zeroed by the caller).
+__global__ __launch_bounds__(256) void march_init_kernel(const int32_t* __restrict__ hit_list,
                                                          int64_t n_hit,
                                                          const int32_t* __restrict__ cnt,
-                                                         const int32_t* __restrict__ active,
                                                          int chunk, int S,
+                                                         int32_t* __restrict__ active,
+                                                         float* __restrict__ total,
                                                          int32_t* __restrict__ hist) {
   const int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (n >= n_hit || !active[n]) return;
-  const int c = min(cnt[hit_list[n]], S);
-  atomicAdd(&hist[(n / chunk) * (S + 1) + c], 1);
+  if (n >= n_hit) return;
+  active[n] = 1;
+  total[n] = 0.0f;
+  atomicAdd(&hist[(n / chunk) * (S + 1) + min(cnt[hit_list[n]], S)], 1);
 }
 
-// span[g] = [start, end) of group g's next chunk (start == end == S when the
-// group is done); next[g] advances to end.  hist[g] is consumed and cleared.
-__global__ __launch_bounds__(256) void march_plan_kernel(int32_t* __restrict__ hist, int groups,
-                                                         int S, int64_t chunk,
+// span[g] = [start, end) of group g's next chunk ([S, S) once the group is
+// done); next[g] advances to end.  A chunk ends before the column that would
+// take it past `chunk` hits (hits in column i = marching rows with more than
+// i samples) and holds at least one column.
+__global__ __launch_bounds__(256) void march_plan_kernel(const int32_t* __restrict__ hist,
+                                                         int groups, int S, int64_t chunk,
                                                          int32_t* __restrict__ span,
                                                          int32_t* __restrict__ next) {
   const int g = blockIdx.x * 256 + threadIdx.x;
   if (g >= groups) return;
-  int32_t* h = hist + int64_t(g) * (S + 1);
+  const int32_t* h = hist + int64_t(g) * (S + 1);
   const int start = next[g];
   int end = S;
   if (start < S) {
-    // hits in column i = active rows with more than i samples
     int64_t col = 0;
     for (int c = start + 1; c <= S; ++c) col += h[c];
     int64_t size = 0;
@@ -63,69 +72,61 @@ __global__ __launch_bounds__(256) void march_plan_kernel(int32_t* __restrict__ h
       col -= h[i + 1];
     }
     end = i;
-  } else {
-    end = S;
   }
   span[2 * g] = start < S ? start : S;
   span[2 * g + 1] = end;
   next[g] = end;
-  for (int c = 0; c <= S; ++c) h[c] = 0;
 }
 
-__global__ __launch_bounds__(256) void march_counts_kernel(const int32_t* __restrict__ hit_list,
-                                                           int64_t n_hit,
-                                                           const int32_t* __restrict__ cnt,
-                                                           const int32_t* __restrict__ active,
-                                                           int chunk,
-                                                           const int32_t* __restrict__ span,
-                                                           int32_t* __restrict__ ecnt) {
-  const int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (n >= n_hit) return;
-  const int64_t g = n / chunk;
-  const int s = span[2 * g], e = min(span[2 * g + 1], cnt[hit_list[n]]);
-  ecnt[n] = (active[n] && e > s) ? e - s : 0;
-}
-
-__global__ __launch_bounds__(256) void march_list_kernel(
-    const int32_t* __restrict__ hit_list, int64_t n_hit, const int32_t* __restrict__ off,
-    int chunk, const int32_t* __restrict__ span, const int32_t* __restrict__ ecnt,
-    const int32_t* __restrict__ eoff, int32_t* __restrict__ list, int8_t* __restrict__ mark) {
-  const int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (n >= n_hit) return;
-  const int c = ecnt[n];
-  if (c == 0) return;
-  const int32_t base = off[hit_list[n]] + span[2 * (n / chunk)];
-  const int32_t o = eoff[n];
-  for (int t = 0; t < c; ++t) {
-    list[o + t] = base + t;
-    if (mark) mark[base + t] = 1;
-  }
-}
-
-// Row totals += the chunk's free energies relu(sigma) * dist * 7, added left
-// to right in float32 (renderer.py:119-121,162-168); rows past `threshold`
-// stop.
+// Each marching row: its chunk's free energies relu(sigma) * dist * 7 added
+// left to right in float32 (renderer.py:119-121,162-168), the samples
+// marked, then the stop test; a row that stops leaves its group's
+// histogram.
 __global__ __launch_bounds__(256) void march_accumulate_kernel(
     const int32_t* __restrict__ hit_list, int64_t n_hit, const int32_t* __restrict__ off,
-    const int32_t* __restrict__ ecnt, int chunk, const int32_t* __restrict__ span,
-    const MarchRec* __restrict__ samples, const float4* __restrict__ field,
-    float* __restrict__ total, int32_t* __restrict__ active, float threshold) {
+    const int32_t* __restrict__ cnt, const MarchRec* __restrict__ samples,
+    const float4* __restrict__ field, int chunk, int S, const int32_t* __restrict__ span,
+    float threshold, int32_t* __restrict__ active, float* __restrict__ total,
+    int32_t* __restrict__ hist, int8_t* __restrict__ mark) {
   const int64_t n = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (n >= n_hit) return;
-  const int c = ecnt[n];
-  if (c == 0) return;
-  const int64_t base = int64_t(off[hit_list[n]]) + span[2 * (n / chunk)];
+  if (n >= n_hit || !active[n]) return;
+  const int64_t g = n / chunk;
+  const int start = span[2 * g];
+  const int32_t r = hit_list[n];
+  const int c = cnt[r];
+  const int e = min(span[2 * g + 1], c);
+  if (e <= start) return;
+  const int64_t base = off[r];
   float part = 0.0f;
-  for (int t = 0; t < c; ++t) {
-    const float sg = field[base + t].x;
-    part = part + ((sg > 0.0f ? sg : 0.0f) * samples[base + t].dist) * 7.0f;
+  for (int j = start; j < e; ++j) {
+    const float sg = field[base + j].x;
+    part = part + ((sg > 0.0f ? sg : 0.0f) * samples[base + j].dist) * 7.0f;
+    mark[base + j] = 1;
   }
   const float acc = total[n] + part;
   total[n] = acc;
-  if (acc > threshold) active[n] = 0;
+  if (acc > threshold) {
+    active[n] = 0;
+    atomicSub(&hist[g * (S + 1) + min(c, S)], 1);
+  }
 }
 
-inline unsigned blocks_of(int64_t n) { return unsigned(std::max<int64_t>(1, ceil_div(n, 256))); }
+// field[slot] = 0 for the slots of the pass's rows that were not marked.
+__global__ __launch_bounds__(256) void march_zero_kernel(const int32_t* __restrict__ hit_list,
+                                                         int64_t n_hit,
+                                                         const int32_t* __restrict__ off,
+                                                         const int32_t* __restrict__ cnt,
+                                                         const int8_t* __restrict__ mark,
+                                                         float4* __restrict__ field) {
+  // one wave per row, lanes over its samples
+  const int64_t n = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (n >= n_hit) return;
+  const int32_t r = hit_list[n];
+  const int64_t base = off[r];
+  const int c = cnt[r];
+  for (int j = threadIdx.x & 63; j < c; j += 64)
+    if (!mark[base + j]) field[base + j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
 
 }  // namespace
 }  // namespace nsvf
@@ -134,53 +135,54 @@ using namespace nsvf;
 
 extern "C" {
 
-int nsvf_march_step_begin(const int32_t* hit_list, int64_t n_hit, const int32_t* cnt,
-                          const int32_t* active, int chunk, int S, int32_t* hist, int32_t* span,
-                          int32_t* next, int32_t* ecnt, void* stream) {
+int64_t nsvf_march_hist_ints(int64_t n_hit, int chunk, int S) {
+  if (n_hit < 0 || chunk <= 0 || S < 0) return -1;
+  return ceil_div(std::max<int64_t>(n_hit, 1), chunk) * (S + 1);
+}
+
+int nsvf_march_begin(const int32_t* hit_list, int64_t n_hit, const int32_t* cnt, int chunk,
+                     int S, int32_t* active, float* total, int32_t* hist, void* stream) {
   clear_error();
   if (n_hit <= 0) return NSVF_OK;
-  if (chunk <= 0 || S < 0)
-    return set_error(NSVF_EINVAL, "march_step_begin: chunk %d, S %d", chunk, S);
-  if (!hit_list || !cnt || !active || !hist || !span || !next || !ecnt)
-    return set_error(NSVF_EINVAL, "march_step_begin: null pointer");
-  const int groups = int(ceil_div(n_hit, chunk));
+  if (chunk <= 0 || S < 0) return set_error(NSVF_EINVAL, "march_begin: chunk %d, S %d", chunk, S);
+  if (!hit_list || !cnt || !active || !total || !hist)
+    return set_error(NSVF_EINVAL, "march_begin: null pointer");
+  hipLaunchKernelGGL(march_init_kernel, dim3(unsigned(ceil_div(n_hit, 256))), dim3(256), 0,
+                     as_stream(stream), hit_list, n_hit, cnt, chunk, S, active, total, hist);
+  return check_launch("march_begin");
+}
+
+int nsvf_march_step(const int32_t* hit_list, int64_t n_hit, const int32_t* off,
+                    const int32_t* cnt, const void* samples, const float* field, int chunk, int S,
+                    float threshold, int32_t* active, float* total, int32_t* hist, int32_t* span,
+                    int32_t* next, int8_t* mark, void* stream) {
+  clear_error();
+  if (n_hit <= 0) return NSVF_OK;
+  if (chunk <= 0 || S < 0) return set_error(NSVF_EINVAL, "march_step: chunk %d, S %d", chunk, S);
+  if (!hit_list || !off || !cnt || !samples || !field || !active || !total || !hist || !span ||
+      !next || !mark)
+    return set_error(NSVF_EINVAL, "march_step: null pointer");
+  const int64_t groups = ceil_div(n_hit, chunk);
   hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL(march_hist_kernel, dim3(blocks_of(n_hit)), dim3(256), 0, st, hit_list,
-                     n_hit, cnt, active, chunk, S, hist);
-  hipLaunchKernelGGL(march_plan_kernel, dim3(blocks_of(groups)), dim3(256), 0, st, hist, groups,
-                     S, int64_t(chunk), span, next);
-  hipLaunchKernelGGL(march_counts_kernel, dim3(blocks_of(n_hit)), dim3(256), 0, st, hit_list,
-                     n_hit, cnt, active, chunk, span, ecnt);
-  return check_launch("march_step_begin");
+  hipLaunchKernelGGL(march_plan_kernel, dim3(unsigned(ceil_div(groups, 256))), dim3(256), 0, st,
+                     hist, int(groups), S, int64_t(chunk), span, next);
+  hipLaunchKernelGGL(march_accumulate_kernel, dim3(unsigned(ceil_div(n_hit, 256))), dim3(256), 0,
+                     st, hit_list, n_hit, off, cnt, static_cast<const MarchRec*>(samples),
+                     reinterpret_cast<const float4*>(field), chunk, S, span, threshold, active,
+                     total, hist, mark);
+  return check_launch("march_step");
 }
 
-int nsvf_march_step_list(const int32_t* hit_list, int64_t n_hit, const int32_t* off, int chunk,
-                         const int32_t* span, const int32_t* ecnt, const int32_t* eoff,
-                         int32_t* list, int8_t* mark, void* stream) {
+int nsvf_march_end(const int32_t* hit_list, int64_t n_hit, const int32_t* off, const int32_t* cnt,
+                   const int8_t* mark, float* field, void* stream) {
   clear_error();
   if (n_hit <= 0) return NSVF_OK;
-  if (chunk <= 0) return set_error(NSVF_EINVAL, "march_step_list: chunk %d", chunk);
-  if (!hit_list || !off || !span || !ecnt || !eoff || !list)
-    return set_error(NSVF_EINVAL, "march_step_list: null pointer");
-  hipLaunchKernelGGL(march_list_kernel, dim3(blocks_of(n_hit)), dim3(256), 0, as_stream(stream),
-                     hit_list, n_hit, off, chunk, span, ecnt, eoff, list, mark);
-  return check_launch("march_step_list");
-}
-
-int nsvf_march_step_end(const int32_t* hit_list, int64_t n_hit, const int32_t* off,
-                        const int32_t* ecnt, int chunk, const int32_t* span, const void* samples,
-                        const float* field, float* total, int32_t* active, float threshold,
-                        void* stream) {
-  clear_error();
-  if (n_hit <= 0) return NSVF_OK;
-  if (chunk <= 0) return set_error(NSVF_EINVAL, "march_step_end: chunk %d", chunk);
-  if (!hit_list || !off || !ecnt || !span || !samples || !field || !total || !active)
-    return set_error(NSVF_EINVAL, "march_step_end: null pointer");
-  hipLaunchKernelGGL(march_accumulate_kernel, dim3(blocks_of(n_hit)), dim3(256), 0,
-                     as_stream(stream), hit_list, n_hit, off, ecnt, chunk, span,
-                     static_cast<const MarchRec*>(samples), reinterpret_cast<const float4*>(field),
-                     total, active, threshold);
-  return check_launch("march_step_end");
+  if (!hit_list || !off || !cnt || !mark || !field)
+    return set_error(NSVF_EINVAL, "march_end: null pointer");
+  hipLaunchKernelGGL(march_zero_kernel, dim3(unsigned(ceil_div(n_hit, 4))), dim3(256), 0,
+                     as_stream(stream), hit_list, n_hit, off, cnt, mark,
+                     reinterpret_cast<float4*>(field));
+  return check_launch("march_end");
 }
 
 }  // extern "C"

@@ -220,9 +220,10 @@ class Renderer:
             raise ValueError(f"field_chunk {field_chunk}: at least 1024 samples")
         self.field_chunk = None if field_chunk is None else int(field_chunk)
         # early ray termination (--raymarching-tolerance, renderer.py:136-174):
-        # > 0 evaluates each pass's field in the reference's chunks of at most
-        # chunk_size rows and hits and stops rays whose free energy passes
-        # -log(tolerance); 0 (the reference's default) evaluates every sample
+        # > 0 keeps, in each pass, the field of exactly the samples the
+        # reference's marching in chunks of at most chunk_size rows and hits
+        # evaluates (rays stop once their free energy passes -log(tolerance))
+        # and zeroes the others; 0 (the reference's default) keeps them all
         if not tolerance >= 0.0 or tolerance >= 1.0:
             raise ValueError(f"tolerance {tolerance}: in [0, 1)")
         if int(chunk_size) <= 0 or int(chunk_size) >= 1 << 31:
@@ -363,18 +364,17 @@ class Renderer:
         chunks decide which fine samples are evaluated)."""
         return self.reuse_coarse and self.tolerance <= 0.0
 
-    def _march(self, sc, ray_start, ray_dir, samples, slots, hit_list, n_hit, off, cnt, S, out,
-               tag):
-        """A pass's field with early ray termination: every slot of `out`
-        zeroed, then the chunks of the reference's marching evaluated in
-        steps (one chunk per row group each; csrc/march.hip).  off / cnt:
-        the pass's frame-indexed slot offsets and counts; S: its longest
-        row.  Marks the evaluated slots in the `{tag}_eval` buffer and the
-        rows still marching in `{tag}_active`."""
+    def _march(self, samples, slots, hit_list, n_hit, off, cnt, S, out, tag):
+        """Early ray termination over a pass whose field `out` holds every
+        sample's value: replays the reference's chunked marching (chunks of
+        at most chunk_size rows and hits, csrc/march.hip), marks the samples
+        it evaluates in the `{tag}_eval` buffer and zeroes the field of the
+        others; `{tag}_active` ends with the rows still marching.  off /
+        cnt: the pass's frame-indexed slot offsets and counts; S: its
+        longest row."""
         L = lib()
-        st = _stream(self.device)
         i32 = torch.int32
-        out.zero_()
+        st = _stream(self.device)
         mark = self._buf(f"{tag}_eval", max(slots, 1), torch.int8)
         mark.zero_()
         active = self._buf(f"{tag}_active", max(n_hit, 1), i32)
@@ -383,32 +383,25 @@ class Renderer:
             return
         chunk = self.chunk_size
         G = -(-n_hit // chunk)
-        hist = self._buf("m_hist", G * (S + 1), i32)
+        hist = self._buf("m_hist", int(L.nsvf_march_hist_ints(n_hit, chunk, S)), i32)
         hist.zero_()
         span = self._buf("m_span", 2 * G, i32)
         nxt = self._buf("m_next", G, i32)
-        nxt.zero_()
+        nxt[:G].zero_()
         total = self._buf("m_total_fe", n_hit, torch.float32)
-        total.zero_()
-        ecnt = self._buf("m_ecnt", n_hit, i32)
-        eoff = self._buf("m_eoff", n_hit, i32)
-        m_dev = self._buf("m_count", 1, i32)
-        bound = min(G * chunk, slots)
-        lst = self._buf("m_list", bound, i32)
         thr = float(np.float32(-math.log(self.tolerance)))
+        check(L.nsvf_march_begin(_p(hit_list), n_hit, _p(cnt), chunk, S, _p(active), _p(total),
+                                 _p(hist), st), "march_begin")
+        # at most S chunks per group; the host looks every 16 steps whether
+        # every group is done
         for step in range(S):
-            check(L.nsvf_march_step_begin(_p(hit_list), n_hit, _p(cnt), _p(active), chunk, S,
-                                          _p(hist), _p(span), _p(nxt), _p(ecnt), st),
-                  "march_step_begin")
-            self._scan(ecnt, eoff, n_hit, m_dev)
-            check(L.nsvf_march_step_list(_p(hit_list), n_hit, _p(off), chunk, _p(span), _p(ecnt),
-                                         _p(eoff), _p(lst), _p(mark), st), "march_step_list")
-            self._field(sc, ray_start, ray_dir, samples, m_dev, bound, out, lst)
-            check(L.nsvf_march_step_end(_p(hit_list), n_hit, _p(off), _p(ecnt), chunk, _p(span),
-                                        _p(samples), _p(out), _p(total), _p(active), thr, st),
-                  "march_step_end")
+            check(L.nsvf_march_step(_p(hit_list), n_hit, _p(off), _p(cnt), _p(samples), _p(out),
+                                    chunk, S, thr, _p(active), _p(total), _p(hist), _p(span),
+                                    _p(nxt), _p(mark), st), "march_step")
             if step % 16 == 15 and int(nxt[:G].min()) >= S:
                 break
+        check(L.nsvf_march_end(_p(hit_list), n_hit, _p(off), _p(cnt), _p(mark), _p(out), st),
+              "march_end")
 
     def _scan(self, counts, offsets, n, total):
         ws = self._buf("scan_ws", int(lib().nsvf_scan_workspace_ints(n)), torch.int32)
@@ -651,11 +644,9 @@ class Renderer:
         flag, rank, c_off, c_cnt, cs, f_bound = g.flag, g.rank, g.off, g.cnt, g.samples, g.f_bound
         slots_f = slots_c + n_hit * (int(math.ceil(fs)) + 2)
         c_field = self._buf("c_field", slots_c * 4, torch.float32)
+        self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
         if self.tolerance > 0:
-            self._march(sc, ray_start, ray_dir, cs, slots_c, hit_list, n_hit, c_off, c_cnt,
-                        int(scal[2]), c_field, "c")
-        else:
-            self._field(sc, ray_start, ray_dir, cs, scal[1:2], slots_c, c_field)
+            self._march(cs, slots_c, hit_list, n_hit, c_off, c_cnt, int(scal[2]), c_field, "c")
         self._mark("coarse_field")
         c_prob = self._buf("c_prob", slots_c, torch.float32)
         ray_rgb = self._buf("ray_rgb", R * 3, torch.float32)
@@ -703,14 +694,13 @@ class Renderer:
                 check(L.nsvf_render_eval_list(_p(src), _p(scal[3:4]), slots_f, _p(boff),
                                               _p(e_list), st), "eval_list")
             self._mark("fine_sample")
-            if self.tolerance > 0:
-                S_f = int(f_cnt.index_select(0, hit_list[:n_hit].long()).max())
-                self._march(sc, ray_start, ray_dir, fsm, slots_f, hit_list, n_hit, f_off, f_cnt,
-                            S_f, f_field, "f")
-            elif reuse:
+            if reuse:
                 self._field(sc, ray_start, ray_dir, fsm, scal[4:5], slots_f, f_field, e_list)
             else:
                 self._field(sc, ray_start, ray_dir, fsm, scal[3:4], slots_f, f_field)
+            if self.tolerance > 0:
+                S_f = int(f_cnt.index_select(0, hit_list[:n_hit].long()).max())
+                self._march(fsm, slots_f, hit_list, n_hit, f_off, f_cnt, S_f, f_field, "f")
             self._mark("fine_field")
             f_prob = self._buf("f_prob", slots_f, torch.float32) if self.track is not None else None
             if reuse:

@@ -126,7 +126,7 @@ def test_tree_depth_of_packed_trees():
 
 def test_new_entry_points_reject_bad_arguments():
     """Argument checks that return before any device work (no GPU needed):
-    the training MLP's built shape and pointers, the marching steps' chunk
+    the training MLP's built shape and pointers, the marching entry points' chunk
     and pointers, and the samplers' 32-bit noise seeds."""
     L = _lib.lib()
     assert L.nsvf_mlp_train_supported(32, 128, 3) == 1
@@ -140,13 +140,15 @@ def test_new_entry_points_reject_bad_arguments():
     assert b"null" in L.nsvf_last_error()
     assert L.nsvf_mlp_train_forward(32, 128, 3, None, None, 0, None, None) == 0
     assert L.nsvf_mlp_train_backward(32, 128, 3, None, None, 10, None, None, None, 0, None, None) == -1
-    assert L.nsvf_march_step_begin(None, 10, None, None, 0, 5, None, None, None, None, None) == -1
+    assert L.nsvf_march_begin(None, 10, None, 0, 5, None, None, None, None) == -1
     assert b"chunk" in L.nsvf_last_error()
-    assert L.nsvf_march_step_begin(None, 10, None, None, 4, 5, None, None, None, None, None) == -1
+    assert L.nsvf_march_begin(None, 10, None, 4, 5, None, None, None, None) == -1
     assert b"null" in L.nsvf_last_error()
-    assert L.nsvf_march_step_list(None, 10, None, 4, None, None, None, None, None, None) == -1
-    assert L.nsvf_march_step_end(None, 10, None, None, 4, None, None, None, None, None, 1.0, None) == -1
-    assert L.nsvf_march_step_begin(None, 0, None, None, 0, 5, None, None, None, None, None) == 0
+    assert L.nsvf_march_step(None, 10, None, None, None, None, 4, 5, 1.0, None, None, None, None,
+                             None, None, None) == -1
+    assert L.nsvf_march_end(None, 10, None, None, None, None, None) == -1
+    assert L.nsvf_march_begin(None, 0, None, 0, 5, None, None, None, None) == 0
+    assert L.nsvf_march_hist_ints(10, 4, 5) == 3 * 6
     rc = L.nsvf_render_sample_coarse(None, None, 10, None, None, None, None, None, 60, 0.1, 1, 64.0,
                                      None, None, None, None, None, 0, None, 1 << 32, None)
     assert rc == -1 and b"noise seed" in L.nsvf_last_error()
