@@ -114,9 +114,20 @@ def test_packed_tree_boxes():
     assert (mn[root] <= mn[leaf]).all() and (mx[root] >= mx[leaf]).all()
 
 
+def chain_end(ch, k):
+    """Node k, or the end of its chain of one-child nodes."""
+    while True:
+        kids = ch[k, :8][ch[k, :8] >= 0]
+        if len(kids) != 1:
+            return k
+        k = int(kids[0])
+
+
 def test_packed_tree_records():
-    """Record k, slot s: child children[k][s]'s (min, scale) and (max, id);
-    empty slots carry id -1; record T holds the root in slot 0 only."""
+    """Record k, slot s: the (min, scale) and (max, id) of child
+    children[k][s], or of the end of its chain of one-child nodes; empty
+    slots carry id -1; record T holds the root (or its chain's end) in slot
+    0 only."""
     sc = shell()
     cen, ch = shell_octree()
     h = np.float32(sc.voxel_size) * np.float32(0.5)
@@ -126,8 +137,16 @@ def test_packed_tree_records():
     assert rec.shape == (T + 1, 16, 4) and rec.dtype == np.float32
     lo, hi = rec[:, 0::2], rec[:, 1::2]                 # [T+1, 8, 4]
     ids = hi[..., 3].view(np.int32)
-    assert np.array_equal(ids[:T], ch[:, :8])
-    assert ids[T, 0] == T - 1 and (ids[T, 1:] == -1).all()
+    one = ((ch[:, :8] >= 0).sum(1) == 1) & (ch[:, 8] > 1)
+    assert one.sum() > T // 4                           # the finest internal level
+    want = np.array([[chain_end(ch, c) if c >= 0 else -1 for c in row] for row in ch[:, :8]])
+    assert np.array_equal(ids[:T], want)
+    assert (scale[want[want >= 0]] == 1).any() and not one[want[want >= 0]].any()
+    assert ids[T, 0] == chain_end(ch, T - 1) and (ids[T, 1:] == -1).all()
+    # a chain's boxes are all its end's box
+    for k in np.nonzero(one)[0][:500]:
+        e = chain_end(ch, k)
+        assert np.array_equal(mn[k], mn[e]) and np.array_equal(mx[k], mx[e])
     ok = ids >= 0
     k = ids[ok]
     assert np.array_equal(lo[ok][:, :3], mn[k]) and np.array_equal(hi[ok][:, :3], mx[k])
