@@ -559,9 +559,11 @@ __global__ __launch_bounds__(kFieldBlock) void field_bf16_kernel(
 // features {32kt + 4g + 0..3, 32kt + 16 + 4g + 0..3} as one 16-byte bf16x8
 // at ((t * NT + j) * KT0 + kt) * 64 + 16g + col, for sample
 // s = t * 16 NT + 16j + col (NT n-tiles of 16 samples per MLP tile; 64 is a
-// multiple of 16 NT).  With kLo the split-BF16 MLP's second operand, the
-// rounding residual bf16(f - bf16(f)) of each feature, goes to the same
-// position lo_off elements further on.
+// multiple of 16 NT), written with nontemporal stores: the tiles are read
+// once, by the MLP kernel, after far more has passed through the caches
+// (the kernel is bound by these writes).  With kLo the split-BF16 MLP's
+// second operand, the rounding residual bf16(f - bf16(f)) of each feature,
+// goes to the same position lo_off elements further on.
 template <int D, int H, int L, int NT = Dims<D, H, L>::NT, bool kLo = false>
 __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRays rays,
                                                          const SampleRec* __restrict__ samples,
@@ -697,8 +699,8 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
       if constexpr (kLo) {
         bf16x8 hi, lo;
         split_pack8(f32x4{f[0], f[1], f[2], f[3]}, f32x4{f[4], f[5], f[6], f[7]}, hi, lo);
-        base[kt * 64 + 16 * g] = hi;
-        base[lo_off + kt * 64 + 16 * g] = lo;
+        __builtin_nontemporal_store(hi, &base[kt * 64 + 16 * g]);
+        __builtin_nontemporal_store(lo, &base[lo_off + kt * 64 + 16 * g]);
       } else {
         const bf16x2 p0 = __builtin_convertvector(f32x2{f[0], f[1]}, bf16x2);
         const bf16x2 p1 = __builtin_convertvector(f32x2{f[2], f[3]}, bf16x2);
@@ -708,7 +710,7 @@ __global__ __launch_bounds__(256) void embed_bf16_kernel(FieldScene sc, FieldRay
                          __builtin_bit_cast(unsigned int, p1),
                          __builtin_bit_cast(unsigned int, p2),
                          __builtin_bit_cast(unsigned int, p3)};
-        base[kt * 64 + 16 * g] = __builtin_bit_cast(bf16x8, u);
+        __builtin_nontemporal_store(__builtin_bit_cast(bf16x8, u), &base[kt * 64 + 16 * g]);
       }
     }
   }

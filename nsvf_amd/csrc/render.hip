@@ -125,21 +125,18 @@ __device__ __forceinline__ unsigned expand_union(int node, int psc, const float 
   int ids[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    los[i] = 0.0f;
-    his[i] = 0.0f;
     const int s = i ^ dmask;
     const float4 mn = lrec[2 * s], mx = lrec[2 * s + 1];
     const int id = __builtin_amdgcn_readfirstlane(__float_as_int(mx.w));
     const int sc = __builtin_amdgcn_readfirstlane(__float_as_int(mn.w));
     ids[i] = id;
-    if (id >= 0 && id < T && sc < psc) {
-      float lo = 0.0f, hi = 0.0f;
-      const bool h = live && ray_box(o, inv, mn, mx, &lo, &hi);
-      if (h) {
-        mine |= 1u << i;
-        los[i] = lo;
-        his[i] = hi;
-      }
+    los[i] = 0.0f;
+    his[i] = 0.0f;
+    if (unsigned(id) < unsigned(T) && sc < psc) {
+      // every lane runs the (branch-free) test; a lane's bounds are read
+      // only where its bit in `mine` is set
+      const bool h = ray_box(o, inv, mn, mx, &los[i], &his[i]) & live;
+      mine |= unsigned(h) << i;
       if (__ballot(h)) {
         U |= 1u << i;
         if (sc == 1) leafs |= 1u << i;
