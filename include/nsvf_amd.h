@@ -553,8 +553,10 @@ int nsvf_render_composite_backward(const int32_t* hit_list,
  * The entry points take the pass's field with every sample evaluated
  * (float4 per slot, sigma first; a sample's value does not depend on the
  * others) and replay the marching:
- *   nsvf_march_begin  active[n] = 1, total[n] = 0, hist[g][c] = rows of
- *                     group g with c samples (hist zeroed by the caller,
+ *   nsvf_march_begin  active[n] = min(cnt, S) + 1 (nonzero while row n
+ *                     marches, 0 once it stops), total[n] = 0,
+ *                     hist[g][c] = rows of group g with c samples
+ *                     (hist zeroed by the caller,
  *                     nsvf_march_hist_ints ints; groups = ceil(n_hit /
  *                     chunk), S = the pass's longest row);
  *   nsvf_march_step   one chunk of every group (span[2 groups] and next

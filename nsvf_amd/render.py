@@ -385,7 +385,7 @@ class Renderer:
         sample's value: replays the reference's chunked marching (chunks of
         at most chunk_size rows and hits, csrc/march.hip), marks the samples
         it evaluates in the `{tag}_eval` buffer and zeroes the field of the
-        others; `{tag}_active` ends with the rows still marching.  off /
+        others; `{tag}_active` ends nonzero for the rows still marching.  off /
         cnt: the pass's frame-indexed slot offsets and counts; S: its
         longest row."""
         L = lib()
