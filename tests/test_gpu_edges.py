@@ -1,7 +1,8 @@
 """Edge cases of the fused renderer against the oracle: a view that hits
-nothing, a one-voxel scene (the octree's root is the leaf), rays that start
-inside voxels (entry depth 0), ray counts that are not a multiple of a wave,
-and a one-hit row budget."""
+nothing, a one-voxel scene (the octree's root is the leaf), voxels far
+apart (chains of one-child nodes down to each), rays that start inside
+voxels (entry depth 0), ray counts that are not a multiple of a wave, and a
+one-hit row budget."""
 import dataclasses
 
 import numpy as np
@@ -67,6 +68,27 @@ def test_one_voxel_scene(device):
     rs = np.tile(np.array([[1.5, 0.7, -0.9]], np.float32), (77, 1))
     rd = (tgt - rs).astype(np.float32)
     rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    r, (rgb, depth, missed) = render(sc, rs, rd, device)
+    ref = check_hits(r, sc, rs, rd)
+    assert ref["hits"].all()
+    assert np.abs(rgb - ref["rgb"]).max() < 1e-5 and np.abs(missed - ref["missed"]).max() < 1e-5
+
+
+def test_far_apart_voxels(device):
+    """Three voxels far apart: each hangs below the root on a chain of
+    one-child nodes, which the packed records skip, so the root's record
+    holds the leaves themselves; hits, their order along each ray and the
+    image against the oracle."""
+    pts = np.array([[-3.0, -3.0, -3.0], [0.0, 0.25, 0.0], [3.0, 3.0, 2.75]], np.float32)
+    sc = IO.scene_from_points(pts, 0.25, embed_dim=32)
+    rng = np.random.default_rng(4)
+    rs = np.tile(np.array([[-6.0, -6.2, -5.9]], np.float32), (90, 1))
+    tgt = pts[rng.integers(0, 3, 90)] + rng.uniform(-0.1, 0.1, (90, 3)).astype(np.float32)
+    rd = (tgt - rs).astype(np.float32)
+    rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    root = DeviceScene(sc, device).tree_records[-2].cpu().numpy()   # node T - 1's record
+    ids, scale = root[1::2, 3].view(np.int32), root[0::2, 3].view(np.int32)
+    assert (ids >= 0).sum() == 3 and (scale[ids >= 0] == 1).all()    # the three leaves
     r, (rgb, depth, missed) = render(sc, rs, rd, device)
     ref = check_hits(r, sc, rs, rd)
     assert ref["hits"].all()

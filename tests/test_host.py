@@ -151,3 +151,27 @@ def test_packed_tree_records():
     k = ids[ok]
     assert np.array_equal(lo[ok][:, :3], mn[k]) and np.array_equal(hi[ok][:, :3], mx[k])
     assert np.array_equal(lo[ok][:, 3].view(np.int32), scale[k])
+
+
+def test_packed_tree_of_far_apart_voxels():
+    """Three voxels far apart hang below the root on chains of one-child
+    nodes; the root's packed record holds the three leaves, in the root's
+    own child slots, with their exact boxes."""
+    from nsvf_amd import geometry, scene_io as IO
+    import torch
+    pts = np.array([[-3.0, -3.0, -3.0], [0.0, 0.25, 0.0], [3.0, 3.0, 2.75]], np.float32)
+    sc = IO.scene_from_points(pts, 0.25, embed_dim=32)
+    h = np.float32(sc.voxel_size) * np.float32(0.5)
+    cen, ch = geometry.build_easy_octree(torch.from_numpy(sc.points), torch.tensor(h))
+    cen, ch = cen.float().numpy(), ch.int().numpy()
+    T = ch.shape[0]
+    assert T > 3 + 1                                    # the chains exist
+    rec = pack_tree(cen, ch, h)
+    ids, scale = rec[T - 1, 1::2, 3].view(np.int32), rec[T - 1, 0::2, 3].view(np.int32)
+    slots = np.nonzero(ch[T - 1, :8] >= 0)[0]
+    assert np.array_equal(np.nonzero(ids >= 0)[0], slots)
+    assert (scale[slots] == 1).all()
+    assert np.array_equal(ids[slots], [chain_end(ch, c) for c in ch[T - 1, slots]])
+    leaf = ids[slots]
+    assert np.array_equal(rec[T - 1, 2 * slots, :3], (cen[leaf] - h).astype(np.float32))
+    assert np.array_equal(rec[T - 1, 2 * slots + 1, :3], (cen[leaf] + h).astype(np.float32))
