@@ -600,7 +600,10 @@ def rows(args, dev, sc, dsc):
     # one training step: forward, MSE against U(0,1) targets, backward --
     # with the HIP training MLP, and with torch's Linear layers for reference
     target = torch.rand((R, 3), device=dev)
-    for key, impl in (("train_step", "hip"), ("train_step_torch_mlp", "torch")):
+    mlp_names = {"hip": "HIP FP32 MFMA", "hip_x3": "HIP split-BF16 forward, FP32 MFMA backward",
+                 "torch": "torch FP32 nn.Linear"}
+    for key, impl in (("train_step", "hip"), ("train_step_x3_forward", "hip_x3"),
+                      ("train_step_torch_mlp", "torch")):
         tf = TrainableField(sc, dev, mlp=impl)
         tr = TrainRenderer(dev)
 
@@ -618,12 +621,12 @@ def rows(args, dev, sc, dsc):
         torch.cuda.synchronize()
         res[key] = {"ms": round((time.perf_counter() - t0) * 1e3 / 5, 2), "rays": R,
                     "samples": M,
-                    "mlp": "HIP FP32 MFMA" if impl == "hip" else "torch FP32 nn.Linear",
+                    "mlp": mlp_names[impl],
                     "kernels": "HIP gather / composite forward and backward"}
         del tf, tr
         torch.cuda.empty_cache()
     # the training MLP kernels alone on the batch's samples
-    from nsvf_amd.train import FieldMLP
+    from nsvf_amd.train import FieldMLP, FieldMLPx3
     tf = TrainableField(sc, dev)
     prm = []
     for w_, b_ in zip(tf.weights, tf.biases):
@@ -632,13 +635,14 @@ def rows(args, dev, sc, dsc):
     dout = torch.randn((M, 4), device=dev)
     fl = sc.mlp.flops_per_sample()
     ms_f = _time_ms(lambda: FieldMLP.apply(emb, *prm), 5)
+    ms_f3 = _time_ms(lambda: FieldMLPx3.apply(emb, *prm), 5)
     ms_fb = _time_ms(lambda: FieldMLP.apply(emb, *prm).backward(dout), 5)
     # backward: recomputed forward + input gradients + weight gradients, plus
     # the first layer once more (about 3 forward FLOPs and one layer)
     ms_b = ms_fb - ms_f
     bw_flops = M * (3 * fl + 2 * dsc.D * dsc.H)
     res["mlp_train"] = {"forward_ms": round(ms_f, 3), "backward_ms": round(ms_b, 3),
-                        "samples": M,
+                        "forward_split_bf16_ms": round(ms_f3, 3), "samples": M,
                         "forward_TFLOPs": round(M * fl / (ms_f * 1e-3) / 1e12, 1),
                         "backward_TFLOPs": round(bw_flops / (ms_b * 1e-3) / 1e12, 1),
                         "peak_TFLOPs": 157.3}

@@ -490,6 +490,14 @@ int nsvf_field_mlp_bf16x3(int D, int H, int L, const void* workspace,
                           int64_t workspace_bytes, const int32_t* m_dev,
                           int64_t m_max, const void* wfrag, const float* bias,
                           float* out, const int32_t* list, void* stream);
+/* The split-BF16 MLP's operand tiles from given features (feats [M, D]
+ * row-major float32, e.g. training's gather output; field.py:217-279 takes
+ * them as its input), written into the workspace as
+ * nsvf_field_embed_bf16x3 writes gathered ones, for nsvf_field_mlp_bf16x3
+ * with m_max = M.                                                           */
+int nsvf_field_tiles_bf16x3(int D, int H, int L, const float* feats, int64_t M,
+                            void* workspace, int64_t workspace_bytes,
+                            void* stream);
 
 /* Front-to-back compositing of each hit ray's samples (one wavefront per
  * ray): free energy relu(sigma) * length * 7, probs, and the ray's rgb,

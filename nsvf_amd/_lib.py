@@ -95,6 +95,7 @@ SIGNATURES = {
     "nsvf_field_embed_bf16x3": ([_I, _I, _I, _P, _P, _P, _F, _P, _P, _P, _P, _P, _L, _P, _L,
                                  _P], _I),
     "nsvf_field_mlp_bf16x3": ([_I, _I, _I, _P, _L, _P, _L, _P, _P, _P, _P, _P], _I),
+    "nsvf_field_tiles_bf16x3": ([_I, _I, _I, _P, _L, _P, _L, _P], _I),
     "nsvf_render_composite": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_composite_backward": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_track_voxel_probs": ([_P, _P, _L, _P, _P, _P, _P, _P, _L, _P], _I),

@@ -1570,6 +1570,57 @@ int launch_mlp_bf16x3(const int32_t* m_dev, int64_t m_max, const void* wfrag, co
   }
 }
 
+// The split-BF16 MLP's operand tiles from features computed elsewhere
+// (training's gather: feats [M, D] row-major float32), laid out and split as
+// embed_bf16_kernel<D, H, L, 2, true> writes gathered ones: a thread per
+// sample, k-tile and lane group g writes features {32kt + 4g + 0..3,
+// 32kt + 16 + 4g + 0..3} as one hi and one lo bf16x8.  Samples up to the
+// next multiple of 64 (the tiles the MLP kernel may read) are written, the
+// ones past M as zeros.
+template <int D, int H, int L>
+__global__ __launch_bounds__(256) void tiles_bf16x3_kernel(const float* __restrict__ feats,
+                                                           int64_t M, int64_t total,
+                                                           bf16x8* __restrict__ emb,
+                                                           int64_t lo_off) {
+  using Dm = Dims3<D, H, L>;
+  constexpr int KT0 = Dm::KT0, NT = Dm::NT, SAMPLES = Dm::SAMPLES;
+  const int64_t tid = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int g = int(tid & 3);
+  const int64_t q = tid >> 2;
+  const int kt = int(q % KT0);
+  const int64_t s = q / KT0;
+  if (s >= total) return;
+  float f[8];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int feat = 32 * kt + 16 * h + 4 * g + i;
+      f[4 * h + i] = (s < M && feat < D) ? feats[s * D + feat] : 0.0f;
+    }
+  bf16x8 hi, lo;
+  split_pack8(f32x4{f[0], f[1], f[2], f[3]}, f32x4{f[4], f[5], f[6], f[7]}, hi, lo);
+  const int64_t t = s / SAMPLES;
+  const int within = int(s - t * SAMPLES);
+  const int64_t idx = ((t * NT + (within >> 4)) * KT0 + kt) * 64 + 16 * g + (within & 15);
+  __builtin_nontemporal_store(hi, &emb[idx]);
+  __builtin_nontemporal_store(lo, &emb[lo_off + idx]);
+}
+
+template <int D, int H, int L>
+int launch_tiles_bf16x3(const float* feats, int64_t M, void* workspace, hipStream_t stream) {
+  if constexpr (x3_supported<D, H, L>()) {
+    const int64_t total = ceil_div(M, 64) * 64;
+    const int64_t threads = total * Dims3<D, H, L>::KT0 * 4;
+    hipLaunchKernelGGL((tiles_bf16x3_kernel<D, H, L>), dim3(unsigned(ceil_div(threads, 256))),
+                       dim3(256), 0, stream, feats, M, total, static_cast<bf16x8*>(workspace),
+                       x3_lo_offset<D, H, L>(M));
+    return check_launch("field_tiles_bf16x3");
+  } else {
+    return set_error(NSVF_EINVAL, "field bf16x3: shape D=%d H=%d L=%d not supported", D, H, L);
+  }
+}
+
 // The fused BF16 field over a sample stream: the run-staged gather + MLP
 // kernel where the weights fit in LDS, else the per-lane gather kernel.
 template <int D, int H, int L>
@@ -1906,6 +1957,26 @@ int nsvf_field_mlp_bf16x3(int D, int H, int L, const void* workspace, int64_t wo
   NSVF_MLP3_CASE(32, 256, 4)
 #undef NSVF_MLP3_CASE
   return set_error(NSVF_EINVAL, "field_mlp_bf16x3: unsupported shape D=%d H=%d L=%d", D, H, L);
+}
+
+int nsvf_field_tiles_bf16x3(int D, int H, int L, const float* feats, int64_t M,
+                            void* workspace, int64_t workspace_bytes, void* stream) {
+  clear_error();
+  if (M < 0) return set_error(NSVF_EINVAL, "field_tiles_bf16x3: M=%lld", static_cast<long long>(M));
+  if (M == 0) return NSVF_OK;
+  if (!feats || !workspace) return set_error(NSVF_EINVAL, "field_tiles_bf16x3: null pointer");
+  if (!nsvf_field_bf16x3_supported(D, H, L))
+    return set_error(NSVF_EINVAL, "field_tiles_bf16x3: unsupported shape D=%d H=%d L=%d", D, H, L);
+  if (workspace_bytes < nsvf_field_workspace_bytes(2, D, H, L, M))
+    return set_error(NSVF_EINVAL, "field_tiles_bf16x3: workspace too small");
+#define NSVF_TILES3_CASE(d, h, l) \
+  if (D == d && H == h && L == l) \
+    return launch_tiles_bf16x3<d, h, l>(feats, M, workspace, as_stream(stream));
+  NSVF_TILES3_CASE(32, 128, 3)
+  NSVF_TILES3_CASE(48, 256, 4)
+  NSVF_TILES3_CASE(32, 256, 4)
+#undef NSVF_TILES3_CASE
+  return set_error(NSVF_EINVAL, "field_tiles_bf16x3: unsupported shape D=%d H=%d L=%d", D, H, L);
 }
 
 int nsvf_field_prune(int precision, int D, int H, int L, const float* centers,

@@ -28,6 +28,7 @@ import torch.nn as nn
 import torch.nn.functional as F
 
 from ._lib import check, lib
+from .field import pack_layer_bf16
 from .render import BG_COLOR, BG_DEPTH, CoarseStream, DeviceScene, Renderer
 from .scene import VoxelScene
 
@@ -257,20 +258,102 @@ class FieldMLP(torch.autograd.Function):
         return (demb, *grads)
 
 
+_X3_INDEX = {}
+
+
+def _x3_index(shapes, device):
+    """For weights of the given [out, in] shapes: the position, in
+    [0, W_0 flat, W_1 flat, ...] (0: zero padding), of every element of the
+    split-BF16 kernel's fragments in their order, [fragments * 512] int64
+    (pack_layer_bf16 applied to the positions; they stay below 2^24, so
+    its float32 arrays carry them exactly)."""
+    key = (tuple(shapes), str(device))
+    idx = _X3_INDEX.get(key)
+    if idx is None:
+        parts, base = [], 1
+        for O, K in shapes:
+            pos = (np.arange(O * K, dtype=np.float64) + base).reshape(O, K).astype(np.float32)
+            parts.append(pack_layer_bf16(pos, ((K + 31) // 32) * 32).reshape(-1))
+            base += O * K
+        if base >= 1 << 24:
+            raise ValueError("split-BF16 training MLP: too many weights")
+        idx = torch.from_numpy(np.concatenate(parts).astype(np.int64)).to(device)
+        _X3_INDEX[key] = idx
+    return idx
+
+
+def pack_mlp_x3(weights, biases):
+    """The split-BF16 MLP kernel's weight fragments and biases from
+    torch-layout weights, on the device (the layout and rounding of
+    field.pack_mlp's "bf16x3": per fragment its 512 hi parts bf16(w), then
+    its lo parts bf16(w - hi), round to nearest even)."""
+    dev = weights[0].device
+    idx = _x3_index([tuple(w.shape) for w in weights], dev)
+    flat = torch.cat([torch.zeros(1, dtype=torch.float32, device=dev)]
+                     + [w.detach().reshape(-1).float() for w in weights])
+    vals = flat[idx].view(-1, 512)
+    hi = vals.to(torch.bfloat16)
+    lo = (vals - hi.float()).to(torch.bfloat16)
+    frags = torch.stack([hi, lo], 1).reshape(-1).view(torch.int16)
+    head = biases[-1].detach().float()
+    bias = torch.cat([b.detach().float() for b in biases[:-1]]
+                     + [F.pad(head, (0, 16 - head.shape[0]))])
+    return frags, bias
+
+
+class FieldMLPx3(FieldMLP):
+    """FieldMLP with its forward on the renderer's split-BF16 MLP kernel
+    (three BF16 MFMAs per product on hi / lo operand parts, FP32
+    accumulation: outputs within about 1e-5 of FP32) -- the gathered
+    features written as its operand tiles, the weights packed on the device
+    each call -- and the same FP32-MFMA backward, which recomputes the
+    activations in FP32 from the input."""
+
+    @staticmethod
+    def forward(ctx, emb, *params):
+        weights, biases = list(params[0::2]), list(params[1::2])
+        D, H, L = _mlp_shape(weights)
+        _need_cuda(emb, "emb")
+        emb = emb.contiguous()
+        if emb.dtype != torch.float32 or emb.dim() != 2 or emb.shape[1] != D:
+            raise RuntimeError(f"emb must be float32 [M, {D}]")
+        L_ = lib()
+        if not L_.nsvf_field_bf16x3_supported(D, H, L):
+            raise RuntimeError(f"split-BF16 MLP not built for D={D} H={H} L={L}")
+        dev, M = emb.device, emb.shape[0]
+        out = torch.empty((M, 4), dtype=torch.float32, device=dev)
+        if M > 0:
+            frags, bias = pack_mlp_x3(weights, biases)
+            ws_bytes = int(L_.nsvf_field_workspace_bytes(2, D, H, L, M))
+            ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+            m_dev = torch.full((1,), M, dtype=torch.int32, device=dev)
+            st = _stream(dev)
+            check(L_.nsvf_field_tiles_bf16x3(D, H, L, _p(emb), M, _p(ws), ws_bytes, st),
+                  "field_tiles_bf16x3")
+            check(L_.nsvf_field_mlp_bf16x3(D, H, L, _p(ws), ws_bytes, _p(m_dev), M, _p(frags),
+                                           _p(bias), _p(out), None, st), "field_mlp_bf16x3")
+        ctx.save_for_backward(emb, pack_mlp_train(weights, biases))
+        ctx.shape = (D, H, L)
+        ctx.param_shapes = [tuple(t.shape) for t in params]
+        return out
+
+
 class TrainableField(nn.Module):
     """The trainable state of a voxel scene: corner embeddings [K,D] and the
     field MLP (L hidden ReLU layers, a 4-wide head: sigma, sigmoid RGB).
 
     mlp="hip": the MLP runs forward and backward on the hand-written
     FP32-MFMA kernels (FieldMLP; built for D=32, H=128, L=3 -- other shapes
-    raise); mlp="torch": torch.nn.functional.linear with torch autograd, as
-    in the reference."""
+    raise); mlp="hip_x3": its forward on the split-BF16 kernel instead
+    (FieldMLPx3, outputs within about 1e-5 of FP32), the same backward;
+    mlp="torch": torch.nn.functional.linear with torch autograd, as in the
+    reference."""
 
     def __init__(self, scene: VoxelScene, device, mlp="hip"):
         super().__init__()
         dev = torch.device(device)
-        if mlp not in ("hip", "torch"):
-            raise ValueError(f"mlp={mlp!r}: 'hip' or 'torch'")
+        if mlp not in ("hip", "hip_x3", "torch"):
+            raise ValueError(f"mlp={mlp!r}: 'hip', 'hip_x3' or 'torch'")
         self.mlp_impl = mlp
         self.values = nn.Parameter(torch.from_numpy(np.array(scene.values, np.float32)).to(dev))
         self.weights = nn.ParameterList(
@@ -282,7 +365,7 @@ class TrainableField(nn.Module):
 
     def mlp(self, emb):
         """[M,D] -> [M,4] = (sigma, r, g, b)."""
-        if self.mlp_impl == "hip":
+        if self.mlp_impl in ("hip", "hip_x3"):
             D, H, L = _mlp_shape(list(self.weights))
             if not mlp_train_supported(D, H, L):
                 raise RuntimeError(f"the HIP training MLP is not built for D={D} H={H} L={L}; "
@@ -290,7 +373,8 @@ class TrainableField(nn.Module):
             params = []
             for w, b in zip(self.weights, self.biases):
                 params += [w, b]
-            return FieldMLP.apply(emb, *params)
+            fn = FieldMLPx3 if self.mlp_impl == "hip_x3" else FieldMLP
+            return fn.apply(emb, *params)
         h = emb
         n = len(self.weights)
         for i in range(n):

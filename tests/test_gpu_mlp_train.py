@@ -13,7 +13,9 @@ import numpy as np
 import pytest
 import torch
 
-from nsvf_amd.train import FieldMLP, mlp_train_supported, pack_mlp_train
+from nsvf_amd.field import pack_mlp
+from nsvf_amd.scene import FieldMLP as SceneMLP
+from nsvf_amd.train import FieldMLP, FieldMLPx3, mlp_train_supported, pack_mlp_train, pack_mlp_x3
 
 pytestmark = pytest.mark.gpu
 
@@ -58,10 +60,10 @@ def off_kinks(emb, ps, dout):
     return out, int(near.sum())
 
 
-def run(emb, ps, dout):
+def run(emb, ps, dout, fn=FieldMLP):
     e = emb.clone().requires_grad_(True)
     q = [p.clone().requires_grad_(True) for p in ps]
-    out = FieldMLP.apply(e, *q)
+    out = fn.apply(e, *q)
     out.backward(dout)
     torch.cuda.synchronize()
     return out.detach(), e.grad, [t.grad for t in q]
@@ -111,3 +113,34 @@ def test_pack_rejects_unbuilt_shape(device):
     w0 = torch.zeros((H, 48), device=device)
     with pytest.raises(RuntimeError, match="not built"):
         pack_mlp_train([w0] + ps[2::2], ps[1::2])
+
+
+def test_split_bf16_weight_pack_on_the_device(device):
+    """pack_mlp_x3 (torch ops on the device, every training step) equals
+    the scene packer's split-BF16 fragments and biases bit for bit."""
+    ps = params(6, device)
+    frags, bias = pack_mlp_x3(ps[0::2], ps[1::2])
+    host = pack_mlp(SceneMLP([p.cpu().numpy() for p in ps[0::2]],
+                             [p.cpu().numpy() for p in ps[1::2]]))
+    assert torch.equal(frags.cpu(), host["bf16x3"]) and torch.equal(bias.cpu(), host["bias"])
+
+
+@pytest.mark.parametrize("M,seed", [(1, 1), (1000, 2), (128 * 300 + 5, 3)])
+def test_split_bf16_forward(device, M, seed):
+    """FieldMLPx3: the features written as the split-BF16 kernel's operand
+    tiles (M not a multiple of the tile), its forward within 1e-4 of float64
+    (three BF16 products per FP32 one; plain BF16 is ~1e-2 here); its
+    backward is FieldMLP's, so every gradient equals FieldMLP's bit for
+    bit."""
+    ps = params(seed, device)
+    g = torch.Generator().manual_seed(seed)
+    emb = torch.randn((M, D), generator=g).mul_(0.3).to(device)
+    dout = torch.randn((M, 4), generator=g).to(device)
+    out, demb, dps = run(emb, ps, dout, FieldMLPx3)
+    r_out = reference(emb, ps, dout)[0]
+    err = float((out.cpu().double() - r_out).abs().max())
+    print(f"M={M}: split-BF16 forward max abs error {err:.2e}", flush=True)
+    assert err < 1e-4
+    _, demb32, dps32 = run(emb, ps, dout)
+    assert torch.equal(demb, demb32)
+    assert all(torch.equal(a, b) for a, b in zip(dps, dps32))

@@ -175,3 +175,22 @@ def test_packed_tree_of_far_apart_voxels():
     leaf = ids[slots]
     assert np.array_equal(rec[T - 1, 2 * slots, :3], (cen[leaf] - h).astype(np.float32))
     assert np.array_equal(rec[T - 1, 2 * slots + 1, :3], (cen[leaf] + h).astype(np.float32))
+
+
+def test_split_bf16_pack_from_torch_weights():
+    """The training packer (torch ops, here on CPU tensors) equals the
+    scene packer's split-BF16 fragments and biases bit for bit: the
+    fragment permutation, zero padding, round-to-nearest-even hi and the lo
+    residual."""
+    import torch
+    from nsvf_amd.field import pack_mlp
+    from nsvf_amd.scene import FieldMLP
+    from nsvf_amd.train import pack_mlp_x3
+    rng = np.random.default_rng(7)
+    shapes = [(128, 32), (128, 128), (128, 128), (4, 128)]
+    W = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    B = [rng.standard_normal(s[0]).astype(np.float32) for s in shapes]
+    W[1][3, 5] = np.float32(1.00390625)             # a tie: rounds to even
+    host = pack_mlp(FieldMLP(W, B))
+    frags, bias = pack_mlp_x3([torch.from_numpy(w) for w in W], [torch.from_numpy(b) for b in B])
+    assert torch.equal(frags, host["bf16x3"]) and torch.equal(bias, host["bias"])
