@@ -127,7 +127,8 @@ def test_tree_depth_of_packed_trees():
 def test_new_entry_points_reject_bad_arguments():
     """Argument checks that return before any device work (no GPU needed):
     the training MLP's built shape and pointers, the marching entry points' chunk
-    and pointers, and the samplers' 32-bit noise seeds."""
+    and pointers, the samplers' 32-bit noise seeds and the split-BF16 tile
+    writer's size, pointers, shape and workspace."""
     L = _lib.lib()
     assert L.nsvf_mlp_train_supported(32, 128, 3) == 1
     assert L.nsvf_mlp_train_supported(48, 256, 4) == 0
@@ -152,3 +153,14 @@ def test_new_entry_points_reject_bad_arguments():
     rc = L.nsvf_render_sample_coarse(None, None, 10, None, None, None, None, None, 60, 0.1, 1, 64.0,
                                      None, None, None, None, None, 0, None, 1 << 32, None)
     assert rc == -1 and b"noise seed" in L.nsvf_last_error()
+    # the split-BF16 operand tiles from given features
+    need = L.nsvf_field_workspace_bytes(2, 32, 128, 3, 100)
+    assert need == 2 * 128 * 4 * 16                     # hi and lo, samples padded to 64
+    assert L.nsvf_field_tiles_bf16x3(32, 128, 3, None, 0, None, 0, None) == 0
+    assert L.nsvf_field_tiles_bf16x3(32, 128, 3, None, -1, None, 0, None) == -1
+    assert L.nsvf_field_tiles_bf16x3(32, 128, 3, None, 100, None, need, None) == -1
+    assert b"null" in L.nsvf_last_error()
+    assert L.nsvf_field_tiles_bf16x3(16, 64, 2, 1, 100, 1, need, None) == -1
+    assert b"shape" in L.nsvf_last_error()
+    assert L.nsvf_field_tiles_bf16x3(32, 128, 3, 1, 100, 1, need - 1, None) == -1
+    assert b"workspace" in L.nsvf_last_error()
