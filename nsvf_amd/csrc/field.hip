@@ -813,13 +813,32 @@ __device__ __forceinline__ void x3_epilogue(const f32x4 (&acc)[2][NT], int j, bf
   split_pack8(r0, r1, oh, ol);
 }
 
+// A quarter of x3_epilogue: word q of the hi and lo operands of n-tile j,
+// from outputs 2q' and 2q' + 1 of half h = q / 2 (q' = q % 2) -- the same
+// values and arithmetic as split_pack8's pair q.
+template <int NT>
+__device__ __forceinline__ void x3_epilogue_unit(const f32x4 (&acc)[2][NT], int j, int q,
+                                                 bf16x8& oh, bf16x8& ol) {
+  const int h = q >> 1, p = (q & 1) * 2;
+  unsigned int hw, lw;
+  split_pack2(relu_bits(acc[h][j][p]), relu_bits(acc[h][j][p + 1]), hw, lw);
+  u32x4 hv = __builtin_bit_cast(u32x4, oh), lv = __builtin_bit_cast(u32x4, ol);
+  hv[q] = hw;
+  lv[q] = lw;
+  oh = __builtin_bit_cast(bf16x8, hv);
+  ol = __builtin_bit_cast(bf16x8, lv);
+}
+
 // One split-BF16 layer, out = relu(W in + bias) split into hi / lo, over
 // m-tile pairs with the weight pairs read two ahead (as mfma_relu_layer).
 // Per output the accumulation order is k-tile 0, 1, ..., and within a
-// k-tile Wl Xh, Wh Xl, Wh Xh.  A pair's epilogue runs one n-tile at a time
-// between the next pair's MFMAs, so its VALU work issues while they execute;
-// the wave raises its priority for the layer so that the SIMD's other wave,
-// if it is between layers, does not take its issue slots.
+// k-tile Wl Xh, Wh Xl, Wh Xh.  A pair's epilogue runs between the next
+// pair's MFMAs, spread evenly over its k-tile groups in quarters (4 NT
+// quarters over 2 KT groups: one per group of a hidden layer, where each
+// group issues 6 MFMAs), so its VALU work issues while they execute
+// (spread, against whole n-tiles in the first NT groups: -2% per MLP
+// launch); the wave raises its priority for the layer so that the SIMD's
+// other wave, if it is between layers, does not take its issue slots.
 template <int KT, int MT, int NT>
 __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B, int lane, int g,
                                                  const bf16x8 (&xh)[KT][NT],
@@ -827,7 +846,6 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
                                                  bf16x8 (&oh)[MT / 2][NT],
                                                  bf16x8 (&ol)[MT / 2][NT]) {
   constexpr int kLead = 2, NF = MT * KT, NB = kLead + 1;
-  static_assert(2 * KT >= NT, "an epilogue slot per n-tile");
   bf16x8 ah[NB], al[NB];
 #pragma unroll
   for (int p = 0; p < kLead; ++p)
@@ -866,7 +884,12 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
           acc[half][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[f % NB], xh[kt][j],
                                                                  acc[half][j], 0, 0, 0);
         const int slot = half * KT + kt;
-        if (kt2 > 0 && slot < NT) x3_epilogue<NT>(prev, slot, oh[kt2 - 1][slot], ol[kt2 - 1][slot]);
+        if (kt2 > 0) {
+          constexpr int U = 4 * NT, S = 2 * KT;
+#pragma unroll
+          for (int u = slot * U / S; u < (slot + 1) * U / S; ++u)
+            x3_epilogue_unit<NT>(prev, u >> 2, u & 3, oh[kt2 - 1][u >> 2], ol[kt2 - 1][u >> 2]);
+        }
         __builtin_amdgcn_sched_barrier(0);
       }
     }
