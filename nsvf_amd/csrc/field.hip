@@ -837,15 +837,21 @@ __device__ __forceinline__ void x3_epilogue_unit(const f32x4 (&acc)[2][NT], int 
 // quarters over 2 KT groups: one per group of a hidden layer, where each
 // group issues 6 MFMAs), so its VALU work issues while they execute
 // (spread, against whole n-tiles in the first NT groups: -2% per MLP
-// launch); the wave raises its priority for the layer so that the SIMD's
-// other wave, if it is between layers, does not take its issue slots.
-template <int KT, int MT, int NT>
+// launch).  The last pair has no later MFMAs in the layer: its
+// accumulators go out in pout, and the next layer (kPendIn: pin) or the
+// head finishes them into its input's last k-tile during its first KT - 1
+// k-tile groups, before that k-tile is read (-1% per launch).  The wave
+// raises its priority for the layer so that the SIMD's other wave, if it
+// is between layers, does not take its issue slots.
+template <int KT, int MT, int NT, bool kPendIn>
 __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B, int lane, int g,
-                                                 const bf16x8 (&xh)[KT][NT],
-                                                 const bf16x8 (&xl)[KT][NT],
+                                                 bf16x8 (&xh)[KT][NT], bf16x8 (&xl)[KT][NT],
+                                                 const f32x4 (&pin)[2][NT],
                                                  bf16x8 (&oh)[MT / 2][NT],
-                                                 bf16x8 (&ol)[MT / 2][NT]) {
+                                                 bf16x8 (&ol)[MT / 2][NT],
+                                                 f32x4 (&pout)[2][NT]) {
   constexpr int kLead = 2, NF = MT * KT, NB = kLead + 1;
+  static_assert(!kPendIn || KT > 1, "a pending input k-tile needs earlier k-tile groups");
   bf16x8 ah[NB], al[NB];
 #pragma unroll
   for (int p = 0; p < kLead; ++p)
@@ -889,6 +895,11 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
 #pragma unroll
           for (int u = slot * U / S; u < (slot + 1) * U / S; ++u)
             x3_epilogue_unit<NT>(prev, u >> 2, u & 3, oh[kt2 - 1][u >> 2], ol[kt2 - 1][u >> 2]);
+        } else if (kPendIn && half == 0 && kt + 1 < KT) {
+          constexpr int U = 4 * NT, S = KT > 1 ? KT - 1 : 1;
+#pragma unroll
+          for (int u = kt * U / S; u < (kt + 1) * U / S; ++u)
+            x3_epilogue_unit<NT>(pin, u >> 2, u & 3, xh[KT - 1][u >> 2], xl[KT - 1][u >> 2]);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -899,7 +910,9 @@ __device__ __forceinline__ void mfma3_relu_layer(const bf16x8* A, const float* B
       for (int j = 0; j < NT; ++j) prev[h][j] = acc[h][j];
   }
 #pragma unroll
-  for (int j = 0; j < NT; ++j) x3_epilogue<NT>(prev, j, oh[MT / 2 - 1][j], ol[MT / 2 - 1][j]);
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) pout[h][j] = prev[h][j];
   __builtin_amdgcn_s_setprio(0);
 }
 
@@ -942,7 +955,8 @@ __global__ __launch_bounds__(64 * kX3Waves, 1) void mlp_bf16x3_kernel(
     StoreSink sink{out, list};
     sink.begin_tile(base, M, Dm::SAMPLES);
     bf16x8 bh[KTH][NT], bl[KTH][NT];
-    mfma3_relu_layer<KT0, MTH, NT>(lw, lb, lane, g, xh, xl, bh, bl);
+    f32x4 pa[2][NT], pb[2][NT];
+    mfma3_relu_layer<KT0, MTH, NT, false>(lw, lb, lane, g, xh, xl, pb, bh, bl, pa);
     // the layer-0 operands are dead: the next tile's load into them
     if (tn < tiles) {
 #pragma unroll
@@ -958,14 +972,14 @@ __global__ __launch_bounds__(64 * kX3Waves, 1) void mlp_bf16x3_kernel(
     bf16x8 nh[KTH][NT], nl[KTH][NT];
 #pragma unroll 1
     for (int l = 1; l + 1 < L; l += 2) {
-      mfma3_relu_layer<KTH, MTH, NT>(lw + 2 * (Dm::F0 + (l - 1) * Dm::FH) * 64, lb + H * l, lane,
-                                     g, bh, bl, nh, nl);
-      mfma3_relu_layer<KTH, MTH, NT>(lw + 2 * (Dm::F0 + l * Dm::FH) * 64, lb + H * (l + 1), lane,
-                                     g, nh, nl, bh, bl);
+      mfma3_relu_layer<KTH, MTH, NT, true>(lw + 2 * (Dm::F0 + (l - 1) * Dm::FH) * 64, lb + H * l,
+                                           lane, g, bh, bl, pa, nh, nl, pb);
+      mfma3_relu_layer<KTH, MTH, NT, true>(lw + 2 * (Dm::F0 + l * Dm::FH) * 64, lb + H * (l + 1),
+                                           lane, g, nh, nl, pb, bh, bl, pa);
     }
     if ((L - 1) % 2) {
-      mfma3_relu_layer<KTH, MTH, NT>(lw + 2 * (Dm::F0 + (L - 2) * Dm::FH) * 64, lb + H * (L - 1),
-                                     lane, g, bh, bl, nh, nl);
+      mfma3_relu_layer<KTH, MTH, NT, true>(lw + 2 * (Dm::F0 + (L - 2) * Dm::FH) * 64,
+                                           lb + H * (L - 1), lane, g, bh, bl, pa, nh, nl, pb);
 #pragma unroll
       for (int kt = 0; kt < KTH; ++kt)
 #pragma unroll
@@ -973,6 +987,10 @@ __global__ __launch_bounds__(64 * kX3Waves, 1) void mlp_bf16x3_kernel(
           bh[kt][j] = nh[kt][j];
           bl[kt][j] = nl[kt][j];
         }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int j = 0; j < NT; ++j) pa[h][j] = pb[h][j];
     }
     // head: 4 outputs in rows 0..3 of one m-tile
     const bf16x8* A = lw + 2 * (Dm::F0 + (L - 1) * Dm::FH) * 64;
@@ -991,6 +1009,13 @@ __global__ __launch_bounds__(64 * kX3Waves, 1) void mlp_bf16x3_kernel(
 #pragma unroll
       for (int j = 0; j < NT; ++j)
         acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[kt][j], acc[j], 0, 0, 0);
+      // the last hidden layer's last pair, into k-tile KTH - 1 before it is read
+      if (kt + 1 < KTH) {
+        constexpr int U = 4 * NT, S = KTH > 1 ? KTH - 1 : 1;
+#pragma unroll
+        for (int u = kt * U / S; u < (kt + 1) * U / S; ++u)
+          x3_epilogue_unit<NT>(pa, u >> 2, u & 3, bh[KTH - 1][u >> 2], bl[KTH - 1][u >> 2]);
+      }
     }
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
