@@ -554,7 +554,7 @@ def rows(args, dev, sc, dsc):
 
     from nsvf_amd import scene as S
     from nsvf_amd._lib import lib
-    from nsvf_amd.render import Renderer
+    from nsvf_amd.render import DeviceScene, Renderer
     from nsvf_amd.train import TrainableField, TrainRenderer, mse_loss, trilinear_gather_backward
     from nsvf_amd.voxels import prune_scores, split_scene
 
@@ -625,6 +625,40 @@ def rows(args, dev, sc, dsc):
                     "kernels": "HIP gather / composite forward and backward"}
         del tf, tr
         torch.cuda.empty_cache()
+    # the training configuration as specified: the shell scene, 4 views x
+    # 2048 random pixels (64x64 views), with and without the traversal's
+    # coherent grouping of the batch's rays
+    shell = S.shell_scene()
+    dshell = DeviceScene(shell, dev)
+    rng = np.random.default_rng(7)
+    parts = []
+    for v in range(4):
+        a_, b_ = S.camera_rays(64, 64, S.sphere_camera(2.5, 100 + v))
+        pick = rng.choice(a_.shape[0], 2048, replace=False)
+        parts.append((a_[pick], b_[pick]))
+    mix = rng.permutation(4 * 2048)
+    crs = torch.from_numpy(np.concatenate([q[0] for q in parts])[mix]).to(dev)
+    crd = torch.from_numpy(np.concatenate([q[1] for q in parts])[mix]).to(dev)
+    ctarget = torch.rand((crs.shape[0], 3), device=dev)
+    c3 = {"rays": int(crs.shape[0]), "scene": f"shell, {shell.num_voxels} voxels"}
+    for key, coherent in (("ms", True), ("ms_rays_in_batch_order", False)):
+        tf = TrainableField(shell, dev)
+        tr = TrainRenderer(dev, coherent=coherent)
+
+        def step():
+            tf.zero_grad(set_to_none=True)
+            rgb, _, _ = tr.forward(dshell, tf, crs, crd)
+            mse_loss(rgb, ctarget).backward()
+
+        step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            step()
+        torch.cuda.synchronize()
+        c3[key] = round((time.perf_counter() - t0) * 1e3 / 20, 3)
+        del tf, tr
+    res["train_step_4x2048_random_rays"] = c3
     # the training MLP kernels alone on the batch's samples
     from nsvf_amd.train import FieldMLP, FieldMLPx3
     tf = TrainableField(sc, dev)

@@ -280,6 +280,28 @@ int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R,
                      int width, const void* records, int T, int depth, int n_max,
                      float step_size, void* hits, int32_t* count, float* tot,
                      int32_t* hit_flag, int32_t* bound, void* stream);
+/* nsvf_render_hits with the traversal's grouping of rays given: lane t of
+ * the kernel traces ray order[t] (order [R] int32 on the device, a
+ * permutation of 0..R-1; entries outside [0, R) are skipped, so their rays
+ * get no output).  For rays that are not image rows -- training batches of
+ * random pixels from several views (fairnr/criterions/rendering_loss.py
+ * renders such batches) -- a grouping by direction and origin keeps each
+ * wave's rays coherent; every output is the same as nsvf_render_hits'.     */
+int nsvf_render_hits_ordered(const float* ray_start, const float* ray_dir,
+                             int64_t R, const int32_t* order, const void* records,
+                             int T, int depth, int n_max, float step_size,
+                             void* hits, int32_t* count, float* tot,
+                             int32_t* hit_flag, int32_t* bound, void* stream);
+/* Sort keys for such a grouping, one pass over the rays: keys[r] (int64) =
+ * direction octant << 42 | origin cell << 30 | Morton code of the direction
+ * (10 bits per axis over [-1, 1]), the origin cell on a 16^3 grid over the
+ * box bounds = {min x, y, z, max x, y, z} (6 floats on the device, e.g. the
+ * origins' bounds).  The ray indices sorted by key are an order for
+ * nsvf_render_hits_ordered that keeps each wave's rays close in origin and
+ * direction.  (No counterpart in the reference, whose traversal is a
+ * thread per ray.)                                                         */
+int nsvf_ray_order_keys(const float* ray_start, const float* ray_dir, int64_t R,
+                        const float* bounds, int64_t* keys, void* stream);
 
 /* One view's pinhole camera, as the reference builds its rays
  * (fairnr/renderer.py:91-112, data/geometry.py:68-122, data_utils.py:57-65):

@@ -56,6 +56,9 @@ SIGNATURES = {
     "nsvf_scan_workspace_ints": ([_L], _L),
     "nsvf_scan_exclusive_i32": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_render_hits": ([_P, _P, _L, _I, _P, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P], _I),
+    "nsvf_render_hits_ordered": ([_P, _P, _L, _P, _P, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P],
+                                 _I),
+    "nsvf_ray_order_keys": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_generate_rays": ([_CAM, _P, _P, _P], _I),
     "nsvf_render_hits_camera": ([_CAM, _P, _I, _I, _I, _F, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "nsvf_render_compact": ([_P, _P, _L, _P, _P], _I),

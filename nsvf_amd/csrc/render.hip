@@ -250,17 +250,24 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
     float* __restrict__ gen_dir, const float4* __restrict__ recs, int T, int n_max,
     float step_size,
     HitRec* __restrict__ hits, int32_t* __restrict__ count, float* __restrict__ tot_out,
-    int32_t* __restrict__ hit_flag, int32_t* __restrict__ bound) {
+    int32_t* __restrict__ hit_flag, int32_t* __restrict__ bound,
+    const int32_t* __restrict__ order) {
   __shared__ float4 lrec_all[kHitsBlock / 64][kMaxLevels * 16];
   float4* lrec = lrec_all[threadIdx.x >> 6];
-  // the lane's ray: consecutive rays, or for rays that are the rows of an
+  // the lane's ray: consecutive rays; for rays that are the rows of an
   // image `width` pixels wide, one 8x8 pixel tile per wave (neighbouring
   // pixels share most of their octree paths, which keeps the union small);
-  // lanes outside R or the image take part in the wave's votes only
+  // with `order`, ray order[t] (a caller's grouping of coherent rays;
+  // entries outside [0, R) are skipped); lanes outside R or the image take
+  // part in the wave's votes only
   const int64_t t = int64_t(blockIdx.x) * kHitsBlock + threadIdx.x;
   int64_t r = t;
   bool valid = t < R;
-  if (width > 0) {
+  if (order) {
+    r = valid ? order[t] : 0;
+    valid = valid && r >= 0 && r < R;
+    r = valid ? r : 0;
+  } else if (width > 0) {
     const int64_t tiles_x = (width + 7) / 8, k = t >> 6;
     const int q = int(t & 63);
     const int64_t y = (k / tiles_x) * 8 + (q >> 3), x = (k % tiles_x) * 8 + (q & 7);
@@ -330,6 +337,41 @@ __global__ __launch_bounds__(kHitsBlock) void hits_kernel(
   // at most ceil(steps) + cnt samples carry a voxel id (one per CDF step and
   // one tail per bin; the compat duplicate replaces a step sample)
   bound[r] = cnt > 0 ? int(ceilf(tot / step_size)) + cnt + 2 : 0;
+}
+
+// Sort key of a ray for the traversal's grouping (nsvf_ray_order_keys):
+// direction octant (bits 42-44), origin cell on a 16^3 grid over the box
+// [bounds[0..2], bounds[3..5]] (bits 30-41), Morton code of the direction
+// quantised to 10 bits per axis (bits 0-29).
+__device__ __forceinline__ int64_t spread3(int64_t q) {
+  q &= 0x3FF;
+  q = (q | (q << 16)) & 0x030000FF;
+  q = (q | (q << 8)) & 0x0300F00F;
+  q = (q | (q << 4)) & 0x030C30C3;
+  q = (q | (q << 2)) & 0x09249249;
+  return q;
+}
+
+__global__ __launch_bounds__(kBlock) void ray_key_kernel(const float* __restrict__ ray_start,
+                                                         const float* __restrict__ ray_dir,
+                                                         int64_t R,
+                                                         const float* __restrict__ bounds,
+                                                         int64_t* __restrict__ keys) {
+  const int64_t r = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (r >= R) return;
+  int64_t oct = 0, cell = 0, morton = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float d = ray_dir[r * 3 + a], o = ray_start[r * 3 + a];
+    oct |= int64_t(d < 0.0f) << a;
+    const float span = fmaxf(bounds[3 + a] - bounds[a], 1e-30f);
+    const int c = min(max(int((o - bounds[a]) / span * 15.999f), 0), 15);
+    cell |= int64_t(c) << (4 * (2 - a));
+    const float dc = fminf(fmaxf(d, -1.0f), 1.0f);
+    const int q = min(max(__float2int_rn((dc + 1.0f) * 511.5f), 0), 1023);
+    morton |= spread3(q) << a;
+  }
+  keys[r] = (oct << 42) | (cell << 30) | morton;
 }
 
 __global__ __launch_bounds__(kBlock) void compact_kernel(const int32_t* __restrict__ flag,
@@ -1104,8 +1146,42 @@ int nsvf_render_hits(const float* ray_start, const float* ray_dir, int64_t R, in
                      dim3(kHitsBlock), 0, as_stream(stream), ray_start, ray_dir, R, width,
                      nsvf_camera{}, 0.0f, 0.0f, nullptr, nullptr,
                      static_cast<const float4*>(records), T, n_max, step_size,
-                     static_cast<HitRec*>(hits), count, tot, hit_flag, bound);
+                     static_cast<HitRec*>(hits), count, tot, hit_flag, bound, nullptr);
   return check_launch("render_hits");
+}
+
+int nsvf_ray_order_keys(const float* ray_start, const float* ray_dir, int64_t R,
+                        const float* bounds, int64_t* keys, void* stream) {
+  clear_error();
+  if (R < 0) return set_error(NSVF_EINVAL, "ray_order_keys: bad size");
+  if (R == 0) return NSVF_OK;
+  if (!ray_start || !ray_dir || !bounds || !keys)
+    return set_error(NSVF_EINVAL, "ray_order_keys: null pointer");
+  hipLaunchKernelGGL(ray_key_kernel, dim3(unsigned(ceil_div(R, kBlock))), dim3(kBlock), 0,
+                     as_stream(stream), ray_start, ray_dir, R, bounds, keys);
+  return check_launch("ray_order_keys");
+}
+
+int nsvf_render_hits_ordered(const float* ray_start, const float* ray_dir, int64_t R,
+                             const int32_t* order, const void* records, int T, int depth,
+                             int n_max, float step_size, void* hits, int32_t* count, float* tot,
+                             int32_t* hit_flag, int32_t* bound, void* stream) {
+  clear_error();
+  if (R < 0 || T <= 0 || n_max <= 0)
+    return set_error(NSVF_EINVAL, "render_hits_ordered: bad size");
+  if (const int rc = check_tree_depth(depth, "render_hits_ordered")) return rc;
+  if (R == 0) return NSVF_OK;
+  if (!ray_start || !ray_dir || !order || !records || !hits || !count || !tot || !hit_flag ||
+      !bound)
+    return set_error(NSVF_EINVAL, "render_hits_ordered: null pointer");
+  if (T >= (1 << 27))
+    return set_error(NSVF_EINVAL, "render_hits_ordered: more than 2^27 nodes");
+  hipLaunchKernelGGL(hits_kernel<false>, dim3(unsigned(ceil_div(R, kHitsBlock))),
+                     dim3(kHitsBlock), 0, as_stream(stream), ray_start, ray_dir, R, 0,
+                     nsvf_camera{}, 0.0f, 0.0f, nullptr, nullptr,
+                     static_cast<const float4*>(records), T, n_max, step_size,
+                     static_cast<HitRec*>(hits), count, tot, hit_flag, bound, order);
+  return check_launch("render_hits_ordered");
 }
 
 static int check_camera(const nsvf_camera* cam, const char* what) {
@@ -1147,7 +1223,8 @@ int nsvf_render_hits_camera(const nsvf_camera* cam, const void* records, int T, 
   hipLaunchKernelGGL(hits_kernel<true>, dim3(unsigned(ceil_div(threads, kHitsBlock))),
                      dim3(kHitsBlock), 0, as_stream(stream), nullptr, nullptr, R, cam->w, *cam,
                      su, sv, ray_start, ray_dir, static_cast<const float4*>(records), T, n_max,
-                     step_size, static_cast<HitRec*>(hits), count, tot, hit_flag, bound);
+                     step_size, static_cast<HitRec*>(hits), count, tot, hit_flag, bound,
+                     nullptr);
   return check_launch("render_hits_camera");
 }
 

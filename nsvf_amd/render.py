@@ -127,6 +127,27 @@ def pack_tree(centers, children, half_voxel):
     return rec.reshape(T + 1, 16, 4)
 
 
+def coherent_order(ray_start, ray_dir):
+    """A traversal grouping for rays that are not image rows: int32 [R] on
+    the rays' device, the ray indices sorted by direction octant (the union
+    walk's child order), then origin cell (a 16^3 grid over the origins'
+    bounds), then the Morton code of the direction (10 bits per axis), so
+    that each wave's 64 rays are close in origin and direction
+    (nsvf_ray_order_keys, one pass, then a sort).  Any order gives the same
+    hits (nsvf_render_hits_ordered)."""
+    rs = ray_start.contiguous().float()
+    rd = ray_dir.contiguous().float()
+    R = rs.shape[0]
+    keys = torch.empty(R, dtype=torch.int64, device=rs.device)
+    if R == 0:
+        return keys.to(torch.int32)
+    lo, hi = torch.aminmax(rs, dim=0)
+    bounds = torch.cat([lo, hi])
+    check(lib().nsvf_ray_order_keys(_p(rs), _p(rd), R, _p(bounds), _p(keys), _stream(rs.device)),
+          "ray_order_keys")
+    return torch.argsort(keys).to(torch.int32)
+
+
 class DeviceScene:
     """A voxel scene resident on one GPU: shifted voxel centres (gather),
     octree over the unshifted centres (intersection), corner ids, embeddings
@@ -491,7 +512,7 @@ class Renderer:
     # ---- phase 1: intersection, compaction, slot offsets
 
     @torch.no_grad()
-    def begin(self, sc: DeviceScene, ray_start, ray_dir, width=0, segments=None):
+    def begin(self, sc: DeviceScene, ray_start, ray_dir, width=0, segments=None, order=None):
         """Enqueues the geometry stages of a view on the current stream:
         intersection, hit compaction and the coarse slot offsets
         (encoder.py:498-535), then an asynchronous copy of the hit-ray count
@@ -502,7 +523,11 @@ class Renderer:
         as one batch, their ray counts in order (at most 16; the rays are
         their concatenation); each view's hierarchical pass then normalises
         over its own longest coarse row, so every view gets the bits it
-        gets alone (with the sampler's evident-intent mode)."""
+        gets alone (with the sampler's evident-intent mode).  order: instead
+        of width, the traversal's grouping of the rays (int32 [R] device
+        tensor, a permutation -- e.g. coherent_order(ray_start, ray_dir) --
+        for rays that are not image rows; the results do not depend on
+        it)."""
         ray_start = ray_start.contiguous()
         ray_dir = ray_dir.contiguous()
         R = ray_start.shape[0]
@@ -515,7 +540,19 @@ class Renderer:
             if len(segments) == 1:
                 segments = None
 
+        if order is not None:
+            order = order.contiguous()
+            if order.dtype != torch.int32 or order.shape != (R,) or order.device != ray_start.device:
+                raise ValueError(f"order must be an int32 [{R}] tensor on the rays' device")
+
         def intersect(sc, P, hits, count, tot, flag, c_bound, st):
+            if order is not None:
+                check(lib().nsvf_render_hits_ordered(_p(ray_start), _p(ray_dir), R, _p(order),
+                                                     _p(sc.tree_records), sc.num_nodes,
+                                                     sc.tree_depth, P, sc.step_size, _p(hits),
+                                                     _p(count), _p(tot), _p(flag), _p(c_bound),
+                                                     st), "render_hits_ordered")
+                return
             check(lib().nsvf_render_hits(_p(ray_start), _p(ray_dir), R, int(width),
                                          _p(sc.tree_records), sc.num_nodes, sc.tree_depth, P,
                                          sc.step_size,
@@ -634,11 +671,12 @@ class Renderer:
                             self._buf("f_bound", R, torch.int32), scal)
 
     @torch.no_grad()
-    def coarse_stream(self, sc: DeviceScene, ray_start, ray_dir):
+    def coarse_stream(self, sc: DeviceScene, ray_start, ray_dir, order=None):
         """The geometry stages of a view: intersection, hit compaction and the
         coarse sample stream (encoder.py:498-556).  Returns a CoarseStream
-        whose tensors are workspace views, valid until the next call."""
-        self.begin(sc, ray_start, ray_dir)
+        whose tensors are workspace views, valid until the next call.
+        order: as in begin()."""
+        self.begin(sc, ray_start, ray_dir, order=order)
         g = self._sample_coarse()
         self._pending = None
         return g

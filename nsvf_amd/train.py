@@ -29,7 +29,7 @@ import torch.nn.functional as F
 
 from ._lib import check, lib
 from .field import pack_layer_bf16
-from .render import BG_COLOR, BG_DEPTH, CoarseStream, DeviceScene, Renderer
+from .render import BG_COLOR, BG_DEPTH, CoarseStream, DeviceScene, Renderer, coherent_order
 from .scene import VoxelScene
 
 
@@ -389,11 +389,15 @@ class TrainRenderer:
     octree, shifted centres and corner ids) with a TrainableField's
     parameters: intersect + coarse samples (non-differentiable kernels), then
     gather, MLP and compositing with autograd, then the background fill
-    (nsvf.py:94-101)."""
+    (nsvf.py:94-101).  coherent: the traversal takes the batch's rays
+    grouped by direction and origin (coherent_order; a batch of random
+    pixels from several views has no image rows to tile), which changes no
+    result."""
 
-    def __init__(self, device, compat=True, deterministic_step=False, seed=0):
+    def __init__(self, device, compat=True, deterministic_step=False, seed=0, coherent=True):
         self.device = torch.device(device)
         self.geometry = Renderer(device, precision="fp32", hierarchical=False, compat=compat)
+        self.coherent = bool(coherent)
         # training samples with noise unless --deterministic-step
         # (encoder.py:546: deterministic = deterministic_step or not training);
         # forward() k of this renderer uses noise seed `seed + k` (mod 2^32)
@@ -418,7 +422,8 @@ class TrainRenderer:
         self.steps += 1
         self.last_seed = seed
         self.geometry.noise_seed = seed
-        g = self.geometry.coarse_stream(sc, rs, rd).owned()
+        order = coherent_order(rs, rd) if self.coherent and rs.shape[0] > 0 else None
+        g = self.geometry.coarse_stream(sc, rs, rd, order=order).owned()
         self.last = g
         rec = g.samples
         s_idx = rec[:, 0]

@@ -164,3 +164,13 @@ def test_new_entry_points_reject_bad_arguments():
     assert b"shape" in L.nsvf_last_error()
     assert L.nsvf_field_tiles_bf16x3(32, 128, 3, 1, 100, 1, need - 1, None) == -1
     assert b"workspace" in L.nsvf_last_error()
+    # the ordered traversal: its order pointer is required
+    assert L.nsvf_render_hits_ordered(1, 1, 10, None, 1, 5, 2, 60, 0.1, 1, 1, 1, 1, 1, None) == -1
+    assert b"null" in L.nsvf_last_error()
+    assert L.nsvf_render_hits_ordered(None, None, 0, None, 1, 5, 2, 60, 0.1, None, None, None,
+                                      None, None, None) == 0
+    assert L.nsvf_render_hits_ordered(1, 1, 10, 1, 1, 5, 25, 60, 0.1, 1, 1, 1, 1, 1, None) == -3
+    assert L.nsvf_ray_order_keys(None, None, 10, None, None, None) == -1
+    assert b"null" in L.nsvf_last_error()
+    assert L.nsvf_ray_order_keys(None, None, 0, None, None, None) == 0
+    assert L.nsvf_ray_order_keys(None, None, -1, None, None, None) == -1

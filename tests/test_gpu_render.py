@@ -16,7 +16,7 @@ import nsvf_oracle as O
 from _cases import shell, shell_rays
 from nsvf_amd import _lib
 from nsvf_amd import scene as S
-from nsvf_amd.render import DeviceScene, Renderer
+from nsvf_amd.render import DeviceScene, Renderer, coherent_order
 
 pytestmark = pytest.mark.gpu
 
@@ -343,6 +343,77 @@ def test_tiled_traversal_order_is_invisible(device):
         assert np.array_equal(a, b)
     with pytest.raises(ValueError):
         Renderer(device).render(dsc, rs, rd, width=7)
+
+
+def test_ordered_traversal_is_invisible(device):
+    """A given lane-to-ray order (nsvf_render_hits_ordered: a random
+    permutation, and the coherent grouping by direction and origin) changes
+    which rays share a wave, never a result: hits, counts, sums and pixels
+    equal the run in ray order bit for bit, on rays from four cameras
+    shuffled as a training batch; a malformed order is refused."""
+    sc = shell()
+    dsc = DeviceScene(sc, device)
+    views = [S.camera_rays(24, 24, S.sphere_camera(2.5, 30 + v)) for v in range(4)]
+    rs_np = np.concatenate([v[0] for v in views])
+    rd_np = np.concatenate([v[1] for v in views])
+    mix = np.random.default_rng(5).permutation(rs_np.shape[0])
+    rs = torch.from_numpy(rs_np[mix]).to(device)
+    rd = torch.from_numpy(rd_np[mix]).to(device)
+    R = rs.shape[0]
+    orders = [None, torch.from_numpy(np.random.default_rng(6).permutation(R).astype(np.int32)).to(device),
+              coherent_order(rs, rd)]
+    assert sorted(orders[2].cpu().tolist()) == list(range(R))
+    outs = []
+    for order in orders:
+        r = Renderer(device, precision="fp32", hierarchical=True)
+        r.begin(dsc, rs, rd, order=order)
+        img = [N(t) for t in r.finish(keep=True)]
+        k = r.kept
+        outs.append((img, N(k["count"]), N(k["hits"]), N(k["tot"])))
+    (i0, c0, h0, t0) = outs[0]
+    assert c0.sum() > 0
+    for (i1, c1, h1, t1) in outs[1:]:
+        assert np.array_equal(c0, c1) and np.array_equal(t0, t1)
+        for ray in np.nonzero(c0)[0]:
+            assert np.array_equal(h0[ray, :c0[ray]], h1[ray, :c0[ray]]), ray
+        for a, b in zip(i0, i1):
+            assert np.array_equal(a, b)
+    with pytest.raises(ValueError):
+        Renderer(device).begin(dsc, rs, rd, order=orders[1][:-1])
+
+
+def test_ray_order_keys(device):
+    """nsvf_ray_order_keys against a NumPy restatement of its layout:
+    octant << 42 | origin cell (16^3 over the bounds) << 30 | Morton code
+    of the direction quantised to 10 bits per axis; -0 counts as positive,
+    as in the traversal's octant."""
+    rng = np.random.default_rng(8)
+    R = 1000
+    rs = rng.uniform(-3, 3, (R, 3)).astype(np.float32)
+    rd = rng.standard_normal((R, 3)).astype(np.float32)
+    rd /= np.linalg.norm(rd, axis=1, keepdims=True)
+    rd[:5, 0] = np.float32(-0.0)
+    lo, hi = rs.min(0), rs.max(0)
+    bounds = torch.from_numpy(np.concatenate([lo, hi])).to(device)
+    keys = torch.empty(R, dtype=torch.int64, device=device)
+    assert _lib.lib().nsvf_ray_order_keys(torch.from_numpy(rs).to(device).data_ptr(),
+                                          torch.from_numpy(rd).to(device).data_ptr(), R,
+                                          bounds.data_ptr(), keys.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+
+    def spread(q):
+        out = np.zeros_like(q)
+        for b in range(10):
+            out |= ((q >> b) & 1) << (3 * b)
+        return out
+    oct_ = sum((rd[:, a] < 0).astype(np.int64) << a for a in range(3))
+    span = np.maximum(hi - lo, np.float32(1e-30))
+    c = np.clip(((rs - lo) / span * np.float32(15.999)).astype(np.int64), 0, 15)
+    cell = (c[:, 0] << 8) | (c[:, 1] << 4) | c[:, 2]
+    q = np.clip(np.rint((np.clip(rd, -1, 1) + np.float32(1)) * np.float32(511.5)), 0, 1023)
+    q = q.astype(np.int64)
+    morton = spread(q[:, 0]) | (spread(q[:, 1]) << 1) | (spread(q[:, 2]) << 2)
+    assert np.array_equal(N(keys), (oct_ << 42) | (cell << 30) | morton)
 
 
 @pytest.mark.parametrize("precision", ["bf16x3", "bf16"])
