@@ -148,6 +148,25 @@ def coherent_order(ray_start, ray_dir):
     return torch.argsort(keys).to(torch.int32)
 
 
+_TILE_ORDERS = {}
+
+
+def image_tile_order(h, w, device):
+    """The ray indices of an h x w image (row-major rays) in 8x8 pixel
+    tiles -- tiles row by row, pixels row by row within a tile -- as int32 on
+    `device` (cached per size): the grouping the traversal uses for a width
+    hint, as an order for images batched with others of other widths."""
+    key = (int(h), int(w), str(device))
+    t = _TILE_ORDERS.get(key)
+    if t is None:
+        i = np.arange(h * w, dtype=np.int64)
+        y, x = i // w, i % w
+        k = ((y // 8) * ((w + 7) // 8) + x // 8) * 64 + (y % 8) * 8 + (x % 8)
+        t = torch.from_numpy(np.argsort(k, kind="stable").astype(np.int32)).to(device)
+        _TILE_ORDERS[key] = t
+    return t
+
+
 class DeviceScene:
     """A voxel scene resident on one GPU: shifted voxel centres (gather),
     octree over the unshifted centres (intersection), corner ids, embeddings

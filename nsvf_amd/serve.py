@@ -113,9 +113,24 @@ class Job:
         self.rid = self.requests[0].rid
         widths = {r.width for r in self.requests}
         # one common image width keeps the traversal's pixel tiles (the
-        # concatenation is an image of the same width); mixed widths run the
-        # traversal over row strips, with the same results
+        # concatenation is an image of the same width); with mixed widths the
+        # traversal takes each image's pixels in the same tiles through a
+        # lane-to-ray order (tiled_order), or row strips when a request has
+        # no width -- the results are the same either way
         self.width = widths.pop() if len(widths) == 1 else 0
+        self.tiled = self.width == 0 and all(r.width > 0 for r in self.requests)
+
+    def tiled_order(self, device):
+        """The concatenated requests' rays in each image's 8x8 pixel tiles
+        (an order for Renderer.begin), for a batch of several widths."""
+        import torch
+
+        from .render import image_tile_order
+        parts, off = [], 0
+        for r in self.requests:
+            parts.append(image_tile_order(r.rays // r.width, r.width, device) + off)
+            off += r.rays
+        return torch.cat(parts)
 
     @property
     def rays(self) -> int:
@@ -395,6 +410,7 @@ class RenderSlot(Slot):
         self._done = None
         self._out = None
         self._job = None
+        self.tiled_batches = 0   # batches of several image widths traced in pixel tiles
         self._cat_bytes = 0
 
     def geometry_bytes(self, job):
@@ -419,7 +435,11 @@ class RenderSlot(Slot):
                     rd = torch.cat([r.ray_dir for r in job.requests])
                     self._cat_bytes = job.rays * 24
                     segments = [r.rays for r in job.requests]
-                self.renderer.begin(sc, rs, rd, job.width, segments=segments)
+                order = None
+                if job.tiled and segments:
+                    order = job.tiled_order(rs.device)
+                    self.tiled_batches += 1
+                self.renderer.begin(sc, rs, rd, job.width, segments=segments, order=order)
         except torch.OutOfMemoryError as e:
             raise SlotOutOfMemory(str(e).splitlines()[0]) from None
 

@@ -160,6 +160,8 @@ def test_serving_sized_scenes_mix_matches_alone(device):
     srv.drain()
     torch.cuda.synchronize()
     assert max(r.batch for r in reqs) > 1
+    print("batches of several widths traced in pixel tiles:",
+          sum(st.slot.tiled_batches for st in srv.scheduler.slots), flush=True)
     for req, (name, rs, rd, w) in zip(reqs, items):
         assert req.state == "done", (req.rid, req.state, req.error)
         r = Renderer(device, precision="bf16x3", hierarchical=True, compat=srv.compat)
