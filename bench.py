@@ -742,11 +742,13 @@ def serving(args, dev):
     800^2 and a camera at radius U(2, 3), arriving as a Poisson process at
     each rate; rays are on the device before the clock starts.  Reports
     rays/s, p50/p99 latency (arrival to completion), and the error of a
-    256^2 view of each scene against the plain FP32 path."""
+    256^2 view of each scene against the plain FP32 path (beside the fused
+    FP32 path's against the same)."""
     import torch
 
     from nsvf_amd import scene as S
     from nsvf_amd.plain import PlainNSVF
+    from nsvf_amd.render import DeviceScene, Renderer
     from nsvf_amd.serve import RenderServer, latency_summary
 
     t0 = time.time()
@@ -831,7 +833,12 @@ def serving(args, dev):
             f"alone time / span {summ['alone_time_per_span']}")
     allocs1 = torch.cuda.memory_stats(dev).get("num_alloc_retries", 0), \
         torch.cuda.memory_stats(dev).get("segment.all.allocated", 0)
-    # error against the plain FP32 path, one 256^2 view per scene
+    # error against the plain FP32 path, one 256^2 view per scene.  The fine
+    # samples' depths are continuous in the coarse field values, so two
+    # renders whose fields differ in the last bits place most fine samples a
+    # few ulps apart, and a ray whose sample crosses a gap between voxels
+    # changes by up to a few 1e-3; the fused FP32 path against the same plain
+    # path shows that spread between two FP32 implementations
     errs = {}
     for name in names:
         rs_np, rd_np = S.camera_rays(256, 256, S.sphere_camera(2.5, 3))
@@ -839,11 +846,17 @@ def serving(args, dev):
         req = srv.submit(name, rs, rd, width=256)
         srv.drain()
         pl = PlainNSVF(scenes[name], dev, compat=srv.compat)
-        prgb, _, _ = pl.render(rs, rd)
-        d = (req.result[0].double() - prgb.double())
-        mse = float(torch.mean(d * d))
-        errs[name] = {"max_abs_rgb": float(d.abs().max()),
-                      "psnr_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None}
+        prgb = pl.render(rs, rd)[0].double()
+        frgb = Renderer(dev, precision="fp32", compat=srv.compat).render(
+            DeviceScene(scenes[name], dev), rs, rd, width=256)[0].double()
+        d = (req.result[0].double() - prgb).abs().amax(1)
+        df = (frgb - prgb).abs().amax(1)
+        mse = float(torch.mean((req.result[0].double() - prgb) ** 2))
+        errs[name] = {"max_abs_rgb": float(d.max()),
+                      "pixels_over_1e-3": int((d > 1e-3).sum()),
+                      "psnr_db": round(10 * math.log10(1.0 / mse), 2) if mse > 0 else None,
+                      "fp32_fused_max_abs_rgb": float(df.max()),
+                      "fp32_fused_pixels_over_1e-3": int((df > 1e-3).sum())}
         del pl
         torch.cuda.empty_cache()
     sat = max(runs, key=lambda r: r["rays_per_sec"] or 0)
