@@ -11,6 +11,8 @@ where traversal order decides the result).
 
 Bars:
   * hit flags, counts and sorted hit rows bit-exact on every ray of the view;
+    the same rows and the same FP32 image with the rays taken in a given
+    lane order (coherent grouping, random permutation);
   * coarse sample streams bit-exact on every hit ray; fine streams bit-exact
     given the GPU's coarse probabilities; sampler mode on and off;
   * FP32 field: RGB, depth and alpha within 1e-5 of FP64 with the same
@@ -34,7 +36,7 @@ import torch
 
 import nsvf_oracle as O
 from nsvf_amd import scene as S
-from nsvf_amd.render import DeviceScene, Renderer
+from nsvf_amd.render import DeviceScene, Renderer, coherent_order
 from nsvf_amd.voxels import prune_scores, split_scene
 
 pytestmark = pytest.mark.gpu
@@ -157,6 +159,33 @@ def test_c2_hit_rows_bit_exact(c2, views):
     steps = g["tot"][g["flag"]] / np.float32(sc.step_size)
     assert np.array_equal(steps.astype(np.float32), ref["steps"])
     log(f"hit rows bit-exact on {rs.shape[0]} rays ({int((cnt == P).sum())} full rows)")
+
+
+def test_c2_ordered_traversal_bit_exact(c2, views, device):
+    """The traversal with a given lane-to-ray order (the coherent grouping
+    by direction and origin, and a random permutation) on the 226k-node
+    tree: hit flags, rows, counts and sums, and the FP32 image, equal the
+    view in 8x8 pixel tiles -- whose rows the previous test holds to the
+    oracle -- bit for bit."""
+    sc, rs, rd, dsc = c2
+    g = views["g32"]
+    rs_t, rd_t = torch.from_numpy(rs).to(device), torch.from_numpy(rd).to(device)
+    R, P = rs.shape[0], sc.max_hits
+    perm = torch.from_numpy(np.random.default_rng(8).permutation(R).astype(np.int32)).to(device)
+    for name, order in (("coherent", coherent_order(rs_t, rd_t)), ("random", perm)):
+        r = Renderer(device, precision="fp32", hierarchical=True, compat=views["compat"])
+        r.begin(dsc, rs_t, rd_t, order=order)
+        img = [N(t) for t in r.finish(keep=True)]
+        k = r.kept
+        cnt = N(k["count"])
+        assert np.array_equal(N(k["flag"]).astype(bool), g["flag"]), name
+        assert np.array_equal(cnt, g["count"]) and np.array_equal(N(k["tot"]), g["tot"]), name
+        m = np.arange(P)[None] < cnt[:, None]
+        assert np.array_equal(np.where(m[..., None], N(k["hits"]), 0),
+                              np.where(m[..., None], g["hits"], 0)), name
+        for a, b in zip(img, (g["rgb"], g["depth"], g["missed"])):
+            assert np.array_equal(a, b), name
+    log(f"ordered traversal bit-exact on {R} rays (compat={views['compat']})")
 
 
 def _stream_equal(cnt, off, cols, ri, rd_, rx):
