@@ -35,15 +35,16 @@ def _stream(dev):
     return ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
 
 
-def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536):
+def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536, x3_chunk=None):
     """[N] float32: max over each voxel's bits^3 lattice of relu(sigma)
     (encoder.py:620-646).  FP32 MFMA by default, as the reference evaluates
     the field in FP32; "bf16" is faster and may flip voxels whose max is
     within about 1% of the threshold; "bf16x3" (split BF16, the renderer's
     served precision) keeps FP32-level accuracy at a fraction of the FP32
-    time."""
+    time.  chunk: voxels per launch of the FP32 / BF16 kernels; x3_chunk:
+    voxels per chunk of the split-BF16 path (default 2048)."""
     if precision == "bf16x3":
-        return _prune_scores_x3(sc, bits)
+        return _prune_scores_x3(sc, bits, **({} if x3_chunk is None else {"chunk_voxels": x3_chunk}))
     prec = PRECISIONS[precision]
     N = sc.centers.shape[0]
     out = torch.empty(N, dtype=torch.float32, device=sc.device)
@@ -57,46 +58,60 @@ def prune_scores(sc: DeviceScene, precision="fp32", bits=16, chunk=65536):
     return out
 
 
-def _prune_scores_x3(sc: DeviceScene, bits, chunk_voxels=256):
+def _prune_scores_x3(sc: DeviceScene, bits, chunk_voxels=2048):
     """prune_scores on the split-BF16 field kernels: each chunk of voxels'
     lattice points (x = c + fp32((2i + 1 - bits) / (bits - 1)) * (vs / 2)
     per axis, i slowest, as the FP32 path places them) become a stream of
     samples -- voxel id, the point as a ray start with direction 0 and
     depth 0, so the gather's x = o + d * z is the point itself -- through
     the stream's gather and MLP kernels, then the per-voxel max of relu
-    sigma."""
+    sigma.  Every point is evaluated on its own, so the chunking changes no
+    value; a chunk of 2048 voxels at bits = 16 is 8.4 M samples (1.1 GB of
+    operand tiles), enough that the per-chunk set-up costs little beside
+    the kernels, and the direction, record and workspace buffers are built
+    once for all chunks."""
     L = lib()
     dev = sc.device
     if not L.nsvf_field_bf16x3_supported(sc.D, sc.H, sc.L):
         raise RuntimeError(f"bf16x3 field not built for D={sc.D} H={sc.H} L={sc.L}")
+    if int(chunk_voxels) < 1:
+        raise ValueError(f"chunk_voxels {chunk_voxels}: at least 1")
     N = sc.centers.shape[0]
+    out = torch.empty(N, dtype=torch.float32, device=dev)
+    if N == 0:
+        return out
     per = bits ** 3
     i = torch.arange(bits, dtype=torch.float32, device=dev)
     off = (2 * i + 1 - bits) / float(bits - 1)          # float32 arithmetic
     half = torch.tensor(np.float32(sc.voxel_size) * np.float32(0.5), device=dev)
     ox, oy, oz = torch.meshgrid(off, off, off, indexing="ij")
     offs = torch.stack([ox.reshape(-1), oy.reshape(-1), oz.reshape(-1)], 1) * half
-    out = torch.empty(N, dtype=torch.float32, device=dev)
+    cv = min(int(chunk_voxels), N)
+    mmax = cv * per
+    dirs = torch.zeros((mmax, 3), dtype=torch.float32, device=dev)
+    rec = torch.zeros((mmax, 4), dtype=torch.int32, device=dev)
+    local = torch.arange(cv, dtype=torch.int32, device=dev).repeat_interleave(per)
+    rec[:, 3] = torch.arange(mmax, dtype=torch.int32, device=dev)
+    pts = torch.empty((cv, per, 3), dtype=torch.float32, device=dev)
+    field = torch.empty((mmax, 4), dtype=torch.float32, device=dev)
+    ws_bytes = int(L.nsvf_field_workspace_bytes(2, sc.D, sc.H, sc.L, mmax))
+    ws = torch.empty((ws_bytes + 15) // 16 * 4, dtype=torch.int32, device=dev)
+    m_dev = torch.full((1,), mmax, dtype=torch.int32, device=dev)
     st = _stream(dev)
     w = sc.mlp["bf16x3"]
-    for v0 in range(0, N, chunk_voxels):
-        n = min(chunk_voxels, N - v0)
+    for v0 in range(0, N, cv):
+        n = min(cv, N - v0)
         m = n * per
-        pts = (sc.centers[v0:v0 + n, None, :] + offs[None]).reshape(m, 3).contiguous()
-        dirs = torch.zeros_like(pts)
-        rec = torch.zeros((m, 4), dtype=torch.int32, device=dev)
-        rec[:, 0] = (torch.arange(n, dtype=torch.int32, device=dev) + v0).repeat_interleave(per)
-        rec[:, 3] = torch.arange(m, dtype=torch.int32, device=dev)
-        m_dev = torch.tensor([m], dtype=torch.int32, device=dev)
-        ws_bytes = int(L.nsvf_field_workspace_bytes(2, sc.D, sc.H, sc.L, m))
-        ws = torch.empty((ws_bytes + 15) // 16 * 4, dtype=torch.int32, device=dev)
-        field = torch.empty((m, 4), dtype=torch.float32, device=dev)
+        if n < cv:
+            m_dev.fill_(m)
+        torch.add(sc.centers[v0:v0 + n, None, :], offs[None], out=pts[:n])
+        torch.add(local[:m], v0, out=rec[:m, 0])
         check(L.nsvf_field_embed_bf16x3(sc.D, sc.H, sc.L, _p(sc.centers), _p(sc.feats),
                                         _p(sc.values), sc.voxel_size, _p(pts), _p(dirs), _p(rec),
                                         None, _p(m_dev), m, _p(ws), ws_bytes, st), "field_embed")
         check(L.nsvf_field_mlp_bf16x3(sc.D, sc.H, sc.L, _p(ws), ws_bytes, _p(m_dev), m, _p(w),
                                       _p(sc.mlp["bias"]), _p(field), None, st), "field_mlp")
-        out[v0:v0 + n] = field[:, 0].view(n, per).clamp(min=0).amax(1)
+        torch.amax(field[:m, 0].view(n, per).clamp(min=0), 1, out=out[v0:v0 + n])
     return out
 
 

@@ -133,3 +133,8 @@ def test_prune_scores_split_bf16_match_oracle(dscene):
     keep_ref = (1 - np.exp(-ref)) > 0.5
     keep = (1 - np.exp(-s3[sel].astype(np.float64))) > 0.5
     assert np.array_equal(keep[clear], keep_ref[clear])
+    # every lattice point is evaluated on its own: other chunkings (a
+    # partial last chunk, one chunk per voxel, one chunk for all) give the
+    # same bits
+    for cv in (256, 1, sc.num_voxels):
+        assert np.array_equal(prune_scores(dsc, "bf16x3", x3_chunk=cv).cpu().numpy(), s3), cv
