@@ -600,9 +600,10 @@ def rows(args, dev, sc, dsc):
     # one training step: forward, MSE against U(0,1) targets, backward --
     # with the HIP training MLP, and with torch's Linear layers for reference
     target = torch.rand((R, 3), device=dev)
-    mlp_names = {"hip": "HIP FP32 MFMA", "hip_x3": "HIP split-BF16 forward, FP32 MFMA backward",
+    mlp_names = {"hip": "HIP FP32 MFMA",
+                 "hip_x3": "HIP split BF16 (FP32 head and weight-gradient sums)",
                  "torch": "torch FP32 nn.Linear"}
-    for key, impl in (("train_step", "hip"), ("train_step_x3_forward", "hip_x3"),
+    for key, impl in (("train_step", "hip"), ("train_step_split_bf16", "hip_x3"),
                       ("train_step_torch_mlp", "torch")):
         tf = TrainableField(sc, dev, mlp=impl)
         tr = TrainRenderer(dev)
@@ -671,12 +672,14 @@ def rows(args, dev, sc, dsc):
     ms_f = _time_ms(lambda: FieldMLP.apply(emb, *prm), 5)
     ms_f3 = _time_ms(lambda: FieldMLPx3.apply(emb, *prm), 5)
     ms_fb = _time_ms(lambda: FieldMLP.apply(emb, *prm).backward(dout), 5)
+    ms_fb3 = _time_ms(lambda: FieldMLPx3.apply(emb, *prm).backward(dout), 5)
     # backward: recomputed forward + input gradients + weight gradients, plus
     # the first layer once more (about 3 forward FLOPs and one layer)
     ms_b = ms_fb - ms_f
     bw_flops = M * (3 * fl + 2 * dsc.D * dsc.H)
     res["mlp_train"] = {"forward_ms": round(ms_f, 3), "backward_ms": round(ms_b, 3),
-                        "forward_split_bf16_ms": round(ms_f3, 3), "samples": M,
+                        "forward_split_bf16_ms": round(ms_f3, 3),
+                        "backward_split_bf16_ms": round(ms_fb3 - ms_f3, 3), "samples": M,
                         "forward_TFLOPs": round(M * fl / (ms_f * 1e-3) / 1e12, 1),
                         "backward_TFLOPs": round(bw_flops / (ms_b * 1e-3) / 1e12, 1),
                         "peak_TFLOPs": 157.3}

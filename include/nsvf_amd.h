@@ -639,6 +639,25 @@ int nsvf_mlp_train_backward(int D, int H, int L, const float* packed,
                             float* demb, float* partial, int64_t partial_floats,
                             float* dparams, void* stream);
 
+/* The same backward with the activations' recomputation and the input
+ * gradients W^T dz in split BF16 (three BF16 MFMAs per product on hi / lo
+ * operand parts, hi = bf16(f), lo = bf16(f - hi), FP32 accumulation: about
+ * FP32's accuracy); the head and the weight-gradient sums stay FP32.  No
+ * counterpart in the reference (its backward is torch autograd over FP32
+ * Linear layers).  x3w: x3_frags(D, H, L) fragments of 1024 bf16 bit
+ * patterns (512 hi, then 512 lo), 16-byte aligned, in the order W_0 [H/16
+ * m-tiles][D/32 k-tiles], W_1..W_{L-1} [H/16][H/32], then the transposes
+ * W_0^T [D/16][H/32], W_1^T..W_{L-1}^T [H/16][H/32]; element (lane, i) of
+ * fragment (mt, kt) of a matrix A is A[16 mt + lane % 16][32 kt + k(i)]
+ * with k(i) = 4 (lane / 16) + i for i < 4, 16 + 4 (lane / 16) + i - 4
+ * otherwise.  `packed` is still nsvf_mlp_train_pack's (biases, head).      */
+int64_t nsvf_mlp_train_x3_frags(int D, int H, int L);
+int nsvf_mlp_train_backward_x3(int D, int H, int L, const float* packed,
+                               const int16_t* x3w, const float* emb, int64_t M,
+                               const float* dout, float* demb, float* partial,
+                               int64_t partial_floats, float* dparams,
+                               void* stream);
+
 /* Frame outputs for all R rays: hit rays' composite plus missed * background
  * (fairnr/models/nsvf.py:89-101); rays that hit nothing get the background. */
 int nsvf_render_finish(int64_t R, const int32_t* hit_flag, const float* ray_rgb,

@@ -78,6 +78,8 @@ SIGNATURES = {
     "nsvf_mlp_train_pack": ([_I, _I, _I, _P, _P, _P, _P], _I),
     "nsvf_mlp_train_forward": ([_I, _I, _I, _P, _P, _L, _P, _P], _I),
     "nsvf_mlp_train_backward": ([_I, _I, _I, _P, _P, _L, _P, _P, _P, _L, _P, _P], _I),
+    "nsvf_mlp_train_x3_frags": ([_I, _I, _I], _L),
+    "nsvf_mlp_train_backward_x3": ([_I, _I, _I, _P, _P, _P, _L, _P, _P, _P, _L, _P, _P], _I),
     "nsvf_render_eval_list_count": ([_P, _P, _L, _P, _P], _I),
     "nsvf_render_eval_list": ([_P, _P, _L, _P, _P, _P], _I),
     "nsvf_render_reuse_fill": ([_P, _P, _L, _P, _P, _P, _P, _P, _P, _P], _I),

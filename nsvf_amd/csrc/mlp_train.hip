@@ -27,6 +27,20 @@
 // the blocks' partials in block order.  The sums therefore run in a fixed
 // order for a given sample count: the gradients are deterministic, with no
 // atomics.
+//
+// Split-BF16 backward (mlp_train_backward_kernel<..., true>).  The same
+// kernel with the forward recomputation of the hidden layers and the input
+// gradients dX^T = W^T dz^T on v_mfma_f32_16x16x32_bf16 in split BF16: an
+// FP32 operand f is hi + lo with hi = bf16(f), lo = bf16(f - hi), and a
+// product is Wl Xh + Wh Xl + Wh Xh (FP32 accumulation; the dropped Wl Xl is
+// 2^-16 of it), so these products keep about FP32's accuracy at three BF16
+// MFMAs per 16x16x32 tile against eight FP32 16x16x4 MFMAs.  The weight
+// fragments (forward and transposed, hi and lo) come from global memory,
+// 288 KB that stay in L2; a k-tile's operand lane (g, c) holds features
+// 32 kt + 4 g + i and 32 kt + 16 + 4 g + i (i < 4) of sample c -- the
+// accumulators of m-tiles 2 kt and 2 kt + 1, so a layer's FP32 output
+// splits into the next layer's operands in registers.  The head (4 wide)
+// and the weight-gradient sums over samples stay in FP32.
 #include <algorithm>
 
 #include "common.h"
@@ -46,6 +60,57 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4;
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// (a, b) as packed bf16 pairs hi = bf16(a, b) and lo = bf16(a - hi, b - hi)
+// (round to nearest even; the differences are exact in FP32).
+__device__ __forceinline__ void split2(float a, float b, unsigned int& hi, unsigned int& lo) {
+  const bf16x2 h = __builtin_convertvector(f32x2{a, b}, bf16x2);
+  const f32x2 hf = __builtin_convertvector(h, f32x2);
+  const bf16x2 l = __builtin_convertvector(f32x2{a - hf[0], b - hf[1]}, bf16x2);
+  hi = __builtin_bit_cast(unsigned int, h);
+  lo = __builtin_bit_cast(unsigned int, l);
+}
+
+// The split operands of k-tile kt from a lane's FP32 values v[4 mt + r]
+// (feature 16 mt + 4 g + r): positions 0-3 from m-tile 2 kt, 4-7 from
+// m-tile 2 kt + 1.
+template <int N>
+__device__ __forceinline__ void split_ktile(const float (&v)[N], int kt, bf16x8& hi,
+                                            bf16x8& lo) {
+  unsigned int h[4], l[4];
+#pragma unroll
+  for (int p = 0; p < 4; ++p) split2(v[8 * kt + 2 * p], v[8 * kt + 2 * p + 1], h[p], l[p]);
+  hi = __builtin_bit_cast(bf16x8, u32x4{h[0], h[1], h[2], h[3]});
+  lo = __builtin_bit_cast(bf16x8, u32x4{l[0], l[1], l[2], l[3]});
+}
+
+// acc + the split product over KT k-tiles with fragments F (per fragment
+// 64 hi then 64 lo bf16x8 lane vectors), fragment kt at F + 128 kt; per
+// k-tile in the order Wl Xh, Wh Xl, Wh Xh -- the renderer's split-BF16 MLP
+// kernel's order, with the bias as the initial accumulator there too, so a
+// recomputed layer equals that kernel's forward bit for bit.
+template <int KT>
+__device__ __forceinline__ f32x4 mfma3_chain(const bf16x8* __restrict__ F,
+                                             const bf16x8 (&xh)[KT], const bf16x8 (&xl)[KT],
+                                             int lane, f32x4 acc) {
+#pragma unroll
+  for (int kt = 0; kt < KT; ++kt) {
+    const bf16x8 wh = F[128 * kt + lane], wl = F[128 * kt + 64 + lane];
+    acc = mfma32(wl, xh[kt], acc);
+    acc = mfma32(wh, xl[kt], acc);
+    acc = mfma32(wh, xh[kt], acc);
+  }
+  return acc;
+}
+
 // Closes one m-tile's loads and MFMAs: left to itself the scheduler hoists
 // the weight loads of every later m-tile (they depend on nothing) and runs
 // out of registers.
@@ -54,7 +119,8 @@ __device__ __forceinline__ void tile_fence() { __builtin_amdgcn_sched_barrier(0)
 // The weights' address as a value the compiler cannot see through: the
 // weight loads of a tile loop are loop-invariant, and hoisted out of it they
 // would need hundreds of registers.
-__device__ __forceinline__ const float* opaque(const float* p) {
+template <class P>
+__device__ __forceinline__ P opaque(P p) {
   asm volatile("" : "+s"(p));
   return p;
 }
@@ -97,6 +163,14 @@ struct TrainDims {
   static constexpr int64_t P_LAYER1 = P_B0 + H;
   static constexpr int64_t P_HEAD = P_LAYER1 + int64_t(L - 1) * (int64_t(H) * H + H);
   static constexpr int64_t PARAMS = P_HEAD + 4 * H + 4;
+  // split-BF16 fragments (1024 bf16 each: 512 hi, 512 lo), 32-wide k-tiles:
+  // layer 0 [MT][D/32], layers 1..L-1 [MT][H/32], then transposed: layer 0
+  // [MT0][H/32], layers 1..L-1 [MT][H/32]
+  static constexpr int KX0 = D / 32, KXH = H / 32;
+  static constexpr int X_HIDDEN = MT * KX0;
+  static constexpr int X_T0 = X_HIDDEN + (L - 1) * MT * KXH;
+  static constexpr int X_THIDDEN = X_T0 + MT0 * KXH;
+  static constexpr int X_FRAGS = X_THIDDEN + (L - 1) * MT * KXH;
 };
 
 // out[mt] = relu(W x + b) over the MT m-tiles: fragments A [MT][KT], input x
@@ -145,6 +219,71 @@ __device__ __forceinline__ void forward_tile(const float* __restrict__ emb, int6
   z = mfma_chain<T::KTH>(packed + T::F_HEAD * 64, h[L - 1], lane);
 }
 
+// ReLU as an integer max on the float's bits (the split-BF16 MLP kernel's).
+__device__ __forceinline__ float relu_bits(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
+}
+
+// dense_relu in split BF16: fragments A [MT][KT] (bf16x8 lane vectors,
+// 128 per fragment), input operands xh / xl of KT k-tiles.
+template <int KT, int MT>
+__device__ __forceinline__ void dense_relu_x3(const bf16x8* __restrict__ A,
+                                              const bf16x8 (&xh)[KT], const bf16x8 (&xl)[KT],
+                                              const float* __restrict__ b, float (&y)[4 * MT],
+                                              int lane, int g) {
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const f32x4 bias = *reinterpret_cast<const f32x4*>(b + 16 * mt + 4 * g);
+    const f32x4 acc = mfma3_chain<KT>(A + 128 * mt * KT, xh, xl, lane, bias);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) y[4 * mt + r] = relu_bits(acc[r]);
+    tile_fence();
+  }
+}
+
+// Layer 0 in split BF16 from the input x0 (k-tile order of forward_tile).
+template <int D, int H, int L>
+__device__ __forceinline__ void layer0_x3(const bf16x8* __restrict__ X, const float (&x0)[D / 4],
+                                          const float* __restrict__ bias, float (&h0)[H / 4],
+                                          int lane, int g) {
+  using T = TrainDims<D, H, L>;
+  bf16x8 xh[T::KX0], xl[T::KX0];
+#pragma unroll
+  for (int kt = 0; kt < T::KX0; ++kt) split_ktile(x0, kt, xh[kt], xl[kt]);
+  dense_relu_x3<T::KX0, T::MT>(X, xh, xl, bias, h0, lane, g);
+}
+
+// forward_tile with the hidden layers in split BF16 (fragments X); the head
+// stays FP32.
+template <int D, int H, int L>
+__device__ __forceinline__ void forward_tile_x3(const float* __restrict__ emb, int64_t M,
+                                                const float* __restrict__ packed,
+                                                const bf16x8* __restrict__ X, int64_t s,
+                                                float (&x0)[D / 4], float (&h)[L][H / 4],
+                                                f32x4& z, int lane, int g) {
+  using T = TrainDims<D, H, L>;
+  const float* bias = packed + T::BIAS_BASE;
+#pragma unroll
+  for (int t = 0; t < D / 16; ++t) {
+    float4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (s < M) v = *reinterpret_cast<const float4*>(emb + s * D + 16 * t + 4 * g);
+    x0[4 * t + 0] = v.x;
+    x0[4 * t + 1] = v.y;
+    x0[4 * t + 2] = v.z;
+    x0[4 * t + 3] = v.w;
+  }
+  layer0_x3<D, H, L>(X, x0, bias, h[0], lane, g);
+#pragma unroll
+  for (int l = 1; l < L; ++l) {
+    bf16x8 xh[T::KXH], xl[T::KXH];
+#pragma unroll
+    for (int kt = 0; kt < T::KXH; ++kt) split_ktile(h[l - 1], kt, xh[kt], xl[kt]);
+    dense_relu_x3<T::KXH, T::MT>(X + 128 * (T::X_HIDDEN + (l - 1) * T::MT * T::KXH), xh, xl,
+                                 bias + l * H, h[l], lane, g);
+  }
+  z = mfma_chain<T::KTH>(packed + T::F_HEAD * 64, h[L - 1], lane);
+}
+
 // out[s] = (sigma, sigmoid(r), sigmoid(g), sigmoid(b)) of emb[s] for s < M.
 template <int D, int H, int L>
 __global__ __launch_bounds__(kTrainBlock) void mlp_train_forward_kernel(
@@ -175,10 +314,13 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_forward_kernel(
 // Backward: demb [M][D] and this block's partial parameter gradients
 // (partial + blockIdx.x * PARAMS, torch order), from dout [M][4] = the
 // gradient of (sigma, r, g, b) after the sigmoid.  Needs H == 16 * waves.
-template <int D, int H, int L>
+// X3: the forward recomputation and the input gradients of layers 0..L-1
+// in split BF16 with the fragments x3w (TrainDims::X_FRAGS of them).
+template <int D, int H, int L, bool X3>
 __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
     const float* __restrict__ emb, int64_t M, const float* __restrict__ packed,
-    const float* __restrict__ dout, float* __restrict__ demb, float* __restrict__ partial) {
+    const bf16x8* __restrict__ x3w, const float* __restrict__ dout, float* __restrict__ demb,
+    float* __restrict__ partial) {
   using T = TrainDims<D, H, L>;
   static_assert(T::MT == kTrainWaves, "one hidden m-tile per wave");
   static_assert(D % 16 == 0 && D <= H, "input width");
@@ -212,7 +354,10 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
     float x0[T::KT0];
     float h[L][T::KTH];
     f32x4 z;
-    forward_tile<D, H, L>(emb, M, pk, s, x0, h, z, lane, g);
+    if constexpr (X3)
+      forward_tile_x3<D, H, L>(emb, M, pk, opaque(x3w), s, x0, h, z, lane, g);
+    else
+      forward_tile<D, H, L>(emb, M, pk, s, x0, h, z, lane, g);
     // dlogits on the lanes of group 0: d sigma as given, d rgb through the
     // sigmoid's derivative y (1 - y)
     f32x4 dl = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -270,7 +415,12 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
       if (l > 0) {
         // the first layer's output again, from x0: keeping it from the
         // forward pass would hold 32 more registers through every later phase
-        if (l == 1) dense_relu<T::KT0, T::MT>(opaque(packed), x0, bias, h[0], lane, g);
+        if (l == 1) {
+          if constexpr (X3)
+            layer0_x3<D, H, L>(opaque(x3w), x0, bias, h[0], lane, g);
+          else
+            dense_relu<T::KT0, T::MT>(opaque(packed), x0, bias, h[0], lane, g);
+        }
 #pragma unroll
         for (int mt = 0; mt < T::MT; ++mt)
           *reinterpret_cast<f32x4*>(my_x + 16 * mt + 4 * g) =
@@ -301,7 +451,31 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
         gb[l] = mfma4(a, 1.0f, gb[l]);
       }
       __syncthreads();
-      if (l > 0) {
+      if constexpr (X3) {
+        bf16x8 dzh[T::KXH], dzl[T::KXH];
+#pragma unroll
+        for (int kt = 0; kt < T::KXH; ++kt) split_ktile(dz, kt, dzh[kt], dzl[kt]);
+        const bf16x8* Xt = opaque(x3w);
+        if (l > 0) {
+          const bf16x8* A = Xt + 128 * (T::X_THIDDEN + (l - 1) * T::MT * T::KXH);
+#pragma unroll
+          for (int mt = 0; mt < T::MT; ++mt) {
+            const f32x4 acc = mfma3_chain<T::KXH>(A + 128 * mt * T::KXH, dzh, dzl, lane,
+                                                  f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dh[4 * mt + r] = acc[r];
+            tile_fence();
+          }
+        } else {
+#pragma unroll
+          for (int mt = 0; mt < T::MT0; ++mt) {
+            const f32x4 acc = mfma3_chain<T::KXH>(Xt + 128 * (T::X_T0 + mt * T::KXH), dzh,
+                                                  dzl, lane, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+            if (s < M) *reinterpret_cast<f32x4*>(demb + s * D + 16 * mt + 4 * g) = acc;
+            tile_fence();
+          }
+        }
+      } else if (l > 0) {
         const float* A = At + (T::T_HIDDEN + (l - 1) * T::MT * T::KTH) * 64;
 #pragma unroll
         for (int mt = 0; mt < T::MT; ++mt) {
@@ -474,29 +648,59 @@ int nsvf_mlp_train_forward(int D, int H, int L, const float* packed, const float
   return check_launch("mlp_train_forward");
 }
 
+int64_t nsvf_mlp_train_x3_frags(int D, int H, int L) {
+  if (!train_shape_ok(D, H, L)) return -1;
+  return TrainDims<32, 128, 3>::X_FRAGS;
+}
+
+static int train_backward(const char* name, int D, int H, int L, const float* packed,
+                          const int16_t* x3w, const float* emb, int64_t M, const float* dout,
+                          float* demb, float* partial, int64_t partial_floats, float* dparams,
+                          void* stream) {
+  if (!train_shape_ok(D, H, L))
+    return set_error(NSVF_EINVAL, "%s: shape D=%d H=%d L=%d (built: 32, 128, 3)", name, D, H, L);
+  if (M < 0) return set_error(NSVF_EINVAL, "%s: M=%lld", name, static_cast<long long>(M));
+  if (!packed || !partial || !dparams || (M > 0 && (!emb || !dout || !demb)))
+    return set_error(NSVF_EINVAL, "%s: null pointer", name);
+  using T = TrainDims<32, 128, 3>;
+  const int nb = backward_blocks(M);
+  if (partial_floats < int64_t(nb) * T::PARAMS)
+    return set_error(NSVF_EINVAL, "%s: partial buffer of %lld floats, %lld needed", name,
+                     static_cast<long long>(partial_floats),
+                     static_cast<long long>(int64_t(nb) * T::PARAMS));
+  hipStream_t st = as_stream(stream);
+  const bf16x8* x3 = reinterpret_cast<const bf16x8*>(x3w);
+  if (x3w)
+    hipLaunchKernelGGL((mlp_train_backward_kernel<32, 128, 3, true>), dim3(nb),
+                       dim3(kTrainBlock), kBackwardLds, st, emb, M, packed, x3, dout, demb,
+                       partial);
+  else
+    hipLaunchKernelGGL((mlp_train_backward_kernel<32, 128, 3, false>), dim3(nb),
+                       dim3(kTrainBlock), kBackwardLds, st, emb, M, packed, x3, dout, demb,
+                       partial);
+  if (check_launch(name) != NSVF_OK) return NSVF_ELAUNCH;
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(ceil_div(T::PARAMS, 256))), dim3(256), 0,
+                     st, partial, nb, T::PARAMS, dparams);
+  return check_launch("mlp_train_sum_partials");
+}
+
 int nsvf_mlp_train_backward(int D, int H, int L, const float* packed, const float* emb,
                             int64_t M, const float* dout, float* demb, float* partial,
                             int64_t partial_floats, float* dparams, void* stream) {
   clear_error();
-  if (!train_shape_ok(D, H, L))
-    return set_error(NSVF_EINVAL, "mlp_train_backward: shape D=%d H=%d L=%d (built: 32, 128, 3)",
-                     D, H, L);
-  if (M < 0) return set_error(NSVF_EINVAL, "mlp_train_backward: M=%lld", static_cast<long long>(M));
-  if (!packed || !partial || !dparams || (M > 0 && (!emb || !dout || !demb)))
-    return set_error(NSVF_EINVAL, "mlp_train_backward: null pointer");
-  using T = TrainDims<32, 128, 3>;
-  const int nb = backward_blocks(M);
-  if (partial_floats < int64_t(nb) * T::PARAMS)
-    return set_error(NSVF_EINVAL, "mlp_train_backward: partial buffer of %lld floats, %lld needed",
-                     static_cast<long long>(partial_floats),
-                     static_cast<long long>(int64_t(nb) * T::PARAMS));
-  hipStream_t st = as_stream(stream);
-  hipLaunchKernelGGL((mlp_train_backward_kernel<32, 128, 3>), dim3(nb), dim3(kTrainBlock),
-                     kBackwardLds, st, emb, M, packed, dout, demb, partial);
-  if (check_launch("mlp_train_backward") != NSVF_OK) return NSVF_ELAUNCH;
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(unsigned(ceil_div(T::PARAMS, 256))), dim3(256), 0,
-                     st, partial, nb, T::PARAMS, dparams);
-  return check_launch("mlp_train_sum_partials");
+  return train_backward("mlp_train_backward", D, H, L, packed, nullptr, emb, M, dout, demb,
+                        partial, partial_floats, dparams, stream);
+}
+
+int nsvf_mlp_train_backward_x3(int D, int H, int L, const float* packed, const int16_t* x3w,
+                               const float* emb, int64_t M, const float* dout, float* demb,
+                               float* partial, int64_t partial_floats, float* dparams,
+                               void* stream) {
+  clear_error();
+  if (!x3w || reinterpret_cast<uintptr_t>(x3w) % 16 != 0)
+    return set_error(NSVF_EINVAL, "mlp_train_backward_x3: fragments null or not 16-byte aligned");
+  return train_backward("mlp_train_backward_x3", D, H, L, packed, x3w, emb, M, dout, demb,
+                        partial, partial_floats, dparams, stream);
 }
 
 }  // extern "C"

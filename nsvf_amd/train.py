@@ -301,13 +301,58 @@ def pack_mlp_x3(weights, biases):
     return frags, bias
 
 
+_X3T_INDEX = {}
+
+
+def _x3_train_index(shapes, device):
+    """For weights of the given [out, in] shapes (hidden layers, then the
+    head): the position, in [0, W_0 flat, W_1 flat, ...] (0: zero padding),
+    of every element of the split-BF16 backward's fragments in their order
+    -- W_0 .. W_{L-1}, then their transposes -- [fragments * 512] int64."""
+    key = (tuple(shapes), str(device))
+    idx = _X3T_INDEX.get(key)
+    if idx is None:
+        pos, base = [], 1
+        for O, K in shapes:
+            pos.append((np.arange(O * K, dtype=np.float64) + base).reshape(O, K).astype(np.float32))
+            base += O * K
+        if base >= 1 << 24:
+            raise ValueError("split-BF16 training MLP: too many weights")
+        hidden = pos[:-1]
+        parts = [pack_layer_bf16(q, ((q.shape[1] + 31) // 32) * 32).reshape(-1) for q in hidden]
+        parts += [pack_layer_bf16(np.ascontiguousarray(q.T), ((q.shape[0] + 31) // 32) * 32)
+                  .reshape(-1) for q in hidden]
+        idx = torch.from_numpy(np.concatenate(parts).astype(np.int64)).to(device)
+        _X3T_INDEX[key] = idx
+    return idx
+
+
+def pack_mlp_train_x3(weights):
+    """The split-BF16 backward's weight fragments (nsvf_mlp_train_backward_x3):
+    per fragment 512 hi parts bf16(w), then 512 lo parts bf16(w - hi), round
+    to nearest even; an int16 device tensor."""
+    dev = weights[0].device
+    D, H, L = _mlp_shape(weights)
+    idx = _x3_train_index([tuple(w.shape) for w in weights], dev)
+    n = int(lib().nsvf_mlp_train_x3_frags(D, H, L))
+    if n < 0 or idx.numel() != n * 512:
+        raise RuntimeError(f"split-BF16 training backward: shape D={D} H={H} L={L} is not built")
+    flat = torch.cat([torch.zeros(1, dtype=torch.float32, device=dev)]
+                     + [w.detach().reshape(-1).float() for w in weights])
+    vals = flat[idx].view(-1, 512)
+    hi = vals.to(torch.bfloat16)
+    lo = (vals - hi.float()).to(torch.bfloat16)
+    return torch.stack([hi, lo], 1).reshape(-1).view(torch.int16)
+
+
 class FieldMLPx3(FieldMLP):
     """FieldMLP with its forward on the renderer's split-BF16 MLP kernel
     (three BF16 MFMAs per product on hi / lo operand parts, FP32
     accumulation: outputs within about 1e-5 of FP32) -- the gathered
     features written as its operand tiles, the weights packed on the device
-    each call -- and the same FP32-MFMA backward, which recomputes the
-    activations in FP32 from the input."""
+    each call -- and the backward kernel with the activations' recomputation
+    and the input gradients in split BF16 too (the head and the weight
+    gradients' sums over samples in FP32), deterministic like FieldMLP's."""
 
     @staticmethod
     def forward(ctx, emb, *params):
@@ -332,10 +377,33 @@ class FieldMLPx3(FieldMLP):
                   "field_tiles_bf16x3")
             check(L_.nsvf_field_mlp_bf16x3(D, H, L, _p(ws), ws_bytes, _p(m_dev), M, _p(frags),
                                            _p(bias), _p(out), None, st), "field_mlp_bf16x3")
-        ctx.save_for_backward(emb, pack_mlp_train(weights, biases))
+        ctx.save_for_backward(emb, pack_mlp_train(weights, biases), pack_mlp_train_x3(weights))
         ctx.shape = (D, H, L)
         ctx.param_shapes = [tuple(t.shape) for t in params]
         return out
+
+    @staticmethod
+    def backward(ctx, dout):
+        emb, packed, x3w = ctx.saved_tensors
+        D, H, L = ctx.shape
+        L_ = lib()
+        dev = emb.device
+        M = emb.shape[0]
+        dout = dout.contiguous().float()
+        demb = torch.empty((M, D), dtype=torch.float32, device=dev)
+        n_part = int(L_.nsvf_mlp_train_partial_floats(D, H, L, M))
+        n_par = int(L_.nsvf_mlp_train_param_floats(D, H, L))
+        partial = torch.empty(n_part, dtype=torch.float32, device=dev)
+        dparams = torch.empty(n_par, dtype=torch.float32, device=dev)
+        check(L_.nsvf_mlp_train_backward_x3(D, H, L, _p(packed), _p(x3w), _p(emb), M, _p(dout),
+                                            _p(demb), _p(partial), n_part, _p(dparams),
+                                            _stream(dev)), "mlp_train_backward_x3")
+        grads, o = [], 0
+        for shp in ctx.param_shapes:
+            n = int(np.prod(shp))
+            grads.append(dparams[o:o + n].view(shp))
+            o += n
+        return (demb, *grads)
 
 
 class TrainableField(nn.Module):
@@ -344,8 +412,9 @@ class TrainableField(nn.Module):
 
     mlp="hip": the MLP runs forward and backward on the hand-written
     FP32-MFMA kernels (FieldMLP; built for D=32, H=128, L=3 -- other shapes
-    raise); mlp="hip_x3": its forward on the split-BF16 kernel instead
-    (FieldMLPx3, outputs within about 1e-5 of FP32), the same backward;
+    raise); mlp="hip_x3": forward and backward in split BF16 (FieldMLPx3:
+    outputs within about 1e-5 of FP32, gradients about as close; the head
+    and the weight gradients' sums stay FP32);
     mlp="torch": torch.nn.functional.linear with torch autograd, as in the
     reference."""
 

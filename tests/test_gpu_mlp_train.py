@@ -126,21 +126,29 @@ def test_split_bf16_weight_pack_on_the_device(device):
 
 
 @pytest.mark.parametrize("M,seed", [(1, 1), (1000, 2), (128 * 300 + 5, 3)])
-def test_split_bf16_forward(device, M, seed):
+def test_split_bf16_forward_backward(device, M, seed):
     """FieldMLPx3: the features written as the split-BF16 kernel's operand
     tiles (M not a multiple of the tile), its forward within 1e-4 of float64
     (three BF16 products per FP32 one; plain BF16 is ~1e-2 here); its
-    backward is FieldMLP's, so every gradient equals FieldMLP's bit for
-    bit."""
+    backward (recomputation and input gradients in split BF16, the head and
+    the weight-gradient sums in FP32) within 1e-4 of the largest float64
+    entry of each gradient off the ReLU kinks (FP32: 2e-5), and bit-for-bit
+    repeatable."""
     ps = params(seed, device)
     g = torch.Generator().manual_seed(seed)
     emb = torch.randn((M, D), generator=g).mul_(0.3).to(device)
-    dout = torch.randn((M, 4), generator=g).to(device)
+    dout, n_kink = off_kinks(emb, ps, torch.randn((M, 4), generator=g).to(device))
     out, demb, dps = run(emb, ps, dout, FieldMLPx3)
-    r_out = reference(emb, ps, dout)[0]
+    r_out, r_demb, r_dps = reference(emb, ps, dout)
     err = float((out.cpu().double() - r_out).abs().max())
-    print(f"M={M}: split-BF16 forward max abs error {err:.2e}", flush=True)
+    errs = {"demb": close(demb, r_demb, 1e-4)[0]}
+    names = [f"{k}{i}" for i in range(L + 1) for k in ("W", "b")]
+    errs.update({name: close(a, b, 1e-4)[0] for name, a, b in zip(names, dps, r_dps)})
+    print(f"M={M}: split-BF16 forward max abs error {err:.2e}; gradients "
+          + " ".join(f"{k} {v:.1e}" for k, v in errs.items()), flush=True)
     assert err < 1e-4
-    _, demb32, dps32 = run(emb, ps, dout)
-    assert torch.equal(demb, demb32)
-    assert all(torch.equal(a, b) for a, b in zip(dps, dps32))
+    for name, e in errs.items():
+        assert e < 1e-4, (name, e)
+    _, demb2, dps2 = run(emb, ps, dout, FieldMLPx3)
+    assert torch.equal(demb, demb2)
+    assert all(torch.equal(a, b) for a, b in zip(dps, dps2))

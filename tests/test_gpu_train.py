@@ -236,18 +236,20 @@ def test_batch_order_traversal_gives_the_same_step(scene, c3, device):
         assert np.array_equal(grads[k], g), k
 
 
-def test_split_bf16_forward_training_step(scene, c3, device):
-    """A step with the MLP's forward on the split-BF16 kernel (mlp="hip_x3"):
-    image within 1e-4 and loss within 1e-5 (relative) of the FP32 step's,
-    MLP gradients within 1e-3 (relative Frobenius norm; measured 2e-5 to
-    2.4e-4) -- the backward is the same FP32 kernel, so the steps differ only
-    through the forward's outputs.  The compositing's relu(sigma) has a kink
-    at 0: on this batch two of the 525k samples have a raw sigma within the
-    forward's 1e-4 of 0 on opposite sides in the two steps, so their
-    d sigma is 0 in one step and not in the other.  Each feeds only its
-    voxel's 8 corner embeddings, which then hold nearly all of the
-    embedding gradient's difference (1.0e-3 relative); the test takes out
-    the 16 rows that differ most and holds the rest to 1e-4."""
+def test_split_bf16_training_step(scene, c3, device):
+    """A step with the MLP in split BF16 (mlp="hip_x3": the renderer's
+    split-BF16 kernel forward; the backward's recomputation and input
+    gradients in split BF16, the head and weight-gradient sums in FP32),
+    against the FP32 step: image within 1e-4 and loss within 1e-5
+    (relative), MLP gradients within 1e-3 (relative Frobenius norm).
+    Where a pre-activation lies within the two forwards' difference (~1e-6)
+    of a ReLU kink the steps take different sides and that sample's
+    gradient differs by a whole term.  At the compositing's relu(sigma) this
+    happens to two of the 525k samples on this batch, each feeding only its
+    voxel's 8 corner embeddings, which hold most of the embedding gradient's
+    difference (1.2e-3 relative): the test takes out the 16 rows that differ
+    most, and holds the rest -- where the hidden layers' kinks still act,
+    spread over many samples -- to 1e-3."""
     loss, rgb, grads = step_grads(scene, c3, device, mlp="hip_x3")
     print(f"hip_x3 step: loss {loss:.8f} vs {c3['loss']:.8f}, "
           f"max |d rgb| {np.abs(rgb - c3['rgb']).max():.2e}", flush=True)
@@ -262,11 +264,15 @@ def test_split_bf16_forward_training_step(scene, c3, device):
           " ".join(f"{k} {v:.2e}" for k, v in rels.items()), flush=True)
     for k, rel in rels.items():
         if k != "values":
-            assert rel < (1e-4 if k == "values_rest" else 1e-3), (k, rel)
+            assert rel < 1e-3, (k, rel)
     assert rels["values"] < 1e-2
 
 
-def test_training_gradients_match_fp64_central_differences(scene, c3):
+@pytest.fixture(scope="module")
+def fd_checks(scene, c3):
+    """64 parameters of the C3 step with FP64 central differences of the
+    oracle's loss on the same samples at eps 1e-4 and 1e-6: (name, position,
+    difference at 1e-4, difference at 1e-6)."""
     sc, _ = scene
     rs, rd, target = c3["rs"], c3["rd"], c3["target"]
     ref = O.render_view(sc, rs, rd, hierarchical=False)
@@ -287,7 +293,6 @@ def test_training_gradients_match_fp64_central_differences(scene, c3):
     # instead, where FP64 rounding still leaves ~1e-10 relative accuracy.
     EPS, FINE = 1e-4, 1e-6
     rng = np.random.default_rng(4)
-    g = c3["grads"]
     checks = []
     # 48 embedding entries of keys that samples touch
     touched = np.unique(obj.keys)
@@ -299,7 +304,7 @@ def test_training_gradients_match_fp64_central_differences(scene, c3):
             vp[k, d] += eps
             vm[k, d] -= eps
             return (obj.loss(vp, sc.mlp, rows) - obj.loss(vm, sc.mlp, rows)) / (2 * eps) / R3
-        checks.append(("values", (k, d), fd(EPS), fd(FINE), g["values"][k, d]))
+        checks.append(("values", (k, d), fd(EPS), fd(FINE)))
     # 16 MLP entries: 3 weights and 1 bias of each layer
     emb = obj.embed(sc.values)
     n_layers = len(sc.mlp.weights)
@@ -309,13 +314,38 @@ def test_training_gradients_match_fp64_central_differences(scene, c3):
         picks.append(("b", (int(rng.integers(W.shape[0])),)))
         for kind, pos in picks:
             a, b = (obj.mlp_central_difference(emb, sc.mlp, i, kind, pos, e) / R3 for e in (EPS, FINE))
-            checks.append((f"{kind}{i}", pos, a, b, g[f"{kind}{i}"][pos]))
+            checks.append((f"{kind}{i}", pos, a, b))
     assert len(checks) == 64
-    floor = 1e-6 * max(abs(c[3]) for c in checks)
-    tol = lambda ref: 1e-4 * abs(ref) + floor
-    kink = [abs(c[2] - c[3]) > tol(c[3]) for c in checks]
+    return checks
+
+
+def check_against_fd(checks, g, floor_rel):
+    """Every gradient within 1e-4 (relative) + floor_rel of the largest of
+    the FP64 difference: the eps 1e-4 one, or where that interval straddles
+    a kink (the two differences disagree) the eps 1e-6 one."""
+    big = max(abs(c[3]) for c in checks)
+    kink = [abs(c[2] - c[3]) > 1e-4 * abs(c[3]) + 1e-6 * big for c in checks]
     kinked = [c for c, k in zip(checks, kink) if k]
-    bad = [c for c, k in zip(checks, kink) if abs(c[4] - (c[3] if k else c[2])) > tol(c[3])]
-    print(f"{len(kinked)} of 64 parameters sit within 1e-4 of a ReLU kink")
+    got = [g[c[0]][c[1]] for c in checks]
+    diff = [abs(x - (c[3] if k else c[2])) for c, k, x in zip(checks, kink, got)]
+    bad = [(c, x) for c, x, d in zip(checks, got, diff) if d > 1e-4 * abs(c[3]) + floor_rel * big]
+    print(f"{len(kinked)} of 64 parameters sit within 1e-4 of a ReLU kink; largest difference "
+          f"{max(diff) / big:.1e} of the largest gradient", flush=True)
     assert not bad, bad
     assert len(kinked) <= 12, kinked
+
+
+def test_training_gradients_match_fp64_central_differences(fd_checks, c3):
+    check_against_fd(fd_checks, c3["grads"], 1e-6)
+
+
+def test_split_bf16_training_gradients_match_fp64_central_differences(scene, c3, fd_checks,
+                                                                      device):
+    """The split-BF16 step's gradients (mlp="hip_x3") at the same 64
+    parameters against the same FP64 central differences: within 1e-4
+    relative + 5e-5 of the largest of them (the FP32 step: + 1e-6).  Its
+    input-gradient products carry ~2^-16 relative error per term, and a
+    weight gradient that is a small sum of large terms over 525k samples
+    keeps their absolute error (1.8e-5 of the largest gradient measured on
+    a W_0 entry 800x smaller than it)."""
+    check_against_fd(fd_checks, step_grads(scene, c3, device, mlp="hip_x3")[2], 5e-5)
