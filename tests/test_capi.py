@@ -141,6 +141,19 @@ def test_new_entry_points_reject_bad_arguments():
     assert b"null" in L.nsvf_last_error()
     assert L.nsvf_mlp_train_forward(32, 128, 3, None, None, 0, None, None) == 0
     assert L.nsvf_mlp_train_backward(32, 128, 3, None, None, 10, None, None, None, 0, None, None) == -1
+    # split-BF16 backward: 144 fragments (W_0 8, W_1 / W_2 32 each, and the
+    # transposes 8, 32, 32); its fragments must be given and 16-byte aligned
+    assert L.nsvf_mlp_train_x3_frags(32, 128, 3) == 144
+    assert L.nsvf_mlp_train_x3_frags(48, 256, 4) == -1
+    assert L.nsvf_mlp_train_backward_x3(32, 128, 3, None, None, None, 10, None, None, None, 0,
+                                        None, None) == -1
+    assert b"fragments" in L.nsvf_last_error()
+    assert L.nsvf_mlp_train_backward_x3(32, 128, 3, None, 8, None, 10, None, None, None, 0,
+                                        None, None) == -1
+    assert b"aligned" in L.nsvf_last_error()
+    assert L.nsvf_mlp_train_backward_x3(48, 256, 4, None, 16, None, 10, None, None, None, 0,
+                                        None, None) == -1
+    assert b"shape" in L.nsvf_last_error()
     assert L.nsvf_march_begin(None, 10, None, 0, 5, None, None, None, None) == -1
     assert b"chunk" in L.nsvf_last_error()
     assert L.nsvf_march_begin(None, 10, None, 4, 5, None, None, None, None) == -1
