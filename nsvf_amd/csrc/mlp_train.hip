@@ -219,6 +219,76 @@ __device__ __forceinline__ void forward_tile(const float* __restrict__ emb, int6
   z = mfma_chain<T::KTH>(packed + T::F_HEAD * 64, h[L - 1], lane);
 }
 
+// Split-BF16 staging of the weight-gradient operands in LDS: per operand a
+// hi and a lo plane of [feature rows][kXRow] bf16 with the block's 128
+// samples along a row, so an MFMA operand (8 samples of one feature) is
+// one 16-byte read; 136 = 128 + 8 staggers the rows over the banks.
+constexpr int kXRow = 136;
+constexpr int kXPlane = kTrainChunk * kXRow;  // bf16 per plane
+
+// The split of a lane's values v[4 mt + r] (feature 16 mt + 4 g + r of
+// block sample s) for m-tiles mt < NMT, into rows of the planes ph / pl.
+template <int NMT, int N>
+__device__ __forceinline__ void stage_split(unsigned short* ph, unsigned short* pl,
+                                            const float (&v)[N], int g, int s) {
+  static_assert(4 * NMT <= N, "values");
+  // one lane address per plane; the rows are constant offsets from it
+  unsigned short* qh = ph + 4 * g * kXRow + s;
+  unsigned short* ql = pl + 4 * g * kXRow + s;
+#pragma unroll
+  for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      unsigned int hw, lw;
+      split2(v[4 * mt + 2 * p], v[4 * mt + 2 * p + 1], hw, lw);
+      const int row = 16 * mt + 2 * p;
+      qh[row * kXRow] = static_cast<unsigned short>(hw);
+      qh[(row + 1) * kXRow] = static_cast<unsigned short>(hw >> 16);
+      ql[row * kXRow] = static_cast<unsigned short>(lw);
+      ql[(row + 1) * kXRow] = static_cast<unsigned short>(lw >> 16);
+    }
+}
+
+__device__ __forceinline__ bf16x8 lds8(const unsigned short* p) {
+  return *reinterpret_cast<const bf16x8*>(p);
+}
+
+// Weight and bias gradients over the block's samples in split BF16:
+// acc[mi] += sum over samples of A row ra x B row rb0 + 16 mi, and (kBias)
+// accb += the sum of A row ra (its product with a row of ones, whose lo
+// part is 0), with A rows from planes (pah, pal) -- zero where !a_on -- and
+// B rows from (pbh, pbl); the lane reads samples 32 ks + 8 g.. of each
+// 32-sample step.  One step at a time with a fence per m-tile, so that the
+// operand reads are not hoisted into registers the kernel does not have.
+template <int NMI, bool kBias>
+__device__ __forceinline__ void dw_x3(const unsigned short* pah, const unsigned short* pal,
+                                      int ra, const unsigned short* pbh,
+                                      const unsigned short* pbl, int rb0, int g, bool a_on,
+                                      f32x4 (&acc)[NMI], f32x4& accb) {
+  const bf16x8 zero = {};
+  const bf16x8 ones = __builtin_bit_cast(
+      bf16x8, u32x4{0x3F803F80u, 0x3F803F80u, 0x3F803F80u, 0x3F803F80u});
+#pragma unroll 1
+  for (int ks = 0; ks < kTrainChunk / 32; ++ks) {
+    const int off = 32 * ks + 8 * g;
+    const bf16x8 ah = a_on ? lds8(pah + ra * kXRow + off) : zero;
+    const bf16x8 al = a_on ? lds8(pal + ra * kXRow + off) : zero;
+#pragma unroll
+    for (int mi = 0; mi < NMI; ++mi) {
+      const int rb = rb0 + 16 * mi;
+      const bf16x8 bh = lds8(pbh + rb * kXRow + off), bl = lds8(pbl + rb * kXRow + off);
+      acc[mi] = mfma32(al, bh, acc[mi]);
+      acc[mi] = mfma32(ah, bl, acc[mi]);
+      acc[mi] = mfma32(ah, bh, acc[mi]);
+      tile_fence();
+    }
+    if (kBias) {
+      accb = mfma32(al, ones, accb);
+      accb = mfma32(ah, ones, accb);
+    }
+  }
+}
+
 // ReLU as an integer max on the float's bits (the split-BF16 MLP kernel's).
 __device__ __forceinline__ float relu_bits(float x) {
   return __int_as_float(max(__float_as_int(x), 0));
@@ -346,6 +416,12 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
 
   float* my_dz = lds_dz + (w * 16 + col) * kLdsRow;
   float* my_x = lds_x + (w * 16 + col) * kLdsRow;
+  // split-BF16 planes (X3): dz hi, dz lo, x hi, x lo
+  unsigned short* pzh = reinterpret_cast<unsigned short*>(smem_raw);
+  unsigned short* pzl = pzh + kXPlane;
+  unsigned short* pxh = pzl + kXPlane;
+  unsigned short* pxl = pxh + kXPlane;
+  const int sb = w * 16 + col;  // this lane's sample in the block
   const int64_t chunks = (M + kTrainChunk - 1) / kTrainChunk;
   for (int64_t ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
     const int64_t s = ch * kTrainChunk + w * 16 + col;
@@ -372,20 +448,36 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
       dl[3] = d.w * (y3 * (1.0f - y3));
     }
     // ---- head: dW_head = dl^T h[L-1], dh = W_head^T dl
-    if (g == 0) *reinterpret_cast<f32x4*>(my_dz) = dl;
+    if constexpr (X3) {
+      if (g == 0) {
+        const float dlv[4] = {dl[0], dl[1], dl[2], dl[3]};
+        stage_split<1>(pzh, pzl, dlv, 0, sb);
+      }
+      stage_split<T::MT>(pxh, pxl, h[L - 1], g, sb);
+      __syncthreads();
+      f32x4 ghv[1] = {gh};
+      if (w == 0)
+        dw_x3<1, true>(pzh, pzl, col, pxh, pxl, 16 * w + col, g, col < 4, ghv, ghb);
+      else
+        dw_x3<1, false>(pzh, pzl, col, pxh, pxl, 16 * w + col, g, col < 4, ghv, ghb);
+      gh = ghv[0];
+      __syncthreads();
+    } else {
+      if (g == 0) *reinterpret_cast<f32x4*>(my_dz) = dl;
 #pragma unroll
-    for (int mt = 0; mt < T::MT; ++mt)
-      *reinterpret_cast<f32x4*>(my_x + 16 * mt + 4 * g) =
-          f32x4{h[L - 1][4 * mt], h[L - 1][4 * mt + 1], h[L - 1][4 * mt + 2], h[L - 1][4 * mt + 3]};
-    __syncthreads();
+      for (int mt = 0; mt < T::MT; ++mt)
+        *reinterpret_cast<f32x4*>(my_x + 16 * mt + 4 * g) = f32x4{
+            h[L - 1][4 * mt], h[L - 1][4 * mt + 1], h[L - 1][4 * mt + 2], h[L - 1][4 * mt + 3]};
+      __syncthreads();
 #pragma unroll 1
-    for (int kk = 0; kk < kTrainChunk / 4; ++kk) {
-      const int row = (4 * kk + g) * kLdsRow;
-      const float a = col < 4 ? lds_dz[row + col] : 0.0f;
-      gh = mfma4(a, lds_x[row + 16 * w + col], gh);
-      if (w == 0) ghb = mfma4(a, 1.0f, ghb);
+      for (int kk = 0; kk < kTrainChunk / 4; ++kk) {
+        const int row = (4 * kk + g) * kLdsRow;
+        const float a = col < 4 ? lds_dz[row + col] : 0.0f;
+        gh = mfma4(a, lds_x[row + 16 * w + col], gh);
+        if (w == 0) ghb = mfma4(a, 1.0f, ghb);
+      }
+      __syncthreads();
     }
-    __syncthreads();
     float dh[T::KTH];
     {
       // operand B of W_head^T: lane group g takes logit g of sample c
@@ -408,10 +500,14 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
       float dz[T::KTH];
 #pragma unroll
       for (int k = 0; k < T::KTH; ++k) dz[k] = h[l][k] > 0.0f ? dh[k] : 0.0f;
+      if constexpr (X3) {
+        stage_split<T::MT>(pzh, pzl, dz, g, sb);
+      } else {
 #pragma unroll
-      for (int mt = 0; mt < T::MT; ++mt)
-        *reinterpret_cast<f32x4*>(my_dz + 16 * mt + 4 * g) =
-            f32x4{dz[4 * mt], dz[4 * mt + 1], dz[4 * mt + 2], dz[4 * mt + 3]};
+        for (int mt = 0; mt < T::MT; ++mt)
+          *reinterpret_cast<f32x4*>(my_dz + 16 * mt + 4 * g) =
+              f32x4{dz[4 * mt], dz[4 * mt + 1], dz[4 * mt + 2], dz[4 * mt + 3]};
+      }
       if (l > 0) {
         // the first layer's output again, from x0: keeping it from the
         // forward pass would hold 32 more registers through every later phase
@@ -421,21 +517,37 @@ __global__ __launch_bounds__(kTrainBlock) void mlp_train_backward_kernel(
           else
             dense_relu<T::KT0, T::MT>(opaque(packed), x0, bias, h[0], lane, g);
         }
+        if constexpr (X3) {
+          stage_split<T::MT>(pxh, pxl, h[l - 1], g, sb);
+        } else {
 #pragma unroll
-        for (int mt = 0; mt < T::MT; ++mt)
-          *reinterpret_cast<f32x4*>(my_x + 16 * mt + 4 * g) =
-              f32x4{h[l - 1][4 * mt], h[l - 1][4 * mt + 1], h[l - 1][4 * mt + 2],
-                    h[l - 1][4 * mt + 3]};
+          for (int mt = 0; mt < T::MT; ++mt)
+            *reinterpret_cast<f32x4*>(my_x + 16 * mt + 4 * g) =
+                f32x4{h[l - 1][4 * mt], h[l - 1][4 * mt + 1], h[l - 1][4 * mt + 2],
+                      h[l - 1][4 * mt + 3]};
+        }
       } else {
+        if constexpr (X3) {
+          stage_split<T::MT0>(pxh, pxl, x0, g, sb);
+        } else {
 #pragma unroll
-        for (int t = 0; t < T::MT0; ++t)
-          *reinterpret_cast<f32x4*>(my_x + 16 * t + 4 * g) =
-              f32x4{x0[4 * t], x0[4 * t + 1], x0[4 * t + 2], x0[4 * t + 3]};
+          for (int t = 0; t < T::MT0; ++t)
+            *reinterpret_cast<f32x4*>(my_x + 16 * t + 4 * g) =
+                f32x4{x0[4 * t], x0[4 * t + 1], x0[4 * t + 2], x0[4 * t + 3]};
+        }
       }
       __syncthreads();
-      // weight gradient rows 16w..16w+15 of layer l, 4 samples per step
+      // weight gradient rows 16w..16w+15 of layer l: split BF16 over the
+      // block's 128 samples, or FP32 4 samples per step
+      if constexpr (X3) {
+        if (l > 0)
+          dw_x3<T::MT, true>(pzh, pzl, 16 * w + col, pxh, pxl, col, g, true,
+                             gw[l > 0 ? l - 1 : 0], gb[l]);
+        else
+          dw_x3<T::MT0, true>(pzh, pzl, 16 * w + col, pxh, pxl, col, g, true, gw0, gb[l]);
+      }
 #pragma unroll 1
-      for (int kk = 0; kk < kTrainChunk / 4; ++kk) {
+      for (int kk = 0; kk < (X3 ? 0 : kTrainChunk / 4); ++kk) {
         const int row = (4 * kk + g) * kLdsRow;
         const float a = lds_dz[row + 16 * w + col];
         if (l > 0) {
@@ -593,6 +705,7 @@ int backward_blocks(int64_t M) {
 }
 
 constexpr size_t kBackwardLds = size_t(2) * kTrainChunk * kLdsRow * sizeof(float);
+constexpr size_t kBackwardLdsX3 = size_t(4) * kXPlane * sizeof(unsigned short);
 
 bool train_shape_ok(int D, int H, int L) { return D == 32 && H == 128 && L == 3; }
 
@@ -672,7 +785,7 @@ static int train_backward(const char* name, int D, int H, int L, const float* pa
   const bf16x8* x3 = reinterpret_cast<const bf16x8*>(x3w);
   if (x3w)
     hipLaunchKernelGGL((mlp_train_backward_kernel<32, 128, 3, true>), dim3(nb),
-                       dim3(kTrainBlock), kBackwardLds, st, emb, M, packed, x3, dout, demb,
+                       dim3(kTrainBlock), kBackwardLdsX3, st, emb, M, packed, x3, dout, demb,
                        partial);
   else
     hipLaunchKernelGGL((mlp_train_backward_kernel<32, 128, 3, false>), dim3(nb),
