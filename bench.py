@@ -601,7 +601,7 @@ def rows(args, dev, sc, dsc):
     # with the HIP training MLP, and with torch's Linear layers for reference
     target = torch.rand((R, 3), device=dev)
     mlp_names = {"hip": "HIP FP32 MFMA",
-                 "hip_x3": "HIP split BF16 (FP32 head and weight-gradient sums)",
+                 "hip_x3": "HIP split BF16 (the backward's head in FP32)",
                  "torch": "torch FP32 nn.Linear"}
     for key, impl in (("train_step", "hip"), ("train_step_split_bf16", "hip_x3"),
                       ("train_step_torch_mlp", "torch")):

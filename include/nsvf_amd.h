@@ -639,10 +639,11 @@ int nsvf_mlp_train_backward(int D, int H, int L, const float* packed,
                             float* demb, float* partial, int64_t partial_floats,
                             float* dparams, void* stream);
 
-/* The same backward with the activations' recomputation and the input
- * gradients W^T dz in split BF16 (three BF16 MFMAs per product on hi / lo
- * operand parts, hi = bf16(f), lo = bf16(f - hi), FP32 accumulation: about
- * FP32's accuracy); the head and the weight-gradient sums stay FP32.  No
+/* The same backward with the activations' recomputation, the input
+ * gradients W^T dz and the parameter gradients in split BF16 (three BF16
+ * MFMAs per product on hi / lo operand parts, hi = bf16(f), lo = bf16(f -
+ * hi), FP32 accumulation: about FP32's accuracy); the head's recomputed
+ * logits and input gradient stay FP32; still deterministic.  No
  * counterpart in the reference (its backward is torch autograd over FP32
  * Linear layers).  x3w: x3_frags(D, H, L) fragments of 1024 bf16 bit
  * patterns (512 hi, then 512 lo), 16-byte aligned, in the order W_0 [H/16

@@ -29,8 +29,9 @@
 // atomics.
 //
 // Split-BF16 backward (mlp_train_backward_kernel<..., true>).  The same
-// kernel with the forward recomputation of the hidden layers and the input
-// gradients dX^T = W^T dz^T on v_mfma_f32_16x16x32_bf16 in split BF16: an
+// kernel with the forward recomputation of the hidden layers, the input
+// gradients dX^T = W^T dz^T and the weight and bias gradients on
+// v_mfma_f32_16x16x32_bf16 in split BF16: an
 // FP32 operand f is hi + lo with hi = bf16(f), lo = bf16(f - hi), and a
 // product is Wl Xh + Wh Xl + Wh Xh (FP32 accumulation; the dropped Wl Xl is
 // 2^-16 of it), so these products keep about FP32's accuracy at three BF16
@@ -39,8 +40,12 @@
 // 288 KB that stay in L2; a k-tile's operand lane (g, c) holds features
 // 32 kt + 4 g + i and 32 kt + 16 + 4 g + i (i < 4) of sample c -- the
 // accumulators of m-tiles 2 kt and 2 kt + 1, so a layer's FP32 output
-// splits into the next layer's operands in registers.  The head (4 wide)
-// and the weight-gradient sums over samples stay in FP32.
+// splits into the next layer's operands in registers.  The weight
+// gradients sum over samples, the MFMA's k: dz and the layer input are
+// staged in LDS as hi and lo planes of [feature][sample] bf16, so an
+// operand (8 samples of one feature) is one 16-byte read, and each wave
+// keeps its rows of every dW in registers as in the FP32 kernel.  The
+// head's recomputed logits and its input gradient (4 wide) stay in FP32.
 #include <algorithm>
 
 #include "common.h"

@@ -350,9 +350,10 @@ class FieldMLPx3(FieldMLP):
     (three BF16 MFMAs per product on hi / lo operand parts, FP32
     accumulation: outputs within about 1e-5 of FP32) -- the gathered
     features written as its operand tiles, the weights packed on the device
-    each call -- and the backward kernel with the activations' recomputation
-    and the input gradients in split BF16 too (the head and the weight
-    gradients' sums over samples in FP32), deterministic like FieldMLP's."""
+    each call -- and the backward kernel with the activations'
+    recomputation, the input gradients and the parameter gradients in split
+    BF16 too (its head's recomputed logits and input gradient in FP32),
+    deterministic like FieldMLP's."""
 
     @staticmethod
     def forward(ctx, emb, *params):
@@ -413,8 +414,9 @@ class TrainableField(nn.Module):
     mlp="hip": the MLP runs forward and backward on the hand-written
     FP32-MFMA kernels (FieldMLP; built for D=32, H=128, L=3 -- other shapes
     raise); mlp="hip_x3": forward and backward in split BF16 (FieldMLPx3:
-    outputs within about 1e-5 of FP32, gradients about as close; the head
-    and the weight gradients' sums stay FP32);
+    outputs within about 1e-5 of FP32, gradients about as close; the
+    backward's head -- recomputed logits and their input gradient -- stays
+    FP32);
     mlp="torch": torch.nn.functional.linear with torch autograd, as in the
     reference."""
 

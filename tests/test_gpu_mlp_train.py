@@ -130,10 +130,10 @@ def test_split_bf16_forward_backward(device, M, seed):
     """FieldMLPx3: the features written as the split-BF16 kernel's operand
     tiles (M not a multiple of the tile), its forward within 1e-4 of float64
     (three BF16 products per FP32 one; plain BF16 is ~1e-2 here); its
-    backward (recomputation and input gradients in split BF16, the head and
-    the weight-gradient sums in FP32) within 1e-4 of the largest float64
-    entry of each gradient off the ReLU kinks (FP32: 2e-5), and bit-for-bit
-    repeatable."""
+    backward (recomputation, input and parameter gradients in split BF16;
+    the head's recomputed logits and input gradient in FP32) within 1e-4 of
+    the largest float64 entry of each gradient off the ReLU kinks (FP32:
+    2e-5), and bit-for-bit repeatable."""
     ps = params(seed, device)
     g = torch.Generator().manual_seed(seed)
     emb = torch.randn((M, D), generator=g).mul_(0.3).to(device)

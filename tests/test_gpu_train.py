@@ -239,7 +239,7 @@ def test_batch_order_traversal_gives_the_same_step(scene, c3, device):
 def test_split_bf16_training_step(scene, c3, device):
     """A step with the MLP in split BF16 (mlp="hip_x3": the renderer's
     split-BF16 kernel forward; the backward's recomputation and input
-    gradients in split BF16, the head and weight-gradient sums in FP32),
+    and parameter gradients in split BF16, the head's recomputation in FP32),
     against the FP32 step: image within 1e-4 and loss within 1e-5
     (relative), MLP gradients within 1e-3 (relative Frobenius norm).
     Where a pre-activation lies within the two forwards' difference (~1e-6)
