@@ -628,7 +628,7 @@ def rows(args, dev, sc, dsc):
         torch.cuda.empty_cache()
     # the training configuration as specified: the shell scene, 4 views x
     # 2048 random pixels (64x64 views), with and without the traversal's
-    # coherent grouping of the batch's rays
+    # coherent grouping of the batch's rays, and with the split-BF16 MLP
     shell = S.shell_scene()
     dshell = DeviceScene(shell, dev)
     rng = np.random.default_rng(7)
@@ -642,8 +642,9 @@ def rows(args, dev, sc, dsc):
     crd = torch.from_numpy(np.concatenate([q[1] for q in parts])[mix]).to(dev)
     ctarget = torch.rand((crs.shape[0], 3), device=dev)
     c3 = {"rays": int(crs.shape[0]), "scene": f"shell, {shell.num_voxels} voxels"}
-    for key, coherent in (("ms", True), ("ms_rays_in_batch_order", False)):
-        tf = TrainableField(shell, dev)
+    for key, coherent, impl in (("ms", True, "hip"), ("ms_rays_in_batch_order", False, "hip"),
+                                ("ms_split_bf16", True, "hip_x3")):
+        tf = TrainableField(shell, dev, mlp=impl)
         tr = TrainRenderer(dev, coherent=coherent)
 
         def step():
