@@ -194,3 +194,33 @@ def test_split_bf16_pack_from_torch_weights():
     host = pack_mlp(FieldMLP(W, B))
     frags, bias = pack_mlp_x3([torch.from_numpy(w) for w in W], [torch.from_numpy(b) for b in B])
     assert torch.equal(frags, host["bf16x3"]) and torch.equal(bias, host["bias"])
+
+
+def test_split_bf16_training_backward_fragments():
+    """pack_mlp_train_x3 against the layout nsvf_mlp_train_backward_x3
+    documents, element by element (no GPU: the packer's torch ops on CPU
+    tensors): fragments W_0 [8][1], W_1, W_2 [8][4], W_0^T [2][4], W_1^T,
+    W_2^T [8][4], each 512 hi then 512 lo; lane l, element i of fragment
+    (mt, kt) of A is A[16 mt + l % 16][32 kt + k(i)], k(i) = 4 (l / 16) + i
+    for i < 4, 16 + 4 (l / 16) + i - 4 otherwise; hi = bf16(a) (nearest
+    even), lo = bf16(a - hi)."""
+    import torch
+    from nsvf_amd.train import pack_mlp_train_x3
+    rng = np.random.default_rng(9)
+    shapes = [(128, 32), (128, 128), (128, 128), (4, 128)]
+    W = [rng.standard_normal(s).astype(np.float32) for s in shapes]
+    got = pack_mlp_train_x3([torch.from_numpy(w) for w in W]).numpy().view(np.uint16)
+    mats = [W[0], W[1], W[2], W[0].T, W[1].T, W[2].T]
+    want = []
+    lane, i = np.arange(64)[:, None], np.arange(8)[None, :]
+    kk = np.where(i < 4, 4 * (lane // 16) + i, 16 + 4 * (lane // 16) + i - 4)
+    for A in mats:
+        A = np.ascontiguousarray(A)
+        for mt in range(A.shape[0] // 16):
+            for kt in range(A.shape[1] // 32):
+                v = A[16 * mt + lane % 16, 32 * kt + kk]            # [64, 8]
+                hi = bf16_round(v)
+                want += [bf16_bits(hi).reshape(-1), bf16_bits(v - hi).reshape(-1)]
+    want = np.concatenate(want)
+    assert got.shape == want.shape == (144 * 1024,)
+    assert np.array_equal(got, want)
